@@ -1,0 +1,167 @@
+/*
+ * pam.h -- C ABI of libpam, the MI355X (gfx950) 2D finite-difference engine:
+ * 8th-order acoustic / pseudo-acoustic VTI / TTI time stepping with Cerjan or
+ * CPML absorbing boundaries, source injection and receiver gathering, RTM
+ * cross-correlation imaging and the FWI adjoint-state gradient.
+ *
+ * Every entry point takes plain pointers and sizes.  Array arguments named
+ * "device" are HIP device pointers owned by the caller (for example a torch
+ * tensor's data_ptr()); the library owns its scratch (wavefields, CPML memory
+ * variables, snapshot pools).  All work is enqueued on the context's stream;
+ * entry points that return results to the host synchronise that stream.
+ * Return value: PAM_OK (0) or a negative PAM_E_* code; pam_last_error() then
+ * holds a message.  Nothing aborts the process.  One context must not be used
+ * from two threads at once; separate contexts may.
+ *
+ * Grid conventions (the reference's): the interior model is [nz][nx] floats,
+ * x fastest; the padded grid adds n_abc cells on every side,
+ * nzp = nz + 2 n_abc, nxp = nx + 2 n_abc.  Source and receiver positions are
+ * padded-grid cell indices (iz, ix), as the reference computes them
+ * (cpp/Survey.cpp:150-158, src/survey.py:131-138).  Seismograms are
+ * [nt_data][nrec] per shot, nt_data = nt - itlag; images and gradients are
+ * interior [nz][nx].
+ */
+#ifndef PAM_H
+#define PAM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PAM_OK 0
+#define PAM_E_ARG (-1)         /* invalid argument or size */
+#define PAM_E_HIP (-2)         /* a HIP runtime call failed */
+#define PAM_E_STATE (-3)       /* call out of order (e.g. no model set) */
+#define PAM_E_NOMEM (-4)       /* device allocation failed */
+#define PAM_E_UNSUPPORTED (-5) /* combination not provided */
+#define PAM_E_NONFINITE (-6)   /* wavefield blew up (stability check) */
+
+enum pam_physics { PAM_ACOUSTIC = 0, PAM_VTI = 1, PAM_TTI = 2 };
+enum pam_abc { PAM_CERJAN = 0, PAM_CPML = 1 };
+enum pam_layout { PAM_INTERIOR = 0, PAM_PADDED = 1 };
+
+/* source-wavefield strategy of the adjoint passes (rtm / gradient) */
+enum pam_storage {
+    PAM_STORE_ALL_F32 = 0,  /* every step saved, float32 */
+    PAM_STORE_ALL_F16 = 1,  /* every step saved, float16 with a per-snapshot scale */
+    PAM_STORE_ALL_BF16 = 2, /* every step saved, bfloat16 with a per-snapshot scale */
+    PAM_STORE_CHECKPOINT = 3 /* uniform checkpoints, segments recomputed (exact) */
+};
+
+/* keys of pam_set_option */
+enum pam_option {
+    PAM_OPT_BATCH = 1,     /* shots advanced together in one launch (default: all) */
+    PAM_OPT_TILE = 2,      /* kernel variant index from pam_tile_count(); -1 = auto */
+    PAM_OPT_GRAPH = 3,     /* 1: replay the step loop as captured hipGraphs (default 1) */
+    PAM_OPT_STORAGE = 4,   /* enum pam_storage for rtm / gradient (default ALL_F32) */
+    PAM_OPT_CKPT_STEP = 5, /* checkpoint interval for PAM_STORE_CHECKPOINT */
+    PAM_OPT_RESET_ADJ_CPML = 6, /* 1 (default): clear CPML memory before the
+                                   backward pass; 0: keep the forward pass's
+                                   values as src/Migration.py does */
+    PAM_OPT_NORMALIZE = 7  /* image/gradient normalisation: 0 none, 1 ilum+1e-5 max
+                              (src/Migration.py:1086, default), 2 ilum>1e-12
+                              (cpp/Migration.cu:470-484) */
+};
+
+typedef struct pam_grid {
+    int nz, nx;       /* interior model size */
+    int n_abc;        /* absorbing width on every side */
+    float dz, dx, dt; /* spacing [m], time step [s] */
+    int nt;           /* total time steps = itlag + nt_data */
+    int itlag;        /* first recorded step */
+} pam_grid;
+
+typedef struct pam_abc_params {
+    float cerjan_factor; /* A[i] = exp(-(factor (N-i))^2); <= 0: the reference's
+                            factor 1/(1.4142*3N) (cpp/Modeling.cu:111-120) */
+    float cpml_d0;       /* -(M+1) ln R, src/Modeling2D.py:120-126 */
+    float cpml_f_pico;   /* fcut/3 */
+} pam_abc_params;
+
+typedef struct pam_ctx pam_ctx;
+
+const char *pam_version(void);
+
+/* Replaces the constructors Modeling(Survey*) (cpp/Modeling.cuh:14) and
+ * wavefield(parameters) (src/Modeling2D.py:23-25) plus initializeFields
+ * (cpp/Modeling.cu:40-82): allocates the wavefields of up to max_shots shots
+ * advanced together.  hip_stream may be NULL (a private stream is created). */
+int pam_create(const pam_grid *grid, int physics, int abc, const pam_abc_params *abc_params,
+               void *hip_stream, int max_shots, pam_ctx **out);
+/* Replaces Modeling::freeMemory (cpp/Modeling.cu:13-38). */
+int pam_destroy(pam_ctx *ctx);
+const char *pam_last_error(const pam_ctx *ctx);
+int pam_set_option(pam_ctx *ctx, int key, long long value);
+/* number of kernel variants (tile shapes) the autotuner may choose from */
+int pam_tile_count(void);
+
+/* Replaces Modeling::setModel / expandModel (cpp/Modeling.cu:130-182,329-382)
+ * and wavefield.ExpandModel (src/Modeling2D.py:38-54): device float arrays,
+ * interior [nz][nx] (expanded by edge replication) or padded [nzp][nxp].
+ * eps/delta are required for VTI and TTI, theta (radians) for TTI. */
+int pam_set_model(pam_ctx *ctx, const float *vp, const float *eps, const float *delta,
+                  const float *theta, int layout);
+/* Replaces Modeling::createWavelet (cpp/Modeling.cu:84-100) /
+ * wavefield.createSourceWavelet (src/Modeling2D.py:27-31): device [nt]. */
+int pam_set_wavelet(pam_ctx *ctx, const float *src);
+/* Replaces the geometry part of Survey (cpp/Survey.cpp:128-176): host arrays of
+ * padded-grid indices, nshot sources and nrec receivers shared by all shots. */
+int pam_set_geometry(pam_ctx *ctx, const int *sz, const int *sx, int nshot, const int *rz,
+                     const int *rx, int nrec);
+
+/* Per-step interface, mirroring Modeling::forward_step(k) (cpp/Modeling.cu:421-431)
+ * and wavefield.forward_stepGPU(k) + store_seismogram (src/Modeling2D.py:292-300,
+ * 363-386) for one shot held in slot 0.  pam_reset clears the slot's fields for
+ * a shot; pam_forward_step advances one step (injection, update, absorbing
+ * boundary, receiver gather into seis row k-itlag when k >= itlag and seis is
+ * not NULL).  Steps must be issued in order k = 0, 1, ... */
+int pam_reset(pam_ctx *ctx, int shot);
+int pam_forward_step(pam_ctx *ctx, int k, float *seis /* device [nt_data][nrec] or NULL */);
+/* Copies a padded [nzp][nxp] field of slot 0 to a device buffer.  which: 0 the
+ * current field c_k (after the injection of step k), 1 the previous field. */
+int pam_get_field(pam_ctx *ctx, int which, float *out);
+
+/* Whole-run forward modeling, replacing Modeling::solveWaveEquation
+ * (cpp/Modeling.cu:433-479) and wavefield.solveWaveEquationGPU
+ * (src/Modeling2D.py:423-467): shots [shot0, shot0+nshots) -> device seis
+ * [nshots][nt_data][nrec].  Shots are advanced PAM_OPT_BATCH at a time. */
+int pam_forward(pam_ctx *ctx, int shot0, int nshots, float *seis);
+
+/* Whole-run RTM, replacing Migration::solveReverseTimeMigrationOntheFly
+ * (cpp/Migration.cu:359-397) and migration.solveBackwardWaveEquationOntheFlyGPU
+ * with fwi=False (src/Migration.py:963-1111): data is the (muted) observed data
+ * [nshots][nt_data][nrec] (device).  image and ilum (device [nz][nx]) receive the
+ * sums over the shots; image is normalised per PAM_OPT_NORMALIZE. */
+int pam_rtm(pam_ctx *ctx, int shot0, int nshots, const float *data, float *image, float *ilum);
+
+/* FWI gradient with fwi=True semantics (src/Migration.py:1039-1092,
+ * src/Inversion.py:55-86): residual = dobs - dcal [nshots][nt_data][nrec].
+ * g_m receives the normalised gradient w.r.t. m = 1/vp^2; g_eps, g_delta,
+ * g_theta (may be NULL) the multiparameter partials of src/CudaKernels.py:1174-1379.
+ * ilum may be NULL. */
+int pam_gradient(pam_ctx *ctx, int shot0, int nshots, const float *residual, float *g_m,
+                 float *g_eps, float *g_delta, float *g_theta, float *ilum);
+
+/* Misfit X = sum_shots 1/2 sum (dobs - dcal)^2 (src/Inversion.py:13-53) with a
+ * float64 reduction; also writes residual = dobs - dcal when residual != NULL.
+ * dobs/dcal/residual are device [nshots][nt_data][nrec]. */
+int pam_misfit(pam_ctx *ctx, int nshots, const float *dobs, const float *dcal, float *residual,
+               double *misfit);
+
+/* Exact discrete adjoint of pam_forward (acoustic physics): given
+ * dseis = dJ/dseis [nshots][nt_data][nrec] (device), accumulates
+ * dJ/dvp into g_vp (device [nz][nx] interior, or padded [nzp][nxp] when
+ * layout = PAM_PADDED).  This is the gradient a torch.autograd.Function
+ * returns; it has no counterpart in the reference, whose gradient
+ * (pam_gradient) is preconditioned and normalised. */
+int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const float *dseis, float *g_vp,
+                        int layout);
+
+/* Elapsed GPU milliseconds of the last whole-run call's step loop, and the
+ * average duration of its dominant stencil kernel (both from HIP events). */
+int pam_last_timing(const pam_ctx *ctx, double *loop_ms, double *kernel_ms, long long *kernel_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PAM_H */

@@ -1,0 +1,439 @@
+"""CPU oracle for the 2D finite-difference engine -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg
+import this module, and only as the checker.  The product package never does.
+
+Two layers:
+
+* ctypes bindings to ``pam_oracle.c`` built three ways (``f64``: float64
+  storage and arithmetic; ``f32``: float32 storage and arithmetic without FMA
+  contraction; ``mixed``: float32 storage, float64 arithmetic, OpenMP over rows
+  -- the Numba kernels' numerics and loop structure, used as the timed CPU
+  baseline);
+* numpy restatements of the small host-side pieces (wavelet, mute, smoothing,
+  stability check, derived sizes) and whole-run drivers (forward shot, RTM and
+  FWI gradient, on-the-fly strategy) that follow the reference's Python loops.
+
+Every function cites the reference file:line it restates.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS: dict = {}
+
+
+def build() -> None:
+    """Compile the three oracle libraries (gcc, make)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _lib(kind: str):
+    if kind not in _LIBS:
+        path = os.path.join(_HERE, f"liboracle_{kind}.so")
+        if not os.path.exists(path):
+            build()
+        lib = ctypes.CDLL(path)
+        _LIBS[kind] = lib
+    return _LIBS[kind]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+# --------------------------------------------------------------------------
+# grid description and derived sizes
+# --------------------------------------------------------------------------
+@dataclass
+class Grid:
+    nz: int
+    nx: int
+    N: int
+    dz: float
+    dx: float
+    dt: float
+    nt: int
+    itlag: int = 0
+
+    @property
+    def nzp(self):
+        return self.nz + 2 * self.N
+
+    @property
+    def nxp(self):
+        return self.nx + 2 * self.N
+
+    @property
+    def nt_data(self):
+        return self.nt - self.itlag
+
+
+def derived_sizes(L, D, T, dx, dz, dt, fcut, rounding="round"):
+    """Derived sizes of cpp/Survey.cpp:101-108 (``round``) and
+    src/survey.py:37-47 (``trunc``): tlag = 2 sqrt(pi)/fcut, itlag, nt_data,
+    nx, nz, nt = itlag + nt_data."""
+    if rounding == "round":
+        f32 = np.float32
+        cround = lambda v: int(np.floor(abs(float(v)) + 0.5)) * (1 if v >= 0 else -1)  # noqa: E731
+        tlag = f32(2.0) * np.sqrt(f32(3.14159265358979323846)) / f32(fcut)
+        itlag = cround(f32(tlag) / f32(dt))
+        nt_data = cround(f32(T) / f32(dt)) + 1
+        nx = cround(f32(L) / f32(dx)) + 1
+        nz = cround(f32(D) / f32(dz)) + 1
+    else:
+        tlag = 2.0 * np.sqrt(np.pi) / fcut
+        itlag = int(tlag / dt)
+        nt_data = int(T / dt) + 1
+        nx = int(L / dx) + 1
+        nz = int(D / dz) + 1
+    return dict(tlag=float(tlag), itlag=itlag, nt_data=nt_data, nx=nx, nz=nz, nt=itlag + nt_data)
+
+
+# --------------------------------------------------------------------------
+# numpy restatements of host-side pieces
+# --------------------------------------------------------------------------
+def ricker_cpp(nt, dt, fcut, tlag, dx, dz):
+    """Plain Ricker of cpp/Modeling.cu:84-100, float32 arithmetic:
+    (1-2*pi^3 f^2 td^2) exp(-pi^3 f^2 td^2) / (dx dz), f = fcut/(3 sqrt(pi))."""
+    f32 = np.float32
+    pi = f32(3.14159265358979323846)
+    scale = f32(1.0) / (f32(dx) * f32(dz))
+    fc = f32(fcut) / (f32(3.0) * np.sqrt(pi, dtype=np.float32))
+    n = np.arange(nt, dtype=np.float32)
+    td = n * f32(dt) - f32(tlag)
+    arg = pi * pi * pi * fc * fc * td * td
+    return ((f32(1.0) - f32(2.0) * arg) * np.exp(-arg) * scale).astype(np.float32)
+
+
+def ricker_py(nt, dt, fcut, tlag, dx, dz):
+    """Ricker of src/utils.py:21-28 with the sqrt(i w) half-derivative filter,
+    scaled by 1/(dx dz) as src/Modeling2D.py:27-31; float64, cast to float32
+    when uploaded (src/Modeling2D.py:427)."""
+    pi = np.pi
+    f = fcut / (3 * np.sqrt(pi))
+    t = np.arange(nt) * dt
+    td = t - tlag
+    src = (1 - 2 * pi * (pi * f * td) ** 2) * np.exp(-pi * (pi * f * td) ** 2)
+    w = 2.0 * pi * np.fft.fftfreq(nt, dt)
+    src = np.real(np.fft.ifft(np.fft.fft(src) * np.sqrt(1j * w)))
+    return (src / (dx * dz)).astype(np.float32)
+
+
+def cerjan_vector(N, factor=None):
+    """src/Modeling2D.py:258-264 (float64) -- factor None: the reference's
+    fb = (N-i)/(1.4142*3N); otherwise fb = factor*(N-i)."""
+    i = np.arange(N, dtype=np.float64)
+    if factor is None or factor <= 0:
+        fb = (N - i) / (1.4142 * 3.0 * N)
+    else:
+        fb = factor * (N - i)
+    return np.exp(-fb * fb)
+
+
+def dampening_const(N, fcut):
+    """src/Modeling2D.py:110-126: R = 1e-3^(N/10) (N >= 200: 1e-3^15),
+    d0 = -(M+1) ln R with M = 2, f_pico = fcut/3."""
+    R = 1e-3 ** (N / 10)
+    if N >= 200:
+        R = 1e-3 ** (150 / 10)
+    return -(2 + 1) * np.log(R), fcut / 3
+
+
+def expand_model(m, N):
+    """src/Modeling2D.py:38-54 (edge replication; corners take corner values)."""
+    return np.pad(np.asarray(m, dtype=np.float32), N, mode="edge")
+
+
+def mute(seis, rec_x, rec_z, sx, sz, dt, shift, window, v0):
+    """Linear top mute, src/utils.py:30-49 (== cpp/Migration.cu:86-115)."""
+    nt, nrec = seis.shape
+    out = np.zeros_like(seis)
+    t = np.arange(nt) * dt
+    for r in range(nrec):
+        dist = np.sqrt((rec_z[r] - sz) ** 2 + (rec_x[r] - sx) ** 2)
+        t1 = dist / v0 + shift
+        t2 = t1 + window
+        ramp = (t >= t1) & (t < t2)
+        out[ramp, r] = ((t[ramp] - t1) / (t2 - t1) * seis[ramp, r]).astype(seis.dtype)
+        full = t >= t2
+        out[full, r] = seis[full, r]
+    return out
+
+
+def gaussian_filter2d(sigma):
+    """src/utils.py:51-74."""
+    ks = int(np.ceil(6.0 * sigma + 1))
+    if ks % 2 == 0:
+        ks += 1
+    h = ks // 2
+    x = np.arange(ks) - h
+    X, Z = np.meshgrid(x, x, indexing="ij")
+    k = (1.0 / (2 * np.pi * sigma * sigma)) * np.exp(-(X * X + Z * Z) / (2 * sigma * sigma))
+    k = k.astype(np.float32)
+    total = np.float64(0.0)
+    for v in (1.0 / (2 * np.pi * sigma * sigma)) * np.exp(-(X * X + Z * Z) / (2 * sigma * sigma)).ravel():
+        total += v
+    return (k / np.float32(total)).astype(np.float32)
+
+
+def smooth_model_py(f, sigma, water_mask, parameter=False):
+    """2-D Gaussian smoothing of slowness (or of a parameter) over the inner
+    region, edge replication of the border, water column restored:
+    src/utils.py:76-129."""
+    f = np.asarray(f, dtype=np.float32)
+    s = f.copy() if parameter else (np.float32(1.0) / f)
+    s_old = s.copy()
+    k = gaussian_filter2d(sigma)
+    ks = k.shape[0]
+    h = ks // 2
+    nz, nx = s.shape
+    from numpy.lib.stride_tricks import sliding_window_view
+
+    if nz > 2 * h and nx > 2 * h:
+        win = sliding_window_view(s_old.astype(np.float64), (ks, ks))
+        conv = np.einsum("abij,ij->ab", win, k.astype(np.float64))
+        total = float(np.sum(k.astype(np.float64)))
+        s[h:nz - h, h:nx - h] = (conv / total).astype(np.float32)
+    for z in range(h):
+        s[z, :] = s[h, :]
+        s[nz - 1 - z, :] = s[nz - 1 - h, :]
+    for x in range(h):
+        s[:, x] = s[:, h]
+        s[:, nx - 1 - x] = s[:, nx - 1 - h]
+    for x in range(nx):
+        for z in range(nz):
+            if water_mask[z, x]:
+                s[z, x] = s_old[z, x]
+            else:
+                break
+    return s if parameter else (np.float32(1.0) / s)
+
+
+def stability_acoustic(vp, fcut, dx, dz, dt):
+    """src/Modeling2D.py:202-223 / cpp/Modeling.cu:287-292: lambda_min = vmin/fcut,
+    dx_lim = lambda_min/4.28, dt_lim = dx_lim/(4 vmax)."""
+    lam = np.min(vp) / fcut
+    dx_lim = lam / 4.28
+    dt_lim = dx_lim / (4.0 * np.max(vp))
+    return dict(dx_lim=dx_lim, dz_lim=dx_lim, dt_lim=dt_lim,
+                ok=bool(dx <= dx_lim and dz <= dx_lim and dt <= dt_lim))
+
+
+# --------------------------------------------------------------------------
+# ctypes layer
+# --------------------------------------------------------------------------
+class Oracle:
+    """Per-kernel bindings.  ``kind``: 'f64' | 'f32' | 'mixed'."""
+
+    def __init__(self, kind="f64"):
+        self.kind = kind
+        self.lib = _lib(kind)
+        self.dtype = np.float64 if kind == "f64" else np.float32
+
+    def arr(self, a):
+        return np.ascontiguousarray(a, dtype=self.dtype)
+
+    def zeros(self, *shape):
+        return np.zeros(shape, dtype=self.dtype)
+
+    def expand(self, m, N):
+        m = np.ascontiguousarray(m, dtype=np.float32)
+        nz, nx = m.shape
+        out = self.zeros(nz + 2 * N, nx + 2 * N)
+        self.lib.or_expand(_ptr(m), _I(nz), _I(nx), _I(N), _ptr(out))
+        return out
+
+    def cerjan(self, N, factor=0.0):
+        A = self.zeros(N)
+        self.lib.or_cerjan(_I(N), _D(factor if factor else 0.0), _ptr(A))
+        return A
+
+    def cpml_profiles(self, vp_exp, N, dx, dz, dt, d0, fpico):
+        nzp, nxp = vp_exp.shape
+        outs = [self.zeros(nzp, nxp) for _ in range(4)]
+        self.lib.or_cpml_profiles(_ptr(self.arr(vp_exp)), _I(nzp), _I(nxp), _I(N), _D(dx), _D(dz),
+                                  _D(dt), _D(d0), _D(fpico), *[_ptr(o) for o in outs])
+        return outs
+
+    # single kernels -------------------------------------------------------
+    def update(self, phys, Uf, Uc, g: Grid, vp, eps=None, dl=None, th=None):
+        L = self.lib
+        if phys == 0:
+            L.or_update_acoustic(_ptr(Uf), _ptr(Uc), _ptr(vp), _I(g.nzp), _I(g.nxp), _D(g.dz), _D(g.dx), _D(g.dt))
+        elif phys == 1:
+            L.or_update_vti(_ptr(Uf), _ptr(Uc), _ptr(vp), _ptr(eps), _ptr(dl), _I(g.nzp), _I(g.nxp),
+                            _D(g.dz), _D(g.dx), _D(g.dt))
+        else:
+            L.or_update_tti(_ptr(Uf), _ptr(Uc), _ptr(vp), _ptr(eps), _ptr(dl), _ptr(th), _I(g.nzp),
+                            _I(g.nxp), _D(g.dz), _D(g.dx), _D(g.dt))
+
+    def adjoint_update(self, phys, Uf, Uc, P, g: Grid, vp, eps=None, dl=None, th=None):
+        L = self.lib
+        if phys == 1:
+            L.or_adjoint_vti(_ptr(Uf), _ptr(Uc), _ptr(P), _ptr(vp), _ptr(eps), _ptr(dl), _I(g.nzp),
+                             _I(g.nxp), _D(g.dz), _D(g.dx), _D(g.dt))
+        elif phys == 2:
+            L.or_adjoint_tti(_ptr(Uf), _ptr(Uc), _ptr(P), _ptr(vp), _ptr(eps), _ptr(dl), _ptr(th),
+                             _I(g.nzp), _I(g.nxp), _D(g.dz), _D(g.dx), _D(g.dt))
+        else:
+            self.update(0, Uf, Uc, g, vp)
+
+    def cerjan_apply(self, U, g: Grid, A):
+        self.lib.or_cerjan_apply(_ptr(U), _I(g.nzp), _I(g.nxp), _I(g.N), _ptr(A))
+
+    def grad_vti(self, P, adj, geps, gdel, nz, nx, dx, dz, eps, dl):
+        self.lib.or_grad_vti(_ptr(P), _ptr(adj), _ptr(geps), _ptr(gdel), _I(nz), _I(nx), _D(dx), _D(dz),
+                             _ptr(eps), _ptr(dl))
+
+    def grad_tti(self, P, adj, geps, gdel, gth, nz, nx, dx, dz, eps, dl, th):
+        self.lib.or_grad_tti(_ptr(P), _ptr(adj), _ptr(geps), _ptr(gdel), _ptr(gth), _I(nz), _I(nx),
+                             _D(dx), _D(dz), _ptr(eps), _ptr(dl), _ptr(th))
+
+    # whole steps -----------------------------------------------------------
+    def make_state(self, g: Grid, abc, vp_exp=None, d0=None, fpico=None):
+        st = dict(U0=self.zeros(g.nzp, g.nxp), U1=self.zeros(g.nzp, g.nxp), cp=None)
+        if abc == 1:
+            prof = self.cpml_profiles(vp_exp, g.N, g.dx, g.dz, g.dt, d0, fpico)
+            st["cp"] = prof + [self.zeros(g.nzp, g.nxp) for _ in range(4)]
+        return st
+
+    def _cp_ptrs(self, cp):
+        if cp is None:
+            return None
+        arr = (ctypes.c_void_p * 8)(*[c.ctypes.data for c in cp])
+        return arr
+
+    def forward_step(self, phys, abc, Uf, Uc, g: Grid, model, A, cp, sz, sx, srcval):
+        vp, eps, dl, th = model
+        cpp = self._cp_ptrs(cp)
+        self.lib.or_forward_step(_I(phys), _I(abc), _ptr(Uf), _ptr(Uc), _ptr(vp), _ptr(eps), _ptr(dl),
+                                 _ptr(th), _ptr(A), cpp, _I(g.nzp), _I(g.nxp), _I(g.N), _D(g.dz),
+                                 _D(g.dx), _D(g.dt), _I(sz), _I(sx), _D(srcval))
+
+    def forward_shot(self, phys, abc, g: Grid, model, src, sz, sx, rz, rx, A=None, cpml=None,
+                     save=False, nsteps=None):
+        """Whole forward shot (src/Modeling2D.py:446-460): returns the seismogram
+        [nt_data][nrec], the final (current, future) fields, and the padded
+        current field after every step when ``save``."""
+        vp, eps, dl, th = [self.arr(m) if m is not None else None for m in model]
+        nt = g.nt if nsteps is None else nsteps
+        st = self.make_state(g, abc, vp, *(cpml or (None, None)))
+        if A is None:
+            A = self.zeros(max(g.N, 1))
+        A = self.arr(A)
+        rz = np.ascontiguousarray(rz, dtype=np.int32)
+        rx = np.ascontiguousarray(rx, dtype=np.int32)
+        nrec = len(rz)
+        seis = self.zeros(max(nt - g.itlag, 0), nrec)
+        sv = self.zeros(nt, g.nzp, g.nxp) if save else None
+        src = self.arr(src)
+        self.lib.or_forward_shot(_I(phys), _I(abc), _ptr(st["U0"]), _ptr(st["U1"]), _ptr(vp), _ptr(eps),
+                                 _ptr(dl), _ptr(th), _ptr(A), self._cp_ptrs(st["cp"]), _I(g.nzp),
+                                 _I(g.nxp), _I(g.N), _D(g.dz), _D(g.dx), _D(g.dt), _I(sz), _I(sx),
+                                 _ptr(src), _I(nt), _I(g.itlag), _I(nrec), _ptr(rz), _ptr(rx),
+                                 _ptr(seis), _ptr(sv))
+        cur, fut = (st["U0"], st["U1"]) if nt % 2 == 0 else (st["U1"], st["U0"])
+        return dict(seis=seis, current=cur, future=fut, save=sv, cp=st["cp"])
+
+    def backward_step(self, phys, abc, Uf, Uc, g: Grid, model, A, cp, P, rz, rx, data_row, dxdz):
+        """src/Migration.py:264-286: inject data/(dx dz) at receivers (if given),
+        then the adjoint update (reference operator choice) and the ABC."""
+        vp, eps, dl, th = model
+        if data_row is not None:
+            rz = np.ascontiguousarray(rz, dtype=np.int32)
+            rx = np.ascontiguousarray(rx, dtype=np.int32)
+            self.lib.or_inject_rec(_ptr(Uc), _I(g.nxp), _I(len(rz)), _ptr(rz), _ptr(rx),
+                                   _ptr(self.arr(data_row)), _D(dxdz))
+        if abc == 0:
+            if phys == 0:
+                self.update(0, Uf, Uc, g, vp)
+            else:
+                self.adjoint_update(phys, Uf, Uc, P, g, vp, eps, dl, th)
+            self.cerjan_apply(Uf, g, A)
+            self.cerjan_apply(Uc, g, A)
+        else:
+            # CPML: the forward operator on the adjoint field (src/Migration.py:270-279)
+            self.forward_step(phys, 1, Uf, Uc, g, model, A, cp, -1, -1, 0.0)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, data, A=None, cpml=None,
+                 fwi=False, multiparameter=False, model_interior=None, reset_cpml=True):
+    """On-the-fly RTM / FWI gradient of src/Migration.py:963-1111 (GPU path):
+    per shot, a forward pass storing the current field after every step and
+    accumulating ilum += S^2 over the interior, then a backward pass
+    t = nt-2..1 with data injection, image += S_t*lambda_t (RTM) or
+    grad += lambda_t*(S_{t+1}-2S_t+S_{t-1})/dt^2 (FWI), and for FWI
+    multiparameter the VTI/TTI partials.  Finally every output is divided by
+    (ilum + 1e-5 max(ilum)).  ``data[s]`` is the (already muted) data or the
+    residual for shot s.  With ``reset_cpml`` the CPML memory variables are
+    cleared before the backward pass (the reference keeps the forward pass's
+    values, src/Migration.py:1000-1039 only resets per shot)."""
+    N = g.N
+    vp, eps, dl, th = [orc.arr(m) if m is not None else None for m in model]
+    image = np.zeros((g.nz, g.nx), dtype=np.float64)
+    ilum = np.zeros((g.nz, g.nx), dtype=np.float64)
+    ge = np.zeros((g.nz, g.nx), dtype=orc.dtype)
+    gd = np.zeros((g.nz, g.nx), dtype=orc.dtype)
+    gt = np.zeros((g.nz, g.nx), dtype=orc.dtype)
+    if A is None:
+        A = orc.zeros(max(N, 1))
+    A = orc.arr(A)
+    sl = (slice(N, N + g.nz), slice(N, N + g.nx))
+    for s, (sz, sx) in enumerate(shots):
+        fw = orc.forward_shot(phys, abc, g, (vp, eps, dl, th), src, sz, sx, rz, rx, A=A, cpml=cpml, save=True)
+        S = fw["save"]
+        ilum_p = np.zeros((g.nz, g.nx), dtype=orc.dtype)
+        for k in range(g.nt):
+            ilum_p += S[k][sl] * S[k][sl]
+        cp = fw["cp"]
+        if cp is not None and reset_cpml:
+            for c in cp[4:]:
+                c[...] = 0
+        Lc = orc.zeros(g.nzp, g.nxp)
+        Lf = orc.zeros(g.nzp, g.nxp)
+        img_p = np.zeros((g.nz, g.nx), dtype=orc.dtype)
+        dxdz = g.dx * g.dz
+        for t in range(g.nt - 2, 0, -1):
+            row = data[s][t - g.itlag] if t >= g.itlag else None
+            orc.backward_step(phys, abc, Lf, Lc, g, (vp, eps, dl, th), A, cp, S[t], rz, rx, row, dxdz)
+            adj = Lc[sl].copy()
+            if fwi:
+                d2 = (S[t + 1][sl] - 2.0 * S[t][sl] + S[t - 1][sl]) / (g.dt * g.dt)
+                img_p += (d2 * adj).astype(orc.dtype)
+                if multiparameter and phys == 1:
+                    orc.grad_vti(np.ascontiguousarray(S[t][sl]), np.ascontiguousarray(adj), ge, gd, g.nz, g.nx,
+                                 g.dx, g.dz, orc.arr(model_interior[1]), orc.arr(model_interior[2]))
+                elif multiparameter and phys == 2:
+                    orc.grad_tti(np.ascontiguousarray(S[t][sl]), np.ascontiguousarray(adj), ge, gd, gt, g.nz,
+                                 g.nx, g.dx, g.dz, orc.arr(model_interior[1]), orc.arr(model_interior[2]),
+                                 orc.arr(model_interior[3]))
+            else:
+                img_p += (S[t][sl] * adj).astype(orc.dtype)
+            Lc, Lf = Lf, Lc
+        image += img_p
+        ilum += ilum_p
+    norm = ilum + 1e-5 * np.max(ilum)
+    out = dict(image=image / norm, raw_image=image, ilum=ilum)
+    if multiparameter:
+        out.update(g_eps=ge / norm, g_delta=gd / norm, g_theta=gt / norm)
+    return out

@@ -1,0 +1,89 @@
+"""ctypes binding of libpam.so (include/pam.h).
+
+The library is built in-tree by ``python -m pseudoacousticmodeling_amd.build``.
+There is no fallback: if the library is missing or fails to load, every entry
+point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpam.so")
+
+PAM_OK = 0
+PAM_ACOUSTIC, PAM_VTI, PAM_TTI = 0, 1, 2
+PAM_CERJAN, PAM_CPML = 0, 1
+PAM_INTERIOR, PAM_PADDED = 0, 1
+PAM_STORE_ALL_F32, PAM_STORE_ALL_F16, PAM_STORE_ALL_BF16, PAM_STORE_CHECKPOINT = 0, 1, 2, 3
+PAM_OPT_BATCH, PAM_OPT_TILE, PAM_OPT_GRAPH, PAM_OPT_STORAGE = 1, 2, 3, 4
+PAM_OPT_CKPT_STEP, PAM_OPT_RESET_ADJ_CPML, PAM_OPT_NORMALIZE = 5, 6, 7
+
+ERRORS = {-1: "PAM_E_ARG", -2: "PAM_E_HIP", -3: "PAM_E_STATE", -4: "PAM_E_NOMEM",
+          -5: "PAM_E_UNSUPPORTED", -6: "PAM_E_NONFINITE"}
+
+# every symbol include/pam.h declares
+EXPORTS = [
+    "pam_version", "pam_create", "pam_destroy", "pam_last_error", "pam_set_option", "pam_tile_count",
+    "pam_set_model", "pam_set_wavelet", "pam_set_geometry", "pam_reset", "pam_forward_step",
+    "pam_get_field", "pam_forward", "pam_rtm", "pam_gradient", "pam_misfit", "pam_forward_adjoint",
+    "pam_last_timing",
+]
+
+
+class PamGrid(ctypes.Structure):
+    _fields_ = [("nz", ctypes.c_int), ("nx", ctypes.c_int), ("n_abc", ctypes.c_int),
+                ("dz", ctypes.c_float), ("dx", ctypes.c_float), ("dt", ctypes.c_float),
+                ("nt", ctypes.c_int), ("itlag", ctypes.c_int)]
+
+
+class PamAbcParams(ctypes.Structure):
+    _fields_ = [("cerjan_factor", ctypes.c_float), ("cpml_d0", ctypes.c_float),
+                ("cpml_f_pico", ctypes.c_float)]
+
+
+class PamError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load libpam.so (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"{LIB_PATH} not found: build it with `python -m pseudoacousticmodeling_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, I, P, D, LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_longlong
+    lib.pam_version.restype = ctypes.c_char_p
+    lib.pam_last_error.restype = ctypes.c_char_p
+    lib.pam_last_error.argtypes = [vp]
+    lib.pam_create.argtypes = [ctypes.POINTER(PamGrid), I, I, ctypes.POINTER(PamAbcParams), vp, I,
+                               ctypes.POINTER(vp)]
+    lib.pam_destroy.argtypes = [vp]
+    lib.pam_set_option.argtypes = [vp, I, LL]
+    lib.pam_set_model.argtypes = [vp, P, P, P, P, I]
+    lib.pam_set_wavelet.argtypes = [vp, P]
+    lib.pam_set_geometry.argtypes = [vp, P, P, I, P, P, I]
+    lib.pam_reset.argtypes = [vp, I]
+    lib.pam_forward_step.argtypes = [vp, I, P]
+    lib.pam_get_field.argtypes = [vp, I, P]
+    lib.pam_forward.argtypes = [vp, I, I, P]
+    lib.pam_rtm.argtypes = [vp, I, I, P, P, P]
+    lib.pam_gradient.argtypes = [vp, I, I, P, P, P, P, P, P]
+    lib.pam_misfit.argtypes = [vp, I, P, P, P, ctypes.POINTER(D)]
+    lib.pam_forward_adjoint.argtypes = [vp, I, I, P, P, I]
+    lib.pam_last_timing.argtypes = [vp, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(LL)]
+    _lib = lib
+    return lib
+
+
+def check(ctx, rc):
+    if rc != PAM_OK:
+        msg = load().pam_last_error(ctx)
+        raise PamError(f"{ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+    return rc

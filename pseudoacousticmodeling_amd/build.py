@@ -1,0 +1,69 @@
+"""Build libpam.so (HIP, gfx950) in-tree with hipcc.
+
+``python -m pseudoacousticmodeling_amd.build`` compiles the kernels and the C
+ABI into ``pseudoacousticmodeling_amd/libpam.so`` and the C++ drivers
+``run_modeling`` / ``run_migration`` next to it.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libpam.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("PAM_OFFLOAD_ARCH", "gfx950")
+
+# Packed FP32 instructions are disabled for all device code: the compiler may
+# form v_pk_* from ordinary float code, and some of their forms have been
+# unreliable on the target machines.
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                # the z-march keeps its row queues in registers only when the
+                # row loop is fully unrolled; the default threshold gives up on
+                # the 4-cells-per-lane variants and spills the queues to scratch
+                "-mllvm", "-pragma-unroll-threshold=500000"]
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", f"-I{os.path.join(ROOT, 'include')}"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_lib(verbose: bool = True) -> str:
+    objs = []
+    os.makedirs(os.path.join(HERE, "_obj"), exist_ok=True)
+    srcs = [("pam_kernels.hip", True), ("pam_api.cpp", False)]
+    procs = []
+    for name, dev in srcs:
+        obj = os.path.join(HERE, "_obj", name + ".o")
+        cmd = [HIPCC, f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, "-c"]
+        if dev:
+            cmd += ["-x", "hip"]
+        cmd += [os.path.join(CSRC, name), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+        objs.append(obj)
+    for p in procs:
+        if p.wait() != 0:
+            raise RuntimeError("hipcc failed")
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def main(argv=None) -> int:
+    build_lib()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

@@ -1,0 +1,848 @@
+// pam_api.cpp -- the C ABI of libpam (include/pam.h): context, model and
+// geometry set-up, and the forward / RTM / gradient time loops that drive the
+// kernels of pam_kernels.hip on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pam.h"
+#include "pam_internal.h"
+
+using namespace pam;
+
+struct pam_ctx {
+    pam_grid g{};
+    int phys = 0, abc = 0;
+    pam_abc_params ap{};
+    int nzp = 0, nxp = 0, P = 0;
+    long long plane = 0;  // nzp * P
+    hipStream_t st = nullptr;
+    bool own_stream = false;
+    int max_shots = 1;
+
+    // model coefficient fields
+    float *v2 = nullptr, *eps = nullptr, *del = nullptr, *cs = nullptr, *sn = nullptr;
+    float *cerx = nullptr, *cerz = nullptr;
+    float *cax = nullptr, *cbx = nullptr, *caz = nullptr, *cbz = nullptr;
+    float *vp_pad = nullptr;
+    bool model_set = false;
+    // wavelet
+    float *wav = nullptr;
+    bool wav_set = false;
+    // geometry
+    std::vector<int> hsz, hsx, hrz, hrx;
+    int nshot = 0, nrec = 0, nslot = 0;
+    int *d_rz = nullptr, *d_rx = nullptr, *d_rrow = nullptr, *d_rhead = nullptr, *d_rnext = nullptr;
+    int *d_ssz = nullptr, *d_ssx = nullptr;  // [max_shots]
+    bool geom_set = false;
+    // wavefields [max_shots][plane]
+    float *U0 = nullptr, *U1 = nullptr;
+    float *psix = nullptr, *psiz = nullptr, *zetax = nullptr, *zetaz = nullptr;
+    // per-step interface state (slot 0)
+    float *cur = nullptr, *prv = nullptr;
+    int next_k = -1;
+    // options
+    int batch = 0, tile = -1, graph = 1, storage = PAM_STORE_ALL_F32, ckpt_step = 0;
+    int reset_adj_cpml = 1, normalize = 1;
+    // adjoint scratch
+    float *tmp = nullptr;
+    long long tmp_bytes = 0;
+    void *save = nullptr;
+    long long save_bytes = 0;
+    float *save_scale = nullptr;
+    float *acc_ilum = nullptr, *acc_img = nullptr;
+    long long acc_bytes = 0;
+    float *maxv = nullptr;
+    float *ilum_tmp = nullptr;
+    long long ilum_tmp_bytes = 0;
+    double *d_acc = nullptr;
+    // timing
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<hipEvent_t> kev;
+    double loop_ms = 0.0, kernel_ms = 0.0;
+    long long launches = 0;
+    int profile = 1;
+    std::string err;
+};
+
+static int fail(pam_ctx *c, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+static int fail(pam_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HCK(call)                                                                                  \
+    do {                                                                                           \
+        hipError_t _e = (call);                                                                    \
+        if (_e != hipSuccess)                                                                      \
+            return fail(ctx, PAM_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), __FILE__, \
+                        __LINE__);                                                                 \
+    } while (0)
+#define KCK(call)                                                                                  \
+    do {                                                                                           \
+        int _e = (call);                                                                           \
+        if (_e != 0)                                                                               \
+            return fail(ctx, PAM_E_HIP, "kernel launch %s failed: %s (%s:%d)", #call,              \
+                        hipGetErrorString((hipError_t)_e), __FILE__, __LINE__);                    \
+    } while (0)
+
+static int dalloc(pam_ctx *ctx, void **p, size_t bytes) {
+    if (*p) return PAM_OK;
+    hipError_t e = hipMalloc(p, bytes ? bytes : 4);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(ctx, PAM_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return PAM_OK;
+}
+#define DALLOC(p, bytes)                                                        \
+    do {                                                                        \
+        int _r = dalloc(ctx, (void **)&(p), (size_t)(bytes));                   \
+        if (_r) return _r;                                                      \
+    } while (0)
+
+static void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+extern "C" const char *pam_version(void) { return "pam-mi355x 0.1 (gfx950)"; }
+extern "C" int pam_tile_count(void) { return tile_count(); }
+
+extern "C" int pam_create(const pam_grid *grid, int physics, int abc, const pam_abc_params *abcp, void *hip_stream,
+                          int max_shots, pam_ctx **out) {
+    if (!out) return PAM_E_ARG;
+    *out = nullptr;
+    pam_ctx *ctx = new pam_ctx();
+    auto bad = [&](const char *msg) {
+        ctx->err = msg;
+        *out = ctx;  // returned so that pam_last_error can report; caller destroys
+        return PAM_E_ARG;
+    };
+    if (!grid) return bad("grid is NULL");
+    if (physics < PAM_ACOUSTIC || physics > PAM_TTI) return bad("physics must be 0 (acoustic), 1 (VTI) or 2 (TTI)");
+    if (abc != PAM_CERJAN && abc != PAM_CPML) return bad("abc must be 0 (Cerjan) or 1 (CPML)");
+    const pam_grid &g = *grid;
+    if (g.nz < 1 || g.nx < 1) return bad("nz and nx must be positive");
+    if (g.n_abc < 0) return bad("n_abc must be >= 0");
+    if (abc == PAM_CPML && g.n_abc < 8) return bad("CPML needs n_abc >= 8");
+    if (g.nz + 2 * g.n_abc < 9 || g.nx + 2 * g.n_abc < 9) return bad("padded grid must be at least 9x9");
+    if (!(g.dx > 0.f) || !(g.dz > 0.f) || !(g.dt > 0.f)) return bad("dx, dz and dt must be positive");
+    if (g.nt < 1 || g.itlag < 0 || g.itlag > g.nt) return bad("need nt >= 1 and 0 <= itlag <= nt");
+    if (max_shots < 1) return bad("max_shots must be >= 1");
+    ctx->g = g;
+    ctx->phys = physics;
+    ctx->abc = abc;
+    if (abcp) ctx->ap = *abcp;
+    ctx->nzp = g.nz + 2 * g.n_abc;
+    ctx->nxp = g.nx + 2 * g.n_abc;
+    ctx->P = (ctx->nxp + 63) / 64 * 64;
+    ctx->plane = (long long)ctx->nzp * ctx->P;
+    ctx->max_shots = max_shots;
+    if (hip_stream) {
+        ctx->st = (hipStream_t)hip_stream;
+    } else {
+        if (hipStreamCreate(&ctx->st) != hipSuccess) return bad("hipStreamCreate failed");
+        ctx->own_stream = true;
+    }
+    *out = ctx;
+    // wavefields and per-shot CPML memory
+    const size_t fb = (size_t)ctx->plane * max_shots * sizeof(float);
+    DALLOC(ctx->U0, fb);
+    DALLOC(ctx->U1, fb);
+    if (abc == PAM_CPML) {
+        DALLOC(ctx->psix, fb);
+        DALLOC(ctx->psiz, fb);
+        DALLOC(ctx->zetax, fb);
+        DALLOC(ctx->zetaz, fb);
+    }
+    const size_t pb = (size_t)ctx->plane * sizeof(float);
+    DALLOC(ctx->v2, pb);
+    if (physics >= 1) {
+        DALLOC(ctx->eps, pb);
+        DALLOC(ctx->del, pb);
+    }
+    if (physics == 2) {
+        DALLOC(ctx->cs, pb);
+        DALLOC(ctx->sn, pb);
+    }
+    DALLOC(ctx->vp_pad, pb);
+    DALLOC(ctx->cerx, (size_t)ctx->P * sizeof(float));
+    DALLOC(ctx->cerz, (size_t)ctx->nzp * sizeof(float));
+    if (abc == PAM_CPML) {
+        DALLOC(ctx->cax, pb);
+        DALLOC(ctx->cbx, pb);
+        DALLOC(ctx->caz, pb);
+        DALLOC(ctx->cbz, pb);
+    }
+    DALLOC(ctx->wav, (size_t)g.nt * sizeof(float));
+    DALLOC(ctx->d_ssz, (size_t)max_shots * sizeof(int));
+    DALLOC(ctx->d_ssx, (size_t)max_shots * sizeof(int));
+    DALLOC(ctx->maxv, 64);
+    DALLOC(ctx->d_acc, 64);
+    HCK(hipMemsetAsync(ctx->U0, 0, fb, ctx->st));
+    HCK(hipMemsetAsync(ctx->U1, 0, fb, ctx->st));
+    // Cerjan factors: 1 in the interior, A[i] in the strips (cpp/Modeling.cu:111-120)
+    {
+        const int N = g.n_abc;
+        std::vector<float> A(std::max(N, 1), 1.0f);
+        for (int i = 0; i < N; ++i) {
+            float fb_;
+            if (ctx->ap.cerjan_factor > 0.f)
+                fb_ = ctx->ap.cerjan_factor * (float)(N - i);
+            else
+                fb_ = (float)(N - i) / (1.4142f * (3.0f * (float)N));
+            A[i] = expf(-fb_ * fb_);
+        }
+        std::vector<float> cx(ctx->P, 1.0f), cz(ctx->nzp, 1.0f);
+        for (int i = 0; i < ctx->nxp; ++i) {
+            float f = 1.0f;
+            if (i < N) f *= A[i];
+            if (i >= ctx->nxp - N) f *= A[ctx->nxp - 1 - i];
+            cx[i] = f;
+        }
+        for (int i = 0; i < ctx->nzp; ++i) {
+            float f = 1.0f;
+            if (i < N) f *= A[i];
+            if (i >= ctx->nzp - N) f *= A[ctx->nzp - 1 - i];
+            cz[i] = f;
+        }
+        HCK(hipMemcpyAsync(ctx->cerx, cx.data(), cx.size() * sizeof(float), hipMemcpyHostToDevice, ctx->st));
+        HCK(hipMemcpyAsync(ctx->cerz, cz.data(), cz.size() * sizeof(float), hipMemcpyHostToDevice, ctx->st));
+        HCK(hipStreamSynchronize(ctx->st));
+    }
+    HCK(hipEventCreate(&ctx->e0));
+    HCK(hipEventCreate(&ctx->e1));
+    return PAM_OK;
+}
+
+extern "C" int pam_destroy(pam_ctx *ctx) {
+    if (!ctx) return PAM_OK;
+    if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+    float *bufs[] = {ctx->v2,  ctx->eps,  ctx->del,   ctx->cs,    ctx->sn,   ctx->cerx, ctx->cerz,
+                     ctx->cax, ctx->cbx,  ctx->caz,   ctx->cbz,   ctx->vp_pad, ctx->wav, ctx->U0,
+                     ctx->U1,  ctx->psix, ctx->psiz,  ctx->zetax, ctx->zetaz, ctx->tmp, ctx->save_scale,
+                     ctx->acc_ilum, ctx->maxv, ctx->ilum_tmp};
+    for (float *b : bufs) dfree(b);
+    dfree(ctx->save);
+    dfree(ctx->d_acc);
+    int *ibufs[] = {ctx->d_rz, ctx->d_rx, ctx->d_rrow, ctx->d_rhead, ctx->d_rnext, ctx->d_ssz, ctx->d_ssx};
+    for (int *b : ibufs) dfree(b);
+    for (auto e : ctx->kev) (void)hipEventDestroy(e);
+    if (ctx->e0) (void)hipEventDestroy(ctx->e0);
+    if (ctx->e1) (void)hipEventDestroy(ctx->e1);
+    if (ctx->own_stream && ctx->st) (void)hipStreamDestroy(ctx->st);
+    delete ctx;
+    return PAM_OK;
+}
+
+extern "C" const char *pam_last_error(const pam_ctx *ctx) { return ctx ? ctx->err.c_str() : "NULL context"; }
+
+extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
+    if (!ctx) return PAM_E_ARG;
+    switch (key) {
+        case PAM_OPT_BATCH:
+            if (v < 0 || v > ctx->max_shots) return fail(ctx, PAM_E_ARG, "batch must be in [0, max_shots]");
+            ctx->batch = (int)v;
+            return PAM_OK;
+        case PAM_OPT_TILE:
+            if (v < -1 || v >= tile_count()) return fail(ctx, PAM_E_ARG, "tile index out of range");
+            ctx->tile = (int)v;
+            return PAM_OK;
+        case PAM_OPT_GRAPH:
+            ctx->graph = v ? 1 : 0;
+            return PAM_OK;
+        case PAM_OPT_STORAGE:
+            if (v < PAM_STORE_ALL_F32 || v > PAM_STORE_CHECKPOINT) return fail(ctx, PAM_E_ARG, "unknown storage");
+            ctx->storage = (int)v;
+            return PAM_OK;
+        case PAM_OPT_CKPT_STEP:
+            if (v < 0) return fail(ctx, PAM_E_ARG, "checkpoint step must be >= 0");
+            ctx->ckpt_step = (int)v;
+            return PAM_OK;
+        case PAM_OPT_RESET_ADJ_CPML:
+            ctx->reset_adj_cpml = v ? 1 : 0;
+            return PAM_OK;
+        case PAM_OPT_NORMALIZE:
+            if (v < 0 || v > 2) return fail(ctx, PAM_E_ARG, "normalize must be 0, 1 or 2");
+            ctx->normalize = (int)v;
+            return PAM_OK;
+        default:
+            return fail(ctx, PAM_E_ARG, "unknown option %d", key);
+    }
+}
+
+extern "C" int pam_set_model(pam_ctx *ctx, const float *vp, const float *eps, const float *delta,
+                             const float *theta, int layout) {
+    if (!ctx) return PAM_E_ARG;
+    if (!vp) return fail(ctx, PAM_E_ARG, "vp is NULL");
+    if (ctx->phys >= 1 && (!eps || !delta)) return fail(ctx, PAM_E_ARG, "VTI/TTI need eps and delta");
+    if (ctx->phys == 2 && !theta) return fail(ctx, PAM_E_ARG, "TTI needs theta");
+    if (layout != PAM_INTERIOR && layout != PAM_PADDED) return fail(ctx, PAM_E_ARG, "layout must be 0 or 1");
+    const int N = ctx->g.n_abc;
+    auto put = [&](const float *src, float *dst) -> int {
+        if (layout == PAM_INTERIOR) return launch_expand(src, ctx->g.nz, ctx->g.nx, N, dst, ctx->P, ctx->st);
+        return launch_copy_padded(src, ctx->nzp, ctx->nxp, dst, ctx->P, ctx->st);
+    };
+    KCK(put(vp, ctx->vp_pad));
+    float *th = nullptr;
+    if (ctx->phys >= 1) {
+        KCK(put(eps, ctx->eps));
+        KCK(put(delta, ctx->del));
+    }
+    if (ctx->phys == 2) {
+        // theta is expanded into sn; k_coeffs reads it before writing cs and sn
+        KCK(put(theta, ctx->sn));
+        th = ctx->sn;
+    }
+    KCK(launch_coeffs(ctx->phys, ctx->vp_pad, th, ctx->g.dt, ctx->nzp, ctx->nxp, ctx->P, ctx->v2, ctx->cs, ctx->sn,
+                      ctx->st));
+    if (ctx->abc == PAM_CPML)
+        KCK(launch_cpml_coeffs(ctx->vp_pad, ctx->nzp, ctx->nxp, ctx->P, N, ctx->g.dx, ctx->g.dz, ctx->g.dt,
+                               ctx->ap.cpml_d0, ctx->ap.cpml_f_pico, ctx->cax, ctx->cbx, ctx->caz, ctx->cbz, ctx->st));
+    HCK(hipStreamSynchronize(ctx->st));
+    ctx->model_set = true;
+    return PAM_OK;
+}
+
+extern "C" int pam_set_wavelet(pam_ctx *ctx, const float *src) {
+    if (!ctx) return PAM_E_ARG;
+    if (!src) return fail(ctx, PAM_E_ARG, "wavelet is NULL");
+    HCK(hipMemcpyAsync(ctx->wav, src, (size_t)ctx->g.nt * sizeof(float), hipMemcpyDeviceToDevice, ctx->st));
+    HCK(hipStreamSynchronize(ctx->st));
+    ctx->wav_set = true;
+    return PAM_OK;
+}
+
+extern "C" int pam_set_geometry(pam_ctx *ctx, const int *sz, const int *sx, int nshot, const int *rz, const int *rx,
+                                int nrec) {
+    if (!ctx) return PAM_E_ARG;
+    if (nshot < 0 || nrec < 0) return fail(ctx, PAM_E_ARG, "negative counts");
+    if ((nshot && (!sz || !sx)) || (nrec && (!rz || !rx))) return fail(ctx, PAM_E_ARG, "NULL geometry array");
+    for (int i = 0; i < nshot; ++i)
+        if (sz[i] < 0 || sz[i] >= ctx->nzp || sx[i] < 0 || sx[i] >= ctx->nxp)
+            return fail(ctx, PAM_E_ARG, "source %d at (%d,%d) outside the %dx%d padded grid", i, sz[i], sx[i],
+                        ctx->nzp, ctx->nxp);
+    for (int i = 0; i < nrec; ++i)
+        if (rz[i] < 0 || rz[i] >= ctx->nzp || rx[i] < 0 || rx[i] >= ctx->nxp)
+            return fail(ctx, PAM_E_ARG, "receiver %d at (%d,%d) outside the %dx%d padded grid", i, rz[i], rx[i],
+                        ctx->nzp, ctx->nxp);
+    ctx->hsz.assign(sz, sz + nshot);
+    ctx->hsx.assign(sx, sx + nshot);
+    ctx->hrz.assign(rz, rz + nrec);
+    ctx->hrx.assign(rx, rx + nrec);
+    ctx->nshot = nshot;
+    ctx->nrec = nrec;
+    // receiver tables: row -> slot, (slot, column) -> first receiver, receiver -> next at the same cell
+    std::vector<int> rrow(ctx->nzp, -1);
+    int nslot = 0;
+    for (int r = 0; r < nrec; ++r)
+        if (rrow[rz[r]] < 0) rrow[rz[r]] = nslot++;
+    std::vector<int> head((size_t)std::max(nslot, 1) * ctx->P, -1), next(std::max(nrec, 1), -1);
+    for (int r = nrec - 1; r >= 0; --r) {
+        const size_t h = (size_t)rrow[rz[r]] * ctx->P + rx[r];
+        next[r] = head[h];
+        head[h] = r;
+    }
+    dfree(ctx->d_rz);
+    dfree(ctx->d_rx);
+    dfree(ctx->d_rrow);
+    dfree(ctx->d_rhead);
+    dfree(ctx->d_rnext);
+    ctx->d_rz = ctx->d_rx = ctx->d_rrow = ctx->d_rhead = ctx->d_rnext = nullptr;
+    DALLOC(ctx->d_rz, std::max(nrec, 1) * sizeof(int));
+    DALLOC(ctx->d_rx, std::max(nrec, 1) * sizeof(int));
+    DALLOC(ctx->d_rrow, rrow.size() * sizeof(int));
+    DALLOC(ctx->d_rhead, head.size() * sizeof(int));
+    DALLOC(ctx->d_rnext, next.size() * sizeof(int));
+    if (nrec) {
+        HCK(hipMemcpyAsync(ctx->d_rz, rz, nrec * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+        HCK(hipMemcpyAsync(ctx->d_rx, rx, nrec * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+    }
+    HCK(hipMemcpyAsync(ctx->d_rrow, rrow.data(), rrow.size() * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+    HCK(hipMemcpyAsync(ctx->d_rhead, head.data(), head.size() * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+    HCK(hipMemcpyAsync(ctx->d_rnext, next.data(), next.size() * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+    HCK(hipStreamSynchronize(ctx->st));
+    ctx->nslot = nslot;
+    ctx->geom_set = true;
+    return PAM_OK;
+}
+
+// ------------------------------------------------------------------ helpers
+static int ready(pam_ctx *ctx) {
+    if (!ctx->model_set) return fail(ctx, PAM_E_STATE, "no model set (pam_set_model)");
+    if (!ctx->wav_set) return fail(ctx, PAM_E_STATE, "no wavelet set (pam_set_wavelet)");
+    if (!ctx->geom_set) return fail(ctx, PAM_E_STATE, "no geometry set (pam_set_geometry)");
+    return PAM_OK;
+}
+
+static int choose_tile(pam_ctx *ctx, int nshots) {
+    if (ctx->tile >= 0) return ctx->tile;
+    // many waves available: wide tiles, tall z-march; few: narrower, shorter tiles
+    const long long cells = (long long)ctx->nzp * ctx->nxp * nshots;
+    if (ctx->phys == 2) return cells > 4000000 ? 0 : 5;
+    if (cells > 4000000) return 1;
+    if (cells > 1000000) return 0;
+    return 5;
+}
+
+static void base_step_args(pam_ctx *ctx, StepArgs &a, int nshots) {
+    std::memset(&a, 0, sizeof(a));
+    const pam_grid &g = ctx->g;
+    a.nzp = ctx->nzp;
+    a.nxp = ctx->nxp;
+    a.P = ctx->P;
+    a.N = g.n_abc;
+    a.nz = g.nz;
+    a.nx = g.nx;
+    a.nshots = nshots;
+    a.sstride = ctx->plane;
+    a.idx2 = 1.0f / (g.dx * g.dx);
+    a.idz2 = 1.0f / (g.dz * g.dz);
+    a.idx = 1.0f / g.dx;
+    a.idz = 1.0f / g.dz;
+    a.idxdz = 1.0f / (g.dz * g.dx);
+    a.idt2 = 1.0f / (g.dt * g.dt);
+    a.dxdz = g.dx * g.dz;
+    a.nt = g.nt;
+    a.itlag = g.itlag;
+    a.nrec = ctx->nrec;
+    a.v2 = ctx->v2;
+    a.eps = ctx->eps;
+    a.del = ctx->del;
+    a.cs = ctx->cs;
+    a.sn = ctx->sn;
+    a.cerx = ctx->cerx;
+    a.cerz = ctx->cerz;
+    a.psix = ctx->psix;
+    a.psiz = ctx->psiz;
+    a.zetax = ctx->zetax;
+    a.zetaz = ctx->zetaz;
+    a.cax = ctx->cax;
+    a.cbx = ctx->cbx;
+    a.caz = ctx->caz;
+    a.cbz = ctx->cbz;
+    a.ssz = ctx->d_ssz;
+    a.ssx = ctx->d_ssx;
+    a.wav = ctx->wav;
+    a.rrow = ctx->d_rrow;
+    a.rhead = ctx->d_rhead;
+    a.rnext = ctx->d_rnext;
+    a.flags = (ctx->abc == PAM_CERJAN) ? F_CERJAN : 0;
+}
+
+static void base_psi_args(pam_ctx *ctx, PsiArgs &p, int nshots, const float *cur) {
+    p.nzp = ctx->nzp;
+    p.nxp = ctx->nxp;
+    p.P = ctx->P;
+    p.N = ctx->g.n_abc;
+    p.nshots = nshots;
+    p.sstride = ctx->plane;
+    p.idx = 1.0f / ctx->g.dx;
+    p.idz = 1.0f / ctx->g.dz;
+    p.cur = cur;
+    p.psix = ctx->psix;
+    p.psiz = ctx->psiz;
+    p.cax = ctx->cax;
+    p.cbx = ctx->cbx;
+    p.caz = ctx->caz;
+    p.cbz = ctx->cbz;
+}
+
+static int zero_shot_state(pam_ctx *ctx, int nshots, bool fields, bool cpml) {
+    const size_t fb = (size_t)ctx->plane * nshots * sizeof(float);
+    if (fields) {
+        HCK(hipMemsetAsync(ctx->U0, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(ctx->U1, 0, fb, ctx->st));
+    }
+    if (cpml && ctx->abc == PAM_CPML) {
+        HCK(hipMemsetAsync(ctx->psix, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(ctx->psiz, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(ctx->zetax, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(ctx->zetaz, 0, fb, ctx->st));
+    }
+    return PAM_OK;
+}
+
+static int upload_sources(pam_ctx *ctx, int shot0, int nshots) {
+    for (int s = 0; s < nshots; ++s)
+        if (shot0 + s < 0 || shot0 + s >= ctx->nshot)
+            return fail(ctx, PAM_E_ARG, "shot %d out of range [0,%d)", shot0 + s, ctx->nshot);
+    HCK(hipMemcpyAsync(ctx->d_ssz, ctx->hsz.data() + shot0, nshots * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+    HCK(hipMemcpyAsync(ctx->d_ssx, ctx->hsx.data() + shot0, nshots * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+    return PAM_OK;
+}
+
+// One forward time step of `nshots` shots: psi pre-pass (CPML) then the fused step.
+static int fwd_step(pam_ctx *ctx, StepArgs &a, int tile, float *cur, float *prv, int k, hipEvent_t *ev) {
+    if (ctx->abc == PAM_CPML) {
+        PsiArgs p;
+        base_psi_args(ctx, p, a.nshots, cur);
+        KCK(launch_psi(p, ctx->st));
+    }
+    a.cur = cur;
+    a.prv = prv;
+    a.k = k;
+    if (ev) HCK(hipEventRecord(ev[0], ctx->st));
+    KCK(launch_step(ctx->phys, ctx->abc == PAM_CPML, tile, a, ctx->st));
+    if (ev) HCK(hipEventRecord(ev[1], ctx->st));
+    return PAM_OK;
+}
+
+static int ensure_events(pam_ctx *ctx, size_t n) {
+    while (ctx->kev.size() < n) {
+        hipEvent_t e;
+        HCK(hipEventCreate(&e));
+        ctx->kev.push_back(e);
+    }
+    return PAM_OK;
+}
+
+// ------------------------------------------------------------ per-step API
+extern "C" int pam_reset(pam_ctx *ctx, int shot) {
+    if (!ctx) return PAM_E_ARG;
+    int r = ready(ctx);
+    if (r) return r;
+    if (shot < 0 || shot >= ctx->nshot) return fail(ctx, PAM_E_ARG, "shot %d out of range", shot);
+    r = zero_shot_state(ctx, 1, true, true);
+    if (r) return r;
+    r = upload_sources(ctx, shot, 1);
+    if (r) return r;
+    KCK(launch_inject_point(ctx->U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, 1, ctx->st));
+    ctx->cur = ctx->U0;
+    ctx->prv = ctx->U1;
+    ctx->next_k = 0;
+    return PAM_OK;
+}
+
+extern "C" int pam_forward_step(pam_ctx *ctx, int k, float *seis) {
+    if (!ctx) return PAM_E_ARG;
+    if (ctx->next_k < 0) return fail(ctx, PAM_E_STATE, "pam_reset must precede pam_forward_step");
+    if (k != ctx->next_k) return fail(ctx, PAM_E_STATE, "steps must be issued in order: expected k=%d", ctx->next_k);
+    if (k >= ctx->g.nt) return fail(ctx, PAM_E_ARG, "k=%d beyond nt=%d", k, ctx->g.nt);
+    StepArgs a;
+    base_step_args(ctx, a, 1);
+    a.flags |= F_SRC;
+    if (seis && k >= ctx->g.itlag) {
+        a.flags |= F_GATHER;
+        a.seis = seis;
+        a.dstride = (long long)(ctx->g.nt - ctx->g.itlag) * ctx->nrec;
+    }
+    int r = fwd_step(ctx, a, choose_tile(ctx, 1), ctx->cur, ctx->prv, k, nullptr);
+    if (r) return r;
+    std::swap(ctx->cur, ctx->prv);
+    ctx->next_k = k + 1;
+    return PAM_OK;
+}
+
+extern "C" int pam_get_field(pam_ctx *ctx, int which, float *out) {
+    if (!ctx || !out) return PAM_E_ARG;
+    if (ctx->next_k < 0) return fail(ctx, PAM_E_STATE, "no shot in progress");
+    const float *src = which == 0 ? ctx->cur : ctx->prv;
+    KCK(launch_pack_padded(src, ctx->nzp, ctx->nxp, ctx->P, out, ctx->st));
+    HCK(hipStreamSynchronize(ctx->st));
+    return PAM_OK;
+}
+
+// ------------------------------------------------------------ whole runs
+static int batch_size(pam_ctx *ctx, int nshots) {
+    int b = ctx->batch > 0 ? ctx->batch : ctx->max_shots;
+    return std::max(1, std::min(b, std::min(nshots, ctx->max_shots)));
+}
+
+static int finish_timing(pam_ctx *ctx, long long nk) {
+    HCK(hipEventSynchronize(ctx->e1));
+    float ms = 0.f;
+    HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
+    ctx->loop_ms += ms;
+    if (nk > 0 && ctx->profile) {
+        double acc = 0.0;
+        for (long long i = 0; i < nk; ++i) {
+            float m = 0.f;
+            HCK(hipEventElapsedTime(&m, ctx->kev[2 * i], ctx->kev[2 * i + 1]));
+            acc += m;
+        }
+        ctx->kernel_ms += acc;
+        ctx->launches += nk;
+    }
+    return PAM_OK;
+}
+
+// forward loop over nt steps for shots already primed in U0 (cur) / U1 (prv);
+// extra flags / args are applied to every step
+static int run_forward_loop(pam_ctx *ctx, StepArgs &a, int tile, bool timed) {
+    float *cur = ctx->U0, *prv = ctx->U1;
+    const int nt = ctx->g.nt;
+    if (timed && ctx->profile) {
+        int r = ensure_events(ctx, 2 * (size_t)nt);
+        if (r) return r;
+    }
+    if (timed) HCK(hipEventRecord(ctx->e0, ctx->st));
+    const int base_flags = a.flags;
+    for (int k = 0; k < nt; ++k) {
+        a.flags = base_flags;
+        if (!(k >= ctx->g.itlag)) a.flags &= ~F_GATHER;
+        int r = fwd_step(ctx, a, tile, cur, prv, k, (timed && ctx->profile) ? &ctx->kev[2 * k] : nullptr);
+        if (r) return r;
+        std::swap(cur, prv);
+    }
+    if (timed) {
+        HCK(hipEventRecord(ctx->e1, ctx->st));
+        int r = finish_timing(ctx, (timed && ctx->profile) ? nt : 0);
+        if (r) return r;
+    }
+    a.flags = base_flags;
+    return PAM_OK;
+}
+
+extern "C" int pam_forward(pam_ctx *ctx, int shot0, int nshots, float *seis) {
+    if (!ctx) return PAM_E_ARG;
+    int r = ready(ctx);
+    if (r) return r;
+    if (nshots < 1) return fail(ctx, PAM_E_ARG, "nshots must be >= 1");
+    if (!seis) return fail(ctx, PAM_E_ARG, "seis is NULL");
+    const long long nd = (long long)(ctx->g.nt - ctx->g.itlag) * ctx->nrec;
+    ctx->loop_ms = ctx->kernel_ms = 0.0;
+    ctx->launches = 0;
+    ctx->next_k = -1;
+    const int B = batch_size(ctx, nshots);
+    for (int b0 = 0; b0 < nshots; b0 += B) {
+        const int nb = std::min(B, nshots - b0);
+        r = zero_shot_state(ctx, nb, true, true);
+        if (r) return r;
+        r = upload_sources(ctx, shot0 + b0, nb);
+        if (r) return r;
+        HCK(hipMemsetAsync(seis + (long long)b0 * nd, 0, (size_t)nb * nd * sizeof(float), ctx->st));
+        KCK(launch_inject_point(ctx->U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, nb, ctx->st));
+        StepArgs a;
+        base_step_args(ctx, a, nb);
+        a.flags |= F_SRC | F_GATHER;
+        a.seis = seis + (long long)b0 * nd;
+        a.dstride = nd;
+        r = run_forward_loop(ctx, a, choose_tile(ctx, nb), true);
+        if (r) return r;
+    }
+    HCK(hipStreamSynchronize(ctx->st));
+    return PAM_OK;
+}
+
+extern "C" int pam_last_timing(const pam_ctx *ctx, double *loop_ms, double *kernel_ms, long long *kernel_launches) {
+    if (!ctx) return PAM_E_ARG;
+    if (loop_ms) *loop_ms = ctx->loop_ms;
+    if (kernel_ms) *kernel_ms = ctx->kernel_ms;
+    if (kernel_launches) *kernel_launches = ctx->launches;
+    return PAM_OK;
+}
+
+// ------------------------------------------------------------ adjoint passes
+static int ensure_scratch(pam_ctx *ctx, void **p, long long *have, long long need) {
+    if (*have >= need && *p) return PAM_OK;
+    dfree(*p);
+    *p = nullptr;
+    *have = 0;
+    int r = dalloc(ctx, p, (size_t)need);
+    if (r) return r;
+    *have = need;
+    return PAM_OK;
+}
+
+// RTM (imaging) or FWI gradient (F_GRADM) over shots, on-the-fly strategy
+static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, bool grad, float *image, float *ilum) {
+    int r = ready(ctx);
+    if (r) return r;
+    if (nshots < 1) return fail(ctx, PAM_E_ARG, "nshots must be >= 1");
+    if (!data || !image) return fail(ctx, PAM_E_ARG, "data and image must not be NULL");
+    const pam_grid &g = ctx->g;
+    const int nt = g.nt;
+    const long long nd = (long long)(nt - g.itlag) * ctx->nrec;
+    const long long nint = (long long)g.nz * g.nx;
+    const bool linear_adj = (ctx->phys >= 1 && ctx->abc == PAM_CERJAN);
+    const int skind = linear_adj ? SV_F32_PAD : SV_F32_INT;
+    if (ctx->storage != PAM_STORE_ALL_F32)
+        return fail(ctx, PAM_E_UNSUPPORTED, "only PAM_STORE_ALL_F32 is available in this build of the adjoint");
+    const long long per_step = (skind == SV_F32_PAD) ? ctx->plane : nint;
+    // choose the batch so that the snapshots fit in free memory
+    size_t freeb = 0, totalb = 0;
+    HCK(hipMemGetInfo(&freeb, &totalb));
+    int B = batch_size(ctx, nshots);
+    const long long per_shot_bytes = per_step * nt * 4 + 2 * nint * 4 + (linear_adj ? 5 * ctx->plane * 4 : 0);
+    const long long have = (long long)ctx->save_bytes + ctx->acc_bytes + ctx->tmp_bytes;
+    const long long budget = (long long)((freeb + have) * 0.85);
+    while (B > 1 && (long long)B * per_shot_bytes > budget) --B;
+    if ((long long)B * per_shot_bytes > budget)
+        return fail(ctx, PAM_E_NOMEM, "one shot's snapshots (%lld bytes) exceed free device memory", per_shot_bytes);
+    r = ensure_scratch(ctx, &ctx->save, &ctx->save_bytes, (long long)B * per_step * nt * 4);
+    if (r) return r;
+    r = ensure_scratch(ctx, (void **)&ctx->acc_ilum, &ctx->acc_bytes, (long long)B * nint * 4 * 2);
+    if (r) return r;
+    ctx->acc_img = ctx->acc_ilum + (long long)B * nint;
+    if (linear_adj) {
+        r = ensure_scratch(ctx, (void **)&ctx->tmp, &ctx->tmp_bytes, (long long)B * 5 * ctx->plane * 4);
+        if (r) return r;
+    }
+    HCK(hipMemsetAsync(image, 0, nint * sizeof(float), ctx->st));
+    float *ilum_acc = ilum;
+    if (!ilum_acc) {
+        r = ensure_scratch(ctx, (void **)&ctx->ilum_tmp, &ctx->ilum_tmp_bytes, nint * 4);
+        if (r) return r;
+        ilum_acc = ctx->ilum_tmp;
+    }
+    HCK(hipMemsetAsync(ilum_acc, 0, nint * sizeof(float), ctx->st));
+    ctx->loop_ms = ctx->kernel_ms = 0.0;
+    ctx->launches = 0;
+    ctx->next_k = -1;
+    const int tile0 = choose_tile(ctx, B);
+    for (int b0 = 0; b0 < nshots; b0 += B) {
+        const int nb = std::min(B, nshots - b0);
+        // ---- forward with snapshots and illumination
+        r = zero_shot_state(ctx, nb, true, true);
+        if (r) break;
+        r = upload_sources(ctx, shot0 + b0, nb);
+        if (r) break;
+        if (skind == SV_F32_PAD) HCK(hipMemsetAsync(ctx->save, 0, (size_t)nb * per_step * nt * 4, ctx->st));
+        HCK(hipMemsetAsync(ctx->acc_ilum, 0, (size_t)nb * nint * 4, ctx->st));
+        HCK(hipMemsetAsync(ctx->acc_img, 0, (size_t)nb * nint * 4, ctx->st));
+        KCK(launch_inject_point(ctx->U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, nb, ctx->st));
+        StepArgs a;
+        base_step_args(ctx, a, nb);
+        a.flags |= F_SRC | F_SAVE | F_ILUM;
+        a.save_kind = skind;
+        a.save = ctx->save;
+        a.save_kstride = per_step;
+        a.save_sstride = per_step * nt;
+        a.ilum = ctx->acc_ilum;
+        a.image = ctx->acc_img;
+        a.acc_sstride = nint;
+        r = run_forward_loop(ctx, a, tile0, false);
+        if (r) break;
+        // ---- backward: lambda fields reuse U0/U1
+        r = zero_shot_state(ctx, nb, true, ctx->reset_adj_cpml != 0);
+        if (r) break;
+        const int t0 = nt - 2;
+        if (t0 < 1) continue;
+        if (t0 >= g.itlag)
+            KCK(launch_inject_rec(ctx->U0, ctx->plane, ctx->P, ctx->d_rz, ctx->d_rx, ctx->nrec,
+                                  data + (long long)b0 * nd, nd, t0 - g.itlag, g.dx * g.dz, nb, ctx->st));
+        float *cur = ctx->U0, *prv = ctx->U1;
+        if (!linear_adj) {
+            StepArgs b;
+            base_step_args(ctx, b, nb);
+            b.flags |= F_RECINJ | (grad ? F_GRADM : F_IMAGE);
+            b.data = data + (long long)b0 * nd;
+            b.dstride = nd;
+            b.save_kind = skind;
+            b.save = ctx->save;
+            b.save_kstride = per_step;
+            b.save_sstride = per_step * nt;
+            b.image = ctx->acc_img;
+            b.acc_sstride = nint;
+            for (int t = t0; t >= 1; --t) {
+                r = fwd_step(ctx, b, tile0, cur, prv, t, nullptr);
+                if (r) return r;
+                std::swap(cur, prv);
+            }
+        } else {
+            AdjArgs c;
+            std::memset(&c, 0, sizeof(c));
+            c.nzp = ctx->nzp;
+            c.nxp = ctx->nxp;
+            c.P = ctx->P;
+            c.N = g.n_abc;
+            c.nz = g.nz;
+            c.nx = g.nx;
+            c.nshots = nb;
+            c.sstride = ctx->plane;
+            c.idx2 = 1.0f / (g.dx * g.dx);
+            c.idz2 = 1.0f / (g.dz * g.dz);
+            c.idx = 1.0f / g.dx;
+            c.idz = 1.0f / g.dz;
+            c.idxdz = 1.0f / (g.dz * g.dx);
+            c.idt2 = 1.0f / (g.dt * g.dt);
+            c.dxdz = g.dx * g.dz;
+            c.nt = nt;
+            c.itlag = g.itlag;
+            c.nrec = ctx->nrec;
+            c.v2 = ctx->v2;
+            c.eps = ctx->eps;
+            c.del = ctx->del;
+            c.cs = ctx->cs;
+            c.sn = ctx->sn;
+            c.cerx = ctx->cerx;
+            c.cerz = ctx->cerz;
+            c.tmp = ctx->tmp;
+            c.tsstride = 5 * ctx->plane;
+            c.rrow = ctx->d_rrow;
+            c.rhead = ctx->d_rhead;
+            c.rnext = ctx->d_rnext;
+            c.data = data + (long long)b0 * nd;
+            c.dstride = nd;
+            c.image = grad ? nullptr : ctx->acc_img;
+            c.acc_sstride = nint;
+            c.psstride = per_step * nt;
+            if (grad) return fail(ctx, PAM_E_UNSUPPORTED, "VTI/TTI Cerjan FWI gradient not available yet");
+            for (int t = t0; t >= 1; --t) {
+                c.cur = cur;
+                c.prv = prv;
+                c.k = t;
+                c.P_ = (const float *)ctx->save + (long long)t * per_step;
+                KCK(launch_adj_products(ctx->phys, c, ctx->st));
+                KCK(launch_adj_update(ctx->phys, c, ctx->st));
+                std::swap(cur, prv);
+            }
+        }
+        KCK(launch_reduce_shots(ctx->acc_img, nint, nb, (int)nint, image, 1, ctx->st));
+        KCK(launch_reduce_shots(ctx->acc_ilum, nint, nb, (int)nint, ilum_acc, 1, ctx->st));
+    }
+    if (r) return r;
+    if (ctx->normalize) {
+        KCK(launch_maxabs(ilum_acc, (int)nint, ctx->maxv, ctx->st));
+        KCK(launch_normalize(image, ilum_acc, (int)nint, ctx->normalize, ctx->maxv, ctx->st));
+    }
+    HCK(hipStreamSynchronize(ctx->st));
+    return PAM_OK;
+}
+
+extern "C" int pam_rtm(pam_ctx *ctx, int shot0, int nshots, const float *data, float *image, float *ilum) {
+    if (!ctx) return PAM_E_ARG;
+    return adjoint_run(ctx, shot0, nshots, data, false, image, ilum);
+}
+
+extern "C" int pam_gradient(pam_ctx *ctx, int shot0, int nshots, const float *residual, float *g_m, float *g_eps,
+                            float *g_delta, float *g_theta, float *ilum) {
+    if (!ctx) return PAM_E_ARG;
+    if (g_eps || g_delta || g_theta)
+        return fail(ctx, PAM_E_UNSUPPORTED, "multiparameter partials not available yet");
+    return adjoint_run(ctx, shot0, nshots, residual, true, g_m, ilum);
+}
+
+extern "C" int pam_misfit(pam_ctx *ctx, int nshots, const float *dobs, const float *dcal, float *residual,
+                          double *misfit) {
+    if (!ctx) return PAM_E_ARG;
+    if (!dobs || !dcal || !misfit) return fail(ctx, PAM_E_ARG, "NULL argument");
+    const long long n = (long long)nshots * (ctx->g.nt - ctx->g.itlag) * ctx->nrec;
+    HCK(hipMemsetAsync(ctx->d_acc, 0, sizeof(double), ctx->st));
+    KCK(launch_residual(dobs, dcal, residual, n, ctx->d_acc, ctx->st));
+    HCK(hipMemcpyAsync(misfit, ctx->d_acc, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HCK(hipStreamSynchronize(ctx->st));
+    return PAM_OK;
+}
+
+extern "C" int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const float *dseis, float *g_vp, int layout) {
+    (void)shot0;
+    (void)nshots;
+    (void)dseis;
+    (void)g_vp;
+    (void)layout;
+    if (!ctx) return PAM_E_ARG;
+    return fail(ctx, PAM_E_UNSUPPORTED, "exact adjoint not available yet");
+}

@@ -1,0 +1,131 @@
+// pam_internal.h -- kernel argument blocks and launch helpers shared by
+// pam_kernels.hip (device code) and pam_api.cpp (the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pam {
+
+// flags of StepArgs::flags (wave-uniform switches inside k_step)
+enum : int {
+    F_CERJAN = 1,       // Cerjan damping (cerx/cerz)
+    F_SRC = 2,          // writer-side injection of wav[k+1] at (ssz, ssx)
+    F_GATHER = 4,       // receiver gather of the input field into seis row k-itlag
+    F_RECINJ = 8,       // writer-side injection of data row (k-1)-itlag / dxdz at receivers
+    F_SAVE = 16,        // store the (damped) input field into save slot k
+    F_ILUM = 32,        // ilum += input^2 on the interior
+    F_IMAGE = 64,       // image += S_k * input on the interior
+    F_GRADM = 128,      // grad += input * (S_{k+1} - 2 S_k + S_{k-1}) / dt^2 on the interior
+    F_ADJ_EXACT = 256,  // (reserved)
+};
+#define MODE_HAS_CERJAN(f) (((f)&F_CERJAN) != 0)
+
+// snapshot storage kinds
+enum : int { SV_NONE = 0, SV_F32_INT = 1, SV_F32_PAD = 2, SV_F16_INT = 3, SV_BF16_INT = 4 };
+
+struct StepArgs {
+    int nzp, nxp, P, N;   // padded grid, row pitch (floats), absorbing width
+    int nz, nx;           // interior
+    int nxt, nzt, nshots; // x tiles, z tiles, shots in this launch
+    long long sstride;    // floats between consecutive shots' fields
+    float idx2, idz2, idx, idz, idxdz, idt2, dxdz;
+    int k;                // step index of the input field
+    int nt, itlag, nrec;
+    int flags;
+
+    const float *cur;     // input field c_k (shot 0 of the launch)
+    float *prv;           // previous field in, new field out (in place)
+    // model coefficient fields, padded [nzp][P], shared by the shots
+    const float *v2;      // (vp*vp)*(dt*dt)
+    const float *eps, *del, *cs, *sn;
+    const float *cerx, *cerz;  // Cerjan factors per column [P] / per row [nzp]
+    // CPML (full-grid fields, zero outside the strips)
+    const float *psix, *psiz;  // new psi (written by k_psi this step)
+    float *zetax, *zetaz;
+    const float *cax, *cbx, *caz, *cbz;
+    // sources (writer injection)
+    const int *ssz, *ssx;      // [nshots]
+    const float *wav;          // [nt]
+    // receivers
+    const int *rrow;           // [nzp] -> slot or -1
+    const int *rhead;          // [nslot][P] -> first receiver at the cell or -1
+    const int *rnext;          // [nrec] -> next receiver at the same cell or -1
+    float *seis;               // gather target [nshots][nt_data][nrec]
+    const float *data;         // injection source [nshots][nt_data][nrec]
+    long long dstride;         // floats between shots in seis/data
+    // snapshots
+    int save_kind;
+    void *save;                // base of shot 0's snapshot buffer
+    long long save_sstride;    // elements between shots
+    long long save_kstride;    // elements between steps
+    const float *save_scale;   // [nt] per-snapshot scale (compressed kinds)
+    const float *save_prev, *save_next;  // (unused; kept for ABI stability)
+    // interior accumulators, per shot [nshots][nz][nx]
+    float *ilum, *image;
+    long long acc_sstride;
+};
+
+// CPML psi pre-pass
+struct PsiArgs {
+    int nzp, nxp, P, N, nshots;
+    long long sstride;
+    float idx, idz;
+    const float *cur;
+    float *psix, *psiz;
+    const float *cax, *cbx, *caz, *cbz;
+};
+
+// linearised VTI/TTI adjoint (Cerjan)
+struct AdjArgs {
+    int nzp, nxp, P, N, nz, nx, nshots;
+    long long sstride;
+    float idx2, idz2, idx, idz, idxdz, idt2, dxdz;
+    int k, nt, itlag, nrec, flags;
+    const float *cur;      // lambda_t
+    float *prv;            // lambda_{t+1} in, lambda_{t-1} out
+    const float *P_;       // source field S_t, padded (damped), shot 0
+    long long psstride;    // stride between shots in P_
+    const float *v2, *eps, *del, *cs, *sn, *cerx, *cerz;
+    float *tmp;            // product scratch [nshots][5][nzp][P]
+    long long tsstride;
+    const int *rrow, *rhead, *rnext;
+    const float *data;
+    long long dstride;
+    float *image;          // per shot
+    long long acc_sstride;
+};
+
+struct TileShape {
+    int V, TZ;
+};
+int tile_count();
+TileShape tile_shape(int idx);
+
+// launchers (return hipError_t as int)
+int launch_step(int phys, bool cpml, int tile, const StepArgs &a, hipStream_t st);
+int launch_psi(const PsiArgs &a, hipStream_t st);
+int launch_adj_products(int phys, const AdjArgs &a, hipStream_t st);
+int launch_adj_update(int phys, const AdjArgs &a, hipStream_t st);
+
+// small utility kernels
+int launch_expand(const float *m, int nz, int nx, int N, float *out, int P, hipStream_t st);
+int launch_copy_padded(const float *m, int nzp, int nxp, float *out, int P, hipStream_t st);
+int launch_coeffs(int phys, const float *vp, const float *theta_in, float dt, int nzp, int nxp, int P, float *v2,
+                  float *cs, float *sn, hipStream_t st);
+int launch_cpml_coeffs(const float *vp, int nzp, int nxp, int P, int N, float dx, float dz, float dt, float d0,
+                       float fpico, float *ax, float *bx, float *az, float *bz, hipStream_t st);
+int launch_inject_point(float *U, long long sstride, const int *ssz, const int *ssx, int P, const float *wav,
+                        int k, int nshots, hipStream_t st);
+int launch_inject_rec(float *U, long long sstride, int P, const int *rz, const int *rx, int nrec,
+                      const float *data, long long dstride, int row, float dxdz, int nshots, hipStream_t st);
+int launch_pack_padded(const float *U, int nzp, int nxp, int P, float *out, hipStream_t st);
+int launch_reduce_shots(const float *parts, long long sstride, int nshots, int n, float *out, int accumulate,
+                        hipStream_t st);
+int launch_normalize(float *image, const float *ilum, int n, int mode, const float *maxv, hipStream_t st);
+int launch_maxabs(const float *x, int n, float *out, hipStream_t st);
+int launch_residual(const float *dobs, const float *dcal, float *res, long long n, double *acc, hipStream_t st);
+int launch_grad_multi(int phys, const float *S, long long sk, const float *lam, long long lk, int nz, int nx,
+                      float dx, float dz, const float *eps, const float *del, const float *theta, float *geps,
+                      float *gdel, float *gth, hipStream_t st);
+
+}  // namespace pam

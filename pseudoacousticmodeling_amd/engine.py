@@ -1,0 +1,165 @@
+"""Thin Python owner of a libpam context.
+
+``Engine`` holds one ``pam_ctx`` and passes torch tensors (device memory and
+the current torch stream) through the C ABI.  It is the layer the reference
+driver facades (``wavefield``, ``migration``, ``fwi``) and the autograd
+function sit on.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+def _f32(t, device):
+    if t is None:
+        return None
+    if not torch.is_tensor(t):
+        t = torch.as_tensor(np.ascontiguousarray(t, dtype=np.float32))
+    return t.to(device=device, dtype=torch.float32).contiguous()
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Engine:
+    """One grid, one physics, one absorbing boundary, up to ``max_shots`` shots
+    advanced together."""
+
+    def __init__(self, nz, nx, n_abc, dz, dx, dt, nt, itlag, physics="acoustic", abc="cerjan",
+                 cerjan_factor=0.0, cpml_d0=0.0, cpml_f_pico=0.0, max_shots=1, device="cuda:0"):
+        self.lib = L.load()
+        if not torch.cuda.is_available():
+            raise RuntimeError("libpam needs a GPU (torch.cuda.is_available() is False)")
+        self.device = torch.device(device)
+        phys = {"acoustic": L.PAM_ACOUSTIC, "VTI": L.PAM_VTI, "TTI": L.PAM_TTI}.get(physics, physics)
+        ab = {"cerjan": L.PAM_CERJAN, "CPML": L.PAM_CPML}.get(abc, abc)
+        self.physics, self.abc = phys, ab
+        self.grid = L.PamGrid(int(nz), int(nx), int(n_abc), float(dz), float(dx), float(dt), int(nt), int(itlag))
+        self.params = L.PamAbcParams(float(cerjan_factor), float(cpml_d0), float(cpml_f_pico))
+        self.nz, self.nx, self.N = int(nz), int(nx), int(n_abc)
+        self.nzp, self.nxp = self.nz + 2 * self.N, self.nx + 2 * self.N
+        self.nt, self.itlag = int(nt), int(itlag)
+        self.nt_data = self.nt - self.itlag
+        self.max_shots = int(max_shots)
+        self.nrec = 0
+        self.nshot = 0
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            ctx = ctypes.c_void_p()
+            rc = self.lib.pam_create(ctypes.byref(self.grid), phys, ab, ctypes.byref(self.params),
+                                     ctypes.c_void_p(stream), self.max_shots, ctypes.byref(ctx))
+        self.ctx = ctx
+        if rc != L.PAM_OK:
+            msg = self.lib.pam_last_error(ctx) if ctx else b"allocation failed"
+            if ctx:
+                self.lib.pam_destroy(ctx)
+                self.ctx = None
+            raise L.PamError(f"pam_create: {L.ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.pam_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        return L.check(self.ctx, rc)
+
+    # -- set-up ----------------------------------------------------------------
+    def set_option(self, key, value):
+        self._ck(self.lib.pam_set_option(self.ctx, int(key), int(value)))
+
+    def set_model(self, vp, eps=None, delta=None, theta=None, layout=L.PAM_INTERIOR):
+        ts = [_f32(x, self.device) for x in (vp, eps, delta, theta)]
+        self._model_keep = ts
+        self._ck(self.lib.pam_set_model(self.ctx, *[_ptr(t) for t in ts], int(layout)))
+
+    def set_wavelet(self, src):
+        t = _f32(src, self.device)
+        if t.numel() != self.nt:
+            raise ValueError(f"wavelet has {t.numel()} samples, nt = {self.nt}")
+        self._ck(self.lib.pam_set_wavelet(self.ctx, _ptr(t)))
+
+    def set_geometry(self, sz, sx, rz, rx):
+        a = [np.ascontiguousarray(np.asarray(v).ravel(), dtype=np.int32) for v in (sz, sx, rz, rx)]
+        self._geom = a
+        p = [x.ctypes.data_as(ctypes.c_void_p) for x in a]
+        self.nshot, self.nrec = len(a[0]), len(a[2])
+        self._ck(self.lib.pam_set_geometry(self.ctx, p[0], p[1], self.nshot, p[2], p[3], self.nrec))
+
+    # -- per step --------------------------------------------------------------
+    def reset(self, shot):
+        self._ck(self.lib.pam_reset(self.ctx, int(shot)))
+
+    def forward_step(self, k, seis=None):
+        self._ck(self.lib.pam_forward_step(self.ctx, int(k), _ptr(seis)))
+
+    def get_field(self, which=0):
+        out = torch.empty((self.nzp, self.nxp), dtype=torch.float32, device=self.device)
+        self._ck(self.lib.pam_get_field(self.ctx, int(which), _ptr(out)))
+        return out
+
+    # -- whole runs ------------------------------------------------------------
+    def forward(self, shot0=0, nshots=None, out=None):
+        nshots = self.nshot - shot0 if nshots is None else nshots
+        if out is None:
+            out = torch.empty((nshots, self.nt_data, self.nrec), dtype=torch.float32, device=self.device)
+        self._ck(self.lib.pam_forward(self.ctx, int(shot0), int(nshots), _ptr(out)))
+        return out
+
+    def rtm(self, data, shot0=0, nshots=None):
+        nshots = data.shape[0] if nshots is None else nshots
+        d = _f32(data, self.device)
+        image = torch.empty((self.nz, self.nx), dtype=torch.float32, device=self.device)
+        ilum = torch.empty_like(image)
+        self._ck(self.lib.pam_rtm(self.ctx, int(shot0), int(nshots), _ptr(d), _ptr(image), _ptr(ilum)))
+        return image, ilum
+
+    def gradient(self, residual, shot0=0, nshots=None, multiparameter=False):
+        nshots = residual.shape[0] if nshots is None else nshots
+        d = _f32(residual, self.device)
+        g = torch.empty((self.nz, self.nx), dtype=torch.float32, device=self.device)
+        ilum = torch.empty_like(g)
+        ge = gd = gt = None
+        if multiparameter:
+            ge = torch.zeros_like(g)
+            gd = torch.zeros_like(g)
+            if self.physics == L.PAM_TTI:
+                gt = torch.zeros_like(g)
+        self._ck(self.lib.pam_gradient(self.ctx, int(shot0), int(nshots), _ptr(d), _ptr(g), _ptr(ge), _ptr(gd),
+                                       _ptr(gt), _ptr(ilum)))
+        return dict(g_m=g, g_eps=ge, g_delta=gd, g_theta=gt, ilum=ilum)
+
+    def misfit(self, dobs, dcal, want_residual=True):
+        dobs = _f32(dobs, self.device)
+        dcal = _f32(dcal, self.device)
+        res = torch.empty_like(dobs) if want_residual else None
+        X = ctypes.c_double()
+        self._ck(self.lib.pam_misfit(self.ctx, int(dobs.shape[0]), _ptr(dobs), _ptr(dcal), _ptr(res),
+                                     ctypes.byref(X)))
+        return X.value, res
+
+    def forward_adjoint(self, dseis, shot0=0, nshots=None, layout=L.PAM_INTERIOR, out=None):
+        nshots = dseis.shape[0] if nshots is None else nshots
+        d = _f32(dseis, self.device)
+        if out is None:
+            shape = (self.nz, self.nx) if layout == L.PAM_INTERIOR else (self.nzp, self.nxp)
+            out = torch.zeros(shape, dtype=torch.float32, device=self.device)
+        self._ck(self.lib.pam_forward_adjoint(self.ctx, int(shot0), int(nshots), _ptr(d), _ptr(out), int(layout)))
+        return out
+
+    def timing(self):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        self._ck(self.lib.pam_last_timing(self.ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return dict(loop_ms=a.value, kernel_ms=b.value, kernel_launches=c.value)

@@ -1,0 +1,67 @@
+"""GPU parity of the forward step (acoustic / VTI / TTI x Cerjan / CPML)
+against the CPU oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, rel_l2
+from tests.problems import make_problem, oracle_forward, make_engine
+
+pytestmark = pytest.mark.gpu
+
+# FP32 GPU vs the FP32 restatement of the same operation order: relative L2 of
+# the seismogram (the north-star bar is 1e-4)
+TOL_F32 = 1e-4
+# TTI with a 1e4-amplified source (so that |den| > 1e-12 and Sd is active even
+# at the source spike): Sd is discontinuous at grad U = 0, and at the spike the
+# gradient direction is set by rounding.  Two CPU builds of the same oracle
+# source (with and without FMA contraction) already differ by 2e-5..4e-5 on
+# this case, and the GPU's reciprocal-multiply / contraction choices by 1e-4.
+TOL_TTI_AMPLIFIED = 3e-4
+
+CASES = [(p, a) for p in (0, 1, 2) for a in (0, 1)]
+
+
+@pytest.mark.parametrize("phys,abc", CASES)
+@pytest.mark.parametrize("amp", [1.0, 1e4])
+def test_forward_matches_oracle(phys, abc, amp):
+    pb = make_problem(phys, abc, amp=amp, seed=10 + 3 * phys + abc)
+    print(f"case phys={phys} abc={abc} amp={amp}", flush=True)
+    ref32 = oracle_forward(Oracle("f32"), pb)
+    ref64 = oracle_forward(Oracle("f64"), pb)
+    eng = make_engine(pb)
+    seis = eng.forward(0, 1)[0].cpu().numpy()
+    e32 = rel_l2(seis, ref32["seis"])
+    e64 = rel_l2(seis, ref64["seis"])
+    print(f"  rel_l2 vs f32 oracle {e32:.3e}, vs f64 oracle {e64:.3e}, |seis|max {np.abs(seis).max():.3e}")
+    assert np.isfinite(seis).all()
+    assert np.abs(seis).max() > 0
+    assert e32 <= (TOL_TTI_AMPLIFIED if (phys == 2 and amp > 1.0) else TOL_F32)
+
+
+@pytest.mark.parametrize("phys,abc", [(0, 0), (1, 1), (2, 0)])
+def test_batched_equals_single(phys, abc):
+    pb = make_problem(phys, abc, nshots=5, seed=3)
+    eng1 = make_engine(pb, max_shots=1)
+    single = np.stack([eng1.forward(s, 1)[0].cpu().numpy() for s in range(5)])
+    eng5 = make_engine(pb, max_shots=5)
+    batched = eng5.forward(0, 5).cpu().numpy()
+    assert np.array_equal(single, batched)
+
+
+@pytest.mark.parametrize("phys,abc", [(0, 0), (1, 1)])
+def test_per_step_api_matches_whole_run(phys, abc):
+    import torch
+
+    pb = make_problem(phys, abc, seed=5)
+    eng = make_engine(pb)
+    whole = eng.forward(0, 1)[0]
+    seis = torch.zeros_like(whole)
+    eng.reset(0)
+    for k in range(pb["g"].nt):
+        eng.forward_step(k, seis)
+    assert torch.equal(seis, whole)
+    # the current field after the last step is the oracle's current field
+    # plus nothing (no injection beyond nt)
+    ref = oracle_forward(Oracle("f32"), pb)
+    cur = eng.get_field(0).cpu().numpy()
+    assert rel_l2(cur, ref["current"]) <= TOL_F32
