@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Throughput of the 2D finite-difference engine on one MI355X.
+
+Workload (one "step" = one pass of the hot path over one batch): forward
+modelling of the 64-shot pseudo-acoustic VTI + 50-cell CPML survey
+(nz=351, nx=1701, dx=dz=12.5 m -> 451 x 1801 padded cells, nt = 98 + 5001
+time steps, 1701 receivers per shot), all 64 shots advanced together in one
+launch per time step, seismograms written to HBM.  Inputs (models, wavelet,
+geometry) are resident on the device before the timed region.
+
+Prints one JSON line:
+  value      Gpt-updates/s = padded cells x time steps x shots / wall time
+  roofline   achieved algorithmic HBM GB/s of the dominant kernel (k_step),
+             from HIP events around every launch of the timed steps, against
+             the 8 TB/s HBM3E peak
+  plain_loop the same 64 shots run one shot per launch (the per-shot loop)
+  cpu_baseline the CPU restatement of the reference's Numba kernels
+             (float32 storage, float64 arithmetic, OpenMP over rows) on a
+             bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gpt-updates/s & %HBM roofline; FWI gradient s/shot, batched vs plain"
+
+
+def step_kernel_bytes(w, nshots):
+    """Algorithmic HBM bytes of one k_step launch advancing `nshots` shots one
+    time step: per padded cell 12 B per shot (U_k read, U_{k-1} read, U_{k+1}
+    write) plus 4 B per model field shared by the shots (acoustic 1, VTI 3,
+    TTI 4); per CPML strip cell (counted once per direction, so corners
+    twice) 12 B per shot (psi read, zeta read and write) plus 8 B of shared
+    damping coefficients."""
+    nzp, nxp, N = w["nz"] + 2 * w["N"], w["nx"] + 2 * w["N"], w["N"]
+    n_m = {0: 1, 1: 3, 2: 4}[w["physics"]]
+    b = (12 * nshots + 4 * n_m) * nzp * nxp
+    if w["abc"] == 1:
+        strip = (nzp - 8) * 2 * (N - 4) + 2 * (N - 4) * (nxp - 8)
+        b += (12 * nshots + 8) * strip
+    return b
+
+
+def make_engine(w, max_shots):
+    from pseudoacousticmodeling_amd.engine import Engine
+
+    eng = Engine(w["nz"], w["nx"], w["N"], w["dz"], w["dx"], w["dt"], w["nt"], w["itlag"],
+                 physics=w["physics"], abc=w["abc"], cerjan_factor=w["cerjan_factor"],
+                 cpml_d0=w["cpml_d0"], cpml_f_pico=w["cpml_f_pico"], max_shots=max_shots)
+    eng.set_model(w["vp"], w["eps"], w["delta"], w["theta"])
+    eng.set_wavelet(w["src"])
+    eng.set_geometry(w["sz"], w["sx"], w["rz"], w["rx"])
+    return eng
+
+
+def cpu_baseline(w, seconds):
+    """Time the oracle's OpenMP build (float32 storage, float64 arithmetic,
+    separate psi / zeta / update passes as the reference's Numba kernels run
+    them) on one shot of the same workload for a bounded number of steps."""
+    import numpy as np
+
+    from oracle.oracle import Grid, Oracle
+
+    orc = Oracle("mixed")
+    g = Grid(nz=w["nz"], nx=w["nx"], N=w["N"], dz=w["dz"], dx=w["dx"], dt=w["dt"], nt=w["nt"], itlag=w["itlag"])
+    model = tuple(orc.expand(m, g.N) if m is not None else None for m in (w["vp"], w["eps"], w["delta"], w["theta"]))
+    cpml = (w["cpml_d0"], w["cpml_f_pico"]) if w["abc"] == 1 else None
+    A = orc.cerjan(g.N, w["cerjan_factor"]) if w["abc"] == 0 else None
+    kw = dict(A=A, cpml=cpml)
+    sz, sx = int(w["sz"][0]), int(w["sx"][0])
+    t0 = time.perf_counter()
+    orc.forward_shot(w["physics"], w["abc"], g, model, w["src"], sz, sx, w["rz"], w["rx"], nsteps=10, **kw)
+    per = (time.perf_counter() - t0) / 10
+    nsteps = int(max(20, min(g.nt, seconds / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    orc.forward_shot(w["physics"], w["abc"], g, model, w["src"], sz, sx, w["rz"], w["rx"], nsteps=nsteps, **kw)
+    dt = time.perf_counter() - t0
+    cells = g.nzp * g.nxp
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": cells * nsteps / dt / 1e9, "unit": "Gpt-updates/s", "cores": threads, "kind": "port",
+            "sample": f"1 shot x {nsteps} of {g.nt} time steps of the same workload ({g.nzp}x{g.nxp} padded), "
+                      f"{dt:.1f} s on {threads} OpenMP threads of {cpu}; float32 storage, float64 arithmetic",
+            "seconds": dt, "nsteps": nsteps, "finite": bool(np.isfinite(model[0]).all())}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nt-data", type=int, default=5001, help="recorded samples (default: the full 5001)")
+    ap.add_argument("--tile", type=int, default=-1, help="k_step tile variant (-1: automatic)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the per-shot loop measurement")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample length (0: skip)")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(shots shard across GPUs by design, but only the one-GPU path is built and measured)",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from pseudoacousticmodeling_amd import _lib, workloads
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible (torch.cuda.is_available() is False)", file=sys.stderr)
+        return 1
+    _lib.load()
+    w = workloads.vti_cpml_64(nt_data=args.nt_data)
+    S = len(w["sx"])
+    print(f"bench: workload {w['name']} {w['nz'] + 2 * w['N']}x{w['nx'] + 2 * w['N']} padded, nt={w['nt']}, "
+          f"{S} shots", file=sys.stderr, flush=True)
+    eng = make_engine(w, S)
+    if args.tile >= 0:
+        eng.set_option(_lib.PAM_OPT_TILE, args.tile)
+    seis = torch.empty((S, w["nt"] - w["itlag"], len(w["rx"])), dtype=torch.float32, device="cuda:0")
+    for _ in range(args.warmup):
+        eng.forward(0, S, out=seis)
+    torch.cuda.synchronize()
+    kern_ms, launches = 0.0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.forward(0, S, out=seis)
+        t = eng.timing()
+        kern_ms += t["kernel_ms"]
+        launches += t["kernel_launches"]
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    finite = bool(torch.isfinite(seis).all().item())
+    upd = workloads.updates(w) * args.steps
+    value = upd / secs / 1e9
+    avg_ms = kern_ms / max(launches, 1)
+    bytes_launch = step_kernel_bytes(w, S)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    print(f"bench: {secs:.3f} s for {args.steps} runs, k_step avg {avg_ms * 1e3:.1f} us over {launches} launches, "
+          f"seis finite={finite}", file=sys.stderr, flush=True)
+
+    plain = None
+    if not args.no_plain:
+        eng.set_option(_lib.PAM_OPT_BATCH, 1)
+        eng.forward(0, 1, out=seis[:1])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.forward(0, S, out=seis)
+        torch.cuda.synchronize()
+        ps = time.perf_counter() - t0
+        t = eng.timing()
+        eng.set_option(_lib.PAM_OPT_BATCH, 0)
+        pavg = t["kernel_ms"] / max(t["kernel_launches"], 1)
+        plain = {"value": workloads.updates(w) / ps / 1e9, "unit": "Gpt-updates/s", "seconds": ps,
+                 "k_step_avg_us": pavg * 1e3,
+                 "k_step_gbs": step_kernel_bytes(w, 1) / (pavg * 1e-3) / 1e9,
+                 "batched_speedup": ps / (secs / args.steps)}
+        print(f"bench: plain per-shot loop {ps:.3f} s", file=sys.stderr, flush=True)
+    eng.close()
+    del seis
+    torch.cuda.empty_cache()
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        print("bench: cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(w, args.cpu_seconds)
+        print(f"bench: cpu {cpu['value']:.3f} Gpt-updates/s ({cpu['sample']})", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "Gpt-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": secs / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{w['name']}: VTI pseudo-acoustic forward modelling + 50-cell CPML, "
+                               f"{w['nz'] + 2 * w['N']}x{w['nx'] + 2 * w['N']} padded cells, nt={w['nt']}, "
+                               f"{S} shots in one batch, {len(w['rx'])} receivers per shot",
+                   "global_batch": S, "nt": w["nt"], "parallelism": "1 GPU, shots batched per launch"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step",
+                     "bytes_per_launch": bytes_launch, "avg_launch_us": avg_ms * 1e3},
+        "plain_loop": plain,
+        "cpu_baseline": cpu,
+        "finite": finite,
+    }
+    if cpu is not None:
+        out["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")}
+    print(json.dumps(out), flush=True)
+    return 0 if finite else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -9,6 +9,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -36,9 +37,12 @@ def _run(cmd):
 def build_lib(verbose: bool = True) -> str:
     objs = []
     os.makedirs(os.path.join(HERE, "_obj"), exist_ok=True)
-    srcs = [("pam_kernels.hip", True), ("pam_api.cpp", False)]
+    srcs = [(f, True) for f in sorted(os.listdir(CSRC)) if f.endswith(".hip")] + [("pam_api.cpp", False)]
     procs = []
+    jobs = int(os.environ.get("MAX_JOBS", "8"))
     for name, dev in srcs:
+        while len([p for p in procs if p.poll() is None]) >= jobs:
+            time.sleep(0.2)
         obj = os.path.join(HERE, "_obj", name + ".o")
         cmd = [HIPCC, f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, "-c"]
         if dev:

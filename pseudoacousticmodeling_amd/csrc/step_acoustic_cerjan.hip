@@ -1,0 +1,8 @@
+// step_acoustic_cerjan.hip -- k_step instances for acoustic physics with the cerjan boundary.
+#include "pam_step.cuh"
+
+namespace pam {
+int launch_step_acoustic_cerjan(int tile, const StepArgs &a, hipStream_t st) {
+    return launch_step_p<0, false>(tile, a, st);
+}
+}  // namespace pam

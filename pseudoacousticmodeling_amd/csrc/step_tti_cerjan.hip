@@ -1,0 +1,8 @@
+// step_tti_cerjan.hip -- k_step instances for tti physics with the cerjan boundary.
+#include "pam_step.cuh"
+
+namespace pam {
+int launch_step_tti_cerjan(int tile, const StepArgs &a, hipStream_t st) {
+    return launch_step_p<2, false>(tile, a, st);
+}
+}  // namespace pam

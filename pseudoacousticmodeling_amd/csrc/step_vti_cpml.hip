@@ -1,0 +1,8 @@
+// step_vti_cpml.hip -- k_step instances for vti physics with the cpml boundary.
+#include "pam_step.cuh"
+
+namespace pam {
+int launch_step_vti_cpml(int tile, const StepArgs &a, hipStream_t st) {
+    return launch_step_p<1, true>(tile, a, st);
+}
+}  // namespace pam

@@ -1,0 +1,116 @@
+"""Host-side set-up of a modelling run: derived sizes, source wavelets and the
+absorbing-boundary constants.
+
+These run once per run on the host (numpy) and feed the C ABI:
+
+* ``derived_sizes``      -- cpp/Survey.cpp:101-108 (``round``) and
+  src/survey.py:37-47 (``trunc``);
+* ``ricker_cpp``         -- cpp/Modeling.cu:84-100 (plain Ricker, float32);
+* ``ricker_py``          -- src/utils.py:21-28 with the 1/(dx dz) scale of
+  src/Modeling2D.py:27-31 (Ricker with a sqrt(i w) half-derivative filter);
+* ``cerjan_vector``      -- cpp/Modeling.cu:111-120 / src/Modeling2D.py:258-264;
+* ``dampening_const``    -- src/Modeling2D.py:110-126 (CPML d0 and f_pico);
+* ``grid_index``         -- source/receiver cell indices, cpp/Survey.cpp:150-158
+  (``round``) and src/survey.py:131-138 (``trunc``);
+* ``stability_acoustic`` -- cpp/Modeling.cu:287-292 / src/Modeling2D.py:202-223.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+PI_F32 = np.float32(3.14159265358979323846)
+
+
+def _cround(v) -> int:
+    """std::round: halves away from zero."""
+    v = float(v)
+    return int(math.floor(abs(v) + 0.5)) * (1 if v >= 0 else -1)
+
+
+def derived_sizes(L, D, T, dx, dz, dt, fcut, rounding="round"):
+    """tlag = 2 sqrt(pi)/fcut, itlag, nt_data, nx, nz and nt = itlag + nt_data.
+
+    ``round`` follows the C++ driver (float32 arithmetic, std::round);
+    ``trunc`` follows the Python driver (float64 arithmetic, int())."""
+    if rounding == "round":
+        f32 = np.float32
+        tlag = f32(2.0) * np.sqrt(PI_F32, dtype=np.float32) / f32(fcut)
+        itlag = _cround(f32(tlag) / f32(dt))
+        nt_data = _cround(f32(T) / f32(dt)) + 1
+        nx = _cround(f32(L) / f32(dx)) + 1
+        nz = _cround(f32(D) / f32(dz)) + 1
+    elif rounding == "trunc":
+        tlag = 2.0 * np.sqrt(np.pi) / fcut
+        itlag = int(tlag / dt)
+        nt_data = int(T / dt) + 1
+        nx = int(L / dx) + 1
+        nz = int(D / dz) + 1
+    else:
+        raise ValueError("rounding must be 'round' or 'trunc'")
+    return dict(tlag=float(tlag), itlag=int(itlag), nt_data=int(nt_data), nx=int(nx), nz=int(nz),
+                nt=int(itlag + nt_data))
+
+
+def grid_index(coord, h, N, rounding="round"):
+    """Padded-grid cell index of a coordinate: round(c/h)+N or int(c/h)+N."""
+    c = np.asarray(coord, dtype=np.float64)
+    if rounding == "round":
+        q = np.asarray(c, dtype=np.float32) / np.float32(h)
+        return (np.sign(q) * np.floor(np.abs(q) + 0.5)).astype(np.int32) + np.int32(N)
+    return (c / h).astype(np.int32) + np.int32(N)
+
+
+def ricker_cpp(nt, dt, fcut, tlag, dx, dz):
+    """(1 - 2 arg) exp(-arg) / (dx dz), arg = pi^3 fc^2 td^2, fc = fcut/(3 sqrt(pi)),
+    td = n dt - tlag; float32 arithmetic.  The peak frequency is fcut/3."""
+    f32 = np.float32
+    scale = f32(1.0) / (f32(dx) * f32(dz))
+    fc = f32(fcut) / (f32(3.0) * np.sqrt(PI_F32, dtype=np.float32))
+    n = np.arange(nt, dtype=np.float32)
+    td = n * f32(dt) - f32(tlag)
+    arg = PI_F32 * PI_F32 * PI_F32 * fc * fc * td * td
+    return ((f32(1.0) - f32(2.0) * arg) * np.exp(-arg) * scale).astype(np.float32)
+
+
+def ricker_py(nt, dt, fcut, tlag, dx, dz):
+    """Ricker followed by the sqrt(i w) half-derivative filter applied with an
+    FFT, scaled by 1/(dx dz); float64, returned as float32 (the cast the
+    reference makes when it uploads the wavelet)."""
+    pi = np.pi
+    f = fcut / (3 * np.sqrt(pi))
+    td = np.arange(nt) * dt - tlag
+    src = (1 - 2 * pi * (pi * f * td) ** 2) * np.exp(-pi * (pi * f * td) ** 2)
+    w = 2.0 * pi * np.fft.fftfreq(nt, dt)
+    src = np.real(np.fft.ifft(np.fft.fft(src) * np.sqrt(1j * w)))
+    return (src / (dx * dz)).astype(np.float32)
+
+
+def cerjan_vector(N, factor=0.0):
+    """A[i] = exp(-fb^2), i < N; fb = (N-i)/(1.4142*3N) for factor <= 0 (the
+    reference's taper), else fb = factor*(N-i).  float32 as on the device."""
+    i = np.arange(N, dtype=np.float32)
+    if factor is None or factor <= 0:
+        fb = (np.float32(N) - i) / (np.float32(1.4142) * (np.float32(3.0) * np.float32(N)))
+    else:
+        fb = np.float32(factor) * (np.float32(N) - i)
+    return np.exp(-fb * fb).astype(np.float32)
+
+
+def dampening_const(N, fcut):
+    """CPML constants: R = 1e-3^(N/10) (1e-3^15 for N >= 200), d0 = -(M+1) ln R
+    with M = 2, f_pico = fcut/3."""
+    R = 1e-3 ** (N / 10)
+    if N >= 200:
+        R = 1e-3 ** (150 / 10)
+    return float(-(2 + 1) * np.log(R)), float(fcut / 3)
+
+
+def stability_acoustic(vp, fcut, dx, dz, dt):
+    """lambda_min = vmin/fcut, dx_lim = lambda_min/4.28, dt_lim = dx_lim/(4 vmax)."""
+    lam = float(np.min(vp)) / fcut
+    dx_lim = lam / 4.28
+    dt_lim = dx_lim / (4.0 * float(np.max(vp)))
+    return dict(dx_lim=dx_lim, dz_lim=dx_lim, dt_lim=dt_lim,
+                ok=bool(dx <= dx_lim and dz <= dx_lim and dt <= dt_lim))

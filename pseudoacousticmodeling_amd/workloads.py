@@ -1,0 +1,84 @@
+"""Seeded synthetic workloads at the benchmark sizes.
+
+Each builder returns a dict with the grid, the interior models (float32
+[nz][nx]; theta in radians), the wavelet, and the source / receiver cell
+indices on the padded grid.  The reference ships no model or geometry files
+(its .bin and .csv inputs are git-ignored), so every workload here is built
+from a fixed seed.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import prep
+
+SEED0 = 20261203
+
+
+def _gauss(f, sigma_cells):
+    from scipy.ndimage import gaussian_filter
+
+    return gaussian_filter(np.asarray(f, dtype=np.float64), sigma_cells, mode="nearest")
+
+
+def _rescale(f, lo, hi):
+    f = np.asarray(f, dtype=np.float64)
+    span = max(float(f.max() - f.min()), 1e-30)
+    return lo + (hi - lo) * (f - f.min()) / span
+
+
+def vti_cpml_64(nshots=64, nt_data=5001, seed=SEED0 + 2):
+    """Pseudo-acoustic VTI with a 50-cell CPML: nz=351, nx=1701, dx=dz=12.5 m,
+    dt=1 ms, 5001 recorded samples; vp a linear gradient 1500->4500 m/s times
+    (1 + 0.1 n), n Gaussian noise smoothed with sigma = 200 m and scaled to
+    [-1, 1]; epsilon smoothed uniform in [0, 0.25]; delta in [0, min(eps, 0.1)];
+    64 surface shots every 250 m from x = 2750 m, Ricker 12 Hz peak
+    (fcut = 36 Hz); 1701 receivers at every grid column, z = 12.5 m."""
+    rng = np.random.default_rng(seed)
+    nz, nx, N, h, dt, fcut = 351, 1701, 50, 12.5, 1e-3, 36.0
+    z = np.arange(nz) * h
+    base = 1500.0 + 3000.0 * z / z[-1]
+    n = _gauss(rng.standard_normal((nz, nx)), 200.0 / h)
+    n = n / max(np.abs(n).max(), 1e-30)
+    vp = (base[:, None] * (1.0 + 0.1 * n)).astype(np.float32)
+    eps = _rescale(_gauss(rng.uniform(0.0, 0.25, (nz, nx)), 8.0), 0.0, 0.25).astype(np.float32)
+    dfrac = _rescale(_gauss(rng.uniform(0.0, 1.0, (nz, nx)), 8.0), 0.0, 1.0)
+    delta = (dfrac * np.minimum(eps, 0.1)).astype(np.float32)
+    tlag = 2.0 * np.sqrt(np.pi) / fcut
+    itlag = int(round(tlag / dt))
+    nt = itlag + nt_data
+    src = prep.ricker_cpp(nt, dt, fcut, np.float32(tlag), h, h)
+    sx = prep.grid_index(2750.0 + 250.0 * np.arange(nshots), h, N)
+    sz = prep.grid_index(np.full(nshots, 12.5), h, N)
+    rx = prep.grid_index(np.arange(nx) * h, h, N)
+    rz = prep.grid_index(np.full(nx, 12.5), h, N)
+    d0, fp = prep.dampening_const(N, fcut)
+    return dict(name="vti_cpml_64", physics=1, abc=1, nz=nz, nx=nx, N=N, dz=h, dx=h, dt=dt, nt=nt,
+                itlag=itlag, fcut=fcut, vp=vp, eps=eps, delta=delta, theta=None, src=src,
+                sz=sz, sx=sx, rz=rz, rx=rx, cpml_d0=d0, cpml_f_pico=fp, cerjan_factor=0.0)
+
+
+def acoustic_cerjan_1():
+    """Plain acoustic forward modelling with a 50-cell Cerjan strip (factor
+    0.0045): nz=201, nx=401, dx=dz=10 m; vp = 1500 m/s above z = 1000 m and
+    2500 m/s below; dt = 1 ms, 2001 recorded samples; Ricker 10 Hz peak
+    (fcut = 30 Hz) at (z=10 m, x=2000 m); 400 receivers every 10 m at z=10 m."""
+    nz, nx, N, h, dt, fcut = 201, 401, 50, 10.0, 1e-3, 30.0
+    vp = np.full((nz, nx), 1500.0, dtype=np.float32)
+    vp[100:, :] = 2500.0
+    sz_ = prep.derived_sizes(4000.0, 2000.0, 2.0, h, h, dt, fcut, "round")
+    itlag, nt = sz_["itlag"], sz_["nt"]
+    src = prep.ricker_cpp(nt, dt, fcut, np.float32(sz_["tlag"]), h, h)
+    sx = prep.grid_index([2000.0], h, N)
+    sz = prep.grid_index([10.0], h, N)
+    rx = prep.grid_index(np.arange(400) * h, h, N)
+    rz = prep.grid_index(np.full(400, 10.0), h, N)
+    return dict(name="acoustic_cerjan_1", physics=0, abc=0, nz=nz, nx=nx, N=N, dz=h, dx=h, dt=dt, nt=nt,
+                itlag=itlag, fcut=fcut, vp=vp, eps=None, delta=None, theta=None, src=src, sz=sz, sx=sx,
+                rz=rz, rx=rx, cpml_d0=0.0, cpml_f_pico=0.0, cerjan_factor=0.0045)
+
+
+def updates(w, nshots=None):
+    """Padded cells x time steps x shots."""
+    ns = len(w["sx"]) if nshots is None else nshots
+    return (w["nz"] + 2 * w["N"]) * (w["nx"] + 2 * w["N"]) * w["nt"] * ns
