@@ -329,14 +329,16 @@ extern "C" int pam_set_geometry(pam_ctx *ctx, const int *sz, const int *sx, int 
     if (!ctx) return PAM_E_ARG;
     if (nshot < 0 || nrec < 0) return fail(ctx, PAM_E_ARG, "negative counts");
     if ((nshot && (!sz || !sx)) || (nrec && (!rz || !rx))) return fail(ctx, PAM_E_ARG, "NULL geometry array");
+    // sources and receivers must lie where the stencil updates the field: the
+    // outer 4-cell rim of the padded grid is never updated
     for (int i = 0; i < nshot; ++i)
-        if (sz[i] < 0 || sz[i] >= ctx->nzp || sx[i] < 0 || sx[i] >= ctx->nxp)
-            return fail(ctx, PAM_E_ARG, "source %d at (%d,%d) outside the %dx%d padded grid", i, sz[i], sx[i],
-                        ctx->nzp, ctx->nxp);
+        if (sz[i] < 4 || sz[i] >= ctx->nzp - 4 || sx[i] < 4 || sx[i] >= ctx->nxp - 4)
+            return fail(ctx, PAM_E_ARG, "source %d at (%d,%d) outside the stencil interior [4,%d)x[4,%d)", i, sz[i],
+                        sx[i], ctx->nzp - 4, ctx->nxp - 4);
     for (int i = 0; i < nrec; ++i)
-        if (rz[i] < 0 || rz[i] >= ctx->nzp || rx[i] < 0 || rx[i] >= ctx->nxp)
-            return fail(ctx, PAM_E_ARG, "receiver %d at (%d,%d) outside the %dx%d padded grid", i, rz[i], rx[i],
-                        ctx->nzp, ctx->nxp);
+        if (rz[i] < 4 || rz[i] >= ctx->nzp - 4 || rx[i] < 4 || rx[i] >= ctx->nxp - 4)
+            return fail(ctx, PAM_E_ARG, "receiver %d at (%d,%d) outside the stencil interior [4,%d)x[4,%d)", i,
+                        rz[i], rx[i], ctx->nzp - 4, ctx->nxp - 4);
     ctx->hsz.assign(sz, sz + nshot);
     ctx->hsx.assign(sx, sx + nshot);
     ctx->hrz.assign(rz, rz + nrec);
@@ -387,13 +389,9 @@ static int ready(pam_ctx *ctx) {
 }
 
 static int choose_tile(pam_ctx *ctx, int nshots) {
+    (void)nshots;
     if (ctx->tile >= 0) return ctx->tile;
-    // many waves available: wide tiles, tall z-march; few: narrower, shorter tiles
-    const long long cells = (long long)ctx->nzp * ctx->nxp * nshots;
-    if (ctx->phys == 2) return cells > 4000000 ? 0 : 5;
-    if (cells > 4000000) return 1;
-    if (cells > 1000000) return 0;
-    return 5;
+    return 0;
 }
 
 static void base_step_args(pam_ctx *ctx, StepArgs &a, int nshots) {

@@ -18,7 +18,7 @@
 //
 // Reference kernels restated (semantics, not code): cpp/Modeling.cu:481-749,
 // src/CudaKernels.py:3-721 and :1174-1715, src/utils.py:380-942.
-#include "pam_step.cuh"
+#include "pam_variants.cuh"
 
 namespace pam {
 

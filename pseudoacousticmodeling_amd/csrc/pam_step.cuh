@@ -446,10 +446,6 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs a) {
     }
 }
 
-// tile shapes (cells per lane V along x, rows per wavefront TZ) of k_step
-static constexpr TileShape kTiles[] = {{4, 16}, {4, 32}, {2, 16}, {2, 32}, {4, 8}, {2, 8}};
-static constexpr int kNumTiles = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
-
 template <int PHYS, bool CPML, int V, int TZ>
 static int launch_step_t(const StepArgs &a0, hipStream_t st) {
     StepArgs a = a0;
@@ -460,19 +456,6 @@ static int launch_step_t(const StepArgs &a0, hipStream_t st) {
     blocks = (blocks + 7) / 8 * 8;
     hipLaunchKernelGGL((k_step<PHYS, CPML, V, TZ>), dim3((unsigned)blocks), dim3(256), 0, st, a);
     return (int)hipGetLastError();
-}
-
-template <int PHYS, bool CPML>
-static int launch_step_p(int tile, const StepArgs &a, hipStream_t st) {
-    switch (tile) {
-        case 0: return launch_step_t<PHYS, CPML, 4, 16>(a, st);
-        case 1: return launch_step_t<PHYS, CPML, 4, 32>(a, st);
-        case 2: return launch_step_t<PHYS, CPML, 2, 16>(a, st);
-        case 3: return launch_step_t<PHYS, CPML, 2, 32>(a, st);
-        case 4: return launch_step_t<PHYS, CPML, 4, 8>(a, st);
-        case 5: return launch_step_t<PHYS, CPML, 2, 8>(a, st);
-        default: return (int)hipErrorInvalidValue;
-    }
 }
 
 }  // namespace pam

@@ -1,0 +1,367 @@
+// pam_tile.cuh -- the LDS-tiled fused time step k_tile.
+//
+// One 256-thread block advances one TX x TZ tile (TX = 64 columns) of one
+// shot by one time step.  The block first issues every global load it needs
+// at once -- the current field U_k over the tile plus a 4-cell halo (into
+// LDS), and its own cells' U_{k-1} and model coefficients (into registers) --
+// so that each wavefront keeps many loads in flight, then computes from LDS.
+// Each thread owns 4 consecutive columns (one float4) of R = TZ/16
+// consecutive rows and keeps a register window of the R+8 rows its
+// z-derivatives need; x-derivatives come from the LDS row with its halo.
+// TTI also stages the first x-derivative of every tile row (with the z-halo)
+// in LDS, so that the mixed derivative is d/dz of it.
+//
+// The rest of the step is fused into the same pass: Cerjan damping (the
+// damping the reference applies to U_k after the update is re-applied when
+// U_k is read back as U_{k-1}), the CPML zeta update with the psi written by
+// k_psi this step, the source injection of the next step (added to the value
+// written), the receiver gather of this step, and for the adjoint passes the
+// data injection, snapshots, illumination and imaging / gradient terms.
+//
+// Blocks are ordered tile-major with the shots of a tile adjacent, so the
+// model coefficients of a tile are fetched from HBM once and reused from L2
+// by the other shots; the XCD remap gives each XCD a contiguous range.
+#pragma once
+#include "pam_step.cuh"
+
+namespace pam {
+
+template <int PHYS, bool CPML, int TZ, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_tile(const StepArgs a) {
+    constexpr int TX = 64;
+    constexpr int R = TZ / 16;        // rows per thread
+    constexpr int ROWS = TZ + 8;      // staged rows (4-row halo each side)
+    constexpr int Q = (TX + 8) / 4;   // float4 per staged row (4-cell halo each side)
+    constexpr int NL = (ROWS * Q + 255) / 256;
+    __shared__ float4 su[ROWS * Q];
+    __shared__ float4 sd[(PHYS == 2) ? ROWS * (TX / 4) : 1];
+
+    const int nb = a.nxt * a.nzt * a.nshots;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    if (b >= nb) return;  // block-uniform
+    const int tile = b / a.nshots;
+    const int s = b - tile * a.nshots;
+    const int tzi = tile / a.nxt;
+    const int txi = tile - tzi * a.nxt;
+    const int x0 = txi * TX, z0 = tzi * TZ;
+    const int P = a.P, nzp = a.nzp, nxp = a.nxp, N = a.N;
+    const long long so = (long long)s * a.sstride;
+    const float *__restrict__ cur = a.cur + so;
+    float *__restrict__ prv = a.prv + so;
+    const int tid = threadIdx.x;
+    const int flags = a.flags;
+    const int cx = tid & 15, rg = tid >> 4;
+    const int col = x0 + 4 * cx;   // first of this thread's 4 columns
+    const int zr0 = z0 + rg * R;   // first of this thread's R rows
+
+    // ---- issue every load: the U_k tile with halo, U_{k-1} and the model
+    float4 st[NL];
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+        const int i = tid + 256 * u;
+        const int r = i / Q, q = i - r * Q;
+        const int gz = z0 - 4 + r, gc = x0 - 4 + 4 * q;
+        st[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < ROWS * Q && gz >= 0 && gz < nzp && gc >= 0 && gc < P)
+            st[u] = *reinterpret_cast<const float4 *>(cur + (long long)gz * P + gc);
+    }
+    float pv[R][4], v2[R][4], e[R][4], dl[R][4], cs[R][4], sn[R][4];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int z = zr0 + r;
+        const bool zin = z < nzp;
+        const long long ro = (long long)(zin ? z : 0) * P + col;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) pv[r][j] = v2[r][j] = e[r][j] = dl[r][j] = cs[r][j] = sn[r][j] = 0.f;
+        if (zin) {
+            vld<4>(pv[r], prv + ro);
+            vld<4>(v2[r], a.v2 + ro);
+            if (PHYS >= 1) {
+                vld<4>(e[r], a.eps + ro);
+                vld<4>(dl[r], a.del + ro);
+            }
+            if (PHYS == 2) {
+                vld<4>(cs[r], a.cs + ro);
+                vld<4>(sn[r], a.sn + ro);
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+        const int i = tid + 256 * u;
+        if (i < ROWS * Q) su[i] = st[u];
+    }
+    __syncthreads();
+
+    if (PHYS == 2) {
+        // first x-derivative (undivided) of every staged row
+        for (int i = tid; i < ROWS * (TX / 4); i += 256) {
+            const int r = i >> 4, c = i & 15;
+            float w[12];
+            const float4 l = su[r * Q + c], m = su[r * Q + c + 1], h = su[r * Q + c + 2];
+            w[0] = l.x; w[1] = l.y; w[2] = l.z; w[3] = l.w;
+            w[4] = m.x; w[5] = m.y; w[6] = m.z; w[7] = m.w;
+            w[8] = h.x; w[9] = h.y; w[10] = h.z; w[11] = h.w;
+            float d2[4], d1[4];
+            xderivs<4>(w, d2, d1);
+            sd[i] = make_float4(d1[0], d1[1], d1[2], d1[3]);
+        }
+        __syncthreads();
+    }
+
+    // ---- z-window of U_k at this thread's columns: local rows rg*R-4 .. rg*R+R+3
+    float zw[R + 8][4];
+#pragma unroll
+    for (int i = 0; i < R + 8; ++i) {
+        const float4 t = su[(rg * R + i) * Q + cx + 1];
+        zw[i][0] = t.x; zw[i][1] = t.y; zw[i][2] = t.z; zw[i][3] = t.w;
+    }
+
+    bool xlive[4], xs[4];
+    float fx[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int c = col + j;
+        xlive[j] = (c >= 4) && (c < nxp - 4);
+        fx[j] = (!CPML && (flags & F_CERJAN) && c < P) ? a.cerx[c] : 1.0f;
+        xs[j] = CPML && ((c >= 4 && c < N) || (c >= nxp - N && c < nxp - 4));
+    }
+    const bool tile_xstrip = CPML && (x0 < N || x0 + TX > nxp - N);
+
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int z = zr0 + r;
+        if (z >= nzp) break;  // rows beyond the grid (last tile row only)
+        const bool zlive = (z >= 4) && (z < nzp - 4);
+        const long long ro = (long long)z * P + col;
+        const float fz = (!CPML && (flags & F_CERJAN)) ? a.cerz[z] : 1.0f;
+        const int lr = rg * R + r + 4;  // staged row of z
+
+        // x-window of row z
+        float w[12];
+        {
+            const float4 l = su[lr * Q + cx], m = su[lr * Q + cx + 1], h = su[lr * Q + cx + 2];
+            w[0] = l.x; w[1] = l.y; w[2] = l.z; w[3] = l.w;
+            w[4] = m.x; w[5] = m.y; w[6] = m.z; w[7] = m.w;
+            w[8] = h.x; w[9] = h.y; w[10] = h.z; w[11] = h.w;
+        }
+        float d2x[4], d1x[4];
+        xderivs<4>(w, d2x, d1x);
+
+        float c0[4], pzz[4], pxx[4], px[4], pz[4], pxz[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float cc = zw[r + 4][j];
+            const float u1p = zw[r + 5][j], u1m = zw[r + 3][j];
+            const float u2p = zw[r + 6][j], u2m = zw[r + 2][j];
+            const float u3p = zw[r + 7][j], u3m = zw[r + 1][j];
+            const float u4p = zw[r + 8][j], u4m = zw[r + 0][j];
+            c0[j] = cc;
+            pzz[j] = (KC0 * cc + KC1 * (u1p + u1m) + KC2 * (u2p + u2m) + KC3 * (u3p + u3m) + KC4 * (u4p + u4m)) *
+                     a.idz2;
+            pxx[j] = d2x[j] * a.idx2;
+            px[j] = 0.f;
+            pz[j] = 0.f;
+            pxz[j] = 0.f;
+            if (PHYS >= 1) {
+                pz[j] = (KA1 * (u1p - u1m) + KA2 * (u2p - u2m) + KA3 * (u3p - u3m) + KA4 * (u4p - u4m)) * a.idz;
+                px[j] = d1x[j] * a.idx;
+            }
+        }
+        if (PHYS == 2) {
+            float q[9][4];
+#pragma unroll
+            for (int h = 0; h < 9; ++h) {
+                if (h == 4) continue;
+                const float4 t = sd[(lr - 4 + h) * (TX / 4) + cx];
+                q[h][0] = t.x; q[h][1] = t.y; q[h][2] = t.z; q[h][3] = t.w;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                pxz[j] = (KA1 * (q[5][j] - q[3][j]) + KA2 * (q[6][j] - q[2][j]) + KA3 * (q[7][j] - q[1][j]) +
+                          KA4 * (q[8][j] - q[0][j])) *
+                         a.idxdz;
+        }
+
+        // ---- receiver gather of the input field (damped as the reference records it)
+        if ((flags & F_GATHER) && a.rrow[z] >= 0) {
+            const int slot = a.rrow[z];
+            const int it = a.k - a.itlag;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                int rr = a.rhead[(long long)slot * P + col + j];
+                const float val = (c0[j] * fx[j]) * fz;
+                while (rr >= 0) {
+                    a.seis[(long long)s * a.dstride + (long long)it * a.nrec + rr] = val;
+                    rr = a.rnext[rr];
+                }
+            }
+        }
+
+        // ---- snapshots / illumination / imaging of the input field
+        if ((flags & (F_SAVE | F_ILUM | F_IMAGE | F_GRADM)) && zlive) {
+            const bool zint = (z >= N) && (z < nzp - N);
+            const long long ib = (long long)(z - N) * a.nx - N;  // interior index = ib + c
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int c = col + j;
+                const bool cin = zint && c >= N && c < nxp - N;
+                const long long ii = ib + c;
+                if (flags & F_SAVE) {
+                    if (a.save_kind == SV_F32_PAD) {
+                        if (xlive[j]) snap_put(a, s, a.k, 0, ro + j, (c0[j] * fx[j]) * fz);
+                    } else if (cin) {
+                        snap_put(a, s, a.k, ii, 0, c0[j]);
+                    }
+                }
+                if (!cin) continue;
+                if (flags & F_ILUM) {
+                    float *acc = a.ilum + (long long)s * a.acc_sstride + ii;
+                    *acc += c0[j] * c0[j];
+                }
+                if (flags & F_IMAGE) {
+                    const float S = snap_get(a, s, a.k, ii, ro + j);
+                    float *im = a.image + (long long)s * a.acc_sstride + ii;
+                    *im += S * c0[j];
+                }
+                if (flags & F_GRADM) {
+                    const float Sp = snap_get(a, s, a.k + 1, ii, ro + j);
+                    const float S0 = snap_get(a, s, a.k, ii, ro + j);
+                    const float Sm = snap_get(a, s, a.k - 1, ii, ro + j);
+                    const float d2t = (Sp - 2.0f * S0 + Sm) * a.idt2;
+                    float *im = a.image + (long long)s * a.acc_sstride + ii;
+                    *im += d2t * c0[j];
+                }
+            }
+        }
+        if (!zlive) continue;
+
+        // ---- CPML: zeta with the new psi, corrected derivatives
+        float PXX[4], PZZ[4], PX[4], PZ[4], zx[4], zz[4], dpx[4], dpz[4];
+        const bool zs = CPML && ((z >= 4 && z < N) || (z >= nzp - N && z < nzp - 4));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            PXX[j] = pxx[j];
+            PZZ[j] = pzz[j];
+            PX[j] = px[j];
+            PZ[j] = pz[j];
+            zx[j] = zz[j] = dpx[j] = dpz[j] = 0.0f;
+        }
+        if (CPML && tile_xstrip) {
+            float pw[12], d2p[4], d1p[4], zo[4], ax[4], bx[4];
+            const float *prow = a.psix + so + (long long)z * P;
+            vldg<4>(pw + 4, prow, col, P);
+            if (col >= 4) vld<4>(pw, prow + col - 4);
+            else pw[0] = pw[1] = pw[2] = pw[3] = 0.f;
+            vldg<4>(pw + 8, prow, col + 4, P);
+            xderivs<4>(pw, d2p, d1p);
+            vld<4>(zo, a.zetax + so + ro);
+            vld<4>(ax, a.cax + ro);
+            vld<4>(bx, a.cbx + ro);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (!xs[j]) continue;
+                const float dps = d1p[j] * a.idx;
+                zx[j] = ax[j] * zo[j] + bx[j] * (pxx[j] + dps);
+                dpx[j] = dps;
+                PX[j] = px[j] + dps;
+                PXX[j] = pxx[j] + dps + zx[j];
+                a.zetax[so + ro + j] = zx[j];
+            }
+        }
+        if (CPML && zs) {
+            float ps[9][4];
+#pragma unroll
+            for (int h = 0; h < 9; ++h) {
+                if (h == 4) continue;
+                vld<4>(ps[h], a.psiz + so + (long long)(z - 4 + h) * P + col);
+            }
+            float zo[4], az[4], bz[4];
+            vld<4>(zo, a.zetaz + so + ro);
+            vld<4>(az, a.caz + ro);
+            vld<4>(bz, a.cbz + ro);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float dps = (KA1 * (ps[5][j] - ps[3][j]) + KA2 * (ps[6][j] - ps[2][j]) +
+                                   KA3 * (ps[7][j] - ps[1][j]) + KA4 * (ps[8][j] - ps[0][j])) *
+                                  a.idz;
+                zz[j] = az[j] * zo[j] + bz[j] * (pzz[j] + dps);
+                dpz[j] = dps;
+                PZ[j] = pz[j] + dps;
+                PZZ[j] = pzz[j] + dps + zz[j];
+                if (xlive[j]) a.zetaz[so + ro + j] = zz[j];
+            }
+        }
+
+        // ---- the update
+        float out[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float pe = (pv[r][j] * fx[j]) * fz;  // U_{k-1} as the reference holds it (damped twice)
+            float u;
+            if (PHYS == 0) {
+                float sum = pxx[j] + pzz[j];
+                if (CPML) {
+                    if (xs[j]) sum += dpx[j];
+                    if (zs) sum += dpz[j];
+                    if (xs[j]) sum += zx[j];
+                    if (zs) sum += zz[j];
+                }
+                u = v2[r][j] * sum + 2.0f * c0[j] - pe;
+            } else if (PHYS == 1) {
+                const float Sd = sd_term(e[r][j], dl[r][j], PX[j], PZ[j]);
+                u = 2.0f * c0[j] - pe + v2[r][j] * ((1.0f + 2.0f * e[r][j]) + Sd) * PXX[j] +
+                    v2[r][j] * (1.0f + Sd) * PZZ[j];
+            } else {
+                const float xi = PX[j] * cs[r][j] - PZ[j] * sn[r][j];
+                const float eta = PX[j] * sn[r][j] + PZ[j] * cs[r][j];
+                const float Sd = sd_term(e[r][j], dl[r][j], xi, eta);
+                const float Ac = (1.0f + 2.0f * e[r][j]) * (cs[r][j] * cs[r][j]) + (sn[r][j] * sn[r][j]) + Sd;
+                const float Bc = (1.0f + 2.0f * e[r][j]) * (sn[r][j] * sn[r][j]) + (cs[r][j] * cs[r][j]) + Sd;
+                u = 2.0f * c0[j] - pe + v2[r][j] * Ac * PXX[j] + v2[r][j] * Bc * PZZ[j] -
+                    2.0f * e[r][j] * v2[r][j] * (2.0f * sn[r][j] * cs[r][j]) * pxz[j];
+            }
+            out[j] = (u * fx[j]) * fz;
+        }
+        // writer-side injection of the next step's source
+        if ((flags & F_SRC) && z == a.ssz[s] && a.k + 1 < a.nt) {
+            const int sx = a.ssx[s];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (col + j == sx) out[j] += a.wav[a.k + 1];
+        }
+        // writer-side injection of the data of the next (earlier) adjoint step
+        if ((flags & F_RECINJ) && a.rrow[z] >= 0 && a.k - 1 - a.itlag >= 0) {
+            const int slot = a.rrow[z];
+            const int tn = a.k - 1 - a.itlag;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                int rr = a.rhead[(long long)slot * P + col + j];
+                while (rr >= 0) {
+                    out[j] += a.data[(long long)s * a.dstride + (long long)tn * a.nrec + rr] / a.dxdz;
+                    rr = a.rnext[rr];
+                }
+            }
+        }
+        if (xlive[0] && xlive[3]) {
+            vst<4>(prv + ro, out);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (xlive[j]) prv[ro + j] = out[j];
+        }
+    }
+}
+
+template <int PHYS, bool CPML, int TZ, int WPE>
+static int launch_tile_t(const StepArgs &a0, hipStream_t st) {
+    StepArgs a = a0;
+    a.nxt = (a.nxp + 63) / 64;
+    a.nzt = (a.nzp + TZ - 1) / TZ;
+    long long blocks = (long long)a.nxt * a.nzt * a.nshots;
+    blocks = (blocks + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_tile<PHYS, CPML, TZ, WPE>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+    return (int)hipGetLastError();
+}
+
+}  // namespace pam

@@ -1,0 +1,30 @@
+// pam_variants.cuh -- the table of step-kernel variants the autotuner and
+// PAM_OPT_TILE choose from: 0..2 the LDS-tiled k_tile (16, 32, 64 rows per
+// block), 3 the register z-march k_step (4 cells per lane, 16 rows per
+// wavefront), 4..6 the LDS-ring z-march k_march (7, 14, 28 chunks of 16 rows
+// per block).
+#pragma once
+#include "pam_step.cuh"
+#include "pam_tile.cuh"
+#include "pam_march.cuh"
+
+namespace pam {
+
+static constexpr TileShape kTiles[] = {{0, 16}, {0, 32}, {0, 64}, {4, 16}, {-1, 7}, {-1, 14}, {-1, 28}};
+static constexpr int kNumTiles = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
+
+template <int PHYS, bool CPML>
+static int launch_step_p(int tile, const StepArgs &a, hipStream_t st) {
+    switch (tile) {
+        case 0: return launch_tile_t<PHYS, CPML, 16, 1>(a, st);
+        case 1: return launch_tile_t<PHYS, CPML, 32, 1>(a, st);
+        case 2: return launch_tile_t<PHYS, CPML, 64, 1>(a, st);
+        case 3: return launch_step_t<PHYS, CPML, 4, 16>(a, st);
+        case 4: return launch_march_t<PHYS, CPML, 7>(a, st);
+        case 5: return launch_march_t<PHYS, CPML, 14>(a, st);
+        case 6: return launch_march_t<PHYS, CPML, 28>(a, st);
+        default: return (int)hipErrorInvalidValue;
+    }
+}
+
+}  // namespace pam

@@ -1,5 +1,5 @@
 // step_vti_cerjan.hip -- k_step instances for vti physics with the cerjan boundary.
-#include "pam_step.cuh"
+#include "pam_variants.cuh"
 
 namespace pam {
 int launch_step_vti_cerjan(int tile, const StepArgs &a, hipStream_t st) {

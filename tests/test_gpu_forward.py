@@ -65,3 +65,21 @@ def test_per_step_api_matches_whole_run(phys, abc):
     ref = oracle_forward(Oracle("f32"), pb)
     cur = eng.get_field(0).cpu().numpy()
     assert rel_l2(cur, ref["current"]) <= TOL_F32
+
+
+@pytest.mark.parametrize("phys,abc", CASES)
+def test_every_kernel_variant_matches_oracle(phys, abc):
+    """Each step-kernel variant (tile shape / kernel family) the autotuner may
+    pick gives the oracle's seismogram."""
+    from pseudoacousticmodeling_amd import _lib
+
+    pb = make_problem(phys, abc, nshots=3, seed=60 + 2 * phys + abc, nz=45, nx=150)
+    ref = np.stack([oracle_forward(Oracle("f32"), pb, shot=s)["seis"] for s in range(3)])
+    eng = make_engine(pb, max_shots=3)
+    n = _lib.load().pam_tile_count()
+    for t in range(n):
+        eng.set_option(_lib.PAM_OPT_TILE, t)
+        seis = eng.forward(0, 3).cpu().numpy()
+        err = rel_l2(seis, ref)
+        print(f"  variant {t}: rel_l2 {err:.3e}", flush=True)
+        assert np.isfinite(seis).all() and err <= TOL_F32, f"variant {t}"
