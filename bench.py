@@ -10,7 +10,8 @@ geometry) are resident on the device before the timed region.
 
 Prints one JSON line:
   value      Gpt-updates/s = padded cells x time steps x shots / wall time
-  roofline   achieved algorithmic HBM GB/s of the dominant kernel (k_step),
+  roofline   achieved algorithmic HBM GB/s of the dominant kernel (the step
+             kernel variant the autotuner picked),
              from HIP events around every launch of the timed steps, against
              the 8 TB/s HBM3E peak
   plain_loop the same 64 shots run one shot per launch (the per-shot loop)
@@ -35,19 +36,20 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Gpt-updates/s & %HBM roofline; FWI gradient s/shot, batched vs plain"
 
 
-def step_kernel_bytes(w, nshots):
-    """Algorithmic HBM bytes of one k_step launch advancing `nshots` shots one
-    time step: per padded cell 12 B per shot (U_k read, U_{k-1} read, U_{k+1}
-    write) plus 4 B per model field shared by the shots (acoustic 1, VTI 3,
-    TTI 4); per CPML strip cell (counted once per direction, so corners
-    twice) 12 B per shot (psi read, zeta read and write) plus 8 B of shared
-    damping coefficients."""
+def step_kernel_bytes(w, nshots, fused_psi=True):
+    """Algorithmic HBM bytes of one step-kernel launch advancing `nshots` shots
+    one time step: per padded cell 12 B per shot (U_k read, U_{k-1} read,
+    U_{k+1} write) plus 4 B per model field shared by the shots (acoustic 1,
+    VTI 3, TTI 4); per CPML strip cell (counted once per direction, so
+    corners twice) 16 B per shot when the kernel updates psi itself (psi read
+    and write, zeta read and write; 12 B when a pre-pass wrote psi) plus 8 B
+    of shared damping coefficients."""
     nzp, nxp, N = w["nz"] + 2 * w["N"], w["nx"] + 2 * w["N"], w["N"]
     n_m = {0: 1, 1: 3, 2: 4}[w["physics"]]
     b = (12 * nshots + 4 * n_m) * nzp * nxp
     if w["abc"] == 1:
         strip = (nzp - 8) * 2 * (N - 4) + 2 * (N - 4) * (nxp - 8)
-        b += (12 * nshots + 8) * strip
+        b += ((16 if fused_psi else 12) * nshots + 8) * strip
     return b
 
 
@@ -78,9 +80,10 @@ def cpu_baseline(w, seconds):
     A = orc.cerjan(g.N, w["cerjan_factor"]) if w["abc"] == 0 else None
     kw = dict(A=A, cpml=cpml)
     sz, sx = int(w["sz"][0]), int(w["sx"][0])
+    orc.forward_shot(w["physics"], w["abc"], g, model, w["src"], sz, sx, w["rz"], w["rx"], nsteps=5, **kw)
     t0 = time.perf_counter()
-    orc.forward_shot(w["physics"], w["abc"], g, model, w["src"], sz, sx, w["rz"], w["rx"], nsteps=10, **kw)
-    per = (time.perf_counter() - t0) / 10
+    orc.forward_shot(w["physics"], w["abc"], g, model, w["src"], sz, sx, w["rz"], w["rx"], nsteps=20, **kw)
+    per = (time.perf_counter() - t0) / 20
     nsteps = int(max(20, min(g.nt, seconds / max(per, 1e-6))))
     t0 = time.perf_counter()
     orc.forward_shot(w["physics"], w["abc"], g, model, w["src"], sz, sx, w["rz"], w["rx"], nsteps=nsteps, **kw)
@@ -102,15 +105,47 @@ def cpu_baseline(w, seconds):
             "seconds": dt, "nsteps": nsteps, "finite": bool(np.isfinite(model[0]).all())}
 
 
+def bandwidth_probe():
+    """Plain device-memory kernels of known size on 256 MiB float arrays
+    (torch's own elementwise kernels): copy (1 read, 1 write), in-place add
+    (2 reads, 1 write -- the access pattern of the leapfrog update, which
+    writes U_{k+1} over U_{k-1}) and a sum (1 read).  Their FETCH_SIZE /
+    WRITE_SIZE counters calibrate the HBM-traffic counters of the step kernel,
+    and their rates are the practical bandwidth of this device."""
+    import torch
+
+    n = 64 << 20
+    a = torch.ones(n, device="cuda:0")
+    b = torch.ones(n, device="cuda:0")
+    res = {}
+    for name, fn, nbytes in (("copy 1R1W", lambda: a.copy_(b), 2 * 4 * n),
+                             ("add_ 2R1W", lambda: a.add_(b), 3 * 4 * n),
+                             ("sum 1R", lambda: b.sum(), 4 * n)):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        res[name] = nbytes / (ms * 1e-3) / 1e9
+        print(f"probe {name}: {ms * 1e3:.1f} us per call, {res[name]:.0f} GB/s", flush=True)
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt-data", type=int, default=5001, help="recorded samples (default: the full 5001)")
-    ap.add_argument("--tile", type=int, default=-1, help="k_step tile variant (-1: automatic)")
+    ap.add_argument("--tile", type=int, default=-1, help="step-kernel variant (-1: the autotuner's choice)")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-shot loop measurement")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample length (0: skip)")
+    ap.add_argument("--probe", action="store_true", help="run only the device bandwidth probe")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -127,6 +162,9 @@ def main(argv=None):
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible (torch.cuda.is_available() is False)", file=sys.stderr)
         return 1
+    if args.probe:
+        bandwidth_probe()
+        return 0
     _lib.load()
     w = workloads.vti_cpml_64(nt_data=args.nt_data)
     S = len(w["sx"])
@@ -149,12 +187,14 @@ def main(argv=None):
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     finite = bool(torch.isfinite(seis).all().item())
+    variant = eng.get_option(_lib.PAM_OPT_TILE)
     upd = workloads.updates(w) * args.steps
     value = upd / secs / 1e9
     avg_ms = kern_ms / max(launches, 1)
-    bytes_launch = step_kernel_bytes(w, S)
+    fused = variant >= 4  # the k_march variants update psi in the step kernel
+    bytes_launch = step_kernel_bytes(w, S, fused)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    print(f"bench: {secs:.3f} s for {args.steps} runs, k_step avg {avg_ms * 1e3:.1f} us over {launches} launches, "
+    print(f"bench: {secs:.3f} s for {args.steps} runs, step kernel variant {variant} avg {avg_ms * 1e3:.1f} us over {launches} launches, "
           f"seis finite={finite}", file=sys.stderr, flush=True)
 
     plain = None
@@ -171,7 +211,7 @@ def main(argv=None):
         pavg = t["kernel_ms"] / max(t["kernel_launches"], 1)
         plain = {"value": workloads.updates(w) / ps / 1e9, "unit": "Gpt-updates/s", "seconds": ps,
                  "k_step_avg_us": pavg * 1e3,
-                 "k_step_gbs": step_kernel_bytes(w, 1) / (pavg * 1e-3) / 1e9,
+                 "k_step_gbs": step_kernel_bytes(w, 1, eng.get_option(_lib.PAM_OPT_TILE) >= 4) / (pavg * 1e-3) / 1e9,
                  "batched_speedup": ps / (secs / args.steps)}
         print(f"bench: plain per-shot loop {ps:.3f} s", file=sys.stderr, flush=True)
     eng.close()
@@ -184,6 +224,12 @@ def main(argv=None):
         cpu = cpu_baseline(w, args.cpu_seconds)
         print(f"bench: cpu {cpu['value']:.3f} Gpt-updates/s ({cpu['sample']})", file=sys.stderr, flush=True)
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{w['name']}.json")
+    if os.path.exists(tpath) and args.nt_data == 5001:
+        rec = json.load(open(tpath))
+        if rec.get("variant") == variant:
+            traffic = rec["traffic_bytes"]
     out = {
         "metric": METRIC, "value": value, "unit": "Gpt-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": secs / args.steps * 1e3, "higher_is_better": True,
@@ -193,7 +239,7 @@ def main(argv=None):
                                f"{S} shots in one batch, {len(w['rx'])} receivers per shot",
                    "global_batch": S, "nt": w["nt"], "parallelism": "1 GPU, shots batched per launch"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "step kernel variant %d" % variant,
                      "bytes_per_launch": bytes_launch, "avg_launch_us": avg_ms * 1e3},
         "plain_loop": plain,
         "cpu_baseline": cpu,

@@ -51,7 +51,9 @@ enum pam_storage {
 /* keys of pam_set_option */
 enum pam_option {
     PAM_OPT_BATCH = 1,     /* shots advanced together in one launch (default: all) */
-    PAM_OPT_TILE = 2,      /* kernel variant index from pam_tile_count(); -1 = auto */
+    PAM_OPT_TILE = 2,      /* step-kernel variant in [0, pam_tile_count()); -1 (default):
+                              the autotuner times every variant on the first run of a
+                              batch size and keeps the fastest */
     PAM_OPT_GRAPH = 3,     /* 1: replay the step loop as captured hipGraphs (default 1) */
     PAM_OPT_STORAGE = 4,   /* enum pam_storage for rtm / gradient (default ALL_F32) */
     PAM_OPT_CKPT_STEP = 5, /* checkpoint interval for PAM_STORE_CHECKPOINT */
@@ -92,6 +94,9 @@ int pam_create(const pam_grid *grid, int physics, int abc, const pam_abc_params 
 int pam_destroy(pam_ctx *ctx);
 const char *pam_last_error(const pam_ctx *ctx);
 int pam_set_option(pam_ctx *ctx, int key, long long value);
+/* Current value of an option; for PAM_OPT_TILE left on auto, the variant the
+ * autotuner chose for the last whole run (-1 before the first). */
+int pam_get_option(const pam_ctx *ctx, int key, long long *value);
 /* number of kernel variants (tile shapes) the autotuner may choose from */
 int pam_tile_count(void);
 

@@ -25,7 +25,8 @@ ERRORS = {-1: "PAM_E_ARG", -2: "PAM_E_HIP", -3: "PAM_E_STATE", -4: "PAM_E_NOMEM"
 
 # every symbol include/pam.h declares
 EXPORTS = [
-    "pam_version", "pam_create", "pam_destroy", "pam_last_error", "pam_set_option", "pam_tile_count",
+    "pam_version", "pam_create", "pam_destroy", "pam_last_error", "pam_set_option", "pam_get_option",
+    "pam_tile_count",
     "pam_set_model", "pam_set_wavelet", "pam_set_geometry", "pam_reset", "pam_forward_step",
     "pam_get_field", "pam_forward", "pam_rtm", "pam_gradient", "pam_misfit", "pam_forward_adjoint",
     "pam_last_timing",
@@ -66,6 +67,7 @@ def load():
                                ctypes.POINTER(vp)]
     lib.pam_destroy.argtypes = [vp]
     lib.pam_set_option.argtypes = [vp, I, LL]
+    lib.pam_get_option.argtypes = [vp, I, ctypes.POINTER(LL)]
     lib.pam_set_model.argtypes = [vp, P, P, P, P, I]
     lib.pam_set_wavelet.argtypes = [vp, P]
     lib.pam_set_geometry.argtypes = [vp, P, P, I, P, P, I]

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -43,10 +44,17 @@ struct pam_ctx {
     bool geom_set = false;
     // wavefields [max_shots][plane]
     float *U0 = nullptr, *U1 = nullptr;
+    // psi is double-buffered across steps: half psi_flip holds the previous
+    // step's psi, the other half receives the new one
     float *psix = nullptr, *psiz = nullptr, *zetax = nullptr, *zetaz = nullptr;
+    long long psi_half = 0;  // floats between the two halves
+    int psi_flip = 0;
     // per-step interface state (slot 0)
     float *cur = nullptr, *prv = nullptr;
     int next_k = -1;
+    // step-kernel variant chosen by the autotuner, per batch size
+    std::map<int, int> tuned;
+    int last_tile = -1;  // variant of the last whole run
     // options
     int batch = 0, tile = -1, graph = 1, storage = PAM_STORE_ALL_F32, ckpt_step = 0;
     int reset_adj_cpml = 1, normalize = 1;
@@ -161,8 +169,9 @@ extern "C" int pam_create(const pam_grid *grid, int physics, int abc, const pam_
     DALLOC(ctx->U0, fb);
     DALLOC(ctx->U1, fb);
     if (abc == PAM_CPML) {
-        DALLOC(ctx->psix, fb);
-        DALLOC(ctx->psiz, fb);
+        DALLOC(ctx->psix, 2 * fb);
+        DALLOC(ctx->psiz, 2 * fb);
+        ctx->psi_half = ctx->plane * max_shots;
         DALLOC(ctx->zetax, fb);
         DALLOC(ctx->zetaz, fb);
     }
@@ -282,6 +291,20 @@ extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
     }
 }
 
+extern "C" int pam_get_option(const pam_ctx *ctx, int key, long long *value) {
+    if (!ctx || !value) return PAM_E_ARG;
+    switch (key) {
+        case PAM_OPT_BATCH: *value = ctx->batch; return PAM_OK;
+        case PAM_OPT_TILE: *value = ctx->tile >= 0 ? ctx->tile : ctx->last_tile; return PAM_OK;
+        case PAM_OPT_GRAPH: *value = ctx->graph; return PAM_OK;
+        case PAM_OPT_STORAGE: *value = ctx->storage; return PAM_OK;
+        case PAM_OPT_CKPT_STEP: *value = ctx->ckpt_step; return PAM_OK;
+        case PAM_OPT_RESET_ADJ_CPML: *value = ctx->reset_adj_cpml; return PAM_OK;
+        case PAM_OPT_NORMALIZE: *value = ctx->normalize; return PAM_OK;
+        default: return PAM_E_ARG;
+    }
+}
+
 extern "C" int pam_set_model(pam_ctx *ctx, const float *vp, const float *eps, const float *delta,
                              const float *theta, int layout) {
     if (!ctx) return PAM_E_ARG;
@@ -388,12 +411,6 @@ static int ready(pam_ctx *ctx) {
     return PAM_OK;
 }
 
-static int choose_tile(pam_ctx *ctx, int nshots) {
-    (void)nshots;
-    if (ctx->tile >= 0) return ctx->tile;
-    return 0;
-}
-
 static void base_step_args(pam_ctx *ctx, StepArgs &a, int nshots) {
     std::memset(&a, 0, sizeof(a));
     const pam_grid &g = ctx->g;
@@ -466,6 +483,9 @@ static int zero_shot_state(pam_ctx *ctx, int nshots, bool fields, bool cpml) {
     if (cpml && ctx->abc == PAM_CPML) {
         HCK(hipMemsetAsync(ctx->psix, 0, fb, ctx->st));
         HCK(hipMemsetAsync(ctx->psiz, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(ctx->psix + ctx->psi_half, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(ctx->psiz + ctx->psi_half, 0, fb, ctx->st));
+        ctx->psi_flip = 0;
         HCK(hipMemsetAsync(ctx->zetax, 0, fb, ctx->st));
         HCK(hipMemsetAsync(ctx->zetaz, 0, fb, ctx->st));
     }
@@ -481,12 +501,24 @@ static int upload_sources(pam_ctx *ctx, int shot0, int nshots) {
     return PAM_OK;
 }
 
-// One forward time step of `nshots` shots: psi pre-pass (CPML) then the fused step.
+// One forward time step of `nshots` shots: psi pre-pass (CPML, unless the
+// kernel variant updates psi itself) then the fused step.
 static int fwd_step(pam_ctx *ctx, StepArgs &a, int tile, float *cur, float *prv, int k, hipEvent_t *ev) {
     if (ctx->abc == PAM_CPML) {
-        PsiArgs p;
-        base_psi_args(ctx, p, a.nshots, cur);
-        KCK(launch_psi(p, ctx->st));
+        const long long in = ctx->psi_flip ? ctx->psi_half : 0, out = ctx->psi_flip ? 0 : ctx->psi_half;
+        a.psix_in = ctx->psix + in;
+        a.psiz_in = ctx->psiz + in;
+        a.psix = ctx->psix + out;
+        a.psiz = ctx->psiz + out;
+        if (!tile_fuses_psi(tile)) {
+            PsiArgs p;
+            base_psi_args(ctx, p, a.nshots, cur);
+            p.psix_in = a.psix_in;
+            p.psiz_in = a.psiz_in;
+            p.psix = a.psix;
+            p.psiz = a.psiz;
+            KCK(launch_psi(p, ctx->st));
+        }
     }
     a.cur = cur;
     a.prv = prv;
@@ -494,7 +526,62 @@ static int fwd_step(pam_ctx *ctx, StepArgs &a, int tile, float *cur, float *prv,
     if (ev) HCK(hipEventRecord(ev[0], ctx->st));
     KCK(launch_step(ctx->phys, ctx->abc == PAM_CPML, tile, a, ctx->st));
     if (ev) HCK(hipEventRecord(ev[1], ctx->st));
+    if (ctx->abc == PAM_CPML) ctx->psi_flip ^= 1;
     return PAM_OK;
+}
+
+// The autotuner: times every step-kernel variant on the wavefields of `nb`
+// shots (a few plain steps each, HIP events on the context's stream) and keeps
+// the fastest for that batch size.  The fields it steps on are cleared before
+// and after, so it must run before a shot is primed.
+static constexpr int kDefaultTile = 5;
+
+static int autotune(pam_ctx *ctx, int nb, int *best) {
+    *best = kDefaultTile;
+    StepArgs a;
+    base_step_args(ctx, a, nb);
+    int r = zero_shot_state(ctx, nb, true, true);
+    if (r) return r;
+    double best_ms = 1e300;
+    for (int t = 0; t < tile_count(); ++t) {
+        float *c = ctx->U0, *p = ctx->U1;
+        for (int i = 0; i < 2; ++i) {
+            r = fwd_step(ctx, a, t, c, p, i, nullptr);
+            if (r) return r;
+            std::swap(c, p);
+        }
+        HCK(hipEventRecord(ctx->e0, ctx->st));
+        for (int i = 0; i < 6; ++i) {
+            r = fwd_step(ctx, a, t, c, p, i, nullptr);
+            if (r) return r;
+            std::swap(c, p);
+        }
+        HCK(hipEventRecord(ctx->e1, ctx->st));
+        HCK(hipEventSynchronize(ctx->e1));
+        float ms = 0.f;
+        HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            *best = t;
+        }
+    }
+    return zero_shot_state(ctx, nb, true, true);
+}
+
+// variant for a batch of nshots: the option if set, else the autotuner's
+// choice (tuned on first use when `may_tune`, else the default)
+static int choose_tile(pam_ctx *ctx, int nshots, bool may_tune) {
+    if (ctx->tile >= 0) return ctx->tile;
+    auto it = ctx->tuned.find(nshots);
+    if (it != ctx->tuned.end()) return it->second;
+    if (!may_tune) return kDefaultTile;
+    int best = kDefaultTile;
+    if (autotune(ctx, nshots, &best) != PAM_OK) {
+        (void)hipGetLastError();
+        best = kDefaultTile;
+    }
+    ctx->tuned[nshots] = best;
+    return best;
 }
 
 static int ensure_events(pam_ctx *ctx, size_t n) {
@@ -512,6 +599,7 @@ extern "C" int pam_reset(pam_ctx *ctx, int shot) {
     int r = ready(ctx);
     if (r) return r;
     if (shot < 0 || shot >= ctx->nshot) return fail(ctx, PAM_E_ARG, "shot %d out of range", shot);
+    (void)choose_tile(ctx, 1, true);
     r = zero_shot_state(ctx, 1, true, true);
     if (r) return r;
     r = upload_sources(ctx, shot, 1);
@@ -536,7 +624,7 @@ extern "C" int pam_forward_step(pam_ctx *ctx, int k, float *seis) {
         a.seis = seis;
         a.dstride = (long long)(ctx->g.nt - ctx->g.itlag) * ctx->nrec;
     }
-    int r = fwd_step(ctx, a, choose_tile(ctx, 1), ctx->cur, ctx->prv, k, nullptr);
+    int r = fwd_step(ctx, a, choose_tile(ctx, 1, false), ctx->cur, ctx->prv, k, nullptr);
     if (r) return r;
     std::swap(ctx->cur, ctx->prv);
     ctx->next_k = k + 1;
@@ -614,6 +702,8 @@ extern "C" int pam_forward(pam_ctx *ctx, int shot0, int nshots, float *seis) {
     ctx->launches = 0;
     ctx->next_k = -1;
     const int B = batch_size(ctx, nshots);
+    const int tile = choose_tile(ctx, B, true);
+    ctx->last_tile = tile;
     for (int b0 = 0; b0 < nshots; b0 += B) {
         const int nb = std::min(B, nshots - b0);
         r = zero_shot_state(ctx, nb, true, true);
@@ -627,7 +717,7 @@ extern "C" int pam_forward(pam_ctx *ctx, int shot0, int nshots, float *seis) {
         a.flags |= F_SRC | F_GATHER;
         a.seis = seis + (long long)b0 * nd;
         a.dstride = nd;
-        r = run_forward_loop(ctx, a, choose_tile(ctx, nb), true);
+        r = run_forward_loop(ctx, a, tile, true);
         if (r) return r;
     }
     HCK(hipStreamSynchronize(ctx->st));
@@ -699,7 +789,8 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     ctx->loop_ms = ctx->kernel_ms = 0.0;
     ctx->launches = 0;
     ctx->next_k = -1;
-    const int tile0 = choose_tile(ctx, B);
+    const int tile0 = choose_tile(ctx, B, true);
+    ctx->last_tile = tile0;
     for (int b0 = 0; b0 < nshots; b0 += B) {
         const int nb = std::min(B, nshots - b0);
         // ---- forward with snapshots and illumination

@@ -39,8 +39,11 @@ struct StepArgs {
     const float *v2;      // (vp*vp)*(dt*dt)
     const float *eps, *del, *cs, *sn;
     const float *cerx, *cerz;  // Cerjan factors per column [P] / per row [nzp]
-    // CPML (full-grid fields, zero outside the strips)
-    const float *psix, *psiz;  // new psi (written by k_psi this step)
+    // CPML (full-grid fields, zero outside the strips); psi is double-buffered
+    // across steps: psix_in/psiz_in hold psi of the previous step, psix/psiz
+    // receive the new psi (written by k_psi, or by k_march itself)
+    const float *psix_in, *psiz_in;
+    float *psix, *psiz;
     float *zetax, *zetaz;
     const float *cax, *cbx, *caz, *cbz;
     // sources (writer injection)
@@ -71,7 +74,8 @@ struct PsiArgs {
     long long sstride;
     float idx, idz;
     const float *cur;
-    float *psix, *psiz;
+    const float *psix_in, *psiz_in;  // psi of the previous step
+    float *psix, *psiz;              // new psi
     const float *cax, *cbx, *caz, *cbz;
 };
 
@@ -100,6 +104,8 @@ struct TileShape {
 };
 int tile_count();
 TileShape tile_shape(int idx);
+// true when the step-kernel variant updates the CPML psi itself (no k_psi pre-pass)
+bool tile_fuses_psi(int idx);
 
 // launchers (return hipError_t as int)
 int launch_step(int phys, bool cpml, int tile, const StepArgs &a, hipStream_t st);

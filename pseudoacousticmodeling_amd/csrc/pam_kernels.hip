@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void k_psi(const PsiArgs a) {
         const long long i = (long long)row * a.P + ix;
         const float d1 = KA1 * (U[i + 1] - U[i - 1]) + KA2 * (U[i + 2] - U[i - 2]) + KA3 * (U[i + 3] - U[i - 3]) +
                          KA4 * (U[i + 4] - U[i - 4]);
-        a.psix[so + i] = a.cax[i] * a.psix[so + i] + a.cbx[i] * (d1 * a.idx);
+        a.psix[so + i] = a.cax[i] * a.psix_in[so + i] + a.cbx[i] * (d1 * a.idx);
     } else if (t < nxs + nzs) {
         const long long u = t - nxs;
         const int rr = (int)(u / (a.nxp - 8));
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void k_psi(const PsiArgs a) {
         const long long P = a.P;
         const float d1 = KA1 * (U[i + P] - U[i - P]) + KA2 * (U[i + 2 * P] - U[i - 2 * P]) +
                          KA3 * (U[i + 3 * P] - U[i - 3 * P]) + KA4 * (U[i + 4 * P] - U[i - 4 * P]);
-        a.psiz[so + i] = a.caz[i] * a.psiz[so + i] + a.cbz[i] * (d1 * a.idz);
+        a.psiz[so + i] = a.caz[i] * a.psiz_in[so + i] + a.cbz[i] * (d1 * a.idz);
     }
 }
 
@@ -408,6 +408,7 @@ TileShape tile_shape(int idx) {
     if (idx < 0 || idx >= kNumTiles) idx = 0;
     return kTiles[idx];
 }
+bool tile_fuses_psi(int idx) { return tile_shape(idx).V < 0; }
 
 int launch_step_acoustic_cerjan(int, const StepArgs &, hipStream_t);
 int launch_step_acoustic_cpml(int, const StepArgs &, hipStream_t);

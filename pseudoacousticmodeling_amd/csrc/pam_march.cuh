@@ -2,12 +2,20 @@
 //
 // One 256-thread block owns a 64-column strip of one shot and marches down
 // a range of rows in chunks of 16.  The current field U_k flows through an
-// LDS ring of 64 rows (each staged row carries a 4-cell halo on each side);
-// while the block computes chunk c from the ring, the global loads of chunk
-// c+1 (its 16 new U_k rows, and U_{k-1} and the model coefficients of its
-// cells) are already in flight in registers, so every wavefront keeps memory
-// busy during the arithmetic.  Each U_k cell is read from HBM once per strip
-// (plus a 4-row halo at the ends of the range); one barrier per chunk.
+// LDS ring of 64 rows (each staged row carries an x-halo); while the block
+// computes chunk c from the ring, the global loads of chunk c+1 (its 16 new
+// U_k rows, and U_{k-1} and the model coefficients of its cells) are already
+// in flight in registers, so every wavefront keeps memory busy during the
+// arithmetic.  Each U_k cell is read from HBM once per strip (plus a halo of
+// a few rows at the ends of the range); one barrier per chunk.
+//
+// CPML is fused as well: the memory variable psi of the step is computed in
+// the pass that uses it.  Rows of the z-strips enter a 32-row LDS ring of new
+// psi_z as the march reaches them (each row once, 4 rows ahead of its use);
+// tiles of the x-strips recompute psi_x on a 4-column halo from an 8-column
+// U halo.  psi is double-buffered across steps (read from psix_in/psiz_in,
+// written to psix/psiz), so halo recomputation never races with a neighbour
+// block's write.  zeta is read and written by the block that owns the cell.
 //
 // Per cell the step is the one of k_tile (the same arithmetic, flags and
 // fused epilogue): thread (cx, rg) owns columns 4cx..4cx+3 of row rg of the
@@ -17,7 +25,8 @@
 
 namespace pam {
 
-constexpr int kMarchRing = 64;  // rows in the LDS ring (power of two)
+constexpr int kMarchRing = 64;  // rows in the LDS ring of U (power of two)
+constexpr int kPsiRing = 32;    // rows in the LDS ring of new psi_z (power of two)
 
 template <int PHYS>
 struct MarchCell {
@@ -43,17 +52,43 @@ __device__ __forceinline__ void march_load_cell(MarchCell<PHYS> &m, const StepAr
     }
 }
 
-template <int PHYS, bool CPML, int CHUNKS>
-__global__ __launch_bounds__(256) void k_march(const StepArgs a) {
-    constexpr int TX = 64, TC = 16;
-    constexpr int Q = (TX + 8) / 4;          // float4 per staged row
-    constexpr int NEW = TC * Q;              // float4 of U entering per chunk
-    constexpr int NP = (NEW + 255) / 256;    // per thread
-    constexpr int NPRO = ((TC + 8) * Q + 255) / 256;
-    __shared__ float4 ring[kMarchRing * Q];
-    __shared__ float4 sd[(PHYS == 2) ? kMarchRing * (TX / 4) : 1];
+__device__ __forceinline__ void f4_to(float *w, float4 t) {
+    w[0] = t.x;
+    w[1] = t.y;
+    w[2] = t.z;
+    w[3] = t.w;
+}
 
-    const int nseg = a.nzt;                  // row segments
+// 4 consecutive cells starting at column c: vector store when all are in the
+// stencil interior [4, nxp-4), else per cell
+__device__ __forceinline__ void store_live4(float *p, const float *v, int c, int nxp) {
+    if (c >= 4 && c + 3 < nxp - 4) {
+        vst<4>(p, v);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (c + j >= 4 && c + j < nxp - 4) p[j] = v[j];
+    }
+}
+
+template <int PHYS, bool CPML, int CHUNKS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((PHYS == 2) ? 1 : 3))) void k_march(
+    const StepArgs a) {
+    constexpr int TX = 64, TC = 16;
+    constexpr int HX = CPML ? 8 : 4;       // x-halo of the staged rows
+    constexpr int LA = CPML ? 8 : 4;       // rows staged above / below a chunk
+    constexpr int Q = (TX + 2 * HX) / 4;   // float4 per staged row
+    constexpr int CQ = HX / 4;             // float4 index of column x0 in a staged row
+    constexpr int NEW = TC * Q;            // float4 of U entering per chunk
+    constexpr int NP = (NEW + 255) / 256;
+    constexpr int NPRO = ((TC + 2 * LA) * Q + 255) / 256;
+    constexpr int G = TX / 4;              // float4 groups of the tile's own columns
+    __shared__ float4 ring[kMarchRing * Q];
+    __shared__ float4 sd[(PHYS == 2) ? kMarchRing * G : 1];
+    __shared__ float4 pzr[CPML ? kPsiRing * G : 1];         // new psi_z rows, own columns
+    __shared__ float4 pxl[CPML ? TC * (G + 2) : 1];         // new psi_x of the chunk, columns x0-4 .. x0+TX+3
+
+    const int nseg = a.nzt;  // row segments
     const int nb = a.nxt * nseg * a.nshots;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= nb) return;  // block-uniform
@@ -63,8 +98,8 @@ __global__ __launch_bounds__(256) void k_march(const StepArgs a) {
     const int txi = strip - seg * a.nxt;
     const int x0 = txi * TX;
     const int P = a.P, nzp = a.nzp, nxp = a.nxp, N = a.N;
-    const int zb = 4 + seg * CHUNKS * TC;           // first row of the range
-    int ze = zb + CHUNKS * TC;                       // one past the last row
+    const int zb = 4 + seg * CHUNKS * TC;  // first row of the range
+    int ze = zb + CHUNKS * TC;             // one past the last row
     if (ze > nzp - 4) ze = nzp - 4;
     const int nchunk = (ze - zb + TC - 1) / TC;
     const long long so = (long long)s * a.sstride;
@@ -74,32 +109,95 @@ __global__ __launch_bounds__(256) void k_march(const StepArgs a) {
     const int flags = a.flags;
     const int cx = tid & 15, rg = tid >> 4;
     const int col = x0 + 4 * cx;
+    const bool xst = CPML && (x0 < N || x0 + TX > nxp - N);  // tile meets an x-strip (block-uniform)
 
-    // ring slot of global row z (z >= -4)
 #define SLOT(z) ((((z) + 8) & (kMarchRing - 1)) * Q)
+#define PSLOT(z) ((((z) + 8) & (kPsiRing - 1)) * G)
 
-    // ---- prologue: rows zb-4 .. zb+TC+3 and the cells of chunk 0
+    // x-halo groups of a staged row are only needed where psi_x is computed
+    auto want_q = [&](int q) { return !CPML || xst || (q >= CQ - 1 && q <= CQ + G); };
+    auto zstrip = [&](int z) { return CPML && ((z >= 4 && z < N) || (z >= nzp - N && z < nzp - 4)); };
+    auto xstrip = [&](int c) { return CPML && ((c >= 4 && c < N) || (c >= nxp - N && c < nxp - 4)); };
+    // new psi_z of the 4 cells of group c in row r: a psi_in + b dU/dz, U from the ring
+    auto psiz_new = [&](float *nv, const float *pin, const float *az, const float *bz, int r, int c) {
+        float uz[9][4];
+#pragma unroll
+        for (int h = 0; h < 9; ++h) {
+            if (h == 4) continue;
+            f4_to(uz[h], ring[SLOT(r - 4 + h) + CQ + c]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float d1 = KA1 * (uz[5][j] - uz[3][j]) + KA2 * (uz[6][j] - uz[2][j]) +
+                             KA3 * (uz[7][j] - uz[1][j]) + KA4 * (uz[8][j] - uz[0][j]);
+            nv[j] = az[j] * pin[j] + bz[j] * (d1 * a.idz);
+        }
+    };
+    // new psi_x of staged group g (columns x0-4+4g .. +3) of row z (chunk row r)
+    // into pxl; written to psix for the tile's own columns of rows < ze
+    auto psix_new_item = [&](int r, int g, int z, const float *pin, const float *ax, const float *bx) {
+        const int c0 = x0 - 4 + 4 * g;
+        float nv[4] = {0.f, 0.f, 0.f, 0.f};
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) any = any || xstrip(c0 + j);
+        if (any && z >= 4 && z < nzp - 4) {
+            float w[12], d2[4], d1[4];
+            const float4 *rw = ring + SLOT(z) + CQ - 2 + g;
+            f4_to(w, rw[0]);
+            f4_to(w + 4, rw[1]);
+            f4_to(w + 8, rw[2]);
+            xderivs<4>(w, d2, d1);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (!xstrip(c0 + j)) continue;
+                nv[j] = ax[j] * pin[j] + bx[j] * (d1[j] * a.idx);
+                if (g >= 1 && g <= G && z < ze) a.psix[so + (long long)z * P + c0 + j] = nv[j];
+            }
+        }
+        pxl[r * (G + 2) + g] = make_float4(nv[0], nv[1], nv[2], nv[3]);
+    };
+
+    // ---- prologue: rows zb-LA .. zb+TC+LA-1 and the cells of chunk 0
     {
         float4 st[NPRO];
 #pragma unroll
         for (int u = 0; u < NPRO; ++u) {
             const int i = tid + 256 * u;
             const int r = i / Q, q = i - r * Q;
-            const int gz = zb - 4 + r, gc = x0 - 4 + 4 * q;
+            const int gz = zb - LA + r, gc = x0 - HX + 4 * q;
             st[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (i < (TC + 8) * Q && gz >= 0 && gz < nzp && gc >= 0 && gc < P)
+            if (i < (TC + 2 * LA) * Q && gz >= 0 && gz < nzp && gc >= 0 && gc < P && want_q(q))
                 st[u] = *reinterpret_cast<const float4 *>(cur + (long long)gz * P + gc);
         }
 #pragma unroll
         for (int u = 0; u < NPRO; ++u) {
             const int i = tid + 256 * u;
             const int r = i / Q, q = i - r * Q;
-            if (i < (TC + 8) * Q) ring[SLOT(zb - 4 + r) + q] = st[u];
+            if (i < (TC + 2 * LA) * Q) ring[SLOT(zb - LA + r) + q] = st[u];
         }
     }
     MarchCell<PHYS> mc, mn;
     march_load_cell<PHYS>(mc, a, prv, zb + rg, col);
     __syncthreads();
+    if (CPML) {
+        // new psi_z of rows zb-4 .. zb+3 (the march computes later rows 4 ahead of use)
+        for (int i = tid; i < 8 * G; i += 256) {
+            const int r = zb - 4 + i / G, c = i - (i / G) * G;
+            float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (zstrip(r)) {
+                const long long ro = (long long)r * P + x0 + 4 * c;
+                float pin[4], az[4], bz[4], nv[4];
+                vld<4>(pin, a.psiz_in + so + ro);
+                vld<4>(az, a.caz + ro);
+                vld<4>(bz, a.cbz + ro);
+                psiz_new(nv, pin, az, bz, r, c);
+                o = make_float4(nv[0], nv[1], nv[2], nv[3]);
+                if (r >= zb) store_live4(a.psiz + so + ro, nv, x0 + 4 * c, nxp);
+            }
+            pzr[PSLOT(r) + c] = o;
+        }
+    }
 
     bool xlive[4], xs[4];
     float fx[4];
@@ -110,60 +208,122 @@ __global__ __launch_bounds__(256) void k_march(const StepArgs a) {
         fx[j] = (!CPML && (flags & F_CERJAN) && c < P) ? a.cerx[c] : 1.0f;
         xs[j] = CPML && ((c >= 4 && c < N) || (c >= nxp - N && c < nxp - 4));
     }
-    const bool tile_xstrip = CPML && (x0 < N || x0 + TX > nxp - N);
 
     for (int ch = 0; ch < nchunk; ++ch) {
         const int zc = zb + ch * TC;
         const bool more = ch + 1 < nchunk;
-        // ---- prefetch chunk ch+1: U rows zc+TC+4 .. zc+2TC+3, and its cells
+        // ---- CPML loads of this chunk, issued before the prefetch so that
+        // waiting for them does not wait for the prefetch (loads complete in
+        // order): psi_z inputs of row zc+4+rg, psi_x inputs of row zc+rg
+        // (own group, and for 32 threads a halo group), zeta of the own cells
+        // rows zc-4 .. zc+TC+3 meet a z-strip: psi_z work and a barrier this chunk
+        const bool zchunk = CPML && ((zc - 4 < N) || (zc + TC + 4 > nzp - N));
+        const int zr = zc + 4 + rg;   // psi_z row of this thread
+        const int z = zc + rg;        // row this thread updates
+        const bool zs = zstrip(z) && z < ze;
+        float pzin[4], paz[4], pbz[4], pxin[4], pax[4], pbx[4], hxin[4], hax[4], hbx[4];
+        float zzo[4], zaz[4], zbz[4], zxo[4];
+        const int hg = (tid < 16) ? 0 : G + 1;  // halo group of threads tid < 32
+        const int hr = tid & 15;                // halo row of threads tid < 32
+        if (CPML) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                pzin[j] = paz[j] = pbz[j] = pxin[j] = pax[j] = pbx[j] = hxin[j] = hax[j] = hbx[j] = zzo[j] = zaz[j] =
+                    zbz[j] = zxo[j] = 0.f;
+            if (zchunk && zstrip(zr)) {
+                const long long ro = (long long)zr * P + col;
+                vld<4>(pzin, a.psiz_in + so + ro);
+                vld<4>(paz, a.caz + ro);
+                vld<4>(pbz, a.cbz + ro);
+            }
+            if (xst && z >= 4 && z < nzp - 4) {
+                const long long ro = (long long)z * P + col;
+                vld<4>(pxin, a.psix_in + so + ro);
+                vld<4>(pax, a.cax + ro);
+                vld<4>(pbx, a.cbx + ro);
+                vld<4>(zxo, a.zetax + so + ro);
+            }
+            if (xst && tid < 32) {
+                const int hz = zc + hr, hc = x0 - 4 + 4 * hg;
+                if (hz >= 4 && hz < nzp - 4 && hc >= 0 && hc < P) {
+                    const long long ro = (long long)hz * P + hc;
+                    vld<4>(hxin, a.psix_in + so + ro);
+                    vld<4>(hax, a.cax + ro);
+                    vld<4>(hbx, a.cbx + ro);
+                }
+            }
+            if (zs) {
+                const long long ro = (long long)z * P + col;
+                vld<4>(zzo, a.zetaz + so + ro);
+                vld<4>(zaz, a.caz + ro);
+                vld<4>(zbz, a.cbz + ro);
+            }
+        }
+        // ---- prefetch chunk ch+1: U rows zc+TC+LA .. zc+2TC+LA-1, and its cells
         float4 pf[NP];
 #pragma unroll
         for (int u = 0; u < NP; ++u) {
             const int i = tid + 256 * u;
             const int r = i / Q, q = i - r * Q;
-            const int gz = zc + TC + 4 + r, gc = x0 - 4 + 4 * q;
+            const int gz = zc + TC + LA + r, gc = x0 - HX + 4 * q;
             pf[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (more && i < NEW && gz < nzp && gc >= 0 && gc < P)
+            if (more && i < NEW && gz < nzp && gc >= 0 && gc < P && want_q(q))
                 pf[u] = *reinterpret_cast<const float4 *>(cur + (long long)gz * P + gc);
         }
         if (more) march_load_cell<PHYS>(mn, a, prv, zc + TC + rg, col);
 
+        bool sync_needed = false;
         if (PHYS == 2) {
-            // first x-derivative of rows zc-4 .. zc+TC+3 (TC+8 rows x 16 float4)
-            for (int i = tid; i < (TC + 8) * (TX / 4); i += 256) {
-                const int r = i >> 4, c = i & 15;
+            // first x-derivative of rows zc-4 .. zc+TC+3
+            for (int i = tid; i < (TC + 8) * G; i += 256) {
+                const int r = i / G, c = i - r * G;
                 const int gz = zc - 4 + r;
-                const float4 *rw = ring + SLOT(gz);
-                float w[12];
-                const float4 l = rw[c], m = rw[c + 1], h = rw[c + 2];
-                w[0] = l.x; w[1] = l.y; w[2] = l.z; w[3] = l.w;
-                w[4] = m.x; w[5] = m.y; w[6] = m.z; w[7] = m.w;
-                w[8] = h.x; w[9] = h.y; w[10] = h.z; w[11] = h.w;
-                float d2[4], d1[4];
+                const float4 *rw = ring + SLOT(gz) + CQ - 1;
+                float w[12], d2[4], d1[4];
+                f4_to(w, rw[c]);
+                f4_to(w + 4, rw[c + 1]);
+                f4_to(w + 8, rw[c + 2]);
                 xderivs<4>(w, d2, d1);
-                sd[((gz + 8) & (kMarchRing - 1)) * (TX / 4) + c] = make_float4(d1[0], d1[1], d1[2], d1[3]);
+                sd[((gz + 8) & (kMarchRing - 1)) * G + c] = make_float4(d1[0], d1[1], d1[2], d1[3]);
             }
-            __syncthreads();
+            sync_needed = true;
         }
+        if (CPML) {
+            // new psi_z of row zr (zero off the z-strips) into the ring
+            float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (zchunk && zstrip(zr)) {
+                float nv[4];
+                psiz_new(nv, pzin, paz, pbz, zr, cx);
+                o = make_float4(nv[0], nv[1], nv[2], nv[3]);
+                if (zr < ze) store_live4(a.psiz + so + (long long)zr * P + col, nv, col, nxp);
+            }
+            pzr[PSLOT(zr) + cx] = o;
+            sync_needed = sync_needed || zchunk;
+            if (xst) {
+                // new psi_x of the chunk rows on columns x0-4 .. x0+TX+3, zero off the x-strips
+                psix_new_item(rg, cx + 1, z, pxin, pax, pbx);
+                if (tid < 32) {
+                    const int hz = zc + hr;
+                    psix_new_item(hr, hg, hz, hxin, hax, hbx);
+                }
+                sync_needed = true;
+            }
+        }
+        if (sync_needed) __syncthreads();  // block-uniform
 
         // ---- compute row z of chunk ch
-        const int z = zc + rg;
         if (z < ze) {
             const long long ro = (long long)z * P + col;
             const float fz = (!CPML && (flags & F_CERJAN)) ? a.cerz[z] : 1.0f;
             float zw[9][4];
 #pragma unroll
-            for (int h = 0; h < 9; ++h) {
-                const float4 t = ring[SLOT(z - 4 + h) + cx + 1];
-                zw[h][0] = t.x; zw[h][1] = t.y; zw[h][2] = t.z; zw[h][3] = t.w;
-            }
+            for (int h = 0; h < 9; ++h) f4_to(zw[h], ring[SLOT(z - 4 + h) + CQ + cx]);
             float w[12];
             {
-                const float4 *rw = ring + SLOT(z);
-                const float4 l = rw[cx], m = rw[cx + 1], h = rw[cx + 2];
-                w[0] = l.x; w[1] = l.y; w[2] = l.z; w[3] = l.w;
-                w[4] = m.x; w[5] = m.y; w[6] = m.z; w[7] = m.w;
-                w[8] = h.x; w[9] = h.y; w[10] = h.z; w[11] = h.w;
+                const float4 *rw = ring + SLOT(z) + CQ - 1 + cx;
+                f4_to(w, rw[0]);
+                f4_to(w + 4, rw[1]);
+                f4_to(w + 8, rw[2]);
             }
             float d2x[4], d1x[4];
             xderivs<4>(w, d2x, d1x);
@@ -189,8 +349,7 @@ __global__ __launch_bounds__(256) void k_march(const StepArgs a) {
 #pragma unroll
                 for (int h = 0; h < 9; ++h) {
                     if (h == 4) continue;
-                    const float4 t = sd[((z - 4 + h + 8) & (kMarchRing - 1)) * (TX / 4) + cx];
-                    q[h][0] = t.x; q[h][1] = t.y; q[h][2] = t.z; q[h][3] = t.w;
+                    f4_to(q[h], sd[((z - 4 + h + 8) & (kMarchRing - 1)) * G + cx]);
                 }
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
@@ -252,7 +411,6 @@ __global__ __launch_bounds__(256) void k_march(const StepArgs a) {
 
             // CPML: zeta with the new psi, corrected derivatives
             float PXX[4], PZZ[4], PX[4], PZ[4], zx[4], zz[4], dpx[4], dpz[4];
-            const bool zs = CPML && ((z >= 4 && z < N) || (z >= nzp - N && z < nzp - 4));
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 PXX[j] = pxx[j];
@@ -261,22 +419,18 @@ __global__ __launch_bounds__(256) void k_march(const StepArgs a) {
                 PZ[j] = pz[j];
                 zx[j] = zz[j] = dpx[j] = dpz[j] = 0.0f;
             }
-            if (CPML && tile_xstrip) {
-                float pw[12], d2p[4], d1p[4], zo[4], ax[4], bx[4];
-                const float *prow = a.psix + so + (long long)z * P;
-                vldg<4>(pw + 4, prow, col, P);
-                if (col >= 4) vld<4>(pw, prow + col - 4);
-                else pw[0] = pw[1] = pw[2] = pw[3] = 0.f;
-                vldg<4>(pw + 8, prow, col + 4, P);
+            if (CPML && xst) {
+                float pw[12], d2p[4], d1p[4];
+                const float4 *pr = pxl + rg * (G + 2) + cx;
+                f4_to(pw, pr[0]);
+                f4_to(pw + 4, pr[1]);
+                f4_to(pw + 8, pr[2]);
                 xderivs<4>(pw, d2p, d1p);
-                vld<4>(zo, a.zetax + so + ro);
-                vld<4>(ax, a.cax + ro);
-                vld<4>(bx, a.cbx + ro);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     if (!xs[j]) continue;
                     const float dps = d1p[j] * a.idx;
-                    zx[j] = ax[j] * zo[j] + bx[j] * (pxx[j] + dps);
+                    zx[j] = pax[j] * zxo[j] + pbx[j] * (pxx[j] + dps);
                     dpx[j] = dps;
                     PX[j] = px[j] + dps;
                     PXX[j] = pxx[j] + dps + zx[j];
@@ -288,18 +442,14 @@ __global__ __launch_bounds__(256) void k_march(const StepArgs a) {
 #pragma unroll
                 for (int h = 0; h < 9; ++h) {
                     if (h == 4) continue;
-                    vld<4>(ps[h], a.psiz + so + (long long)(z - 4 + h) * P + col);
+                    f4_to(ps[h], pzr[PSLOT(z - 4 + h) + cx]);
                 }
-                float zo[4], az[4], bz[4];
-                vld<4>(zo, a.zetaz + so + ro);
-                vld<4>(az, a.caz + ro);
-                vld<4>(bz, a.cbz + ro);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const float dps = (KA1 * (ps[5][j] - ps[3][j]) + KA2 * (ps[6][j] - ps[2][j]) +
                                        KA3 * (ps[7][j] - ps[1][j]) + KA4 * (ps[8][j] - ps[0][j])) *
                                       a.idz;
-                    zz[j] = az[j] * zo[j] + bz[j] * (pzz[j] + dps);
+                    zz[j] = zaz[j] * zzo[j] + zbz[j] * (pzz[j] + dps);
                     dpz[j] = dps;
                     PZ[j] = pz[j] + dps;
                     PZZ[j] = pzz[j] + dps + zz[j];
@@ -369,12 +519,13 @@ __global__ __launch_bounds__(256) void k_march(const StepArgs a) {
         for (int u = 0; u < NP; ++u) {
             const int i = tid + 256 * u;
             const int r = i / Q, q = i - r * Q;
-            if (more && i < NEW) ring[SLOT(zc + TC + 4 + r) + q] = pf[u];
+            if (more && i < NEW) ring[SLOT(zc + TC + LA + r) + q] = pf[u];
         }
         mc = mn;
         __syncthreads();
     }
 #undef SLOT
+#undef PSLOT
 }
 
 template <int PHYS, bool CPML, int CHUNKS>

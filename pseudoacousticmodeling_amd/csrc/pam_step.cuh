@@ -79,7 +79,8 @@ __device__ __forceinline__ float sd_term(float e, float dl, float px, float pz) 
     const float px2 = px * px, pz2 = pz * pz;
     const float num = -2.0f * (e - dl) * px2 * pz2;
     const float den = (1.0f + 2.0f * e) * (px2 * px2) + (pz2 * pz2) + 2.0f * (1.0f + dl) * px2 * pz2;
-    return (fabsf(den) > 1e-12f) ? num / den : 0.0f;
+    // v_rcp_f32 (1 ulp) instead of the IEEE division sequence: den > 1e-12 here
+    return (fabsf(den) > 1e-12f) ? num * __builtin_amdgcn_rcpf(den) : 0.0f;
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD

@@ -80,6 +80,11 @@ class Engine:
     def set_option(self, key, value):
         self._ck(self.lib.pam_set_option(self.ctx, int(key), int(value)))
 
+    def get_option(self, key):
+        v = ctypes.c_longlong()
+        self._ck(self.lib.pam_get_option(self.ctx, int(key), ctypes.byref(v)))
+        return v.value
+
     def set_model(self, vp, eps=None, delta=None, theta=None, layout=L.PAM_INTERIOR):
         ts = [_f32(x, self.device) for x in (vp, eps, delta, theta)]
         self._model_keep = ts
