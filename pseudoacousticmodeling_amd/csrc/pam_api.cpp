@@ -32,6 +32,7 @@ struct pam_ctx {
     float *cerx = nullptr, *cerz = nullptr;
     float *cax = nullptr, *cbx = nullptr, *caz = nullptr, *cbz = nullptr;
     float *vp_pad = nullptr;
+    float *th_pad = nullptr;  // tilt in radians, padded (TTI)
     bool model_set = false;
     // wavelet
     float *wav = nullptr;
@@ -66,6 +67,8 @@ struct pam_ctx {
     float *save_scale = nullptr;
     float *acc_ilum = nullptr, *acc_img = nullptr;
     long long acc_bytes = 0;
+    float *acc_multi = nullptr;  // per-shot eps / delta / theta partials
+    long long acc_multi_bytes = 0;
     float *maxv = nullptr;
     float *ilum_tmp = nullptr;
     long long ilum_tmp_bytes = 0;
@@ -184,6 +187,7 @@ extern "C" int pam_create(const pam_grid *grid, int physics, int abc, const pam_
     if (physics == 2) {
         DALLOC(ctx->cs, pb);
         DALLOC(ctx->sn, pb);
+        DALLOC(ctx->th_pad, pb);
     }
     DALLOC(ctx->vp_pad, pb);
     DALLOC(ctx->cerx, (size_t)ctx->P * sizeof(float));
@@ -241,7 +245,7 @@ extern "C" int pam_destroy(pam_ctx *ctx) {
     float *bufs[] = {ctx->v2,  ctx->eps,  ctx->del,   ctx->cs,    ctx->sn,   ctx->cerx, ctx->cerz,
                      ctx->cax, ctx->cbx,  ctx->caz,   ctx->cbz,   ctx->vp_pad, ctx->wav, ctx->U0,
                      ctx->U1,  ctx->psix, ctx->psiz,  ctx->zetax, ctx->zetaz, ctx->tmp, ctx->save_scale,
-                     ctx->acc_ilum, ctx->maxv, ctx->ilum_tmp};
+                     ctx->acc_ilum, ctx->maxv, ctx->ilum_tmp, ctx->th_pad, ctx->acc_multi};
     for (float *b : bufs) dfree(b);
     dfree(ctx->save);
     dfree(ctx->d_acc);
@@ -324,9 +328,8 @@ extern "C" int pam_set_model(pam_ctx *ctx, const float *vp, const float *eps, co
         KCK(put(delta, ctx->del));
     }
     if (ctx->phys == 2) {
-        // theta is expanded into sn; k_coeffs reads it before writing cs and sn
-        KCK(put(theta, ctx->sn));
-        th = ctx->sn;
+        KCK(put(theta, ctx->th_pad));
+        th = ctx->th_pad;
     }
     KCK(launch_coeffs(ctx->phys, ctx->vp_pad, th, ctx->g.dt, ctx->nzp, ctx->nxp, ctx->P, ctx->v2, ctx->cs, ctx->sn,
                       ctx->st));
@@ -745,7 +748,8 @@ static int ensure_scratch(pam_ctx *ctx, void **p, long long *have, long long nee
 }
 
 // RTM (imaging) or FWI gradient (F_GRADM) over shots, on-the-fly strategy
-static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, bool grad, float *image, float *ilum) {
+static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, bool grad, float *image, float *ilum,
+                       float *g_eps = nullptr, float *g_del = nullptr, float *g_th = nullptr) {
     int r = ready(ctx);
     if (r) return r;
     if (nshots < 1) return fail(ctx, PAM_E_ARG, "nshots must be >= 1");
@@ -759,12 +763,15 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     if (ctx->storage != PAM_STORE_ALL_F32)
         return fail(ctx, PAM_E_UNSUPPORTED, "only PAM_STORE_ALL_F32 is available in this build of the adjoint");
     const long long per_step = (skind == SV_F32_PAD) ? ctx->plane : nint;
+    const bool multi = grad && (g_eps || g_del || g_th);
+    const int nmulti = (ctx->phys == 2) ? 3 : 2;
     // choose the batch so that the snapshots fit in free memory
     size_t freeb = 0, totalb = 0;
     HCK(hipMemGetInfo(&freeb, &totalb));
     int B = batch_size(ctx, nshots);
-    const long long per_shot_bytes = per_step * nt * 4 + 2 * nint * 4 + (linear_adj ? 5 * ctx->plane * 4 : 0);
-    const long long have = (long long)ctx->save_bytes + ctx->acc_bytes + ctx->tmp_bytes;
+    const long long per_shot_bytes = per_step * nt * 4 + 2 * nint * 4 + (linear_adj ? 5 * ctx->plane * 4 : 0) +
+                                     (multi ? nmulti * nint * 4 : 0);
+    const long long have = (long long)ctx->save_bytes + ctx->acc_bytes + ctx->tmp_bytes + ctx->acc_multi_bytes;
     const long long budget = (long long)((freeb + have) * 0.85);
     while (B > 1 && (long long)B * per_shot_bytes > budget) --B;
     if ((long long)B * per_shot_bytes > budget)
@@ -777,6 +784,13 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     if (linear_adj) {
         r = ensure_scratch(ctx, (void **)&ctx->tmp, &ctx->tmp_bytes, (long long)B * 5 * ctx->plane * 4);
         if (r) return r;
+    }
+    float *out_multi[3] = {g_eps, g_del, g_th};
+    if (multi) {
+        r = ensure_scratch(ctx, (void **)&ctx->acc_multi, &ctx->acc_multi_bytes, (long long)B * nmulti * nint * 4);
+        if (r) return r;
+        for (int m = 0; m < nmulti; ++m)
+            if (out_multi[m]) HCK(hipMemsetAsync(out_multi[m], 0, nint * sizeof(float), ctx->st));
     }
     HCK(hipMemsetAsync(image, 0, nint * sizeof(float), ctx->st));
     float *ilum_acc = ilum;
@@ -801,6 +815,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         if (skind == SV_F32_PAD) HCK(hipMemsetAsync(ctx->save, 0, (size_t)nb * per_step * nt * 4, ctx->st));
         HCK(hipMemsetAsync(ctx->acc_ilum, 0, (size_t)nb * nint * 4, ctx->st));
         HCK(hipMemsetAsync(ctx->acc_img, 0, (size_t)nb * nint * 4, ctx->st));
+        if (multi) HCK(hipMemsetAsync(ctx->acc_multi, 0, (size_t)B * nmulti * nint * 4, ctx->st));
         KCK(launch_inject_point(ctx->U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, nb, ctx->st));
         StepArgs a;
         base_step_args(ctx, a, nb);
@@ -823,6 +838,17 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
             KCK(launch_inject_rec(ctx->U0, ctx->plane, ctx->P, ctx->d_rz, ctx->d_rx, ctx->nrec,
                                   data + (long long)b0 * nd, nd, t0 - g.itlag, g.dx * g.dz, nb, ctx->st));
         float *cur = ctx->U0, *prv = ctx->U1;
+        const long long pint = (long long)g.n_abc * ctx->P + g.n_abc;  // interior origin in a padded field
+        // eps / delta / theta partials of step t (src/Migration.py:1053-1064)
+        auto partials = [&](int t, const float *lam) -> int {
+            const float *S = (const float *)ctx->save + (long long)t * per_step + (skind == SV_F32_PAD ? pint : 0);
+            const int sP = (skind == SV_F32_PAD) ? ctx->P : g.nx;
+            float *acc = ctx->acc_multi;
+            return launch_grad_multi(ctx->phys, S, sP, per_step * nt, lam + pint, ctx->P, ctx->plane, nb, g.nz, g.nx,
+                                     g.dx, g.dz, ctx->eps + pint, ctx->del + pint,
+                                     ctx->th_pad ? ctx->th_pad + pint : nullptr, ctx->P, acc,
+                                     acc + (long long)B * nint, nmulti == 3 ? acc + 2LL * B * nint : nullptr, ctx->st);
+        };
         if (!linear_adj) {
             StepArgs b;
             base_step_args(ctx, b, nb);
@@ -838,6 +864,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
             for (int t = t0; t >= 1; --t) {
                 r = fwd_step(ctx, b, tile0, cur, prv, t, nullptr);
                 if (r) return r;
+                if (multi) KCK(partials(t, cur));
                 std::swap(cur, prv);
             }
         } else {
@@ -875,10 +902,11 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
             c.rnext = ctx->d_rnext;
             c.data = data + (long long)b0 * nd;
             c.dstride = nd;
-            c.image = grad ? nullptr : ctx->acc_img;
+            c.image = ctx->acc_img;
             c.acc_sstride = nint;
             c.psstride = per_step * nt;
-            if (grad) return fail(ctx, PAM_E_UNSUPPORTED, "VTI/TTI Cerjan FWI gradient not available yet");
+            c.gradm = grad ? 1 : 0;
+            c.pkstride = per_step;
             for (int t = t0; t >= 1; --t) {
                 c.cur = cur;
                 c.prv = prv;
@@ -886,16 +914,26 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                 c.P_ = (const float *)ctx->save + (long long)t * per_step;
                 KCK(launch_adj_products(ctx->phys, c, ctx->st));
                 KCK(launch_adj_update(ctx->phys, c, ctx->st));
+                if (multi) KCK(partials(t, cur));
                 std::swap(cur, prv);
             }
         }
         KCK(launch_reduce_shots(ctx->acc_img, nint, nb, (int)nint, image, 1, ctx->st));
         KCK(launch_reduce_shots(ctx->acc_ilum, nint, nb, (int)nint, ilum_acc, 1, ctx->st));
+        if (multi)
+            for (int m = 0; m < nmulti; ++m)
+                if (out_multi[m])
+                    KCK(launch_reduce_shots(ctx->acc_multi + (long long)m * B * nint, nint, nb, (int)nint,
+                                            out_multi[m], 1, ctx->st));
     }
     if (r) return r;
     if (ctx->normalize) {
         KCK(launch_maxabs(ilum_acc, (int)nint, ctx->maxv, ctx->st));
         KCK(launch_normalize(image, ilum_acc, (int)nint, ctx->normalize, ctx->maxv, ctx->st));
+        if (multi)
+            for (int m = 0; m < nmulti; ++m)
+                if (out_multi[m])
+                    KCK(launch_normalize(out_multi[m], ilum_acc, (int)nint, ctx->normalize, ctx->maxv, ctx->st));
     }
     HCK(hipStreamSynchronize(ctx->st));
     return PAM_OK;
@@ -909,9 +947,10 @@ extern "C" int pam_rtm(pam_ctx *ctx, int shot0, int nshots, const float *data, f
 extern "C" int pam_gradient(pam_ctx *ctx, int shot0, int nshots, const float *residual, float *g_m, float *g_eps,
                             float *g_delta, float *g_theta, float *ilum) {
     if (!ctx) return PAM_E_ARG;
-    if (g_eps || g_delta || g_theta)
-        return fail(ctx, PAM_E_UNSUPPORTED, "multiparameter partials not available yet");
-    return adjoint_run(ctx, shot0, nshots, residual, true, g_m, ilum);
+    if ((g_eps || g_delta || g_theta) && ctx->phys == PAM_ACOUSTIC)
+        return fail(ctx, PAM_E_ARG, "eps / delta / theta partials need VTI or TTI physics");
+    if (g_theta && ctx->phys != PAM_TTI) return fail(ctx, PAM_E_ARG, "the theta partial needs TTI physics");
+    return adjoint_run(ctx, shot0, nshots, residual, true, g_m, ilum, g_eps, g_delta, g_theta);
 }
 
 extern "C" int pam_misfit(pam_ctx *ctx, int nshots, const float *dobs, const float *dcal, float *residual,

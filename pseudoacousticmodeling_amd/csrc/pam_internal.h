@@ -97,6 +97,8 @@ struct AdjArgs {
     long long dstride;
     float *image;          // per shot
     long long acc_sstride;
+    int gradm;             // image += lambda (S_{t+1} - 2 S_t + S_{t-1}) / dt^2 instead of lambda S_t
+    long long pkstride;    // elements between consecutive saved steps of P_
 };
 
 struct TileShape {
@@ -130,8 +132,14 @@ int launch_reduce_shots(const float *parts, long long sstride, int nshots, int n
 int launch_normalize(float *image, const float *ilum, int n, int mode, const float *maxv, hipStream_t st);
 int launch_maxabs(const float *x, int n, float *out, hipStream_t st);
 int launch_residual(const float *dobs, const float *dcal, float *res, long long n, double *acc, hipStream_t st);
-int launch_grad_multi(int phys, const float *S, long long sk, const float *lam, long long lk, int nz, int nx,
-                      float dx, float dz, const float *eps, const float *del, const float *theta, float *geps,
-                      float *gdel, float *gth, hipStream_t st);
+// multiparameter partials (src/CudaKernels.py:1174-1379) of nshots shots:
+// S (saved source field at step t) and lam (adjoint field) are given at their
+// interior origin with row pitches sP / lP and shot strides ss / ls; the
+// partials accumulate into g* + shot * nz * nx ([nz][nx] per shot); the model
+// fields eps / del / theta (radians) are given at their interior origin with
+// row pitch mP
+int launch_grad_multi(int phys, const float *S, int sP, long long ss, const float *lam, int lP, long long ls,
+                      int nshots, int nz, int nx, float dx, float dz, const float *eps, const float *del,
+                      const float *theta, int mP, float *geps, float *gdel, float *gth, hipStream_t st);
 
 }  // namespace pam

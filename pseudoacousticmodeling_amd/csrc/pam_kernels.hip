@@ -186,8 +186,13 @@ __global__ __launch_bounds__(256) void k_adj_update(const AdjArgs a) {
     a.prv[so + i] = out;
     if (a.image && iz >= a.N && iz < a.nzp - a.N && ix >= a.N && ix < a.nxp - a.N) {
         const long long ii = (long long)(iz - a.N) * a.nx + (ix - a.N);
-        const float S = a.P_[(long long)s * a.psstride + i];
-        a.image[(long long)s * a.acc_sstride + ii] += S * lam;
+        const float *Ps = a.P_ + (long long)s * a.psstride + i;
+        if (a.gradm) {
+            const float d2t = (Ps[a.pkstride] - 2.0f * Ps[0] + Ps[-a.pkstride]) * a.idt2;
+            a.image[(long long)s * a.acc_sstride + ii] += d2t * lam;
+        } else {
+            a.image[(long long)s * a.acc_sstride + ii] += Ps[0] * lam;
+        }
     }
 }
 
@@ -332,14 +337,18 @@ __global__ void k_residual(const float *dobs, const float *dcal, float *res, lon
 // multiparameter partials on interior arrays, float64 arithmetic
 // (src/CudaKernels.py:1174-1379)
 template <int PHYS>
-__global__ void k_grad_multi(const float *S, const float *lam, int nz, int nx, float dx, float dz, const float *eps,
-                             const float *del, const float *theta, float *geps, float *gdel, float *gth) {
+__global__ void k_grad_multi(const float *S, int sP, long long ss, const float *lam, int lP, long long ls, int nz,
+                             int nx, float dx, float dz, const float *eps, const float *del, const float *theta, int mP,
+                             float *geps, float *gdel, float *gth) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     const int iz = blockIdx.y;
+    const int s = blockIdx.z;
     if (ix < 4 || ix >= nx - 4 || iz < 4 || iz >= nz - 4) return;
-    const long long i = (long long)iz * nx + ix;
-    const long long n = nx;
-    const float *c = S + i;
+    const long long i = (long long)iz * nx + ix;  // interior index (models, partials)
+    const long long n = sP;
+    const float *c = S + (long long)s * ss + (long long)iz * sP + ix;
+    const long long go = (long long)s * nz * nx;
+    const long long mi = (long long)iz * mP + ix;  // model index
     const double pxx = (double)((KC0 * c[0] + KC1 * (c[1] + c[-1]) + KC2 * (c[2] + c[-2]) + KC3 * (c[3] + c[-3]) +
                                  KC4 * (c[4] + c[-4])) /
                                 (dx * dx));
@@ -351,8 +360,8 @@ __global__ void k_grad_multi(const float *S, const float *lam, int nz, int nx, f
     const double pz = (double)((KA1 * (c[n] - c[-n]) + KA2 * (c[2 * n] - c[-2 * n]) + KA3 * (c[3 * n] - c[-3 * n]) +
                                 KA4 * (c[4 * n] - c[-4 * n])) /
                                dz);
-    const double e = eps[i], dl = del[i];
-    const double adj = lam[i];
+    const double e = eps[mi], dl = del[mi];
+    const double adj = lam[(long long)s * ls + (long long)iz * lP + ix];
     if (PHYS == 1) {
         const double num = -2.0 * (e - dl) * (px * px) * (pz * pz);
         const double den =
@@ -364,8 +373,8 @@ __global__ void k_grad_multi(const float *S, const float *lam, int nz, int nx, f
             sde = (dn_e * den - num * dd_e) / (den * den);
             sdd = (dn_d * den - num * dd_d) / (den * den);
         }
-        geps[i] = (float)((double)geps[i] + adj * ((-2.0 - sde) * pxx - sde * pzz));
-        gdel[i] = (float)((double)gdel[i] + adj * (-sdd * (pxx + pzz)));
+        geps[go + i] = (float)((double)geps[go + i] + adj * ((-2.0 - sde) * pxx - sde * pzz));
+        gdel[go + i] = (float)((double)gdel[go + i] + adj * (-sdd * (pxx + pzz)));
     } else {
         double acc = 0.0;
         const float aa[4] = {KA1, KA2, KA3, KA4};
@@ -376,7 +385,7 @@ __global__ void k_grad_multi(const float *S, const float *lam, int nz, int nx, f
                         (c[jj * n + ii] - c[-jj * n + ii] + c[-jj * n - ii] - c[jj * n - ii]);
         acc = (double)(accf / (dz * dx));
         const double pxz = acc;
-        const double th = theta[i];
+        const double th = theta[mi];
         const double cs = cos(th), sn = sin(th);
         const double h = px * cs - pz * sn, q = px * sn + pz * cs;
         const double h2 = h * h, q2 = q * q;
@@ -395,10 +404,10 @@ __global__ void k_grad_multi(const float *S, const float *lam, int nz, int nx, f
         }
         const double s2t = sin(2.0 * th), c2t = cos(2.0 * th);
         const double dA = 2.0 * e * s2t - sdt, dB = -2.0 * e * s2t - sdt, dC = 4.0 * e * c2t;
-        geps[i] = (float)((double)geps[i] +
-                          adj * ((-(2.0 * cs * cs + sde)) * pxx - (2.0 * sn * sn + sde) * pzz + 2.0 * s2t * pxz));
-        gdel[i] = (float)((double)gdel[i] + adj * (-sdd * (pxx + pzz)));
-        gth[i] = (float)((double)gth[i] + adj * (dA * pxx + dB * pzz + dC * pxz));
+        geps[go + i] = (float)((double)geps[go + i] +
+                               adj * ((-(2.0 * cs * cs + sde)) * pxx - (2.0 * sn * sn + sde) * pzz + 2.0 * s2t * pxz));
+        gdel[go + i] = (float)((double)gdel[go + i] + adj * (-sdd * (pxx + pzz)));
+        gth[go + i] = (float)((double)gth[go + i] + adj * (dA * pxx + dB * pzz + dC * pxz));
     }
 }
 
@@ -506,18 +515,16 @@ int launch_residual(const float *dobs, const float *dcal, float *res, long long 
     hipLaunchKernelGGL(k_residual, dim3((unsigned)blocks), dim3(256), 0, st, dobs, dcal, res, n, acc);
     return (int)hipGetLastError();
 }
-int launch_grad_multi(int phys, const float *S, long long sk, const float *lam, long long lk, int nz, int nx, float dx,
-                      float dz, const float *eps, const float *del, const float *theta, float *geps, float *gdel,
-                      float *gth, hipStream_t st) {
-    (void)sk;
-    (void)lk;
-    dim3 g((nx + 255) / 256, nz);
+int launch_grad_multi(int phys, const float *S, int sP, long long ss, const float *lam, int lP, long long ls,
+                      int nshots, int nz, int nx, float dx, float dz, const float *eps, const float *del,
+                      const float *theta, int mP, float *geps, float *gdel, float *gth, hipStream_t st) {
+    dim3 g((nx + 255) / 256, nz, nshots);
     if (phys == 1)
-        hipLaunchKernelGGL(k_grad_multi<1>, g, dim3(256), 0, st, S, lam, nz, nx, dx, dz, eps, del, theta, geps, gdel,
-                           gth);
+        hipLaunchKernelGGL(k_grad_multi<1>, g, dim3(256), 0, st, S, sP, ss, lam, lP, ls, nz, nx, dx, dz, eps, del,
+                           theta, mP, geps, gdel, gth);
     else
-        hipLaunchKernelGGL(k_grad_multi<2>, g, dim3(256), 0, st, S, lam, nz, nx, dx, dz, eps, del, theta, geps, gdel,
-                           gth);
+        hipLaunchKernelGGL(k_grad_multi<2>, g, dim3(256), 0, st, S, sP, ss, lam, lP, ls, nz, nx, dx, dz, eps, del,
+                           theta, mP, geps, gdel, gth);
     return (int)hipGetLastError();
 }
 

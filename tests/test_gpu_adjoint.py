@@ -23,15 +23,16 @@ def _data(pb, nshots, seed):
     return np.stack([oracle_forward(orc, pert, shot=s)["seis"] for s in range(nshots)]).astype(np.float32)
 
 
-def _oracle_adjoint(pb, data, fwi):
+def _oracle_adjoint(pb, data, fwi, multiparameter=False):
     orc = Oracle("f32")
     g = pb["g"]
     model = tuple(orc.expand(m, g.N) if m is not None else None for m in (pb["vp"], pb["eps"], pb["dl"], pb["th"]))
     A = orc.cerjan(g.N) if pb["abc"] == 0 else None
     cpml = (pb["d0"], pb["fp"]) if pb["abc"] == 1 else None
     shots = list(zip(pb["sz"].tolist(), pb["sx"].tolist()))
+    interior = (pb["vp"], pb["eps"], pb["dl"], pb["th"])
     return rtm_onthefly(orc, pb["phys"], pb["abc"], g, model, pb["src"], shots, pb["rz"], pb["rx"], data, A=A,
-                        cpml=cpml, fwi=fwi)
+                        cpml=cpml, fwi=fwi, multiparameter=multiparameter, model_interior=interior)
 
 
 @pytest.mark.parametrize("phys,abc", [(p, a) for p in (0, 1, 2) for a in (0, 1)])
@@ -52,7 +53,7 @@ def test_rtm_matches_oracle(phys, abc):
     assert e_img <= TOL
 
 
-@pytest.mark.parametrize("phys,abc", [(0, 0), (0, 1), (1, 1), (2, 1)])
+@pytest.mark.parametrize("phys,abc", [(p, a) for p in (0, 1, 2) for a in (0, 1)])
 def test_gradient_matches_oracle(phys, abc):
     print(f"case gradient phys={phys} abc={abc}", flush=True)
     pb = make_problem(phys, abc, nshots=2, seed=80 + 2 * phys + abc, nt=240)
@@ -78,3 +79,23 @@ def test_rtm_batched_equals_one_shot_at_a_time():
     img1, il1 = eng.rtm(data)
     assert rel_l2(img1.cpu().numpy(), img3.cpu().numpy()) < 1e-6
     assert rel_l2(il1.cpu().numpy(), il3.cpu().numpy()) < 1e-6
+
+
+@pytest.mark.parametrize("phys,abc", [(1, 0), (1, 1), (2, 0), (2, 1)])
+def test_multiparameter_partials_match_oracle(phys, abc):
+    """eps / delta (/ theta) partials of the FWI gradient, src/Migration.py:1053-1064 with
+    src/CudaKernels.py:1174-1379, normalised by the illumination."""
+    print(f"case partials phys={phys} abc={abc}", flush=True)
+    pb = make_problem(phys, abc, nshots=2, seed=95 + 2 * phys + abc, nt=240)
+    data = _data(pb, 2, seed=11)
+    ref = _oracle_adjoint(pb, data, fwi=True, multiparameter=True)
+    eng = make_engine(pb, max_shots=2)
+    out = eng.gradient(data, multiparameter=True)
+    names = [("g_eps", "g_eps"), ("g_delta", "g_delta")] + ([("g_theta", "g_theta")] if phys == 2 else [])
+    assert rel_l2(out["g_m"].cpu().numpy(), ref["image"]) <= TOL
+    for gk, rk in names:
+        g = out[gk].cpu().numpy()
+        err = rel_l2(g, ref[rk])
+        print(f"  {gk} rel_l2 {err:.3e}", flush=True)
+        assert np.isfinite(g).all() and np.abs(g).max() > 0
+        assert err <= TOL
