@@ -17,6 +17,18 @@
 
 using namespace pam;
 
+// The fields one set of shots is advanced with: the two wavefield buffers and
+// the CPML memory variables ([max_shots][plane] each; psi twice, double-
+// buffered across steps, psi_flip selecting the half that holds the previous
+// step's psi).  The context holds the main set and, for checkpointed adjoint
+// runs, a second set that recomputes source-field segments while the main set
+// carries the adjoint field.
+struct FieldSet {
+    float *U0 = nullptr, *U1 = nullptr;
+    float *psix = nullptr, *psiz = nullptr, *zetax = nullptr, *zetaz = nullptr;
+    int psi_flip = 0;
+};
+
 struct pam_ctx {
     pam_grid g{};
     int phys = 0, abc = 0;
@@ -36,6 +48,7 @@ struct pam_ctx {
     bool model_set = false;
     // wavelet
     float *wav = nullptr;
+    float wav_absmax = 0.f;
     bool wav_set = false;
     // geometry
     std::vector<int> hsz, hsx, hrz, hrx;
@@ -43,13 +56,9 @@ struct pam_ctx {
     int *d_rz = nullptr, *d_rx = nullptr, *d_rrow = nullptr, *d_rhead = nullptr, *d_rnext = nullptr;
     int *d_ssz = nullptr, *d_ssx = nullptr;  // [max_shots]
     bool geom_set = false;
-    // wavefields [max_shots][plane]
-    float *U0 = nullptr, *U1 = nullptr;
-    // psi is double-buffered across steps: half psi_flip holds the previous
-    // step's psi, the other half receives the new one
-    float *psix = nullptr, *psiz = nullptr, *zetax = nullptr, *zetaz = nullptr;
-    long long psi_half = 0;  // floats between the two halves
-    int psi_flip = 0;
+    // wavefields and CPML memory: the main set and the recompute set
+    FieldSet fs, aux;
+    long long psi_half = 0;  // floats between the two halves of psi
     // per-step interface state (slot 0)
     float *cur = nullptr, *prv = nullptr;
     int next_k = -1;
@@ -59,6 +68,13 @@ struct pam_ctx {
     // options
     int batch = 0, tile = -1, graph = 1, storage = PAM_STORE_ALL_F32, ckpt_step = 0;
     int reset_adj_cpml = 1, normalize = 1;
+    // checkpoints of the main set ([segment][field][shot][plane]) and the
+    // psi_flip at each
+    float *ckpt = nullptr;
+    long long ckpt_bytes = 0;
+    std::vector<int> ckpt_flip;
+    unsigned int *smax = nullptr;  // per-snapshot max |S_k| (compressed storage)
+    long long smax_bytes = 0;
     // adjoint scratch
     float *tmp = nullptr;
     long long tmp_bytes = 0;
@@ -169,14 +185,14 @@ extern "C" int pam_create(const pam_grid *grid, int physics, int abc, const pam_
     *out = ctx;
     // wavefields and per-shot CPML memory
     const size_t fb = (size_t)ctx->plane * max_shots * sizeof(float);
-    DALLOC(ctx->U0, fb);
-    DALLOC(ctx->U1, fb);
+    DALLOC(ctx->fs.U0, fb);
+    DALLOC(ctx->fs.U1, fb);
     if (abc == PAM_CPML) {
-        DALLOC(ctx->psix, 2 * fb);
-        DALLOC(ctx->psiz, 2 * fb);
+        DALLOC(ctx->fs.psix, 2 * fb);
+        DALLOC(ctx->fs.psiz, 2 * fb);
         ctx->psi_half = ctx->plane * max_shots;
-        DALLOC(ctx->zetax, fb);
-        DALLOC(ctx->zetaz, fb);
+        DALLOC(ctx->fs.zetax, fb);
+        DALLOC(ctx->fs.zetaz, fb);
     }
     const size_t pb = (size_t)ctx->plane * sizeof(float);
     DALLOC(ctx->v2, pb);
@@ -203,8 +219,8 @@ extern "C" int pam_create(const pam_grid *grid, int physics, int abc, const pam_
     DALLOC(ctx->d_ssx, (size_t)max_shots * sizeof(int));
     DALLOC(ctx->maxv, 64);
     DALLOC(ctx->d_acc, 64);
-    HCK(hipMemsetAsync(ctx->U0, 0, fb, ctx->st));
-    HCK(hipMemsetAsync(ctx->U1, 0, fb, ctx->st));
+    HCK(hipMemsetAsync(ctx->fs.U0, 0, fb, ctx->st));
+    HCK(hipMemsetAsync(ctx->fs.U1, 0, fb, ctx->st));
     // Cerjan factors: 1 in the interior, A[i] in the strips (cpp/Modeling.cu:111-120)
     {
         const int N = g.n_abc;
@@ -243,12 +259,14 @@ extern "C" int pam_destroy(pam_ctx *ctx) {
     if (!ctx) return PAM_OK;
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
     float *bufs[] = {ctx->v2,  ctx->eps,  ctx->del,   ctx->cs,    ctx->sn,   ctx->cerx, ctx->cerz,
-                     ctx->cax, ctx->cbx,  ctx->caz,   ctx->cbz,   ctx->vp_pad, ctx->wav, ctx->U0,
-                     ctx->U1,  ctx->psix, ctx->psiz,  ctx->zetax, ctx->zetaz, ctx->tmp, ctx->save_scale,
+                     ctx->cax, ctx->cbx,  ctx->caz,   ctx->cbz,   ctx->vp_pad, ctx->wav, ctx->fs.U0,
+                     ctx->fs.U1,  ctx->fs.psix, ctx->fs.psiz,  ctx->fs.zetax, ctx->fs.zetaz, ctx->tmp, ctx->save_scale,
+                     ctx->aux.U0, ctx->aux.U1, ctx->aux.psix, ctx->aux.psiz, ctx->aux.zetax, ctx->aux.zetaz, ctx->ckpt,
                      ctx->acc_ilum, ctx->maxv, ctx->ilum_tmp, ctx->th_pad, ctx->acc_multi};
     for (float *b : bufs) dfree(b);
     dfree(ctx->save);
     dfree(ctx->d_acc);
+    dfree(ctx->smax);
     int *ibufs[] = {ctx->d_rz, ctx->d_rx, ctx->d_rrow, ctx->d_rhead, ctx->d_rnext, ctx->d_ssz, ctx->d_ssx};
     for (int *b : ibufs) dfree(b);
     for (auto e : ctx->kev) (void)hipEventDestroy(e);
@@ -345,7 +363,11 @@ extern "C" int pam_set_wavelet(pam_ctx *ctx, const float *src) {
     if (!ctx) return PAM_E_ARG;
     if (!src) return fail(ctx, PAM_E_ARG, "wavelet is NULL");
     HCK(hipMemcpyAsync(ctx->wav, src, (size_t)ctx->g.nt * sizeof(float), hipMemcpyDeviceToDevice, ctx->st));
+    std::vector<float> h(ctx->g.nt);
+    HCK(hipMemcpyAsync(h.data(), src, h.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->st));
     HCK(hipStreamSynchronize(ctx->st));
+    ctx->wav_absmax = 0.f;
+    for (float v : h) ctx->wav_absmax = std::max(ctx->wav_absmax, std::fabs(v));
     ctx->wav_set = true;
     return PAM_OK;
 }
@@ -442,10 +464,10 @@ static void base_step_args(pam_ctx *ctx, StepArgs &a, int nshots) {
     a.sn = ctx->sn;
     a.cerx = ctx->cerx;
     a.cerz = ctx->cerz;
-    a.psix = ctx->psix;
-    a.psiz = ctx->psiz;
-    a.zetax = ctx->zetax;
-    a.zetaz = ctx->zetaz;
+    a.psix = ctx->fs.psix;
+    a.psiz = ctx->fs.psiz;
+    a.zetax = ctx->fs.zetax;
+    a.zetaz = ctx->fs.zetaz;
     a.cax = ctx->cax;
     a.cbx = ctx->cbx;
     a.caz = ctx->caz;
@@ -469,30 +491,33 @@ static void base_psi_args(pam_ctx *ctx, PsiArgs &p, int nshots, const float *cur
     p.idx = 1.0f / ctx->g.dx;
     p.idz = 1.0f / ctx->g.dz;
     p.cur = cur;
-    p.psix = ctx->psix;
-    p.psiz = ctx->psiz;
+    p.psix = ctx->fs.psix;
+    p.psiz = ctx->fs.psiz;
     p.cax = ctx->cax;
     p.cbx = ctx->cbx;
     p.caz = ctx->caz;
     p.cbz = ctx->cbz;
 }
 
-static int zero_shot_state(pam_ctx *ctx, int nshots, bool fields, bool cpml) {
+static int zero_fields(pam_ctx *ctx, FieldSet &f, int nshots, bool fields, bool cpml) {
     const size_t fb = (size_t)ctx->plane * nshots * sizeof(float);
     if (fields) {
-        HCK(hipMemsetAsync(ctx->U0, 0, fb, ctx->st));
-        HCK(hipMemsetAsync(ctx->U1, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(f.U0, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(f.U1, 0, fb, ctx->st));
     }
     if (cpml && ctx->abc == PAM_CPML) {
-        HCK(hipMemsetAsync(ctx->psix, 0, fb, ctx->st));
-        HCK(hipMemsetAsync(ctx->psiz, 0, fb, ctx->st));
-        HCK(hipMemsetAsync(ctx->psix + ctx->psi_half, 0, fb, ctx->st));
-        HCK(hipMemsetAsync(ctx->psiz + ctx->psi_half, 0, fb, ctx->st));
-        ctx->psi_flip = 0;
-        HCK(hipMemsetAsync(ctx->zetax, 0, fb, ctx->st));
-        HCK(hipMemsetAsync(ctx->zetaz, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(f.psix, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(f.psiz, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(f.psix + ctx->psi_half, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(f.psiz + ctx->psi_half, 0, fb, ctx->st));
+        f.psi_flip = 0;
+        HCK(hipMemsetAsync(f.zetax, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(f.zetaz, 0, fb, ctx->st));
     }
     return PAM_OK;
+}
+static int zero_shot_state(pam_ctx *ctx, int nshots, bool fields, bool cpml) {
+    return zero_fields(ctx, ctx->fs, nshots, fields, cpml);
 }
 
 static int upload_sources(pam_ctx *ctx, int shot0, int nshots) {
@@ -504,15 +529,18 @@ static int upload_sources(pam_ctx *ctx, int shot0, int nshots) {
     return PAM_OK;
 }
 
-// One forward time step of `nshots` shots: psi pre-pass (CPML, unless the
-// kernel variant updates psi itself) then the fused step.
-static int fwd_step(pam_ctx *ctx, StepArgs &a, int tile, float *cur, float *prv, int k, hipEvent_t *ev) {
+// One forward time step of `nshots` shots on the field set f: psi pre-pass
+// (CPML, unless the kernel variant updates psi itself) then the fused step.
+static int fwd_step_fs(pam_ctx *ctx, FieldSet &f, StepArgs &a, int tile, float *cur, float *prv, int k,
+                       hipEvent_t *ev) {
     if (ctx->abc == PAM_CPML) {
-        const long long in = ctx->psi_flip ? ctx->psi_half : 0, out = ctx->psi_flip ? 0 : ctx->psi_half;
-        a.psix_in = ctx->psix + in;
-        a.psiz_in = ctx->psiz + in;
-        a.psix = ctx->psix + out;
-        a.psiz = ctx->psiz + out;
+        const long long in = f.psi_flip ? ctx->psi_half : 0, out = f.psi_flip ? 0 : ctx->psi_half;
+        a.psix_in = f.psix + in;
+        a.psiz_in = f.psiz + in;
+        a.psix = f.psix + out;
+        a.psiz = f.psiz + out;
+        a.zetax = f.zetax;
+        a.zetaz = f.zetaz;
         if (!tile_fuses_psi(tile)) {
             PsiArgs p;
             base_psi_args(ctx, p, a.nshots, cur);
@@ -529,8 +557,11 @@ static int fwd_step(pam_ctx *ctx, StepArgs &a, int tile, float *cur, float *prv,
     if (ev) HCK(hipEventRecord(ev[0], ctx->st));
     KCK(launch_step(ctx->phys, ctx->abc == PAM_CPML, tile, a, ctx->st));
     if (ev) HCK(hipEventRecord(ev[1], ctx->st));
-    if (ctx->abc == PAM_CPML) ctx->psi_flip ^= 1;
+    if (ctx->abc == PAM_CPML) f.psi_flip ^= 1;
     return PAM_OK;
+}
+static int fwd_step(pam_ctx *ctx, StepArgs &a, int tile, float *cur, float *prv, int k, hipEvent_t *ev) {
+    return fwd_step_fs(ctx, ctx->fs, a, tile, cur, prv, k, ev);
 }
 
 // The autotuner: times every step-kernel variant on the wavefields of `nb`
@@ -547,7 +578,7 @@ static int autotune(pam_ctx *ctx, int nb, int *best) {
     if (r) return r;
     double best_ms = 1e300;
     for (int t = 0; t < tile_count(); ++t) {
-        float *c = ctx->U0, *p = ctx->U1;
+        float *c = ctx->fs.U0, *p = ctx->fs.U1;
         for (int i = 0; i < 2; ++i) {
             r = fwd_step(ctx, a, t, c, p, i, nullptr);
             if (r) return r;
@@ -607,9 +638,9 @@ extern "C" int pam_reset(pam_ctx *ctx, int shot) {
     if (r) return r;
     r = upload_sources(ctx, shot, 1);
     if (r) return r;
-    KCK(launch_inject_point(ctx->U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, 1, ctx->st));
-    ctx->cur = ctx->U0;
-    ctx->prv = ctx->U1;
+    KCK(launch_inject_point(ctx->fs.U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, 1, ctx->st));
+    ctx->cur = ctx->fs.U0;
+    ctx->prv = ctx->fs.U1;
     ctx->next_k = 0;
     return PAM_OK;
 }
@@ -670,7 +701,7 @@ static int finish_timing(pam_ctx *ctx, long long nk) {
 // forward loop over nt steps for shots already primed in U0 (cur) / U1 (prv);
 // extra flags / args are applied to every step
 static int run_forward_loop(pam_ctx *ctx, StepArgs &a, int tile, bool timed) {
-    float *cur = ctx->U0, *prv = ctx->U1;
+    float *cur = ctx->fs.U0, *prv = ctx->fs.U1;
     const int nt = ctx->g.nt;
     if (timed && ctx->profile) {
         int r = ensure_events(ctx, 2 * (size_t)nt);
@@ -714,7 +745,7 @@ extern "C" int pam_forward(pam_ctx *ctx, int shot0, int nshots, float *seis) {
         r = upload_sources(ctx, shot0 + b0, nb);
         if (r) return r;
         HCK(hipMemsetAsync(seis + (long long)b0 * nd, 0, (size_t)nb * nd * sizeof(float), ctx->st));
-        KCK(launch_inject_point(ctx->U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, nb, ctx->st));
+        KCK(launch_inject_point(ctx->fs.U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, nb, ctx->st));
         StepArgs a;
         base_step_args(ctx, a, nb);
         a.flags |= F_SRC | F_GATHER;
@@ -747,7 +778,71 @@ static int ensure_scratch(pam_ctx *ctx, void **p, long long *have, long long nee
     return PAM_OK;
 }
 
-// RTM (imaging) or FWI gradient (F_GRADM) over shots, on-the-fly strategy
+static int alloc_fieldset(pam_ctx *ctx, FieldSet &f) {
+    const size_t fb = (size_t)ctx->plane * ctx->max_shots * sizeof(float);
+    DALLOC(f.U0, fb);
+    DALLOC(f.U1, fb);
+    if (ctx->abc == PAM_CPML) {
+        DALLOC(f.psix, 2 * fb);
+        DALLOC(f.psiz, 2 * fb);
+        DALLOC(f.zetax, fb);
+        DALLOC(f.zetaz, fb);
+        HCK(hipMemsetAsync(f.psix, 0, 2 * fb, ctx->st));
+        HCK(hipMemsetAsync(f.psiz, 0, 2 * fb, ctx->st));
+    }
+    return PAM_OK;
+}
+
+// fields held by a checkpoint: the two wavefields, and for CPML the previous
+// step's psi (x, z) and zeta (x, z)
+static int ckpt_fields(const pam_ctx *ctx) { return ctx->abc == PAM_CPML ? 6 : 2; }
+
+// copy (cur, prv, CPML state) of nb shots of the main set into checkpoint j
+static int save_ckpt(pam_ctx *ctx, int j, int nb, const float *cur, const float *prv) {
+    const size_t fb = (size_t)ctx->plane * nb * sizeof(float);
+    const long long fl = ctx->plane * nb;  // floats per checkpointed field
+    float *c = ctx->ckpt + (long long)j * ckpt_fields(ctx) * fl;
+    HCK(hipMemcpyAsync(c, cur, fb, hipMemcpyDeviceToDevice, ctx->st));
+    HCK(hipMemcpyAsync(c + fl, prv, fb, hipMemcpyDeviceToDevice, ctx->st));
+    if (ctx->abc == PAM_CPML) {
+        const long long in = ctx->fs.psi_flip ? ctx->psi_half : 0;
+        HCK(hipMemcpyAsync(c + 2 * fl, ctx->fs.psix + in, fb, hipMemcpyDeviceToDevice, ctx->st));
+        HCK(hipMemcpyAsync(c + 3 * fl, ctx->fs.psiz + in, fb, hipMemcpyDeviceToDevice, ctx->st));
+        HCK(hipMemcpyAsync(c + 4 * fl, ctx->fs.zetax, fb, hipMemcpyDeviceToDevice, ctx->st));
+        HCK(hipMemcpyAsync(c + 5 * fl, ctx->fs.zetaz, fb, hipMemcpyDeviceToDevice, ctx->st));
+    }
+    ctx->ckpt_flip[j] = ctx->fs.psi_flip;
+    return PAM_OK;
+}
+
+// restore checkpoint j into the recompute set (U0 = cur, U1 = prv)
+static int restore_ckpt(pam_ctx *ctx, int j, int nb) {
+    const size_t fb = (size_t)ctx->plane * nb * sizeof(float);
+    const long long fl = ctx->plane * nb;
+    const float *c = ctx->ckpt + (long long)j * ckpt_fields(ctx) * fl;
+    FieldSet &f = ctx->aux;
+    HCK(hipMemcpyAsync(f.U0, c, fb, hipMemcpyDeviceToDevice, ctx->st));
+    HCK(hipMemcpyAsync(f.U1, c + fl, fb, hipMemcpyDeviceToDevice, ctx->st));
+    if (ctx->abc == PAM_CPML) {
+        f.psi_flip = ctx->ckpt_flip[j];
+        const long long in = f.psi_flip ? ctx->psi_half : 0;
+        HCK(hipMemcpyAsync(f.psix + in, c + 2 * fl, fb, hipMemcpyDeviceToDevice, ctx->st));
+        HCK(hipMemcpyAsync(f.psiz + in, c + 3 * fl, fb, hipMemcpyDeviceToDevice, ctx->st));
+        HCK(hipMemcpyAsync(f.zetax, c + 4 * fl, fb, hipMemcpyDeviceToDevice, ctx->st));
+        HCK(hipMemcpyAsync(f.zetaz, c + 5 * fl, fb, hipMemcpyDeviceToDevice, ctx->st));
+    }
+    return PAM_OK;
+}
+
+// RTM (imaging) or FWI gradient (F_GRADM) over shots.  The source field is
+// kept by one of the strategies of enum pam_storage:
+//  - every step in float32 (src/Migration.py:996-1037);
+//  - every step in float16 / bfloat16 scaled by the snapshot's max |S_k| over
+//    the interior, found by the step that produces the field (F_SMAX);
+//  - uniform checkpoints of the whole state every C steps in HBM, each segment
+//    recomputed into a (C+2)-step buffer on a second field set just before
+//    the adjoint pass crosses it (src/Migration.py:1114-1263 keeps them in host
+//    files); the recomputed fields are bit-identical to the first pass.
 static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, bool grad, float *image, float *ilum,
                        float *g_eps = nullptr, float *g_del = nullptr, float *g_th = nullptr) {
     int r = ready(ctx);
@@ -756,27 +851,42 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     if (!data || !image) return fail(ctx, PAM_E_ARG, "data and image must not be NULL");
     const pam_grid &g = ctx->g;
     const int nt = g.nt;
+    if (nt < 3) return fail(ctx, PAM_E_ARG, "the adjoint pass needs nt >= 3");
     const long long nd = (long long)(nt - g.itlag) * ctx->nrec;
     const long long nint = (long long)g.nz * g.nx;
     const bool linear_adj = (ctx->phys >= 1 && ctx->abc == PAM_CERJAN);
-    const int skind = linear_adj ? SV_F32_PAD : SV_F32_INT;
-    if (ctx->storage != PAM_STORE_ALL_F32)
-        return fail(ctx, PAM_E_UNSUPPORTED, "only PAM_STORE_ALL_F32 is available in this build of the adjoint");
-    const long long per_step = (skind == SV_F32_PAD) ? ctx->plane : nint;
+    const int storage = ctx->storage;
+    const bool ckpt = storage == PAM_STORE_CHECKPOINT;
+    const bool half = storage == PAM_STORE_ALL_F16 || storage == PAM_STORE_ALL_BF16;
+    if (half && linear_adj)
+        return fail(ctx, PAM_E_UNSUPPORTED,
+                    "float16 / bfloat16 snapshots feed the pressure-form adjoint only; the linearised VTI/TTI "
+                    "adjoint (Cerjan) reads float32 padded snapshots: use PAM_STORE_ALL_F32 or PAM_STORE_CHECKPOINT");
+    const int skind = linear_adj ? SV_F32_PAD
+                                 : (storage == PAM_STORE_ALL_F16 ? SV_F16_INT
+                                                                 : (storage == PAM_STORE_ALL_BF16 ? SV_BF16_INT : SV_F32_INT));
+    const int elem = half ? 2 : 4;
+    const long long per_step = (skind == SV_F32_PAD) ? ctx->plane : nint;  // elements per snapshot
+    const int C = ckpt ? (ctx->ckpt_step > 0 ? ctx->ckpt_step : (int)std::ceil(std::sqrt((double)nt))) : nt;
+    const int nseg = ckpt ? (nt - 2) / C + 1 : 1;  // segments of backward steps t in [1, nt-2]
+    const int nbuf = ckpt ? C + 2 : nt;            // snapshots held at once
     const bool multi = grad && (g_eps || g_del || g_th);
     const int nmulti = (ctx->phys == 2) ? 3 : 2;
-    // choose the batch so that the snapshots fit in free memory
+    // choose the batch so that the snapshots (and checkpoints) fit in free memory
     size_t freeb = 0, totalb = 0;
     HCK(hipMemGetInfo(&freeb, &totalb));
     int B = batch_size(ctx, nshots);
-    const long long per_shot_bytes = per_step * nt * 4 + 2 * nint * 4 + (linear_adj ? 5 * ctx->plane * 4 : 0) +
-                                     (multi ? nmulti * nint * 4 : 0);
-    const long long have = (long long)ctx->save_bytes + ctx->acc_bytes + ctx->tmp_bytes + ctx->acc_multi_bytes;
-    const long long budget = (long long)((freeb + have) * 0.85);
+    const long long per_shot_bytes = per_step * nbuf * elem + 2 * nint * 4 + (linear_adj ? 5 * ctx->plane * 4 : 0) +
+                                     (multi ? nmulti * nint * 4 : 0) + (half ? (long long)nt * 4 : 0) +
+                                     (ckpt ? (long long)nseg * ckpt_fields(ctx) * ctx->plane * 4 : 0);
+    const long long have = (long long)ctx->save_bytes + ctx->acc_bytes + ctx->tmp_bytes + ctx->acc_multi_bytes +
+                           ctx->ckpt_bytes + ctx->smax_bytes;
+    long long budget = (long long)((freeb + have) * 0.85);
+    if (ckpt && !ctx->aux.U0) budget -= (long long)ctx->plane * ctx->max_shots * 4 * (ctx->abc == PAM_CPML ? 8 : 2);
     while (B > 1 && (long long)B * per_shot_bytes > budget) --B;
     if ((long long)B * per_shot_bytes > budget)
         return fail(ctx, PAM_E_NOMEM, "one shot's snapshots (%lld bytes) exceed free device memory", per_shot_bytes);
-    r = ensure_scratch(ctx, &ctx->save, &ctx->save_bytes, (long long)B * per_step * nt * 4);
+    r = ensure_scratch(ctx, &ctx->save, &ctx->save_bytes, (long long)B * per_step * nbuf * elem);
     if (r) return r;
     r = ensure_scratch(ctx, (void **)&ctx->acc_ilum, &ctx->acc_bytes, (long long)B * nint * 4 * 2);
     if (r) return r;
@@ -784,6 +894,18 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     if (linear_adj) {
         r = ensure_scratch(ctx, (void **)&ctx->tmp, &ctx->tmp_bytes, (long long)B * 5 * ctx->plane * 4);
         if (r) return r;
+    }
+    if (half) {
+        r = ensure_scratch(ctx, (void **)&ctx->smax, &ctx->smax_bytes, (long long)B * nt * 4);
+        if (r) return r;
+    }
+    if (ckpt) {
+        r = ensure_scratch(ctx, (void **)&ctx->ckpt, &ctx->ckpt_bytes,
+                           (long long)nseg * ckpt_fields(ctx) * ctx->plane * B * 4);
+        if (r) return r;
+        r = alloc_fieldset(ctx, ctx->aux);
+        if (r) return r;
+        ctx->ckpt_flip.assign(nseg, 0);
     }
     float *out_multi[3] = {g_eps, g_del, g_th};
     if (multi) {
@@ -803,73 +925,91 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     ctx->loop_ms = ctx->kernel_ms = 0.0;
     ctx->launches = 0;
     ctx->next_k = -1;
-    const int tile0 = choose_tile(ctx, B, true);
+    int tile0 = choose_tile(ctx, B, true);
+    if (half && tile_shape(tile0).V > 0) tile0 = kDefaultTile;  // k_step has no fused max
     ctx->last_tile = tile0;
+    const long long pint = (long long)g.n_abc * ctx->P + g.n_abc;  // interior origin in a padded field
+    HCK(hipEventRecord(ctx->e0, ctx->st));
     for (int b0 = 0; b0 < nshots; b0 += B) {
         const int nb = std::min(B, nshots - b0);
-        // ---- forward with snapshots and illumination
+        // ---- forward pass: illumination, and snapshots or checkpoints
         r = zero_shot_state(ctx, nb, true, true);
-        if (r) break;
+        if (r) return r;
         r = upload_sources(ctx, shot0 + b0, nb);
-        if (r) break;
-        if (skind == SV_F32_PAD) HCK(hipMemsetAsync(ctx->save, 0, (size_t)nb * per_step * nt * 4, ctx->st));
+        if (r) return r;
+        if (skind == SV_F32_PAD) HCK(hipMemsetAsync(ctx->save, 0, (size_t)nb * per_step * nbuf * elem, ctx->st));
         HCK(hipMemsetAsync(ctx->acc_ilum, 0, (size_t)nb * nint * 4, ctx->st));
         HCK(hipMemsetAsync(ctx->acc_img, 0, (size_t)nb * nint * 4, ctx->st));
         if (multi) HCK(hipMemsetAsync(ctx->acc_multi, 0, (size_t)B * nmulti * nint * 4, ctx->st));
-        KCK(launch_inject_point(ctx->U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, nb, ctx->st));
+        KCK(launch_inject_point(ctx->fs.U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, nb, ctx->st));
+        if (half) {
+            HCK(hipMemsetAsync(ctx->smax, 0, (size_t)nb * nt * 4, ctx->st));
+            // scale of S_0 (imaging) or a bound of |D_0| (gradient): 3 max|wavelet| bounds
+            // |S_1 - 2 S_0| under the CFL condition
+            KCK(launch_smax0(ctx->smax, nt, ctx->d_ssz, ctx->d_ssx, g.n_abc, ctx->nzp, ctx->nxp, ctx->wav, nb,
+                             grad ? 3.0f * ctx->wav_absmax : 0.0f, ctx->st));
+        }
         StepArgs a;
         base_step_args(ctx, a, nb);
-        a.flags |= F_SRC | F_SAVE | F_ILUM;
+        a.flags |= F_SRC | F_ILUM | (ckpt ? 0 : F_SAVE) | (half ? F_SMAX : 0) | (half && grad ? F_D2T : 0);
         a.save_kind = skind;
         a.save = ctx->save;
+        a.save_k0 = 0;
         a.save_kstride = per_step;
-        a.save_sstride = per_step * nt;
+        a.save_sstride = per_step * nbuf;
+        a.save_scale = reinterpret_cast<const float *>(ctx->smax);
+        a.smax = ctx->smax;
         a.ilum = ctx->acc_ilum;
         a.image = ctx->acc_img;
         a.acc_sstride = nint;
-        r = run_forward_loop(ctx, a, tile0, false);
-        if (r) break;
-        // ---- backward: lambda fields reuse U0/U1
+        {
+            float *cur = ctx->fs.U0, *prv = ctx->fs.U1;
+            int j = 0;
+            for (int k = 0; k < nt; ++k) {
+                if (ckpt && j < nseg && k == std::max(0, j * C - 1)) {
+                    r = save_ckpt(ctx, j, nb, cur, prv);
+                    if (r) return r;
+                    ++j;
+                }
+                r = fwd_step(ctx, a, tile0, cur, prv, k, nullptr);
+                if (r) return r;
+                std::swap(cur, prv);
+            }
+        }
+        // ---- backward pass: the adjoint field reuses the main set
         r = zero_shot_state(ctx, nb, true, ctx->reset_adj_cpml != 0);
-        if (r) break;
+        if (r) return r;
         const int t0 = nt - 2;
-        if (t0 < 1) continue;
         if (t0 >= g.itlag)
-            KCK(launch_inject_rec(ctx->U0, ctx->plane, ctx->P, ctx->d_rz, ctx->d_rx, ctx->nrec,
+            KCK(launch_inject_rec(ctx->fs.U0, ctx->plane, ctx->P, ctx->d_rz, ctx->d_rx, ctx->nrec,
                                   data + (long long)b0 * nd, nd, t0 - g.itlag, g.dx * g.dz, nb, ctx->st));
-        float *cur = ctx->U0, *prv = ctx->U1;
-        const long long pint = (long long)g.n_abc * ctx->P + g.n_abc;  // interior origin in a padded field
+        float *cur = ctx->fs.U0, *prv = ctx->fs.U1;
+        int k0 = 0;  // step held in snapshot slot 0
         // eps / delta / theta partials of step t (src/Migration.py:1053-1064)
         auto partials = [&](int t, const float *lam) -> int {
-            const float *S = (const float *)ctx->save + (long long)t * per_step + (skind == SV_F32_PAD ? pint : 0);
+            const float *S = (const float *)ctx->save + (long long)(t - k0) * per_step + (skind == SV_F32_PAD ? pint : 0);
             const int sP = (skind == SV_F32_PAD) ? ctx->P : g.nx;
             float *acc = ctx->acc_multi;
-            return launch_grad_multi(ctx->phys, S, sP, per_step * nt, lam + pint, ctx->P, ctx->plane, nb, g.nz, g.nx,
+            return launch_grad_multi(ctx->phys, S, sP, per_step * nbuf, lam + pint, ctx->P, ctx->plane, nb, g.nz, g.nx,
                                      g.dx, g.dz, ctx->eps + pint, ctx->del + pint,
                                      ctx->th_pad ? ctx->th_pad + pint : nullptr, ctx->P, acc,
                                      acc + (long long)B * nint, nmulti == 3 ? acc + 2LL * B * nint : nullptr, ctx->st);
         };
-        if (!linear_adj) {
-            StepArgs b;
-            base_step_args(ctx, b, nb);
-            b.flags |= F_RECINJ | (grad ? F_GRADM : F_IMAGE);
-            b.data = data + (long long)b0 * nd;
-            b.dstride = nd;
-            b.save_kind = skind;
-            b.save = ctx->save;
-            b.save_kstride = per_step;
-            b.save_sstride = per_step * nt;
-            b.image = ctx->acc_img;
-            b.acc_sstride = nint;
-            for (int t = t0; t >= 1; --t) {
-                r = fwd_step(ctx, b, tile0, cur, prv, t, nullptr);
-                if (r) return r;
-                if (multi) KCK(partials(t, cur));
-                std::swap(cur, prv);
-            }
-        } else {
-            AdjArgs c;
-            std::memset(&c, 0, sizeof(c));
+        StepArgs b;
+        base_step_args(ctx, b, nb);
+        b.flags |= F_RECINJ | (grad ? F_GRADM : F_IMAGE) | (half && grad ? F_D2T : 0);
+        b.data = data + (long long)b0 * nd;
+        b.dstride = nd;
+        b.save_kind = skind;
+        b.save = ctx->save;
+        b.save_kstride = per_step;
+        b.save_sstride = per_step * nbuf;
+        b.save_scale = reinterpret_cast<const float *>(ctx->smax);
+        b.image = ctx->acc_img;
+        b.acc_sstride = nint;
+        AdjArgs c;
+        std::memset(&c, 0, sizeof(c));
+        if (linear_adj) {
             c.nzp = ctx->nzp;
             c.nxp = ctx->nxp;
             c.P = ctx->P;
@@ -904,16 +1044,44 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
             c.dstride = nd;
             c.image = ctx->acc_img;
             c.acc_sstride = nint;
-            c.psstride = per_step * nt;
+            c.psstride = per_step * nbuf;
             c.gradm = grad ? 1 : 0;
             c.pkstride = per_step;
-            for (int t = t0; t >= 1; --t) {
-                c.cur = cur;
-                c.prv = prv;
-                c.k = t;
-                c.P_ = (const float *)ctx->save + (long long)t * per_step;
-                KCK(launch_adj_products(ctx->phys, c, ctx->st));
-                KCK(launch_adj_update(ctx->phys, c, ctx->st));
+        }
+        for (int seg = nseg - 1; seg >= 0; --seg) {
+            int hi = t0, lo = 1;
+            if (ckpt) {
+                lo = std::max(1, seg * C);
+                hi = std::min(t0, (seg + 1) * C - 1);
+                if (lo > hi) continue;
+                // recompute the source field of steps k0 .. hi+1 from checkpoint seg
+                k0 = std::max(0, seg * C - 1);
+                const int k1 = std::min(nt - 1, hi + 1);
+                r = restore_ckpt(ctx, seg, nb);
+                if (r) return r;
+                StepArgs f = a;
+                f.flags = (a.flags & (F_CERJAN | F_SRC)) | F_SAVE;
+                f.save_k0 = k0;
+                float *fc = ctx->aux.U0, *fp = ctx->aux.U1;
+                for (int k = k0; k <= k1; ++k) {
+                    r = fwd_step_fs(ctx, ctx->aux, f, tile0, fc, fp, k, nullptr);
+                    if (r) return r;
+                    std::swap(fc, fp);
+                }
+            }
+            b.save_k0 = k0;
+            for (int t = hi; t >= lo; --t) {
+                if (!linear_adj) {
+                    r = fwd_step(ctx, b, tile0, cur, prv, t, nullptr);
+                    if (r) return r;
+                } else {
+                    c.cur = cur;
+                    c.prv = prv;
+                    c.k = t;
+                    c.P_ = (const float *)ctx->save + (long long)(t - k0) * per_step;
+                    KCK(launch_adj_products(ctx->phys, c, ctx->st));
+                    KCK(launch_adj_update(ctx->phys, c, ctx->st));
+                }
                 if (multi) KCK(partials(t, cur));
                 std::swap(cur, prv);
             }
@@ -926,7 +1094,6 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                     KCK(launch_reduce_shots(ctx->acc_multi + (long long)m * B * nint, nint, nb, (int)nint,
                                             out_multi[m], 1, ctx->st));
     }
-    if (r) return r;
     if (ctx->normalize) {
         KCK(launch_maxabs(ilum_acc, (int)nint, ctx->maxv, ctx->st));
         KCK(launch_normalize(image, ilum_acc, (int)nint, ctx->normalize, ctx->maxv, ctx->st));
@@ -935,6 +1102,11 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                 if (out_multi[m])
                     KCK(launch_normalize(out_multi[m], ilum_acc, (int)nint, ctx->normalize, ctx->maxv, ctx->st));
     }
+    HCK(hipEventRecord(ctx->e1, ctx->st));
+    HCK(hipEventSynchronize(ctx->e1));
+    float ms = 0.f;
+    HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
+    ctx->loop_ms = ms;
     HCK(hipStreamSynchronize(ctx->st));
     return PAM_OK;
 }

@@ -17,6 +17,9 @@ enum : int {
     F_IMAGE = 64,       // image += S_k * input on the interior
     F_GRADM = 128,      // grad += input * (S_{k+1} - 2 S_k + S_{k-1}) / dt^2 on the interior
     F_ADJ_EXACT = 256,  // (reserved)
+    F_SMAX = 512,       // max |new field| over the interior -> smax[shot][k+1] (scale of snapshot k+1)
+    F_D2T = 1024,       // snapshots hold D_k = S_{k+1} - 2 S_k + S_{k-1} (formed in float32 by step k),
+                        // F_SMAX then takes max |D_k| (the lagged scale of D_{k+1}) and F_GRADM reads D_k
 };
 #define MODE_HAS_CERJAN(f) (((f)&F_CERJAN) != 0)
 
@@ -61,7 +64,9 @@ struct StepArgs {
     void *save;                // base of shot 0's snapshot buffer
     long long save_sstride;    // elements between shots
     long long save_kstride;    // elements between steps
-    const float *save_scale;   // [nt] per-snapshot scale (compressed kinds)
+    int save_k0;               // step held in save slot 0 (slot of step k is k - save_k0)
+    const float *save_scale;   // [nshots][nt] max |S_k| over the interior (compressed kinds)
+    unsigned int *smax;        // F_SMAX target, [nshots][nt] float bits (atomicMax)
     const float *save_prev, *save_next;  // (unused; kept for ABI stability)
     // interior accumulators, per shot [nshots][nz][nx]
     float *ilum, *image;
@@ -124,6 +129,10 @@ int launch_cpml_coeffs(const float *vp, int nzp, int nxp, int P, int N, float dx
                        float fpico, float *ax, float *bx, float *az, float *bz, hipStream_t st);
 int launch_inject_point(float *U, long long sstride, const int *ssz, const int *ssx, int P, const float *wav,
                         int k, int nshots, hipStream_t st);
+// max |S_0| over the interior of each shot (the injected source sample) into smax[s * nt]
+// (bound > 0: smax[s * nt] = bound instead -- the scale of D_0)
+int launch_smax0(unsigned int *smax, int nt, const int *ssz, const int *ssx, int N, int nzp, int nxp,
+                 const float *wav, int nshots, float bound, hipStream_t st);
 int launch_inject_rec(float *U, long long sstride, int P, const int *rz, const int *rx, int nrec,
                       const float *data, long long dstride, int row, float dxdz, int nshots, hipStream_t st);
 int launch_pack_padded(const float *U, int nzp, int nxp, int P, float *out, hipStream_t st);

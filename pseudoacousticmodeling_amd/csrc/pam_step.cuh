@@ -106,24 +106,31 @@ __device__ __forceinline__ float sv_load<__hip_bfloat16>(const void *p, long lon
     return __bfloat162float(reinterpret_cast<const __hip_bfloat16 *>(p)[i]) * inv_scale;
 }
 
-// snapshot value of cell (interior index ii, padded index pi) at step k for shot s
+// snapshot value of cell (interior index ii, padded index pi) at step k for shot s.
+// Compressed kinds store S / max|S_k| (max over the interior, save_scale).
 __device__ __forceinline__ float snap_get(const StepArgs &a, int s, int k, long long ii, long long pi) {
-    const long long base = (long long)s * a.save_sstride + (long long)k * a.save_kstride;
+    const long long base = (long long)s * a.save_sstride + (long long)(k - a.save_k0) * a.save_kstride;
     switch (a.save_kind) {
         case SV_F32_INT:
             return sv_load<float>(a.save, base + ii, 1.0f);
         case SV_F32_PAD:
             return sv_load<float>(a.save, base + pi, 1.0f);
         case SV_F16_INT:
-            return sv_load<__half>(a.save, base + ii, 1.0f / a.save_scale[(long long)s * a.nt + k]);
+            return __half2float(reinterpret_cast<const __half *>(a.save)[base + ii]) * a.save_scale[(long long)s * a.nt + k];
         case SV_BF16_INT:
-            return sv_load<__hip_bfloat16>(a.save, base + ii, 1.0f / a.save_scale[(long long)s * a.nt + k]);
+            return __bfloat162float(reinterpret_cast<const __hip_bfloat16 *>(a.save)[base + ii]) *
+                   a.save_scale[(long long)s * a.nt + k];
         default:
             return 0.0f;
     }
 }
 __device__ __forceinline__ void snap_put(const StepArgs &a, int s, int k, long long ii, long long pi, float v) {
-    const long long base = (long long)s * a.save_sstride + (long long)k * a.save_kstride;
+    const long long base = (long long)s * a.save_sstride + (long long)(k - a.save_k0) * a.save_kstride;
+    float inv = 0.0f;
+    if (a.save_kind == SV_F16_INT || a.save_kind == SV_BF16_INT) {
+        const float m = a.save_scale[(long long)s * a.nt + k];
+        inv = (m > 0.0f) ? 1.0f / m : 0.0f;
+    }
     switch (a.save_kind) {
         case SV_F32_INT:
             reinterpret_cast<float *>(a.save)[base + ii] = v;
@@ -132,14 +139,30 @@ __device__ __forceinline__ void snap_put(const StepArgs &a, int s, int k, long l
             reinterpret_cast<float *>(a.save)[base + pi] = v;
             break;
         case SV_F16_INT:
-            reinterpret_cast<__half *>(a.save)[base + ii] = __float2half(v * a.save_scale[(long long)s * a.nt + k]);
+            reinterpret_cast<__half *>(a.save)[base + ii] = __float2half(v * inv);
             break;
         case SV_BF16_INT:
-            reinterpret_cast<__hip_bfloat16 *>(a.save)[base + ii] =
-                __float2bfloat16(v * a.save_scale[(long long)s * a.nt + k]);
+            reinterpret_cast<__hip_bfloat16 *>(a.save)[base + ii] = __float2bfloat16(v * inv);
             break;
         default:
             break;
+    }
+}
+
+// block-wide max of a non-negative float, then one atomicMax on its bits
+// (non-negative floats order like their bit patterns); every thread of the
+// block must call it
+__device__ __forceinline__ void block_max_to(unsigned int *dst, float v) {
+    __shared__ float red[8];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = red[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmaxf(m, red[i]);
+        atomicMax(dst, __float_as_uint(m));
     }
 }
 

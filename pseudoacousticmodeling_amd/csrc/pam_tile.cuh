@@ -127,6 +127,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         xs[j] = CPML && ((c >= 4 && c < N) || (c >= nxp - N && c < nxp - 4));
     }
     const bool tile_xstrip = CPML && (x0 < N || x0 + TX > nxp - N);
+    float tmax = 0.0f;  // F_SMAX: max |new field| over this thread's interior cells
 
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 const int c = col + j;
                 const bool cin = zint && c >= N && c < nxp - N;
                 const long long ii = ib + c;
-                if (flags & F_SAVE) {
+                if ((flags & F_SAVE) && !(flags & F_D2T)) {
                     if (a.save_kind == SV_F32_PAD) {
                         if (xlive[j]) snap_put(a, s, a.k, 0, ro + j, (c0[j] * fx[j]) * fz);
                     } else if (cin) {
@@ -225,10 +226,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                     *im += S * c0[j];
                 }
                 if (flags & F_GRADM) {
-                    const float Sp = snap_get(a, s, a.k + 1, ii, ro + j);
-                    const float S0 = snap_get(a, s, a.k, ii, ro + j);
-                    const float Sm = snap_get(a, s, a.k - 1, ii, ro + j);
-                    const float d2t = (Sp - 2.0f * S0 + Sm) * a.idt2;
+                    const float Sp = (flags & F_D2T) ? 0.0f : snap_get(a, s, a.k + 1, ii, ro + j);
+                    const float S0 = (flags & F_D2T) ? 0.0f : snap_get(a, s, a.k, ii, ro + j);
+                    const float Sm = (flags & F_D2T) ? 0.0f : snap_get(a, s, a.k - 1, ii, ro + j);
+                    const float d2t =
+                            ((flags & F_D2T) ? snap_get(a, s, a.k, ii, ro + j) : (Sp - 2.0f * S0 + Sm)) * a.idt2;
                     float *im = a.image + (long long)s * a.acc_sstride + ii;
                     *im += d2t * c0[j];
                 }
@@ -343,6 +345,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 }
             }
         }
+        if (flags & F_SMAX) {
+            const bool zint = z >= N && z < nzp - N;
+            const long long ib = (long long)(z - N) * a.nx - N;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (!(zint && col + j >= N && col + j < nxp - N)) continue;
+                // interior cells are undamped: D_k = S_{k+1} - 2 S_k + S_{k-1}
+                const float d = out[j] - 2.0f * c0[j] + pv[r][j];
+                if ((flags & F_D2T) && (flags & F_SAVE)) snap_put(a, s, a.k, ib + col + j, 0, d);
+                tmax = fmaxf(tmax, fabsf((flags & F_D2T) ? d : out[j]));
+            }
+        }
         if (xlive[0] && xlive[3]) {
             vst<4>(prv + ro, out);
         } else {
@@ -351,6 +365,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 if (xlive[j]) prv[ro + j] = out[j];
         }
     }
+    if ((flags & F_SMAX) && a.k + 1 < a.nt) block_max_to(a.smax + (long long)s * a.nt + a.k + 1, tmax);
 }
 
 template <int PHYS, bool CPML, int TZ, int WPE>

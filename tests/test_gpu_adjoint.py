@@ -23,8 +23,8 @@ def _data(pb, nshots, seed):
     return np.stack([oracle_forward(orc, pert, shot=s)["seis"] for s in range(nshots)]).astype(np.float32)
 
 
-def _oracle_adjoint(pb, data, fwi, multiparameter=False):
-    orc = Oracle("f32")
+def _oracle_adjoint(pb, data, fwi, multiparameter=False, kind="f32"):
+    orc = Oracle(kind)
     g = pb["g"]
     model = tuple(orc.expand(m, g.N) if m is not None else None for m in (pb["vp"], pb["eps"], pb["dl"], pb["th"]))
     A = orc.cerjan(g.N) if pb["abc"] == 0 else None
