@@ -11,10 +11,12 @@ geometry) are resident on the device before the timed region.
 Prints one JSON line:
   value      Gpt-updates/s = padded cells x time steps x shots / wall time
   roofline   achieved algorithmic HBM GB/s of the dominant kernel (the step
-             kernel variant the autotuner picked),
+             kernel variant the autotuner picked, at the batch size it picked),
              from HIP events around every launch of the timed steps, against
              the 8 TB/s HBM3E peak
   plain_loop the same 64 shots run one shot per launch (the per-shot loop)
+  fwi_gradient seconds per shot of the FWI gradient of the same survey, all
+             64 shots batched vs the per-shot loop
   cpu_baseline the CPU restatement of the reference's Numba kernels
              (float32 storage, float64 arithmetic, OpenMP over rows) on a
              bounded sample of the same workload
@@ -146,6 +148,10 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true", help="skip the per-shot loop measurement")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample length (0: skip)")
     ap.add_argument("--probe", action="store_true", help="run only the device bandwidth probe")
+    ap.add_argument("--no-fwi", action="store_true", help="skip the FWI-gradient legs")
+    ap.add_argument("--batch", type=int, default=0, help="shots advanced per launch (0: all)")
+    ap.add_argument("--fwi-plain-shots", type=int, default=8,
+                    help="shots timed in the per-shot gradient loop (seconds per shot are reported)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -173,6 +179,8 @@ def main(argv=None):
     eng = make_engine(w, S)
     if args.tile >= 0:
         eng.set_option(_lib.PAM_OPT_TILE, args.tile)
+    if args.batch > 0:
+        eng.set_option(_lib.PAM_OPT_BATCH, args.batch)
     seis = torch.empty((S, w["nt"] - w["itlag"], len(w["rx"])), dtype=torch.float32, device="cuda:0")
     for _ in range(args.warmup):
         eng.forward(0, S, out=seis)
@@ -188,13 +196,14 @@ def main(argv=None):
     secs = time.perf_counter() - t0
     finite = bool(torch.isfinite(seis).all().item())
     variant = eng.get_option(_lib.PAM_OPT_TILE)
+    per_launch = eng.get_option(_lib.PAM_OPT_BATCH)
     upd = workloads.updates(w) * args.steps
     value = upd / secs / 1e9
     avg_ms = kern_ms / max(launches, 1)
     fused = variant >= 4  # the k_march variants update psi in the step kernel
-    bytes_launch = step_kernel_bytes(w, S, fused)
+    bytes_launch = step_kernel_bytes(w, per_launch, fused)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    print(f"bench: {secs:.3f} s for {args.steps} runs, step kernel variant {variant} avg {avg_ms * 1e3:.1f} us over {launches} launches, "
+    print(f"bench: {secs:.3f} s for {args.steps} runs, {per_launch} shots per launch, step kernel variant {variant} avg {avg_ms * 1e3:.1f} us over {launches} launches, "
           f"seis finite={finite}", file=sys.stderr, flush=True)
 
     plain = None
@@ -214,6 +223,35 @@ def main(argv=None):
                  "k_step_gbs": step_kernel_bytes(w, 1, eng.get_option(_lib.PAM_OPT_TILE) >= 4) / (pavg * 1e-3) / 1e9,
                  "batched_speedup": ps / (secs / args.steps)}
         print(f"bench: plain per-shot loop {ps:.3f} s", file=sys.stderr, flush=True)
+    fwi = None
+    if not args.no_fwi:
+        # FWI gradient of the same survey: adjoint-state gradient w.r.t. m = 1/vp^2 of all
+        # 64 shots, source field kept by uniform checkpoints in HBM (every step would
+        # need 780 GB); the residual is a recorded gather of the same shape
+        eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_CHECKPOINT)
+        eng.set_option(_lib.PAM_OPT_BATCH, args.batch)
+        res = seis
+        eng.gradient(res)  # allocates checkpoints, tunes
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out_g = eng.gradient(res)
+        torch.cuda.synchronize()
+        gb = time.perf_counter() - t0
+        gfinite = bool(torch.isfinite(out_g["g_m"]).all().item())
+        np_ = max(1, min(args.fwi_plain_shots, S))
+        eng.set_option(_lib.PAM_OPT_BATCH, 1)
+        eng.gradient(res[:1])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.gradient(res[:np_])
+        torch.cuda.synchronize()
+        gp = time.perf_counter() - t0
+        eng.set_option(_lib.PAM_OPT_BATCH, args.batch)
+        fwi = {"unit": "s/shot", "batched": gb / S, "plain": gp / np_, "speedup": (gp / np_) / (gb / S),
+               "shots_batched": S, "shots_plain_loop": np_, "storage": "uniform checkpoints in HBM",
+               "finite": gfinite}
+        print(f"bench: FWI gradient {gb / S * 1e3:.1f} ms/shot batched, {gp / np_ * 1e3:.1f} ms/shot plain",
+              file=sys.stderr, flush=True)
     eng.close()
     del seis
     torch.cuda.empty_cache()
@@ -228,7 +266,7 @@ def main(argv=None):
     tpath = os.path.join(ROOT, "profiles", f"traffic_{w['name']}.json")
     if os.path.exists(tpath) and args.nt_data == 5001:
         rec = json.load(open(tpath))
-        if rec.get("variant") == variant:
+        if rec.get("variant") == variant and rec.get("shots_per_launch") == per_launch:
             traffic = rec["traffic_bytes"]
     out = {
         "metric": METRIC, "value": value, "unit": "Gpt-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -237,11 +275,13 @@ def main(argv=None):
         "config": {"workload": f"{w['name']}: VTI pseudo-acoustic forward modelling + 50-cell CPML, "
                                f"{w['nz'] + 2 * w['N']}x{w['nx'] + 2 * w['N']} padded cells, nt={w['nt']}, "
                                f"{S} shots in one batch, {len(w['rx'])} receivers per shot",
-                   "global_batch": S, "nt": w["nt"], "parallelism": "1 GPU, shots batched per launch"},
+                   "global_batch": S, "nt": w["nt"], "shots_per_launch": per_launch,
+                   "parallelism": "1 GPU, shots batched per launch"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "step kernel variant %d" % variant,
                      "bytes_per_launch": bytes_launch, "avg_launch_us": avg_ms * 1e3},
         "plain_loop": plain,
+        "fwi_gradient": fwi,
         "cpu_baseline": cpu,
         "finite": finite,
     }

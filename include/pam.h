@@ -50,7 +50,9 @@ enum pam_storage {
 
 /* keys of pam_set_option */
 enum pam_option {
-    PAM_OPT_BATCH = 1,     /* shots advanced together in one launch (default: all) */
+    PAM_OPT_BATCH = 1,     /* shots advanced together in one launch; 0 (default): the
+                              autotuner times max_shots, max_shots/2, ... 8 and keeps the
+                              fastest per shot-step (small batches stay in the MALL) */
     PAM_OPT_TILE = 2,      /* step-kernel variant in [0, pam_tile_count()); -1 (default):
                               the autotuner times every variant on the first run of a
                               batch size and keeps the fastest */
@@ -94,8 +96,9 @@ int pam_create(const pam_grid *grid, int physics, int abc, const pam_abc_params 
 int pam_destroy(pam_ctx *ctx);
 const char *pam_last_error(const pam_ctx *ctx);
 int pam_set_option(pam_ctx *ctx, int key, long long value);
-/* Current value of an option; for PAM_OPT_TILE left on auto, the variant the
- * autotuner chose for the last whole run (-1 before the first). */
+/* Current value of an option; for PAM_OPT_TILE / PAM_OPT_BATCH left on auto,
+ * the variant / batch the autotuner chose for the last whole run (-1 / 0
+ * before the first). */
 int pam_get_option(const pam_ctx *ctx, int key, long long *value);
 /* number of kernel variants (tile shapes) the autotuner may choose from */
 int pam_tile_count(void);

@@ -64,7 +64,10 @@ struct pam_ctx {
     int next_k = -1;
     // step-kernel variant chosen by the autotuner, per batch size
     std::map<int, int> tuned;
-    int last_tile = -1;  // variant of the last whole run
+    std::map<int, double> tuned_ms;  // its time per shot-step
+    int auto_batch = 0;              // shots per launch chosen by the autotuner (0: not yet)
+    int last_tile = -1;              // variant of the last whole run
+    int last_batch = 0;              // shots per launch of the last whole run
     // options
     int batch = 0, tile = -1, graph = 1, storage = PAM_STORE_ALL_F32, ckpt_step = 0;
     int reset_adj_cpml = 1, normalize = 1;
@@ -284,6 +287,7 @@ extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
     switch (key) {
         case PAM_OPT_BATCH:
             if (v < 0 || v > ctx->max_shots) return fail(ctx, PAM_E_ARG, "batch must be in [0, max_shots]");
+            ctx->auto_batch = 0;
             ctx->batch = (int)v;
             return PAM_OK;
         case PAM_OPT_TILE:
@@ -316,7 +320,7 @@ extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
 extern "C" int pam_get_option(const pam_ctx *ctx, int key, long long *value) {
     if (!ctx || !value) return PAM_E_ARG;
     switch (key) {
-        case PAM_OPT_BATCH: *value = ctx->batch; return PAM_OK;
+        case PAM_OPT_BATCH: *value = ctx->batch > 0 ? ctx->batch : ctx->last_batch; return PAM_OK;
         case PAM_OPT_TILE: *value = ctx->tile >= 0 ? ctx->tile : ctx->last_tile; return PAM_OK;
         case PAM_OPT_GRAPH: *value = ctx->graph; return PAM_OK;
         case PAM_OPT_STORAGE: *value = ctx->storage; return PAM_OK;
@@ -570,8 +574,9 @@ static int fwd_step(pam_ctx *ctx, StepArgs &a, int tile, float *cur, float *prv,
 // and after, so it must run before a shot is primed.
 static constexpr int kDefaultTile = 5;
 
-static int autotune(pam_ctx *ctx, int nb, int *best) {
+static int autotune(pam_ctx *ctx, int nb, int *best, double *best_per_shot_step) {
     *best = kDefaultTile;
+    *best_per_shot_step = 1e300;
     StepArgs a;
     base_step_args(ctx, a, nb);
     int r = zero_shot_state(ctx, nb, true, true);
@@ -599,6 +604,7 @@ static int autotune(pam_ctx *ctx, int nb, int *best) {
             *best = t;
         }
     }
+    *best_per_shot_step = best_ms / (6.0 * nb);
     return zero_shot_state(ctx, nb, true, true);
 }
 
@@ -610,11 +616,35 @@ static int choose_tile(pam_ctx *ctx, int nshots, bool may_tune) {
     if (it != ctx->tuned.end()) return it->second;
     if (!may_tune) return kDefaultTile;
     int best = kDefaultTile;
-    if (autotune(ctx, nshots, &best) != PAM_OK) {
+    double per = 1e300;
+    if (autotune(ctx, nshots, &best, &per) != PAM_OK) {
         (void)hipGetLastError();
         best = kDefaultTile;
     }
     ctx->tuned[nshots] = best;
+    ctx->tuned_ms[nshots] = per;
+    return best;
+}
+
+// Shots advanced per launch when PAM_OPT_BATCH is 0: the autotuner times the
+// batch sizes max_shots, max_shots/2, ... down to 8 (each with its best
+// kernel variant) and keeps the one with the shortest time per shot-step.
+// Small batches keep the working set in the 256 MB MALL, large ones
+// amortise launches and model reads; the best size depends on the grid.
+static int auto_batch(pam_ctx *ctx) {
+    if (ctx->auto_batch > 0) return ctx->auto_batch;
+    int best = ctx->max_shots;
+    double best_per = 1e300;
+    for (int nb = ctx->max_shots; nb >= 1; nb /= 2) {
+        (void)choose_tile(ctx, nb, true);
+        const double per = ctx->tuned_ms.count(nb) ? ctx->tuned_ms[nb] : 1e300;
+        if (per < best_per) {
+            best_per = per;
+            best = nb;
+        }
+        if (nb <= 8) break;
+    }
+    ctx->auto_batch = best;
     return best;
 }
 
@@ -676,7 +706,7 @@ extern "C" int pam_get_field(pam_ctx *ctx, int which, float *out) {
 
 // ------------------------------------------------------------ whole runs
 static int batch_size(pam_ctx *ctx, int nshots) {
-    int b = ctx->batch > 0 ? ctx->batch : ctx->max_shots;
+    const int b = ctx->batch > 0 ? ctx->batch : auto_batch(ctx);
     return std::max(1, std::min(b, std::min(nshots, ctx->max_shots)));
 }
 
@@ -738,6 +768,7 @@ extern "C" int pam_forward(pam_ctx *ctx, int shot0, int nshots, float *seis) {
     const int B = batch_size(ctx, nshots);
     const int tile = choose_tile(ctx, B, true);
     ctx->last_tile = tile;
+    ctx->last_batch = B;
     for (int b0 = 0; b0 < nshots; b0 += B) {
         const int nb = std::min(B, nshots - b0);
         r = zero_shot_state(ctx, nb, true, true);
@@ -872,6 +903,10 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     const int nbuf = ckpt ? C + 2 : nt;            // snapshots held at once
     const bool multi = grad && (g_eps || g_del || g_th);
     const int nmulti = (ctx->phys == 2) ? 3 : 2;
+    // the gradient needs S only through S_{t+1} - 2 S_t + S_{t-1}: store that
+    // difference (formed by the forward step in float32, the same arithmetic the
+    // adjoint step would do) unless the partials or the linearised adjoint need S
+    const bool d2t = grad && !multi && !linear_adj;
     // choose the batch so that the snapshots (and checkpoints) fit in free memory
     size_t freeb = 0, totalb = 0;
     HCK(hipMemGetInfo(&freeb, &totalb));
@@ -928,6 +963,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     int tile0 = choose_tile(ctx, B, true);
     if (half && tile_shape(tile0).V > 0) tile0 = kDefaultTile;  // k_step has no fused max
     ctx->last_tile = tile0;
+    ctx->last_batch = B;
     const long long pint = (long long)g.n_abc * ctx->P + g.n_abc;  // interior origin in a padded field
     HCK(hipEventRecord(ctx->e0, ctx->st));
     for (int b0 = 0; b0 < nshots; b0 += B) {
@@ -951,7 +987,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         }
         StepArgs a;
         base_step_args(ctx, a, nb);
-        a.flags |= F_SRC | F_ILUM | (ckpt ? 0 : F_SAVE) | (half ? F_SMAX : 0) | (half && grad ? F_D2T : 0);
+        a.flags |= F_SRC | F_ILUM | (ckpt ? 0 : F_SAVE) | (half ? F_SMAX : 0) | (d2t ? F_D2T : 0);
         a.save_kind = skind;
         a.save = ctx->save;
         a.save_k0 = 0;
@@ -997,7 +1033,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         };
         StepArgs b;
         base_step_args(ctx, b, nb);
-        b.flags |= F_RECINJ | (grad ? F_GRADM : F_IMAGE) | (half && grad ? F_D2T : 0);
+        b.flags |= F_RECINJ | (grad ? F_GRADM : F_IMAGE) | (d2t ? F_D2T : 0);
         b.data = data + (long long)b0 * nd;
         b.dstride = nd;
         b.save_kind = skind;
@@ -1060,7 +1096,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                 r = restore_ckpt(ctx, seg, nb);
                 if (r) return r;
                 StepArgs f = a;
-                f.flags = (a.flags & (F_CERJAN | F_SRC)) | F_SAVE;
+                f.flags = (a.flags & (F_CERJAN | F_SRC | F_D2T)) | F_SAVE;
                 f.save_k0 = k0;
                 float *fc = ctx->aux.U0, *fp = ctx->aux.U1;
                 for (int k = k0; k <= k1; ++k) {

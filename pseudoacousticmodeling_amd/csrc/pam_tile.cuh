@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 }
             }
         }
-        if (flags & F_SMAX) {
+        if ((flags & F_SMAX) || ((flags & F_D2T) && (flags & F_SAVE))) {
             const bool zint = z >= N && z < nzp - N;
             const long long ib = (long long)(z - N) * a.nx - N;
 #pragma unroll
