@@ -62,9 +62,12 @@ enum pam_option {
     PAM_OPT_RESET_ADJ_CPML = 6, /* 1 (default): clear CPML memory before the
                                    backward pass; 0: keep the forward pass's
                                    values as src/Migration.py does */
-    PAM_OPT_NORMALIZE = 7  /* image/gradient normalisation: 0 none, 1 ilum+1e-5 max
+    PAM_OPT_NORMALIZE = 7, /* image/gradient normalisation: 0 none, 1 ilum+1e-5 max
                               (src/Migration.py:1086, default), 2 ilum>1e-12
                               (cpp/Migration.cu:470-484) */
+    PAM_OPT_ADJ_RANGE = 8  /* adjoint steps: 0 (default) t = nt-2 .. 1 as src/Migration.py:1039;
+                              1 t = nt-1 .. 0 as cpp/Migration.cu:383-390 (imaging only: the
+                              gradient needs S_{t+1} and S_{t-1}) */
 };
 
 typedef struct pam_grid {

@@ -64,8 +64,28 @@ def build_lib(verbose: bool = True) -> str:
     return LIB
 
 
+def build_cli(verbose: bool = True) -> list:
+    """The C++ drivers run_modeling / run_migration (csrc/cli), linked against
+    libpam.so next to them (rpath $ORIGIN/..)."""
+    out = []
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    for name in ("run_modeling", "run_migration"):
+        exe = os.path.join(bindir, name)
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+               os.path.join(CSRC, "cli", name + ".cpp"), "-o", exe + ".tmp", f"-L{HERE}", "-lpam",
+               "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
+
+
 def main(argv=None) -> int:
     build_lib()
+    build_cli()
     return 0
 
 

@@ -70,7 +70,7 @@ struct pam_ctx {
     int last_batch = 0;              // shots per launch of the last whole run
     // options
     int batch = 0, tile = -1, graph = 1, storage = PAM_STORE_ALL_F32, ckpt_step = 0;
-    int reset_adj_cpml = 1, normalize = 1;
+    int reset_adj_cpml = 1, normalize = 1, adj_range = 0;
     // checkpoints of the main set ([segment][field][shot][plane]) and the
     // psi_flip at each
     float *ckpt = nullptr;
@@ -312,6 +312,10 @@ extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
             if (v < 0 || v > 2) return fail(ctx, PAM_E_ARG, "normalize must be 0, 1 or 2");
             ctx->normalize = (int)v;
             return PAM_OK;
+        case PAM_OPT_ADJ_RANGE:
+            if (v < 0 || v > 1) return fail(ctx, PAM_E_ARG, "adjoint range must be 0 or 1");
+            ctx->adj_range = (int)v;
+            return PAM_OK;
         default:
             return fail(ctx, PAM_E_ARG, "unknown option %d", key);
     }
@@ -327,6 +331,7 @@ extern "C" int pam_get_option(const pam_ctx *ctx, int key, long long *value) {
         case PAM_OPT_CKPT_STEP: *value = ctx->ckpt_step; return PAM_OK;
         case PAM_OPT_RESET_ADJ_CPML: *value = ctx->reset_adj_cpml; return PAM_OK;
         case PAM_OPT_NORMALIZE: *value = ctx->normalize; return PAM_OK;
+        case PAM_OPT_ADJ_RANGE: *value = ctx->adj_range; return PAM_OK;
         default: return PAM_E_ARG;
     }
 }
@@ -899,7 +904,11 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     const int elem = half ? 2 : 4;
     const long long per_step = (skind == SV_F32_PAD) ? ctx->plane : nint;  // elements per snapshot
     const int C = ckpt ? (ctx->ckpt_step > 0 ? ctx->ckpt_step : (int)std::ceil(std::sqrt((double)nt))) : nt;
-    const int nseg = ckpt ? (nt - 2) / C + 1 : 1;  // segments of backward steps t in [1, nt-2]
+    // backward steps t in [t_lo, t_hi]: [1, nt-2] (Python reference) or [0, nt-1] (C++)
+    if (grad && ctx->adj_range == 1)
+        return fail(ctx, PAM_E_ARG, "the FWI gradient needs the adjoint range nt-2 .. 1 (PAM_OPT_ADJ_RANGE 0)");
+    const int t_hi = ctx->adj_range ? nt - 1 : nt - 2, t_lo = ctx->adj_range ? 0 : 1;
+    const int nseg = ckpt ? t_hi / C + 1 : 1;  // segments of backward steps
     const int nbuf = ckpt ? C + 2 : nt;            // snapshots held at once
     const bool multi = grad && (g_eps || g_del || g_th);
     const int nmulti = (ctx->phys == 2) ? 3 : 2;
@@ -1015,7 +1024,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         // ---- backward pass: the adjoint field reuses the main set
         r = zero_shot_state(ctx, nb, true, ctx->reset_adj_cpml != 0);
         if (r) return r;
-        const int t0 = nt - 2;
+        const int t0 = t_hi;
         if (t0 >= g.itlag)
             KCK(launch_inject_rec(ctx->fs.U0, ctx->plane, ctx->P, ctx->d_rz, ctx->d_rx, ctx->nrec,
                                   data + (long long)b0 * nd, nd, t0 - g.itlag, g.dx * g.dz, nb, ctx->st));
@@ -1085,9 +1094,9 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
             c.pkstride = per_step;
         }
         for (int seg = nseg - 1; seg >= 0; --seg) {
-            int hi = t0, lo = 1;
+            int hi = t0, lo = t_lo;
             if (ckpt) {
-                lo = std::max(1, seg * C);
+                lo = std::max(t_lo, seg * C);
                 hi = std::min(t0, (seg + 1) * C - 1);
                 if (lo > hi) continue;
                 // recompute the source field of steps k0 .. hi+1 from checkpoint seg

@@ -1,0 +1,133 @@
+"""The RTM / FWI-gradient driver: class ``migration`` of src/Migration.py with
+the same method names and outputs, running on the MI355X through libpam.
+
+``SolveBackwardWaveEquation`` reads the recorded (or residual) gathers with the
+reference's file names, smooths the model and mutes the data as the reference
+does for RTM (src/Migration.py:963-1030), and writes
+
+* ``migrated_image_{approx}_Nx{nx}_Nz{nz}.bin`` (RTM, imageFolder), or
+* ``gradient_{approx}_Nx{nx}_Nz{nz}.bin`` and, for multiparameter FWI,
+  ``epsilon_`` / ``delta_`` / ``theta_gradient_...`` (gradientsFolder),
+
+each raw float32 [nz][nx], normalised by ilum + 1e-5 max(ilum)
+(src/Migration.py:1086-1110).
+
+Source-field strategies (Parameters.json "migration"): "onthefly" keeps every
+step in HBM (float32, or float16 / bfloat16 with "storage"), "checkpoint"
+keeps uniform checkpoints every "step" steps in HBM and recomputes segments.
+"SB" (saving boundaries) and "RBC" (random boundaries) are not provided.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _lib, prep
+
+STORAGE = {"float32": _lib.PAM_STORE_ALL_F32, "float16": _lib.PAM_STORE_ALL_F16,
+           "bfloat16": _lib.PAM_STORE_ALL_BF16}
+
+
+class migration:
+    def __init__(self, wavefield, parameters):
+        self.pmt = parameters
+        self.wf = wavefield
+
+    def initializeMigrationfields(self):
+        p = self.pmt
+        self.migrated_image = np.zeros((p.nz, p.nx), dtype=np.float32)
+        self.ilum = np.zeros((p.nz, p.nx), dtype=np.float32)
+        if p.fwi and p.multiparameter and p.approximation in ("VTI", "TTI"):
+            self.epsilon_grad = np.zeros((p.nz, p.nx), dtype=np.float32)
+            self.delta_grad = np.zeros((p.nz, p.nx), dtype=np.float32)
+            if p.approximation == "TTI":
+                self.theta_grad = np.zeros((p.nz, p.nx), dtype=np.float32)
+        print(f"info: Wavefields initialized: {p.nx}x{p.nz}x{p.nt}")
+
+    def loadSeismogram(self, shot):
+        p = self.pmt
+        if p.fwi:
+            f = f"{p.seismogramFolder}residual_shot_{shot + 1}_Nt{p.nt_data}_Nrec{p.Nrec}.bin"
+        else:
+            f = f"{p.seismogramFolder}seismogram_shot_{shot + 1}_Nt{p.nt_data}_Nrec{p.Nrec}_fcut{p.fcut}.bin"
+        return np.fromfile(f, dtype=np.float32).reshape(p.nt_data, p.Nrec)
+
+    def _models(self):
+        """Migration models: smoothed slowness (and parameters) for RTM
+        (src/Migration.py:967-992), the current models for FWI."""
+        p = self.pmt
+        wf = self.wf
+        if p.fwi:
+            vp = getattr(self, "vp", wf.vp)
+            eps = getattr(self, "epsilon", getattr(wf, "epsilon", None))
+            dl = getattr(self, "delta", getattr(wf, "delta", None))
+            th = getattr(self, "theta", getattr(wf, "theta", None))
+            return vp, eps, dl, th
+        water = np.abs(wf.vp - np.min(wf.vp)) < 1e-3
+        self.vp = prep.smooth_model(wf.vp, p.sigma, water)
+        eps = dl = th = None
+        if p.approximation in ("VTI", "TTI"):
+            eps = prep.smooth_model(wf.epsilon, p.sigma, np.abs(wf.epsilon - np.min(wf.epsilon)) < 1e-3, True)
+            dl = prep.smooth_model(wf.delta, p.sigma, np.abs(wf.delta - np.min(wf.delta)) < 1e-3, True)
+        if p.approximation == "TTI":
+            th = prep.smooth_model(wf.theta, p.sigma, np.abs(wf.theta - np.min(wf.theta)) < 1e-3, True)
+        self.epsilon, self.delta, self.theta = eps, dl, th
+        return self.vp, eps, dl, th
+
+    def _configure(self, eng):
+        p = self.pmt
+        if p.migration == "onthefly":
+            eng.set_option(_lib.PAM_OPT_STORAGE, STORAGE[getattr(p, "storage", "float32")])
+        elif p.migration == "checkpoint":
+            eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_CHECKPOINT)
+            eng.set_option(_lib.PAM_OPT_CKPT_STEP, int(p.step) if p.step else 0)
+        else:
+            raise NotImplementedError(f"migration strategy {p.migration!r} is not provided "
+                                      "(use 'onthefly' or 'checkpoint')")
+
+    def data(self):
+        """All shots' adjoint sources on the host: the muted gathers (RTM) or
+        the residuals (FWI), [Nshot][nt_data][Nrec]."""
+        p = self.pmt
+        out = np.zeros((p.Nshot, p.nt_data, p.Nrec), dtype=np.float32)
+        for s in range(p.Nshot):
+            d = self.loadSeismogram(s)
+            if not p.fwi:
+                d = prep.mute(d, p.rec_x, p.rec_z, p.shot_x[s], p.shot_z[s], p.dt, p.shift, p.window, p.v0)
+            out[s] = d
+        return out
+
+    def SolveBackwardWaveEquation(self, data=None):
+        """RTM image (fwi false) or FWI gradient (fwi true) of every shot.
+        ``data`` (host or device [Nshot][nt_data][Nrec]) replaces the file
+        reads when given."""
+        start = time.time()
+        p = self.pmt
+        vp, eps, dl, th = self._models()
+        eng = self.wf.upload_models(vp, eps, dl, th)
+        self._configure(eng)
+        d = self.data() if data is None else data
+        if p.fwi:
+            out = eng.gradient(d, multiparameter=bool(p.multiparameter and p.approximation in ("VTI", "TTI")))
+            self.migrated_image = out["g_m"].cpu().numpy()
+            self.ilum = out["ilum"].cpu().numpy()
+            f = f"{p.gradientsFolder}gradient_{p.approximation}_Nx{p.nx}_Nz{p.nz}.bin"
+            if out["g_eps"] is not None:
+                self.epsilon_grad = out["g_eps"].cpu().numpy()
+                self.delta_grad = out["g_delta"].cpu().numpy()
+                self.epsilon_grad.tofile(f"{p.gradientsFolder}epsilon_gradient_{p.approximation}_Nx{p.nx}_Nz{p.nz}.bin")
+                self.delta_grad.tofile(f"{p.gradientsFolder}delta_gradient_{p.approximation}_Nx{p.nx}_Nz{p.nz}.bin")
+            if out["g_theta"] is not None:
+                self.theta_grad = out["g_theta"].cpu().numpy()
+                self.theta_grad.tofile(f"{p.gradientsFolder}theta_gradient_{p.approximation}_Nx{p.nx}_Nz{p.nz}.bin")
+        else:
+            img, ilum = eng.rtm(d)
+            self.migrated_image = img.cpu().numpy()
+            self.ilum = ilum.cpu().numpy()
+            f = f"{p.imageFolder}migrated_image_{p.approximation}_Nx{p.nx}_Nz{p.nz}.bin"
+        self.outputFile = f
+        self.migrated_image.astype(np.float32).tofile(f)
+        print(f"info: Final image saved to {f}")
+        print(f"info: Migration solved in {time.time() - start:.2f} seconds")
+        return self.migrated_image

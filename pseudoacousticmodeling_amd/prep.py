@@ -114,3 +114,70 @@ def stability_acoustic(vp, fcut, dx, dz, dt):
     dt_lim = dx_lim / (4.0 * float(np.max(vp)))
     return dict(dx_lim=dx_lim, dz_lim=dx_lim, dt_lim=dt_lim,
                 ok=bool(dx <= dx_lim and dz <= dx_lim and dt <= dt_lim))
+
+
+def expand_model(m, N):
+    """Edge-replicating pad by N cells on every side, corners taking the corner
+    value: src/Modeling2D.py:38-54 / cpp/Modeling.cu:130-182 (the device path
+    pads on the GPU; this host copy serves drivers that write padded files)."""
+    return np.pad(np.asarray(m, dtype=np.float32), N, mode="edge")
+
+
+def mute(seis, rec_x, rec_z, shot_x, shot_z, dt, shift, window, v0):
+    """Linear top mute of one shot gather [nt][nrec]: zero before
+    t1 = dist/v0 + shift, a linear ramp to t2 = t1 + window, unchanged after
+    (src/utils.py:30-49, cpp/Migration.cu:86-115)."""
+    seis = np.asarray(seis, dtype=np.float32)
+    nt, nrec = seis.shape
+    dist = np.sqrt((np.asarray(rec_z, dtype=np.float64) - shot_z) ** 2 +
+                   (np.asarray(rec_x, dtype=np.float64) - shot_x) ** 2)
+    t1 = dist / v0 + shift
+    t2 = t1 + window
+    t = (np.arange(nt) * dt)[:, None]
+    ramp = (t >= t1[None, :]) & (t < t2[None, :])
+    w = np.where(t >= t2[None, :], 1.0, np.where(ramp, (t - t1[None, :]) / (t2 - t1)[None, :], 0.0))
+    return (w * seis).astype(np.float32)
+
+
+def gaussian_filter2d(sigma):
+    """Normalised 2-D Gaussian kernel of size ceil(6 sigma + 1) made odd
+    (src/utils.py:51-74)."""
+    ks = int(np.ceil(6.0 * sigma + 1))
+    if ks % 2 == 0:
+        ks += 1
+    h = ks // 2
+    x = (np.arange(ks) - h).astype(np.float64)
+    val = (1.0 / (2.0 * np.pi * sigma * sigma)) * np.exp(-(x[:, None] ** 2 + x[None, :] ** 2) / (2.0 * sigma * sigma))
+    return (val.astype(np.float32) / np.float32(val.sum())).astype(np.float32)
+
+
+def smooth_model(f, sigma, water_mask, parameter=False):
+    """Gaussian smoothing of slowness (or of a parameter when ``parameter``)
+    over the cells a full kernel fits, edge replication of the border rows and
+    columns, and restoration of the water column from the top of each column
+    down to its first non-water cell (src/utils.py:76-129)."""
+    f = np.asarray(f, dtype=np.float32)
+    s = f.copy() if parameter else (np.float32(1.0) / f)
+    s_old = s.copy()
+    k = gaussian_filter2d(sigma)
+    ks = k.shape[0]
+    h = ks // 2
+    nz, nx = s.shape
+    if nz > 2 * h and nx > 2 * h:
+        from numpy.lib.stride_tricks import sliding_window_view
+
+        win = sliding_window_view(s_old, (ks, ks))
+        acc = np.einsum("abij,ij->ab", win.astype(np.float64), k.astype(np.float64))
+        s[h:nz - h, h:nx - h] = (acc / float(k.astype(np.float64).sum())).astype(np.float32)
+    for z in range(h):
+        s[z, :] = s[h, :]
+        s[nz - 1 - z, :] = s[nz - 1 - h, :]
+    for x in range(h):
+        s[:, x] = s[:, h]
+        s[:, nx - 1 - x] = s[:, nx - 1 - h]
+    wm = np.asarray(water_mask, dtype=bool)
+    # the water column: rows above the first non-water cell of each column
+    first_dry = np.where(wm.all(axis=0), nz, np.argmin(wm, axis=0))
+    rows = np.arange(nz)[:, None] < first_dry[None, :]
+    s[rows] = s_old[rows]
+    return s if parameter else (np.float32(1.0) / s).astype(np.float32)
