@@ -222,6 +222,67 @@ def smooth_model_py(f, sigma, water_mask, parameter=False):
     return s if parameter else (np.float32(1.0) / s)
 
 
+def mute_cpp(seis, rec_x, rec_z, sx, sz, dt, shift, window, v0):
+    """Linear top mute of the C++ driver, cpp/Migration.cu:86-115: the same
+    rule as ``mute`` with every quantity a float32 (sqrtf of the float
+    offsets, t = it * dt in float32, the ramp factor multiplied in place)."""
+    f32 = np.float32
+    out = np.array(seis, dtype=np.float32, copy=True)
+    nt, nrec = out.shape
+    for r in range(nrec):
+        ddz = f32(rec_z[r]) - f32(sz)
+        ddx = f32(rec_x[r]) - f32(sx)
+        dist = np.sqrt(f32(ddz * ddz + ddx * ddx), dtype=np.float32)
+        t1 = f32(dist / f32(v0)) + f32(shift)
+        t2 = f32(t1 + f32(window))
+        for it in range(nt):
+            t = f32(it) * f32(dt)
+            if t < t1:
+                out[it, r] = 0.0
+            elif t < t2:
+                out[it, r] = out[it, r] * f32((t - t1) / (t2 - t1))
+    return out
+
+
+def gaussian_kernel1d_cpp(sigma):
+    """Normalised 1-D Gaussian of size ceil(6 sigma + 1) made odd, float32
+    with a sequential float32 sum: cpp/Migration.cu:147-169."""
+    f32 = np.float32
+    ks = int(np.ceil(f32(6.0) * f32(sigma) + f32(1.0)))
+    if ks % 2 == 0:
+        ks += 1
+    h = ks // 2
+    k = np.zeros(ks, dtype=np.float32)
+    total = f32(0.0)
+    for i in range(ks):
+        x = i - h
+        k[i] = np.exp(f32(-(x * x)) / (f32(2.0) * f32(sigma) * f32(sigma)), dtype=np.float32)
+        total = f32(total + k[i])
+    return (k / total).astype(np.float32)
+
+
+def smooth_model_cpp(f, sigma, mask, parameter=False):
+    """Separable Gaussian smoothing of the C++ driver, cpp/Migration.cu:171-236:
+    slowness (or the parameter) filtered along x then along z with clamped
+    (edge-replicated) indices, every masked cell restored, slowness turned back
+    into velocity.  float32 with float32 accumulation in tap order."""
+    f = np.asarray(f, dtype=np.float32)
+    old = f.copy() if parameter else (np.float32(1.0) / f).astype(np.float32)
+    k = gaussian_kernel1d_cpp(sigma)
+    h = k.shape[0] // 2
+    nz, nx = old.shape
+    xi = np.arange(nx)
+    zi = np.arange(nz)
+    tmp = np.zeros_like(old)
+    for i in range(-h, h + 1):
+        tmp = (tmp + k[i + h] * old[:, np.clip(xi + i, 0, nx - 1)]).astype(np.float32)
+    out = np.zeros_like(old)
+    for i in range(-h, h + 1):
+        out = (out + k[i + h] * tmp[np.clip(zi + i, 0, nz - 1), :]).astype(np.float32)
+    out = np.where(np.asarray(mask, dtype=bool), old, out).astype(np.float32)
+    return out if parameter else (np.float32(1.0) / out).astype(np.float32)
+
+
 def stability_acoustic(vp, fcut, dx, dz, dt):
     """src/Modeling2D.py:202-223 / cpp/Modeling.cu:287-292: lambda_min = vmin/fcut,
     dx_lim = lambda_min/4.28, dt_lim = dx_lim/(4 vmax)."""
@@ -377,7 +438,8 @@ def rel_l2(a, b):
 
 
 def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, data, A=None, cpml=None,
-                 fwi=False, multiparameter=False, model_interior=None, reset_cpml=True):
+                 fwi=False, multiparameter=False, model_interior=None, reset_cpml=True, t_range="py",
+                 normalize=1):
     """On-the-fly RTM / FWI gradient of src/Migration.py:963-1111 (GPU path):
     per shot, a forward pass storing the current field after every step and
     accumulating ilum += S^2 over the interior, then a backward pass
@@ -387,7 +449,16 @@ def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, dat
     (ilum + 1e-5 max(ilum)).  ``data[s]`` is the (already muted) data or the
     residual for shot s.  With ``reset_cpml`` the CPML memory variables are
     cleared before the backward pass (the reference keeps the forward pass's
-    values, src/Migration.py:1000-1039 only resets per shot)."""
+    values, src/Migration.py:1000-1039 only resets per shot).
+
+    ``t_range="cpp"`` and ``normalize=2`` give the C++ driver's variant
+    (cpp/Migration.cu:383-394, 470-483): the backward pass runs t = nt-1..0
+    (imaging only) and the image is divided by ilum where ilum > 1e-12,
+    zero elsewhere."""
+    if t_range not in ("py", "cpp") or normalize not in (1, 2):
+        raise ValueError("t_range 'py'|'cpp', normalize 1|2")
+    if t_range == "cpp" and fwi:
+        raise ValueError("the C++ backward range applies to imaging only")
     N = g.N
     vp, eps, dl, th = [orc.arr(m) if m is not None else None for m in model]
     image = np.zeros((g.nz, g.nx), dtype=np.float64)
@@ -413,7 +484,8 @@ def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, dat
         Lf = orc.zeros(g.nzp, g.nxp)
         img_p = np.zeros((g.nz, g.nx), dtype=orc.dtype)
         dxdz = g.dx * g.dz
-        for t in range(g.nt - 2, 0, -1):
+        t_hi, t_lo = (g.nt - 2, 1) if t_range == "py" else (g.nt - 1, 0)
+        for t in range(t_hi, t_lo - 1, -1):
             row = data[s][t - g.itlag] if t >= g.itlag else None
             orc.backward_step(phys, abc, Lf, Lc, g, (vp, eps, dl, th), A, cp, S[t], rz, rx, row, dxdz)
             adj = Lc[sl].copy()
@@ -432,7 +504,10 @@ def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, dat
             Lc, Lf = Lf, Lc
         image += img_p
         ilum += ilum_p
-    norm = ilum + 1e-5 * np.max(ilum)
+    if normalize == 1:
+        norm = ilum + 1e-5 * np.max(ilum)
+    else:
+        norm = np.where(ilum > 1e-12, ilum, np.inf)
     out = dict(image=image / norm, raw_image=image, ilum=ilum)
     if multiparameter:
         out.update(g_eps=ge / norm, g_delta=gd / norm, g_theta=gt / norm)

@@ -132,10 +132,11 @@ struct Survey {
     int N_abc = 0, nx = 0, nz = 0, nt = 0, itlag = 0, nt_data = 0, nx_abc = 0, nz_abc = 0;
     float sigma = 0, shift = 0, window = 0, v0 = 1500;
     bool reciprocity = false, mirror = false;
-    int idx_water = 0, step = 0;
+    int idx_water = 0, step = 0, last_save = 0;
+    bool snap = false;
     float cerjan_factor = 0;
     std::string rec_file, src_file, vpFile, epsilonFile, deltaFile, thetaFile;
-    std::string seismogramFolder, imageFolder;
+    std::string seismogramFolder, imageFolder, snapshotFolder;
     std::vector<float> rec_x, rec_z, shot_x, shot_z;
     std::vector<int> rx, rz, sx, sz;
     int Nrec = 0, Nshot = 0;
@@ -178,6 +179,8 @@ struct Survey {
         window = num("window", 0.f);
         v0 = num("v0", 1500.f);
         step = (int)num("step", 0.f);
+        last_save = (int)num("last_save", 0.f);
+        snap = flag("snap");
         idx_water = (int)num("idx_water", 0.f);
         cerjan_factor = num("cerjan_factor", 0.f);
         reciprocity = flag("reciprocity");
@@ -190,6 +193,7 @@ struct Survey {
         thetaFile = str("thetaFile");
         seismogramFolder = str("seismogramFolder");
         imageFolder = str("imageFolder");
+        snapshotFolder = str("snapshotFolder");
         // derived sizes, cpp/Survey.cpp:101-108
         tlag = 2.0f * std::sqrt(kPi) / fcut;
         itlag = cround(tlag / dt);

@@ -4,11 +4,49 @@
 // (default ../inputs/Parameters.json, the reference's fixed path);
 // `run_modeling <Parameters.json> --check` prints the parsed survey only.
 // Writes seismogram_shot_{s+1}_Nt{nt_data}_Nrec{Nrec}_fcut{fcut:.1f}.bin,
-// raw float32 [nt_data][Nrec], into seismogramFolder.
+// raw float32 [nt_data][Nrec], into seismogramFolder.  With "snap": true it
+// also writes the interior [nz][nx] of the field after step k for every k
+// that is a multiple of "step" up to "last_save", as
+// {approx}forward_shot_{s+1}Nx{nx}_Nz{nz}_Nt{nt}_frame{k}.bin in
+// snapshotFolder; as in cpp/Modeling.cu:433-470 the last one taken is
+// written under the name frame{last_save}.
 #include <chrono>
 #include <iostream>
 
 #include "pam_cli.h"
+
+// Snapshots through the per-step interface: one shot at a time in slot 0.
+static void write_snapshots(const cli::Survey &s, pam_ctx *ctx) {
+    if (s.step <= 0) throw std::invalid_argument("snap needs step > 0");
+    const size_t np = (size_t)s.nz_abc * s.nx_abc;
+    float *d_field = nullptr;
+    CLI_HIP(hipMalloc(&d_field, np * sizeof(float)));
+    std::vector<float> pad(np), frame((size_t)s.nz * s.nx);
+    auto name = [&](int shot, int k) {
+        return s.snapshotFolder + s.approximation + "forward_shot_" + std::to_string(shot + 1) + "Nx" +
+               std::to_string(s.nx) + "_Nz" + std::to_string(s.nz) + "_Nt" + std::to_string(s.nt) + "_frame" +
+               std::to_string(k) + ".bin";
+    };
+    for (int shot = 0; shot < s.Nshot; ++shot) {
+        CLI_PAM(ctx, pam_reset(ctx, shot));
+        int held = -1;  // the frame taken last and not yet written
+        for (int k = 0; k < s.nt; ++k) {
+            CLI_PAM(ctx, pam_forward_step(ctx, k, nullptr));
+            if (k <= s.last_save && k % s.step == 0) {
+                if (held >= 0) cli::write_bin(name(shot, held), frame.data(), frame.size());
+                CLI_PAM(ctx, pam_get_field(ctx, 1, d_field));
+                CLI_HIP(hipMemcpy(pad.data(), d_field, np * sizeof(float), hipMemcpyDeviceToHost));
+                for (int z = 0; z < s.nz; ++z)
+                    std::copy(pad.begin() + (size_t)(z + s.N_abc) * s.nx_abc + s.N_abc,
+                              pad.begin() + (size_t)(z + s.N_abc) * s.nx_abc + s.N_abc + s.nx,
+                              frame.begin() + (size_t)z * s.nx);
+                held = k;
+            }
+        }
+        if (held >= 0) cli::write_bin(name(shot, s.last_save), frame.data(), frame.size());
+    }
+    (void)hipFree(d_field);
+}
 
 int main(int argc, char **argv) {
     const auto t0 = std::chrono::system_clock::now();
@@ -52,6 +90,7 @@ int main(int argc, char **argv) {
                                ".bin",
                            h.data(), nd);
         }
+        if (s.snap) write_snapshots(s, ctx);
         double loop_ms = 0, kern_ms = 0;
         long long launches = 0;
         pam_last_timing(ctx, &loop_ms, &kern_ms, &launches);
