@@ -159,14 +159,18 @@ int pam_gradient(pam_ctx *ctx, int shot0, int nshots, const float *residual, flo
 int pam_misfit(pam_ctx *ctx, int nshots, const float *dobs, const float *dcal, float *residual,
                double *misfit);
 
-/* Exact discrete adjoint of pam_forward (acoustic physics): given
- * dseis = dJ/dseis [nshots][nt_data][nrec] (device), accumulates
- * dJ/dvp into g_vp (device [nz][nx] interior, or padded [nzp][nxp] when
- * layout = PAM_PADDED).  This is the gradient a torch.autograd.Function
- * returns; it has no counterpart in the reference, whose gradient
- * (pam_gradient) is preconditioned and normalised. */
-int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const float *dseis, float *g_vp,
-                        int layout);
+/* Exact discrete adjoint of pam_forward (acoustic physics, Cerjan or CPML):
+ * given dseis = dJ/dseis [nshots][nt_data][nrec] (device) for the seismograms
+ * pam_forward produces for shots shot0 .. shot0+nshots-1, writes dJ/dvp into
+ * g_vp (device [nz][nx] when layout = PAM_INTERIOR -- the transpose of the
+ * edge-replicating expansion applied --, or [nzp][nxp] when PAM_PADDED).  It
+ * is the transpose of the forward map's Jacobian, including the dependence of
+ * the CPML profiles on vp: the backward a torch.autograd.Function needs.  The
+ * forward states are kept per step in float32 (PAM_STORE_ALL_F32) or as
+ * uniform checkpoints recomputed per segment (PAM_STORE_CHECKPOINT, interval
+ * PAM_OPT_CKPT_STEP or ceil(sqrt(nt))).  No counterpart in the reference,
+ * whose gradient (pam_gradient) is preconditioned and normalised. */
+int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const float *dseis, float *g_vp, int layout);
 
 /* Elapsed GPU milliseconds of the last whole-run call's step loop, and the
  * average duration of its dominant stencil kernel (both from HIP events). */

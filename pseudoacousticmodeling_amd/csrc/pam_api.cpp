@@ -92,6 +92,9 @@ struct pam_ctx {
     float *ilum_tmp = nullptr;
     long long ilum_tmp_bytes = 0;
     double *d_acc = nullptr;
+    // exact-adjoint scratch: saved forward states, adjoint work fields, gradient
+    float *ex_save = nullptr, *ex_work = nullptr, *ex_gacc = nullptr;
+    long long ex_save_bytes = 0, ex_work_bytes = 0, ex_gacc_bytes = 0;
     // timing
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::vector<hipEvent_t> kev;
@@ -265,7 +268,8 @@ extern "C" int pam_destroy(pam_ctx *ctx) {
                      ctx->cax, ctx->cbx,  ctx->caz,   ctx->cbz,   ctx->vp_pad, ctx->wav, ctx->fs.U0,
                      ctx->fs.U1,  ctx->fs.psix, ctx->fs.psiz,  ctx->fs.zetax, ctx->fs.zetaz, ctx->tmp, ctx->save_scale,
                      ctx->aux.U0, ctx->aux.U1, ctx->aux.psix, ctx->aux.psiz, ctx->aux.zetax, ctx->aux.zetaz, ctx->ckpt,
-                     ctx->acc_ilum, ctx->maxv, ctx->ilum_tmp, ctx->th_pad, ctx->acc_multi};
+                     ctx->acc_ilum, ctx->maxv, ctx->ilum_tmp, ctx->th_pad, ctx->acc_multi,
+                     ctx->ex_save, ctx->ex_work, ctx->ex_gacc};
     for (float *b : bufs) dfree(b);
     dfree(ctx->save);
     dfree(ctx->d_acc);
@@ -1182,12 +1186,208 @@ extern "C" int pam_misfit(pam_ctx *ctx, int nshots, const float *dobs, const flo
     return PAM_OK;
 }
 
+// ------------------------------------------------------------ exact adjoint
+// Copy the state of step k held by field set f (C_k = cur, and for CPML the
+// psi and zeta the step just wrote) into a snapshot slot: fields `blk` floats
+// apart, shots one plane apart.
+static int exact_copy_state(pam_ctx *ctx, const FieldSet &f, const float *cur, float *slot, int nb, long long blk) {
+    const size_t fb = (size_t)ctx->plane * nb * sizeof(float);
+    HCK(hipMemcpyAsync(slot, cur, fb, hipMemcpyDeviceToDevice, ctx->st));
+    if (ctx->abc == PAM_CPML) {
+        const long long newest = f.psi_flip ? ctx->psi_half : 0;
+        HCK(hipMemcpyAsync(slot + blk, f.psix + newest, fb, hipMemcpyDeviceToDevice, ctx->st));
+        HCK(hipMemcpyAsync(slot + 2 * blk, f.psiz + newest, fb, hipMemcpyDeviceToDevice, ctx->st));
+        HCK(hipMemcpyAsync(slot + 3 * blk, f.zetax, fb, hipMemcpyDeviceToDevice, ctx->st));
+        HCK(hipMemcpyAsync(slot + 4 * blk, f.zetaz, fb, hipMemcpyDeviceToDevice, ctx->st));
+    }
+    return PAM_OK;
+}
+
+// Backward of pam_forward (acoustic physics): a forward pass that keeps the
+// state of every step in HBM (or uniform checkpoints of the whole state, the
+// segments recomputed on the second field set, with PAM_STORE_CHECKPOINT),
+// then the reverse sweep of pam_exact.hip, shots in batches.
 extern "C" int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const float *dseis, float *g_vp, int layout) {
-    (void)shot0;
-    (void)nshots;
-    (void)dseis;
-    (void)g_vp;
-    (void)layout;
     if (!ctx) return PAM_E_ARG;
-    return fail(ctx, PAM_E_UNSUPPORTED, "exact adjoint not available yet");
+    int r = ready(ctx);
+    if (r) return r;
+    if (ctx->phys != PAM_ACOUSTIC) return fail(ctx, PAM_E_UNSUPPORTED, "the exact adjoint covers acoustic physics");
+    if (nshots < 1) return fail(ctx, PAM_E_ARG, "nshots must be >= 1");
+    if (!dseis || !g_vp) return fail(ctx, PAM_E_ARG, "dseis and g_vp must not be NULL");
+    if (layout != PAM_INTERIOR && layout != PAM_PADDED) return fail(ctx, PAM_E_ARG, "layout must be 0 or 1");
+    if (ctx->storage == PAM_STORE_ALL_F16 || ctx->storage == PAM_STORE_ALL_BF16)
+        return fail(ctx, PAM_E_UNSUPPORTED,
+                    "the exact adjoint keeps float32 state: use PAM_STORE_ALL_F32 or PAM_STORE_CHECKPOINT");
+    const pam_grid &g = ctx->g;
+    const int nt = g.nt;
+    const long long nd = (long long)(nt - g.itlag) * ctx->nrec;
+    const bool cpml = ctx->abc == PAM_CPML;
+    const bool ckpt = ctx->storage == PAM_STORE_CHECKPOINT;
+    const int nf = cpml ? 5 : 1;           // saved fields per step
+    const int nwork = cpml ? 6 : 0;        // mu (x, z), nu (x, z) in and out
+    const int nacc = cpml ? 5 : 1;         // dJ/dV, and dJ/da, dJ/db in x and z
+    const int C = ckpt ? (ctx->ckpt_step > 0 ? ctx->ckpt_step : (int)std::ceil(std::sqrt((double)nt))) : nt;
+    const int nseg = ckpt ? (nt - 1) / C + 1 : 1;
+    const int nbuf = ckpt ? C + 1 : nt;    // slots k0 .. hi of a segment
+    const long long pl = ctx->plane;
+    size_t freeb = 0, totalb = 0;
+    HCK(hipMemGetInfo(&freeb, &totalb));
+    int B = batch_size(ctx, nshots);
+    const long long per_shot = pl * 4 * ((long long)nbuf * nf + nwork + nacc) +
+                               (ckpt ? (long long)nseg * ckpt_fields(ctx) * pl * 4 : 0);
+    const long long have = ctx->ex_save_bytes + ctx->ex_work_bytes + ctx->ckpt_bytes;
+    long long budget = (long long)((freeb + have) * 0.85);
+    if (ckpt && !ctx->aux.U0) budget -= pl * ctx->max_shots * 4 * (cpml ? 8 : 2);
+    while (B > 1 && (long long)B * per_shot > budget) --B;
+    if ((long long)B * per_shot > budget)
+        return fail(ctx, PAM_E_NOMEM, "one shot's saved states (%lld bytes) exceed free device memory", per_shot);
+    const long long blk = (long long)B * pl;  // floats per field block of a slot
+    r = ensure_scratch(ctx, (void **)&ctx->ex_save, &ctx->ex_save_bytes, blk * nbuf * nf * 4);
+    if (r) return r;
+    r = ensure_scratch(ctx, (void **)&ctx->ex_work, &ctx->ex_work_bytes, blk * (nwork + nacc) * 4);
+    if (r) return r;
+    r = ensure_scratch(ctx, (void **)&ctx->ex_gacc, &ctx->ex_gacc_bytes, pl * 4);
+    if (r) return r;
+    if (ckpt) {
+        r = ensure_scratch(ctx, (void **)&ctx->ckpt, &ctx->ckpt_bytes, (long long)nseg * ckpt_fields(ctx) * pl * B * 4);
+        if (r) return r;
+        r = alloc_fieldset(ctx, ctx->aux);
+        if (r) return r;
+        ctx->ckpt_flip.assign(nseg, 0);
+    }
+    HCK(hipMemsetAsync(ctx->ex_gacc, 0, (size_t)pl * 4, ctx->st));
+    ctx->loop_ms = ctx->kernel_ms = 0.0;
+    ctx->launches = 0;
+    ctx->next_k = -1;
+    const int tile = choose_tile(ctx, B, true);
+    ctx->last_tile = tile;
+    ctx->last_batch = B;
+    auto slot = [&](int j) { return ctx->ex_save + (long long)j * nf * blk; };
+    HCK(hipEventRecord(ctx->e0, ctx->st));
+    for (int b0 = 0; b0 < nshots; b0 += B) {
+        const int nb = std::min(B, nshots - b0);
+        // ---- forward pass: the state of every step, or checkpoints
+        r = zero_shot_state(ctx, nb, true, true);
+        if (r) return r;
+        r = upload_sources(ctx, shot0 + b0, nb);
+        if (r) return r;
+        KCK(launch_inject_point(ctx->fs.U0, pl, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, nb, ctx->st));
+        StepArgs a;
+        base_step_args(ctx, a, nb);
+        a.flags |= F_SRC;
+        {
+            float *cur = ctx->fs.U0, *prv = ctx->fs.U1;
+            int j = 0;
+            for (int k = 0; k < nt; ++k) {
+                if (ckpt && j < nseg && k == std::max(0, j * C - 1)) {
+                    r = save_ckpt(ctx, j, nb, cur, prv);
+                    if (r) return r;
+                    ++j;
+                }
+                r = fwd_step(ctx, a, tile, cur, prv, k, nullptr);
+                if (r) return r;
+                if (!ckpt) {
+                    r = exact_copy_state(ctx, ctx->fs, cur, slot(k), nb, blk);
+                    if (r) return r;
+                }
+                std::swap(cur, prv);
+            }
+        }
+        // ---- reverse sweep
+        r = zero_shot_state(ctx, nb, true, false);
+        if (r) return r;
+        HCK(hipMemsetAsync(ctx->ex_work, 0, (size_t)blk * (nwork + nacc) * 4, ctx->st));
+        ExactArgs e;
+        std::memset(&e, 0, sizeof(e));
+        e.nzp = ctx->nzp;
+        e.nxp = ctx->nxp;
+        e.P = ctx->P;
+        e.N = g.n_abc;
+        e.nshots = nb;
+        e.cpml = cpml ? 1 : 0;
+        e.sstride = pl;
+        e.fstride = pl;
+        e.idx2 = 1.0f / (g.dx * g.dx);
+        e.idz2 = 1.0f / (g.dz * g.dz);
+        e.idx = 1.0f / g.dx;
+        e.idz = 1.0f / g.dz;
+        e.v2 = ctx->v2;
+        e.cerx = ctx->cerx;
+        e.cerz = ctx->cerz;
+        e.cax = ctx->cax;
+        e.cbx = ctx->cbx;
+        e.caz = ctx->caz;
+        e.cbz = ctx->cbz;
+        float *X = ctx->fs.U0, *Y = ctx->fs.U1;
+        float *w = ctx->ex_work;
+        float *nxi = w + 2 * blk, *nzi = w + 3 * blk, *nxo = w + 4 * blk, *nzo = w + 5 * blk;
+        if (cpml) {
+            e.mux = w;
+            e.muz = w + blk;
+            e.gax = w + (nwork + 1) * blk;
+            e.gbx = w + (nwork + 2) * blk;
+            e.gaz = w + (nwork + 3) * blk;
+            e.gbz = w + (nwork + 4) * blk;
+        }
+        e.gV = w + nwork * blk;
+        const float *data = dseis + (long long)b0 * nd;
+        for (int seg = nseg - 1; seg >= 0; --seg) {
+            const int lo = ckpt ? seg * C : 0;
+            const int hi = ckpt ? std::min(nt - 1, (seg + 1) * C - 1) : nt - 1;
+            const int k0 = ckpt ? std::max(0, lo - 1) : 0;
+            if (ckpt) {
+                r = restore_ckpt(ctx, seg, nb);
+                if (r) return r;
+                float *fc = ctx->aux.U0, *fp = ctx->aux.U1;
+                for (int k = k0; k <= hi; ++k) {
+                    r = fwd_step_fs(ctx, ctx->aux, a, tile, fc, fp, k, nullptr);
+                    if (r) return r;
+                    r = exact_copy_state(ctx, ctx->aux, fc, slot(k - k0), nb, blk);
+                    if (r) return r;
+                    std::swap(fc, fp);
+                }
+            }
+            for (int t = hi; t >= lo; --t) {
+                const float *S = slot(t - k0);
+                const float *Sm = (t >= 1 && t - 1 >= k0) ? slot(t - 1 - k0) : nullptr;
+                e.X = X;
+                e.Y = Y;
+                e.C = S;
+                if (cpml) {
+                    e.psx = S + blk;
+                    e.psz = S + 2 * blk;
+                    e.zex = S + 3 * blk;
+                    e.zez = S + 4 * blk;
+                    e.psxm = Sm ? Sm + blk : nullptr;
+                    e.pszm = Sm ? Sm + 2 * blk : nullptr;
+                    e.zexm = Sm ? Sm + 3 * blk : nullptr;
+                    e.zezm = Sm ? Sm + 4 * blk : nullptr;
+                    e.nux_in = nxi;
+                    e.nuz_in = nzi;
+                    e.nux_out = nxo;
+                    e.nuz_out = nzo;
+                    KCK(launch_exact_mu(e, ctx->st));
+                }
+                KCK(launch_exact_lambda(e, ctx->st));
+                std::swap(nxi, nxo);
+                std::swap(nzi, nzo);
+                if (t >= g.itlag)
+                    KCK(launch_exact_rec(Y, pl, ctx->P, ctx->d_rz, ctx->d_rx, ctx->nrec, data, nd, t - g.itlag,
+                                         cpml ? nullptr : ctx->cerx, ctx->cerz, nb, ctx->st));
+                std::swap(X, Y);
+            }
+        }
+        KCK(launch_exact_finish(e, ctx->vp_pad, g.dx, g.dz, g.dt, ctx->ap.cpml_d0, ctx->ap.cpml_f_pico, ctx->ex_gacc,
+                                ctx->st));
+    }
+    if (layout == PAM_PADDED)
+        KCK(launch_pack_padded(ctx->ex_gacc, ctx->nzp, ctx->nxp, ctx->P, g_vp, ctx->st));
+    else
+        KCK(launch_expand_T(ctx->ex_gacc, ctx->nzp, ctx->nxp, ctx->P, g.n_abc, g_vp, ctx->st));
+    HCK(hipEventRecord(ctx->e1, ctx->st));
+    HCK(hipEventSynchronize(ctx->e1));
+    float ms = 0.f;
+    HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
+    ctx->loop_ms = ms;
+    return PAM_OK;
 }

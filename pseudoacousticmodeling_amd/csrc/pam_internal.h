@@ -16,7 +16,7 @@ enum : int {
     F_ILUM = 32,        // ilum += input^2 on the interior
     F_IMAGE = 64,       // image += S_k * input on the interior
     F_GRADM = 128,      // grad += input * (S_{k+1} - 2 S_k + S_{k-1}) / dt^2 on the interior
-    F_ADJ_EXACT = 256,  // (reserved)
+    F_ADJ_EXACT = 256,  // (unused bit)
     F_SMAX = 512,       // max |new field| over the interior -> smax[shot][k+1] (scale of snapshot k+1)
     F_D2T = 1024,       // snapshots hold D_k = S_{k+1} - 2 S_k + S_{k-1} (formed in float32 by step k),
                         // F_SMAX then takes max |D_k| (the lagged scale of D_{k+1}) and F_GRADM reads D_k
@@ -150,5 +150,35 @@ int launch_residual(const float *dobs, const float *dcal, float *res, long long 
 int launch_grad_multi(int phys, const float *S, int sP, long long ss, const float *lam, int lP, long long ls,
                       int nshots, int nz, int nx, float dx, float dz, const float *eps, const float *del,
                       const float *theta, int mP, float *geps, float *gdel, float *gth, hipStream_t st);
+
+// Exact discrete adjoint of the acoustic forward map (pam_forward_adjoint).
+// Fields are padded [nzp][P] per shot, shots `sstride` floats apart; the saved
+// forward state of step k (C_k = current field after injection, and for CPML
+// psi_k / zeta_k after the step) is `fstride` floats apart per shot.
+struct ExactArgs {
+    int nzp, nxp, P, N, nshots, cpml;
+    long long sstride, fstride;
+    float idx2, idz2, idx, idz;
+    const float *v2, *cerx, *cerz, *cax, *cbx, *caz, *cbz;
+    const float *X;                          // lambda_{k+1}
+    float *Y;                                // lambda_{k+2} in, lambda_k out
+    float *mux, *muz;                        // mu_{k+1} in, mu_k out (CPML)
+    const float *nux_in, *nuz_in;            // nu_{k+1} (CPML)
+    float *nux_out, *nuz_out;                // nu_k (CPML)
+    const float *C, *psx, *psz, *zex, *zez;  // saved state of step k
+    const float *psxm, *pszm, *zexm, *zezm;  // saved psi / zeta of step k-1 (NULL at k = 0)
+    float *gV, *gax, *gbx, *gaz, *gbz;       // per-shot accumulators
+};
+int launch_exact_mu(const ExactArgs &a, hipStream_t st);
+int launch_exact_lambda(const ExactArgs &a, hipStream_t st);
+// Y[rz, rx] += w * data[row] with w the Cerjan factor of the cell (1 for CPML)
+int launch_exact_rec(float *Y, long long sstride, int P, const int *rz, const int *rx, int nrec, const float *data,
+                     long long dstride, int row, const float *cerx, const float *cerz, int nshots, hipStream_t st);
+// gacc += sum over shots of dJ/dvp from the per-shot accumulators (V and, for
+// CPML, the profile derivatives)
+int launch_exact_finish(const ExactArgs &a, const float *vp, float dx, float dz, float dt, float d0, float fpico,
+                        float *gacc, hipStream_t st);
+// out [nz][nx] = transpose of the edge-replicating expansion of gpad
+int launch_expand_T(const float *gpad, int nzp, int nxp, int P, int N, float *out, hipStream_t st);
 
 }  // namespace pam
