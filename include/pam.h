@@ -45,7 +45,15 @@ enum pam_storage {
     PAM_STORE_ALL_F32 = 0,  /* every step saved, float32 */
     PAM_STORE_ALL_F16 = 1,  /* every step saved, float16 with a per-snapshot scale */
     PAM_STORE_ALL_BF16 = 2, /* every step saved, bfloat16 with a per-snapshot scale */
-    PAM_STORE_CHECKPOINT = 3 /* uniform checkpoints, segments recomputed (exact) */
+    PAM_STORE_CHECKPOINT = 3, /* uniform checkpoints, segments recomputed (exact) */
+    /* boundary saving (src/Migration.py:1266-1410): the 4-cell rims just inside
+       the interior edge of every step's source field are kept, and the backward
+       pass rebuilds the source field on the interior by running the undamped
+       step backwards in time (zero on the absorbing layer); rims in float32, or
+       float16 / bfloat16 scaled by each step's max |rim value| */
+    PAM_STORE_BOUNDARY_F32 = 4,
+    PAM_STORE_BOUNDARY_F16 = 5,
+    PAM_STORE_BOUNDARY_BF16 = 6
 };
 
 /* keys of pam_set_option */

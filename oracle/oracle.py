@@ -341,6 +341,17 @@ class Oracle:
             L.or_update_tti(_ptr(Uf), _ptr(Uc), _ptr(vp), _ptr(eps), _ptr(dl), _ptr(th), _I(g.nzp),
                             _I(g.nxp), _D(g.dz), _D(g.dx), _D(g.dt))
 
+    def operator(self, phys, Uc, g: Grid, vp, eps=None, dl=None, th=None):
+        """The operator term V L(Uc) of the undamped update (zero on the
+        outer 4-cell rim): S_{k+1} - 2 S_k + S_{k-1} of a source-free step."""
+        out = self.zeros(g.nzp, g.nxp)
+        self.lib.or_set_operator_only(_I(1))
+        try:
+            self.update(phys, out, Uc, g, vp, eps, dl, th)
+        finally:
+            self.lib.or_set_operator_only(_I(0))
+        return out
+
     def adjoint_update(self, phys, Uf, Uc, P, g: Grid, vp, eps=None, dl=None, th=None):
         L = self.lib
         if phys == 1:
@@ -437,9 +448,111 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
 
 
+def _bf16_round(x):
+    """float32 -> bfloat16 (round to nearest even) -> float32, as the device's
+    float -> bfloat16 conversion does for finite values."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def quantize_scaled(v, kind):
+    """Values of one step stored in float16 / bfloat16 as v / m with
+    m = max |v| (float32 arithmetic: inv = 1/m, q = v * inv), read back as
+    q * m; ``kind`` 'f32' returns v unchanged (in its own precision)."""
+    if kind == "f32":
+        return np.array(v, copy=True)
+    v = np.asarray(v, dtype=np.float32)
+    m = np.float32(np.max(np.abs(v))) if v.size else np.float32(0.0)
+    inv = np.float32(1.0) / m if m > 0 else np.float32(0.0)
+    q = v * inv
+    q = q.astype(np.float16).astype(np.float32) if kind == "f16" else _bf16_round(q)
+    return q * m
+
+
+def rim_mask(nz, nx):
+    """The 4-cell rims just inside the interior edge (src/Migration.py:288-298:
+    top / bottom 4 rows, left / right 4 columns of the interior)."""
+    m = np.zeros((nz, nx), dtype=bool)
+    m[:4, :] = m[nz - 4:, :] = True
+    m[:, :4] = m[:, nx - 4:] = True
+    return m
+
+
+class _SaveAll:
+    """Source field kept at every step (src/Migration.py:996-1037)."""
+
+    def __init__(self, S, g: Grid):
+        self.S, self.sl = S, (slice(g.N, g.N + g.nz), slice(g.N, g.N + g.nx))
+
+    def get(self, t, need_d):
+        S, sl = self.S, self.sl
+        d = (S[t + 1][sl] - 2.0 * S[t][sl] + S[t - 1][sl]) if need_d else None
+        return S[t], d
+
+
+class _BoundaryRebuild:
+    """Boundary-saving source field: the rims of every S_k (quantised per step
+    as ``kind``: 'f32' | 'f16' | 'bf16'), and S rebuilt backwards in time from
+    S_nt, S_{nt-1} by S_{k-1} = v^2 dt^2 L(S_k) + 2 S_k - S_{k+1}
+    (+ wav[k+1] at a source cell of the inner region) with the undamped update,
+    zero on the absorbing layer, the saved rims written back after each step.
+    D_k = S_{k-1} - 2 S_k + S_{k+1} is formed as the step goes as the operator
+    term v^2 dt^2 L(S_k) (+ wav[k+1] at the source), which the leapfrog makes
+    equal to the difference of the three levels without cancelling them; its
+    rim cells come from the saved rims.  src/Migration.py:1266-1410 keeps the same rims; its
+    time alignment (the reconstructed field paired with the wrong step) is not
+    followed."""
+
+    def __init__(self, orc, phys, g: Grid, model, S, U_nt, src, sz, sx, kind):
+        self.orc, self.phys, self.g, self.model = orc, phys, g, model
+        self.src, self.sz, self.sx = src, sz, sx
+        N = g.N
+        self.sl = (slice(N, N + g.nz), slice(N, N + g.nx))
+        self.mask = rim_mask(g.nz, g.nx)
+        self.rims = [quantize_scaled(S[k][self.sl][self.mask], kind) for k in range(g.nt)]
+        self.hi = self._interior_only(U_nt)          # S_{lev+1}
+        self.lo = self._interior_only(S[g.nt - 1])   # S_lev
+        self.lev = g.nt - 1
+
+    def _interior_only(self, U):
+        out = np.zeros_like(U)
+        out[self.sl] = U[self.sl]
+        return out
+
+    def _step(self, need_d=False):
+        g, sl, lev = self.g, self.sl, self.lev
+        vp, eps, dl, th = self.model
+        Uf, Uc = self.hi, self.lo
+        d = self.orc.operator(self.phys, Uc, g, vp, eps, dl, th)[sl].copy() if need_d else None
+        self.orc.update(self.phys, Uf, Uc, g, vp, eps, dl, th)  # Uf := S_{lev-1} (undamped)
+        keep = Uf[sl].copy()
+        Uf[...] = 0
+        Uf[sl] = keep
+        iz, ix = self.sz - g.N, self.sx - g.N
+        if lev + 1 < g.nt and 4 <= iz < g.nz - 4 and 4 <= ix < g.nx - 4:
+            Uf[self.sz, self.sx] += self.src[lev + 1]
+            if need_d:
+                d[iz, ix] += self.src[lev + 1]
+        Uf[sl][self.mask] = self.rims[lev - 1]
+        if need_d:
+            d[self.mask] = (self.rims[lev - 1] - 2.0 * self.rims[lev]) + self.rims[lev + 1]
+        self.hi, self.lo = Uc, Uf
+        self.lev = lev - 1
+        return d
+
+    def get(self, t, need_d):
+        while self.lev > t:
+            self._step()
+        if not need_d:
+            return self.lo, None
+        d = self._step(True)
+        return self.hi, d
+
+
 def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, data, A=None, cpml=None,
                  fwi=False, multiparameter=False, model_interior=None, reset_cpml=True, t_range="py",
-                 normalize=1):
+                 normalize=1, storage="all", rim_kind="f32"):
     """On-the-fly RTM / FWI gradient of src/Migration.py:963-1111 (GPU path):
     per shot, a forward pass storing the current field after every step and
     accumulating ilum += S^2 over the interior, then a backward pass
@@ -454,11 +567,17 @@ def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, dat
     ``t_range="cpp"`` and ``normalize=2`` give the C++ driver's variant
     (cpp/Migration.cu:383-394, 470-483): the backward pass runs t = nt-1..0
     (imaging only) and the image is divided by ilum where ilum > 1e-12,
-    zero elsewhere."""
+    zero elsewhere.
+
+    ``storage="boundary"`` takes the source field from the boundary-saving
+    rebuild (src/Migration.py:1266-1410; ``_BoundaryRebuild``) with rims kept
+    as ``rim_kind``; ilum still comes from the forward pass."""
     if t_range not in ("py", "cpp") or normalize not in (1, 2):
         raise ValueError("t_range 'py'|'cpp', normalize 1|2")
     if t_range == "cpp" and fwi:
         raise ValueError("the C++ backward range applies to imaging only")
+    if storage not in ("all", "boundary"):
+        raise ValueError("storage 'all'|'boundary'")
     N = g.N
     vp, eps, dl, th = [orc.arr(m) if m is not None else None for m in model]
     image = np.zeros((g.nz, g.nx), dtype=np.float64)
@@ -476,6 +595,11 @@ def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, dat
         ilum_p = np.zeros((g.nz, g.nx), dtype=orc.dtype)
         for k in range(g.nt):
             ilum_p += S[k][sl] * S[k][sl]
+        if storage == "all":
+            prov = _SaveAll(S, g)
+        else:
+            prov = _BoundaryRebuild(orc, phys, g, (vp, eps, dl, th), S, fw["current"], orc.arr(src), sz, sx,
+                                    rim_kind)
         cp = fw["cp"]
         if cp is not None and reset_cpml:
             for c in cp[4:]:
@@ -486,21 +610,22 @@ def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, dat
         dxdz = g.dx * g.dz
         t_hi, t_lo = (g.nt - 2, 1) if t_range == "py" else (g.nt - 1, 0)
         for t in range(t_hi, t_lo - 1, -1):
+            St, Dt = prov.get(t, fwi)
             row = data[s][t - g.itlag] if t >= g.itlag else None
-            orc.backward_step(phys, abc, Lf, Lc, g, (vp, eps, dl, th), A, cp, S[t], rz, rx, row, dxdz)
+            orc.backward_step(phys, abc, Lf, Lc, g, (vp, eps, dl, th), A, cp, St, rz, rx, row, dxdz)
             adj = Lc[sl].copy()
             if fwi:
-                d2 = (S[t + 1][sl] - 2.0 * S[t][sl] + S[t - 1][sl]) / (g.dt * g.dt)
+                d2 = Dt / (g.dt * g.dt)
                 img_p += (d2 * adj).astype(orc.dtype)
                 if multiparameter and phys == 1:
-                    orc.grad_vti(np.ascontiguousarray(S[t][sl]), np.ascontiguousarray(adj), ge, gd, g.nz, g.nx,
+                    orc.grad_vti(np.ascontiguousarray(St[sl]), np.ascontiguousarray(adj), ge, gd, g.nz, g.nx,
                                  g.dx, g.dz, orc.arr(model_interior[1]), orc.arr(model_interior[2]))
                 elif multiparameter and phys == 2:
-                    orc.grad_tti(np.ascontiguousarray(S[t][sl]), np.ascontiguousarray(adj), ge, gd, gt, g.nz,
+                    orc.grad_tti(np.ascontiguousarray(St[sl]), np.ascontiguousarray(adj), ge, gd, gt, g.nz,
                                  g.nx, g.dx, g.dz, orc.arr(model_interior[1]), orc.arr(model_interior[2]),
                                  orc.arr(model_interior[3]))
             else:
-                img_p += (S[t][sl] * adj).astype(orc.dtype)
+                img_p += (St[sl] * adj).astype(orc.dtype)
             Lc, Lf = Lf, Lc
         image += img_p
         ilum += ilum_p

@@ -222,6 +222,13 @@ void or_inject_rec(store *U, int nxp, int nrec, const int *rz, const int *rx, co
 
 /* Acoustic leapfrog, src/CudaKernels.py:268-305 / src/utils.py:396-410:
  * Uf = (vp*vp)*(dt*dt)*(pxx+pzz) + 2*Uc - Uf on the interior. */
+/* When set, the three update routines store only the operator term
+ * V L(Uc) of the leapfrog (Uf = V L(Uc) instead of V L(Uc) + 2 Uc - Uf): the
+ * second time difference S_{k+1} - 2 S_k + S_{k-1} of an undamped step, which
+ * the boundary-saving rebuild needs without cancelling time levels. */
+static int g_op_only = 0;
+void or_set_operator_only(int on) { g_op_only = on; }
+
 void or_update_acoustic(store *Uf, const store *Uc, const store *vp, int nzp, int nxp, double dz,
                         double dx, double dt) {
     const real idx2 = R(dx) * R(dx), idz2 = R(dz) * R(dz), dt2 = R(dt) * R(dt);
@@ -231,7 +238,8 @@ void or_update_acoustic(store *Uf, const store *Uc, const store *vp, int nzp, in
             long c = (long)j * nxp + i;
             real pxx = d2(Uc, c, 1) / idx2;
             real pzz = d2(Uc, c, nxp) / idz2;
-            Uf[c] = (store)(((real)vp[c] * vp[c]) * dt2 * (pxx + pzz) + R(2.0) * Uc[c] - Uf[c]);
+            const real op = ((real)vp[c] * vp[c]) * dt2 * (pxx + pzz);
+            Uf[c] = (store)(g_op_only ? op : op + R(2.0) * Uc[c] - Uf[c]);
         }
 }
 
@@ -256,8 +264,9 @@ void or_update_vti(store *Uf, const store *Uc, const store *vp, const store *eps
             real e = eps[c], dl = del[c];
             real Sd = sd_term(e, dl, px, pz);
             real v2 = (real)vp[c] * vp[c] * dt2;
-            Uf[c] = (store)(R(2.0) * Uc[c] - Uf[c] + v2 * ((R(1.0) + R(2.0) * e) + Sd) * pxx +
-                            v2 * (R(1.0) + Sd) * pzz);
+            Uf[c] = (store)(g_op_only ? v2 * ((R(1.0) + R(2.0) * e) + Sd) * pxx + v2 * (R(1.0) + Sd) * pzz
+                                      : R(2.0) * Uc[c] - Uf[c] + v2 * ((R(1.0) + R(2.0) * e) + Sd) * pxx +
+                                            v2 * (R(1.0) + Sd) * pzz);
         }
 }
 
@@ -280,8 +289,9 @@ void or_update_tti(store *Uf, const store *Uc, const store *vp, const store *eps
             real v2 = (real)vp[c] * vp[c] * dt2;
             real Ac = (R(1.0) + R(2.0) * e) * (cs * cs) + (sn * sn) + Sd;
             real Bc = (R(1.0) + R(2.0) * e) * (sn * sn) + (cs * cs) + Sd;
-            Uf[c] = (store)(R(2.0) * Uc[c] - Uf[c] + v2 * Ac * pxx + v2 * Bc * pzz -
-                            R(2.0) * e * v2 * (R(2.0) * sn * cs) * pxz);
+            Uf[c] = (store)(g_op_only ? v2 * Ac * pxx + v2 * Bc * pzz - R(2.0) * e * v2 * (R(2.0) * sn * cs) * pxz
+                                      : R(2.0) * Uc[c] - Uf[c] + v2 * Ac * pxx + v2 * Bc * pzz -
+                                            R(2.0) * e * v2 * (R(2.0) * sn * cs) * pxz);
         }
 }
 

@@ -17,6 +17,7 @@ PAM_ACOUSTIC, PAM_VTI, PAM_TTI = 0, 1, 2
 PAM_CERJAN, PAM_CPML = 0, 1
 PAM_INTERIOR, PAM_PADDED = 0, 1
 PAM_STORE_ALL_F32, PAM_STORE_ALL_F16, PAM_STORE_ALL_BF16, PAM_STORE_CHECKPOINT = 0, 1, 2, 3
+PAM_STORE_BOUNDARY_F32, PAM_STORE_BOUNDARY_F16, PAM_STORE_BOUNDARY_BF16 = 4, 5, 6
 PAM_OPT_BATCH, PAM_OPT_TILE, PAM_OPT_GRAPH, PAM_OPT_STORAGE = 1, 2, 3, 4
 PAM_OPT_CKPT_STEP, PAM_OPT_RESET_ADJ_CPML, PAM_OPT_NORMALIZE, PAM_OPT_ADJ_RANGE = 5, 6, 7, 8
 

@@ -28,6 +28,7 @@ int main(int argc, char **argv) {
                   << " method." << std::endl;
         int storage = PAM_STORE_ALL_F32;
         if (s.migration == "checkpoint") storage = PAM_STORE_CHECKPOINT;
+        else if (s.migration == "SB") storage = PAM_STORE_BOUNDARY_F32;
         else if (s.migration != "onthefly") {
             std::cerr << "Error: Invalid migration method: " << s.migration << std::endl;
             return 1;

@@ -86,6 +86,8 @@ struct pam_ctx {
     float *save_scale = nullptr;
     float *acc_ilum = nullptr, *acc_img = nullptr;
     long long acc_bytes = 0;
+    float *sbd = nullptr;  // boundary saving: D_t of the shots ([shot][nz][nx])
+    long long sbd_bytes = 0;
     float *acc_multi = nullptr;  // per-shot eps / delta / theta partials
     long long acc_multi_bytes = 0;
     float *maxv = nullptr;
@@ -269,7 +271,7 @@ extern "C" int pam_destroy(pam_ctx *ctx) {
                      ctx->fs.U1,  ctx->fs.psix, ctx->fs.psiz,  ctx->fs.zetax, ctx->fs.zetaz, ctx->tmp, ctx->save_scale,
                      ctx->aux.U0, ctx->aux.U1, ctx->aux.psix, ctx->aux.psiz, ctx->aux.zetax, ctx->aux.zetaz, ctx->ckpt,
                      ctx->acc_ilum, ctx->maxv, ctx->ilum_tmp, ctx->th_pad, ctx->acc_multi,
-                     ctx->ex_save, ctx->ex_work, ctx->ex_gacc};
+                     ctx->sbd, ctx->ex_save, ctx->ex_work, ctx->ex_gacc};
     for (float *b : bufs) dfree(b);
     dfree(ctx->save);
     dfree(ctx->d_acc);
@@ -302,7 +304,7 @@ extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
             ctx->graph = v ? 1 : 0;
             return PAM_OK;
         case PAM_OPT_STORAGE:
-            if (v < PAM_STORE_ALL_F32 || v > PAM_STORE_CHECKPOINT) return fail(ctx, PAM_E_ARG, "unknown storage");
+            if (v < PAM_STORE_ALL_F32 || v > PAM_STORE_BOUNDARY_BF16) return fail(ctx, PAM_E_ARG, "unknown storage");
             ctx->storage = (int)v;
             return PAM_OK;
         case PAM_OPT_CKPT_STEP:
@@ -882,7 +884,13 @@ static int restore_ckpt(pam_ctx *ctx, int j, int nb) {
 //  - uniform checkpoints of the whole state every C steps in HBM, each segment
 //    recomputed into a (C+2)-step buffer on a second field set just before
 //    the adjoint pass crosses it (src/Migration.py:1114-1263 keeps them in host
-//    files); the recomputed fields are bit-identical to the first pass.
+//    files); the recomputed fields are bit-identical to the first pass;
+//  - boundary saving: the 4-cell rims inside the interior edge of every step
+//    (float32, or float16 / bfloat16 scaled per step), the interior rebuilt
+//    backwards in time on the second field set one step ahead of the adjoint
+//    pass (src/Migration.py:1266-1410; the reference's time alignment is
+//    re-derived: S_{k-1} = v^2 dt^2 L(S_k) + 2 S_k - S_{k+1} + wav[k+1] at the
+//    source, rims written back, zero on the absorbing layer).
 static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, bool grad, float *image, float *ilum,
                        float *g_eps = nullptr, float *g_del = nullptr, float *g_th = nullptr) {
     int r = ready(ctx);
@@ -898,15 +906,23 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     const int storage = ctx->storage;
     const bool ckpt = storage == PAM_STORE_CHECKPOINT;
     const bool half = storage == PAM_STORE_ALL_F16 || storage == PAM_STORE_ALL_BF16;
+    const bool sb = storage >= PAM_STORE_BOUNDARY_F32;
+    const int sb_kind = sb ? storage - PAM_STORE_BOUNDARY_F32 : 0;  // 0 float32, 1 float16, 2 bfloat16
+    if (sb && (g.nz < 9 || g.nx < 9 || g.n_abc < 4))
+        return fail(ctx, PAM_E_ARG, "boundary saving needs nz, nx >= 9 and n_abc >= 4");
     if (half && linear_adj)
         return fail(ctx, PAM_E_UNSUPPORTED,
                     "float16 / bfloat16 snapshots feed the pressure-form adjoint only; the linearised VTI/TTI "
                     "adjoint (Cerjan) reads float32 padded snapshots: use PAM_STORE_ALL_F32 or PAM_STORE_CHECKPOINT");
-    const int skind = linear_adj ? SV_F32_PAD
-                                 : (storage == PAM_STORE_ALL_F16 ? SV_F16_INT
-                                                                 : (storage == PAM_STORE_ALL_BF16 ? SV_BF16_INT : SV_F32_INT));
-    const int elem = half ? 2 : 4;
-    const long long per_step = (skind == SV_F32_PAD) ? ctx->plane : nint;  // elements per snapshot
+    // boundary saving: the adjoint step reads D_t from a one-step interior
+    // buffer (gradient) or S_t from the rebuilt padded field (imaging)
+    const int skind = sb ? (grad ? SV_F32_INT : SV_F32_PAD)
+                         : linear_adj ? SV_F32_PAD
+                                      : (storage == PAM_STORE_ALL_F16
+                                             ? SV_F16_INT
+                                             : (storage == PAM_STORE_ALL_BF16 ? SV_BF16_INT : SV_F32_INT));
+    const int elem = (half || sb_kind) ? 2 : 4;
+    const long long per_step = sb ? sb_width(g.nz, g.nx) : (skind == SV_F32_PAD) ? ctx->plane : nint;  // elements per step
     const int C = ckpt ? (ctx->ckpt_step > 0 ? ctx->ckpt_step : (int)std::ceil(std::sqrt((double)nt))) : nt;
     // backward steps t in [t_lo, t_hi]: [1, nt-2] (Python reference) or [0, nt-1] (C++)
     if (grad && ctx->adj_range == 1)
@@ -919,18 +935,20 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     // the gradient needs S only through S_{t+1} - 2 S_t + S_{t-1}: store that
     // difference (formed by the forward step in float32, the same arithmetic the
     // adjoint step would do) unless the partials or the linearised adjoint need S
-    const bool d2t = grad && !multi && !linear_adj;
+    const bool d2t = grad && (sb || (!multi && !linear_adj));
     // choose the batch so that the snapshots (and checkpoints) fit in free memory
     size_t freeb = 0, totalb = 0;
     HCK(hipMemGetInfo(&freeb, &totalb));
     int B = batch_size(ctx, nshots);
     const long long per_shot_bytes = per_step * nbuf * elem + 2 * nint * 4 + (linear_adj ? 5 * ctx->plane * 4 : 0) +
                                      (multi ? nmulti * nint * 4 : 0) + (half ? (long long)nt * 4 : 0) +
-                                     (ckpt ? (long long)nseg * ckpt_fields(ctx) * ctx->plane * 4 : 0);
+                                     (ckpt ? (long long)nseg * ckpt_fields(ctx) * ctx->plane * 4 : 0) +
+                                     (sb ? (long long)nt * 4 + nint * 4 : 0);
     const long long have = (long long)ctx->save_bytes + ctx->acc_bytes + ctx->tmp_bytes + ctx->acc_multi_bytes +
-                           ctx->ckpt_bytes + ctx->smax_bytes;
+                           ctx->ckpt_bytes + ctx->smax_bytes + ctx->sbd_bytes;
     long long budget = (long long)((freeb + have) * 0.85);
-    if (ckpt && !ctx->aux.U0) budget -= (long long)ctx->plane * ctx->max_shots * 4 * (ctx->abc == PAM_CPML ? 8 : 2);
+    if ((ckpt || sb) && !ctx->aux.U0)
+        budget -= (long long)ctx->plane * ctx->max_shots * 4 * (ctx->abc == PAM_CPML ? 8 : 2);
     while (B > 1 && (long long)B * per_shot_bytes > budget) --B;
     if ((long long)B * per_shot_bytes > budget)
         return fail(ctx, PAM_E_NOMEM, "one shot's snapshots (%lld bytes) exceed free device memory", per_shot_bytes);
@@ -943,8 +961,14 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         r = ensure_scratch(ctx, (void **)&ctx->tmp, &ctx->tmp_bytes, (long long)B * 5 * ctx->plane * 4);
         if (r) return r;
     }
-    if (half) {
+    if (half || sb) {
         r = ensure_scratch(ctx, (void **)&ctx->smax, &ctx->smax_bytes, (long long)B * nt * 4);
+        if (r) return r;
+    }
+    if (sb) {
+        r = ensure_scratch(ctx, (void **)&ctx->sbd, &ctx->sbd_bytes, (long long)B * nint * 4);
+        if (r) return r;
+        r = alloc_fieldset(ctx, ctx->aux);
         if (r) return r;
     }
     if (ckpt) {
@@ -974,7 +998,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     ctx->launches = 0;
     ctx->next_k = -1;
     int tile0 = choose_tile(ctx, B, true);
-    if (half && tile_shape(tile0).V > 0) tile0 = kDefaultTile;  // k_step has no fused max
+    if ((half || sb) && tile_shape(tile0).V > 0) tile0 = kDefaultTile;  // k_step has no fused max / D_t
     ctx->last_tile = tile0;
     ctx->last_batch = B;
     const long long pint = (long long)g.n_abc * ctx->P + g.n_abc;  // interior origin in a padded field
@@ -986,7 +1010,8 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         if (r) return r;
         r = upload_sources(ctx, shot0 + b0, nb);
         if (r) return r;
-        if (skind == SV_F32_PAD) HCK(hipMemsetAsync(ctx->save, 0, (size_t)nb * per_step * nbuf * elem, ctx->st));
+        if (skind == SV_F32_PAD && !sb)
+            HCK(hipMemsetAsync(ctx->save, 0, (size_t)nb * per_step * nbuf * elem, ctx->st));
         HCK(hipMemsetAsync(ctx->acc_ilum, 0, (size_t)nb * nint * 4, ctx->st));
         HCK(hipMemsetAsync(ctx->acc_img, 0, (size_t)nb * nint * 4, ctx->st));
         if (multi) HCK(hipMemsetAsync(ctx->acc_multi, 0, (size_t)B * nmulti * nint * 4, ctx->st));
@@ -1000,7 +1025,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         }
         StepArgs a;
         base_step_args(ctx, a, nb);
-        a.flags |= F_SRC | F_ILUM | (ckpt ? 0 : F_SAVE) | (half ? F_SMAX : 0) | (d2t ? F_D2T : 0);
+        a.flags |= F_SRC | F_ILUM | ((ckpt || sb) ? 0 : F_SAVE) | (half ? F_SMAX : 0) | ((d2t && !sb) ? F_D2T : 0);
         a.save_kind = skind;
         a.save = ctx->save;
         a.save_k0 = 0;
@@ -1013,6 +1038,16 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         a.acc_sstride = nint;
         {
             float *cur = ctx->fs.U0, *prv = ctx->fs.U1;
+            // boundary saving: rims of S_k into step k of the strip buffer
+            auto sb_save = [&](const float *U, int k) -> int {
+                KCK(launch_sb_save(U, ctx->plane, ctx->P, g.n_abc, g.nz, g.nx, nb, sb_kind, ctx->save,
+                                   per_step * nt, k, reinterpret_cast<float *>(ctx->smax), nt, ctx->st));
+                return PAM_OK;
+            };
+            if (sb) {
+                r = sb_save(cur, 0);
+                if (r) return r;
+            }
             int j = 0;
             for (int k = 0; k < nt; ++k) {
                 if (ckpt && j < nseg && k == std::max(0, j * C - 1)) {
@@ -1022,7 +1057,18 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                 }
                 r = fwd_step(ctx, a, tile0, cur, prv, k, nullptr);
                 if (r) return r;
+                if (sb && k + 1 < nt) {
+                    r = sb_save(prv, k + 1);
+                    if (r) return r;
+                }
                 std::swap(cur, prv);
+            }
+            // the rebuild starts from S_nt and S_{nt-1} (interiors; zero on the absorbing layer)
+            if (sb) {
+                KCK(launch_copy_interior(cur, ctx->aux.U0, ctx->plane, ctx->nzp, ctx->P, g.n_abc, g.nz, g.nx, nb,
+                                         ctx->st));
+                KCK(launch_copy_interior(prv, ctx->aux.U1, ctx->plane, ctx->nzp, ctx->P, g.n_abc, g.nz, g.nx, nb,
+                                         ctx->st));
             }
         }
         // ---- backward pass: the adjoint field reuses the main set
@@ -1034,12 +1080,42 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                                   data + (long long)b0 * nd, nd, t0 - g.itlag, g.dx * g.dz, nb, ctx->st));
         float *cur = ctx->fs.U0, *prv = ctx->fs.U1;
         int k0 = 0;  // step held in snapshot slot 0
+        // boundary saving: the rebuilt source field, S_{lev+1} in ra and S_lev in rb
+        float *ra = ctx->aux.U0, *rb = ctx->aux.U1;
+        int lev = nt - 1;
+        // one reverse step: S_{lev-1} = v^2 dt^2 L(S_lev) + 2 S_lev - S_{lev+1} over
+        // the padded grid (undamped, no CPML terms) into ra, then the rims of
+        // step lev-1, the cleared outer band and wav[lev+1] at the source; with
+        // with_d, D_lev = S_{lev-1} - 2 S_lev + S_{lev+1} into sbd on the way
+        auto rev_step = [&](bool with_d) -> int {
+            StepArgs f;
+            base_step_args(ctx, f, nb);
+            f.flags = with_d ? (F_D2T | F_SAVE | F_D2T_OP) : 0;
+            f.save_kind = SV_F32_INT;
+            f.save = ctx->sbd;
+            f.save_k0 = lev;
+            f.save_kstride = nint;
+            f.save_sstride = nint;
+            f.cur = rb;
+            f.prv = ra;
+            f.k = lev;
+            KCK(launch_step(ctx->phys, false, tile0, f, ctx->st));
+            KCK(launch_sb_restore(ra, ctx->plane, ctx->P, g.n_abc, g.nz, g.nx, nb, sb_kind, ctx->save, per_step * nt,
+                                  reinterpret_cast<const float *>(ctx->smax), nt, lev - 1,
+                                  with_d ? ctx->sbd : nullptr, nint, ctx->d_ssz, ctx->d_ssx, ctx->wav, lev + 1,
+                                  ctx->st));
+            std::swap(ra, rb);
+            --lev;
+            return PAM_OK;
+        };
         // eps / delta / theta partials of step t (src/Migration.py:1053-1064)
         auto partials = [&](int t, const float *lam) -> int {
-            const float *S = (const float *)ctx->save + (long long)(t - k0) * per_step + (skind == SV_F32_PAD ? pint : 0);
-            const int sP = (skind == SV_F32_PAD) ? ctx->P : g.nx;
+            const float *S = sb ? ra + pint
+                                : (const float *)ctx->save + (long long)(t - k0) * per_step + (skind == SV_F32_PAD ? pint : 0);
+            const int sP = (sb || skind == SV_F32_PAD) ? ctx->P : g.nx;
+            const long long ss = sb ? ctx->plane : per_step * nbuf;
             float *acc = ctx->acc_multi;
-            return launch_grad_multi(ctx->phys, S, sP, per_step * nbuf, lam + pint, ctx->P, ctx->plane, nb, g.nz, g.nx,
+            return launch_grad_multi(ctx->phys, S, sP, ss, lam + pint, ctx->P, ctx->plane, nb, g.nz, g.nx,
                                      g.dx, g.dz, ctx->eps + pint, ctx->del + pint,
                                      ctx->th_pad ? ctx->th_pad + pint : nullptr, ctx->P, acc,
                                      acc + (long long)B * nint, nmulti == 3 ? acc + 2LL * B * nint : nullptr, ctx->st);
@@ -1053,6 +1129,10 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         b.save = ctx->save;
         b.save_kstride = per_step;
         b.save_sstride = per_step * nbuf;
+        if (sb) {
+            b.save = grad ? (void *)ctx->sbd : nullptr;
+            b.save_kstride = b.save_sstride = grad ? nint : ctx->plane;
+        }
         b.save_scale = reinterpret_cast<const float *>(ctx->smax);
         b.image = ctx->acc_img;
         b.acc_sstride = nint;
@@ -1120,6 +1200,20 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
             }
             b.save_k0 = k0;
             for (int t = hi; t >= lo; --t) {
+                if (sb) {
+                    // rebuild S_t (imaging: into rb) or S_{t-1} and D_t (gradient: S_t in ra)
+                    while (lev > t) {
+                        r = rev_step(false);
+                        if (r) return r;
+                    }
+                    if (grad) {
+                        r = rev_step(true);
+                        if (r) return r;
+                    } else {
+                        b.save = rb;
+                    }
+                    b.save_k0 = t;
+                }
                 if (!linear_adj) {
                     r = fwd_step(ctx, b, tile0, cur, prv, t, nullptr);
                     if (r) return r;
@@ -1128,6 +1222,12 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                     c.prv = prv;
                     c.k = t;
                     c.P_ = (const float *)ctx->save + (long long)(t - k0) * per_step;
+                    if (sb) {
+                        c.P_ = grad ? ra : rb;
+                        c.psstride = ctx->plane;
+                        c.D_ = grad ? ctx->sbd : nullptr;
+                        c.dsstride = nint;
+                    }
                     KCK(launch_adj_products(ctx->phys, c, ctx->st));
                     KCK(launch_adj_update(ctx->phys, c, ctx->st));
                 }
@@ -1215,7 +1315,7 @@ extern "C" int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const fl
     if (nshots < 1) return fail(ctx, PAM_E_ARG, "nshots must be >= 1");
     if (!dseis || !g_vp) return fail(ctx, PAM_E_ARG, "dseis and g_vp must not be NULL");
     if (layout != PAM_INTERIOR && layout != PAM_PADDED) return fail(ctx, PAM_E_ARG, "layout must be 0 or 1");
-    if (ctx->storage == PAM_STORE_ALL_F16 || ctx->storage == PAM_STORE_ALL_BF16)
+    if (ctx->storage != PAM_STORE_ALL_F32 && ctx->storage != PAM_STORE_CHECKPOINT)
         return fail(ctx, PAM_E_UNSUPPORTED,
                     "the exact adjoint keeps float32 state: use PAM_STORE_ALL_F32 or PAM_STORE_CHECKPOINT");
     const pam_grid &g = ctx->g;

@@ -20,6 +20,9 @@ enum : int {
     F_SMAX = 512,       // max |new field| over the interior -> smax[shot][k+1] (scale of snapshot k+1)
     F_D2T = 1024,       // snapshots hold D_k = S_{k+1} - 2 S_k + S_{k-1} (formed in float32 by step k),
                         // F_SMAX then takes max |D_k| (the lagged scale of D_{k+1}) and F_GRADM reads D_k
+    F_D2T_OP = 2048,    // with F_D2T | F_SAVE: D_k is the operator term v^2 dt^2 L(c_k) of the update
+                        // instead of out - 2 c + prev (equal in exact arithmetic; free of the
+                        // cancellation between time levels when the levels carry rounding noise)
 };
 #define MODE_HAS_CERJAN(f) (((f)&F_CERJAN) != 0)
 
@@ -104,6 +107,9 @@ struct AdjArgs {
     long long acc_sstride;
     int gradm;             // image += lambda (S_{t+1} - 2 S_t + S_{t-1}) / dt^2 instead of lambda S_t
     long long pkstride;    // elements between consecutive saved steps of P_
+    const float *D_;       // with gradm: D_t = S_{t+1} - 2 S_t + S_{t-1} on the interior
+                           // ([nshots][nz][nx], dsstride apart) instead of three levels of P_
+    long long dsstride;
 };
 
 struct TileShape {
@@ -150,6 +156,23 @@ int launch_residual(const float *dobs, const float *dcal, float *res, long long 
 int launch_grad_multi(int phys, const float *S, int sP, long long ss, const float *lam, int lP, long long ls,
                       int nshots, int nz, int nx, float dx, float dz, const float *eps, const float *del,
                       const float *theta, int mP, float *geps, float *gdel, float *gth, hipStream_t st);
+
+// Boundary saving: the 4-cell rims just inside the interior edge of a padded
+// field, sb_width(nz, nx) elements per step in the order [top 4][nx]
+// [bottom 4][nx] [left nz-8][4] [right nz-8][4]; kind 0 float32, 1 float16,
+// 2 bfloat16 (16-bit kinds hold v / m, m = the step's max |v| in scale[shot][k]).
+long long sb_width(int nz, int nx);
+int launch_sb_save(const float *U, long long sstride, int P, int N, int nz, int nx, int nshots, int kind,
+                   void *strips, long long strip_sstride, int k, float *scale, int nt, hipStream_t st);
+// rims of step L into U, the 4-cell band outside the interior cleared, wav[wk]
+// added at a source cell of the inner region (wk in [0, nt)); with D, the rim
+// cells of D_{L+1} re-formed from the saved rims of steps L, L+1, L+2
+int launch_sb_restore(float *U, long long sstride, int P, int N, int nz, int nx, int nshots, int kind,
+                      const void *strips, long long strip_sstride, const float *scale, int nt, int L, float *D,
+                      long long dsstride, const int *ssz, const int *ssx, const float *wav, int wk, hipStream_t st);
+// dst = src on the interior, 0 on the absorbing layer (padded fields)
+int launch_copy_interior(const float *src, float *dst, long long sstride, int nzp, int P, int N, int nz, int nx,
+                         int nshots, hipStream_t st);
 
 // Exact discrete adjoint of the acoustic forward map (pam_forward_adjoint).
 // Fields are padded [nzp][P] per shot, shots `sstride` floats apart; the saved

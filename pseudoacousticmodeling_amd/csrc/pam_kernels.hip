@@ -188,7 +188,9 @@ __global__ __launch_bounds__(256) void k_adj_update(const AdjArgs a) {
         const long long ii = (long long)(iz - a.N) * a.nx + (ix - a.N);
         const float *Ps = a.P_ + (long long)s * a.psstride + i;
         if (a.gradm) {
-            const float d2t = (Ps[a.pkstride] - 2.0f * Ps[0] + Ps[-a.pkstride]) * a.idt2;
+            const float d2t = (a.D_ ? a.D_[(long long)s * a.dsstride + ii]
+                                    : (Ps[a.pkstride] - 2.0f * Ps[0] + Ps[-a.pkstride])) *
+                              a.idt2;
             a.image[(long long)s * a.acc_sstride + ii] += d2t * lam;
         } else {
             a.image[(long long)s * a.acc_sstride + ii] += Ps[0] * lam;

@@ -460,7 +460,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((PHYS == 2)
             }
 
             // the update
-            float out[4];
+            float out[4], opv[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const float pe = (mc.pv[j] * fx[j]) * fz;
@@ -474,10 +474,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((PHYS == 2)
                         if (zs) sum += zz[j];
                     }
                     u = mc.v2[j] * sum + 2.0f * c0[j] - pe;
+                    if (flags & F_D2T_OP) opv[j] = mc.v2[j] * sum;
                 } else if (PHYS == 1) {
                     const float Sd = sd_term(mc.e[j], mc.dl[j], PX[j], PZ[j]);
                     u = 2.0f * c0[j] - pe + mc.v2[j] * ((1.0f + 2.0f * mc.e[j]) + Sd) * PXX[j] +
                         mc.v2[j] * (1.0f + Sd) * PZZ[j];
+                    if (flags & F_D2T_OP)
+                        opv[j] = mc.v2[j] * ((1.0f + 2.0f * mc.e[j]) + Sd) * PXX[j] + mc.v2[j] * (1.0f + Sd) * PZZ[j];
                 } else {
                     const float xi = PX[j] * mc.cs[j] - PZ[j] * mc.sn[j];
                     const float eta = PX[j] * mc.sn[j] + PZ[j] * mc.cs[j];
@@ -486,6 +489,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((PHYS == 2)
                     const float Bc = (1.0f + 2.0f * mc.e[j]) * (mc.sn[j] * mc.sn[j]) + (mc.cs[j] * mc.cs[j]) + Sd;
                     u = 2.0f * c0[j] - pe + mc.v2[j] * Ac * PXX[j] + mc.v2[j] * Bc * PZZ[j] -
                         2.0f * mc.e[j] * mc.v2[j] * (2.0f * mc.sn[j] * mc.cs[j]) * pxz[j];
+                    if (flags & F_D2T_OP)
+                        opv[j] = mc.v2[j] * Ac * PXX[j] + mc.v2[j] * Bc * PZZ[j] -
+                                 2.0f * mc.e[j] * mc.v2[j] * (2.0f * mc.sn[j] * mc.cs[j]) * pxz[j];
                 }
                 out[j] = (u * fx[j]) * fz;
             }
@@ -514,7 +520,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((PHYS == 2)
                 for (int j = 0; j < 4; ++j) {
                     if (!(zint && col + j >= N && col + j < nxp - N)) continue;
                     // interior cells are undamped: D_k = S_{k+1} - 2 S_k + S_{k-1}
-                    const float d = out[j] - 2.0f * c0[j] + mc.pv[j];
+                    const float d = (flags & F_D2T_OP) ? opv[j] : out[j] - 2.0f * c0[j] + mc.pv[j];
                     if ((flags & F_D2T) && (flags & F_SAVE)) snap_put(a, s, a.k, ib + col + j, 0, d);
                     tmax = fmaxf(tmax, fabsf((flags & F_D2T) ? d : out[j]));
                 }

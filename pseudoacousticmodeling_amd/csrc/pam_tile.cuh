@@ -296,7 +296,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         }
 
         // ---- the update
-        float out[4];
+        float out[4], opv[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const float pe = (pv[r][j] * fx[j]) * fz;  // U_{k-1} as the reference holds it (damped twice)
@@ -310,10 +310,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                     if (zs) sum += zz[j];
                 }
                 u = v2[r][j] * sum + 2.0f * c0[j] - pe;
+                if (flags & F_D2T_OP) opv[j] = v2[r][j] * sum;
             } else if (PHYS == 1) {
                 const float Sd = sd_term(e[r][j], dl[r][j], PX[j], PZ[j]);
                 u = 2.0f * c0[j] - pe + v2[r][j] * ((1.0f + 2.0f * e[r][j]) + Sd) * PXX[j] +
                     v2[r][j] * (1.0f + Sd) * PZZ[j];
+                if (flags & F_D2T_OP)
+                    opv[j] = v2[r][j] * ((1.0f + 2.0f * e[r][j]) + Sd) * PXX[j] + v2[r][j] * (1.0f + Sd) * PZZ[j];
             } else {
                 const float xi = PX[j] * cs[r][j] - PZ[j] * sn[r][j];
                 const float eta = PX[j] * sn[r][j] + PZ[j] * cs[r][j];
@@ -322,6 +325,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 const float Bc = (1.0f + 2.0f * e[r][j]) * (sn[r][j] * sn[r][j]) + (cs[r][j] * cs[r][j]) + Sd;
                 u = 2.0f * c0[j] - pe + v2[r][j] * Ac * PXX[j] + v2[r][j] * Bc * PZZ[j] -
                     2.0f * e[r][j] * v2[r][j] * (2.0f * sn[r][j] * cs[r][j]) * pxz[j];
+                if (flags & F_D2T_OP)
+                    opv[j] = v2[r][j] * Ac * PXX[j] + v2[r][j] * Bc * PZZ[j] -
+                             2.0f * e[r][j] * v2[r][j] * (2.0f * sn[r][j] * cs[r][j]) * pxz[j];
             }
             out[j] = (u * fx[j]) * fz;
         }
@@ -352,7 +358,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
             for (int j = 0; j < 4; ++j) {
                 if (!(zint && col + j >= N && col + j < nxp - N)) continue;
                 // interior cells are undamped: D_k = S_{k+1} - 2 S_k + S_{k-1}
-                const float d = out[j] - 2.0f * c0[j] + pv[r][j];
+                const float d = (flags & F_D2T_OP) ? opv[j] : out[j] - 2.0f * c0[j] + pv[r][j];
                 if ((flags & F_D2T) && (flags & F_SAVE)) snap_put(a, s, a.k, ib + col + j, 0, d);
                 tmax = fmaxf(tmax, fabsf((flags & F_D2T) ? d : out[j]));
             }

@@ -14,8 +14,11 @@ each raw float32 [nz][nx], normalised by ilum + 1e-5 max(ilum)
 
 Source-field strategies (Parameters.json "migration"): "onthefly" keeps every
 step in HBM (float32, or float16 / bfloat16 with "storage"), "checkpoint"
-keeps uniform checkpoints every "step" steps in HBM and recomputes segments.
-"SB" (saving boundaries) and "RBC" (random boundaries) are not provided.
+keeps uniform checkpoints every "step" steps in HBM and recomputes segments,
+"SB" keeps the 4-cell rims inside the interior edge of every step (float32,
+or float16 / bfloat16 with "storage") and rebuilds the source field backwards
+in time (src/Migration.py:1266-1410).  "RBC" (random boundaries) is not
+provided.
 """
 from __future__ import annotations
 
@@ -27,6 +30,8 @@ from . import _lib, prep
 
 STORAGE = {"float32": _lib.PAM_STORE_ALL_F32, "float16": _lib.PAM_STORE_ALL_F16,
            "bfloat16": _lib.PAM_STORE_ALL_BF16}
+SB_STORAGE = {"float32": _lib.PAM_STORE_BOUNDARY_F32, "float16": _lib.PAM_STORE_BOUNDARY_F16,
+              "bfloat16": _lib.PAM_STORE_BOUNDARY_BF16}
 
 
 class migration:
@@ -82,9 +87,11 @@ class migration:
         elif p.migration == "checkpoint":
             eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_CHECKPOINT)
             eng.set_option(_lib.PAM_OPT_CKPT_STEP, int(p.step) if p.step else 0)
+        elif p.migration == "SB":
+            eng.set_option(_lib.PAM_OPT_STORAGE, SB_STORAGE[getattr(p, "storage", "float32")])
         else:
             raise NotImplementedError(f"migration strategy {p.migration!r} is not provided "
-                                      "(use 'onthefly' or 'checkpoint')")
+                                      "(use 'onthefly', 'checkpoint' or 'SB')")
 
     def data(self):
         """All shots' adjoint sources on the host: the muted gathers (RTM) or

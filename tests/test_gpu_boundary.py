@@ -1,0 +1,154 @@
+"""GPU parity of the boundary-saving source-field strategy (PAM_STORE_BOUNDARY_*)
+through the C ABI: the RTM image and FWI gradient (and partials) with the
+source field rebuilt backwards in time from saved rims, against the oracle's
+restatement of the same strategy (float32, rims kept the same way), for
+float32, float16 and bfloat16 rims, for the pressure-form adjoint and the
+linearised VTI / TTI adjoint.
+
+Tolerances.  Time reversal in float32 cannot undo the forward pass's
+rounding, so any float32 rebuild carries noise that the second time
+derivative of the gradient amplifies (the spatial operator's gain on that
+noise over its gain on the wavefield); two float32 rebuilds with different
+rounding (FMA on the GPU, none in the oracle) differ by a few 1e-4 in the
+gradient while the image stays within 1e-4.  Quantised rims round nearly
+equal values of the two implementations differently where they sit at a
+rounding boundary.  So each case has a tolerance against the same-strategy
+oracle (below), and every case must also be at least as close to the FP64
+restatement of the strategy as the float32 restatement is (the GPU rebuild is
+as accurate as the algorithm allows in float32).  Distances to the GPU's
+save-every-step result are printed: they measure the strategy's own error
+(rims quantisation, zero source field on the absorbing layer for the
+linearised adjoint)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle.oracle import rel_l2
+from pseudoacousticmodeling_amd import _lib
+from tests.problems import make_problem, make_engine
+from tests.test_gpu_adjoint import _data
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+# against the same-strategy float32 oracle: (rims, image / gradient)
+TOLS = {("f32", False): 1e-4, ("f32", True): 1e-3, ("f16", False): 1e-3, ("f16", True): 5e-3,
+        ("bf16", False): 5e-3, ("bf16", True): 5e-2}
+KINDS = {"f32": _lib.PAM_STORE_BOUNDARY_F32, "f16": _lib.PAM_STORE_BOUNDARY_F16,
+         "bf16": _lib.PAM_STORE_BOUNDARY_BF16}
+
+
+def _oracle(pb, data, fwi, rim_kind, kind="f32", multiparameter=False, t_range="py", storage="boundary"):
+    orc = O.Oracle(kind)
+    g = pb["g"]
+    model = tuple(orc.expand(m, g.N) if m is not None else None for m in (pb["vp"], pb["eps"], pb["dl"], pb["th"]))
+    A = orc.cerjan(g.N) if pb["abc"] == 0 else None
+    cpml = (pb["d0"], pb["fp"]) if pb["abc"] == 1 else None
+    shots = list(zip(pb["sz"].tolist(), pb["sx"].tolist()))
+    interior = (pb["vp"], pb["eps"], pb["dl"], pb["th"])
+    return O.rtm_onthefly(orc, pb["phys"], pb["abc"], g, model, pb["src"], shots, pb["rz"], pb["rx"], data, A=A,
+                          cpml=cpml, fwi=fwi, multiparameter=multiparameter, model_interior=interior,
+                          storage=storage, rim_kind=rim_kind, t_range=t_range,
+                          normalize=2 if t_range == "cpp" else 1)
+
+
+def _gpu(eng, data, grad, multi=False):
+    if grad:
+        out = eng.gradient(data, multiparameter=multi)
+        return {k: v.cpu().numpy() for k, v in out.items() if v is not None}
+    img, il = eng.rtm(data)
+    return {"image": img.cpu().numpy(), "ilum": il.cpu().numpy()}
+
+
+CASES = [
+    # (phys, abc, grad, rim kind)
+    (0, 0, False, "f32"), (0, 0, True, "f32"), (0, 1, True, "f32"), (0, 1, False, "f16"),
+    (0, 0, True, "bf16"), (1, 1, True, "f16"), (2, 1, False, "bf16"),
+    # linearised adjoint (Cerjan VTI / TTI): S_t padded, zero on the absorbing layer
+    (1, 0, False, "f32"), (1, 0, True, "f32"), (2, 0, True, "bf16"),
+]
+
+
+@pytest.mark.parametrize("phys,abc,grad,rim", CASES)
+def test_boundary_saving_matches_oracle(phys, abc, grad, rim):
+    print(f"case boundary phys={phys} abc={abc} grad={grad} rims={rim}", flush=True)
+    pb = make_problem(phys, abc, nshots=2, seed=400 + 2 * phys + abc, nt=240)
+    data = _data(pb, 2, seed=40)
+    eng = make_engine(pb, max_shots=2)
+    key = "g_m" if grad else "image"
+    ref_all = _gpu(eng, data, grad)[key]
+    eng.set_option(_lib.PAM_OPT_STORAGE, KINDS[rim])
+    got = _gpu(eng, data, grad)
+    ref = _oracle(pb, data, grad, rim)
+    o64 = _oracle(pb, data, grad, rim, kind="f64")
+    e = rel_l2(got[key], ref["image"])
+    e64, eo64 = rel_l2(got[key], o64["image"]), rel_l2(ref["image"], o64["image"])
+    print(f"  rel_l2 vs float32 oracle (same strategy) {e:.3e}; vs FP64 oracle (same strategy) {e64:.3e} "
+          f"(float32 oracle vs FP64: {eo64:.3e}); strategy error vs GPU save-all {rel_l2(got[key], ref_all):.3e}",
+          flush=True)
+    assert np.isfinite(got[key]).all() and np.abs(got[key]).max() > 0
+    assert rel_l2(got["ilum"], ref["ilum"]) <= TOL
+    assert e <= TOLS[(rim, grad)]
+    assert e64 <= 2.0 * eo64 + 1e-5
+
+
+@pytest.mark.parametrize("phys", [1, 2])
+def test_boundary_saving_partials_match_oracle(phys):
+    print(f"case boundary partials phys={phys}", flush=True)
+    pb = make_problem(phys, 1, nshots=2, seed=420 + phys, nt=240)
+    data = _data(pb, 2, seed=42)
+    eng = make_engine(pb, max_shots=2)
+    eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_BOUNDARY_F32)
+    got = _gpu(eng, data, True, multi=True)
+    ref = _oracle(pb, data, True, "f32", multiparameter=True)
+    o64 = _oracle(pb, data, True, "f32", kind="f64", multiparameter=True)
+    for k, rk in (("g_m", "image"), ("g_eps", "g_eps"), ("g_delta", "g_delta")) + (
+            (("g_theta", "g_theta"),) if phys == 2 else ()):
+        e, e64, eo64 = rel_l2(got[k], ref[rk]), rel_l2(got[k], o64[rk]), rel_l2(ref[rk], o64[rk])
+        print(f"  {k}: rel_l2 vs float32 oracle {e:.3e}, vs FP64 oracle {e64:.3e} "
+              f"(float32 oracle vs FP64 {eo64:.3e})", flush=True)
+        # g_m carries the second time derivative of the rebuilt field, the
+        # partials its second space derivatives: the same noise amplification
+        assert e <= TOLS[("f32", True)]
+        assert e64 <= 2.0 * eo64 + 1e-5
+
+
+def test_boundary_saving_cpp_range():
+    """The C++ driver's imaging range t = nt-1 .. 0 (the rebuild runs down to
+    S_0), boundary saving in float32.  The image is compared before
+    normalisation: the C++ driver's division by ilum wherever ilum > 1e-12
+    turns rounding-level differences in barely illuminated cells into
+    arbitrary ones."""
+    pb = make_problem(0, 0, nshots=1, seed=430, nt=200)
+    data = _data(pb, 1, seed=43)
+    eng = make_engine(pb, max_shots=1)
+    eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_BOUNDARY_F32)
+    eng.set_option(_lib.PAM_OPT_ADJ_RANGE, 1)
+    eng.set_option(_lib.PAM_OPT_NORMALIZE, 0)
+    got = _gpu(eng, data, False)
+    ref = _oracle(pb, data, False, "f32", t_range="cpp")
+    e = rel_l2(got["image"], ref["raw_image"])
+    print(f"  cpp range: rel_l2 {e:.3e}", flush=True)
+    assert e <= TOL
+
+
+def test_boundary_saving_batched_equals_single_shot():
+    pb = make_problem(0, 1, nshots=3, seed=440, nt=200)
+    data = _data(pb, 3, seed=44)
+    eng = make_engine(pb, max_shots=3)
+    eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_BOUNDARY_F16)
+    g3 = eng.gradient(data)["g_m"].cpu().numpy()
+    eng.set_option(_lib.PAM_OPT_BATCH, 1)
+    g1 = eng.gradient(data)["g_m"].cpu().numpy()
+    assert rel_l2(g1, g3) < 1e-6
+
+
+def test_boundary_saving_rejected_by_the_exact_adjoint():
+    pb = make_problem(0, 0, nshots=1, seed=450, nt=120)
+    eng = make_engine(pb)
+    eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_BOUNDARY_F32)
+    import torch
+
+    dseis = torch.zeros((1, pb["g"].nt - pb["g"].itlag, len(pb["rx"])), device="cuda")
+    with pytest.raises(_lib.PamError, match="PAM_E_UNSUPPORTED"):
+        eng.forward_adjoint(dseis)
