@@ -64,7 +64,10 @@ enum pam_option {
     PAM_OPT_TILE = 2,      /* step-kernel variant in [0, pam_tile_count()); -1 (default):
                               the autotuner times every variant on the first run of a
                               batch size and keeps the fastest */
-    PAM_OPT_GRAPH = 3,     /* 1: replay the step loop as captured hipGraphs (default 1) */
+    PAM_OPT_GRAPH = 3,     /* 1 (default): a whole run's step loop is captured once into a
+                              hipGraph (per batch of shots, cached while the model, geometry,
+                              options and output buffers stay the same) and replayed; 0: the
+                              steps are launched one by one */
     PAM_OPT_STORAGE = 4,   /* enum pam_storage for rtm / gradient (default ALL_F32) */
     PAM_OPT_CKPT_STEP = 5, /* checkpoint interval for PAM_STORE_CHECKPOINT */
     PAM_OPT_RESET_ADJ_CPML = 6, /* 1 (default): clear CPML memory before the
@@ -73,9 +76,12 @@ enum pam_option {
     PAM_OPT_NORMALIZE = 7, /* image/gradient normalisation: 0 none, 1 ilum+1e-5 max
                               (src/Migration.py:1086, default), 2 ilum>1e-12
                               (cpp/Migration.cu:470-484) */
-    PAM_OPT_ADJ_RANGE = 8  /* adjoint steps: 0 (default) t = nt-2 .. 1 as src/Migration.py:1039;
+    PAM_OPT_ADJ_RANGE = 8, /* adjoint steps: 0 (default) t = nt-2 .. 1 as src/Migration.py:1039;
                               1 t = nt-1 .. 0 as cpp/Migration.cu:383-390 (imaging only: the
                               gradient needs S_{t+1} and S_{t-1}) */
+    PAM_OPT_PROFILE = 9    /* 1: HIP events around every step kernel of pam_forward (the
+                              kernel time pam_last_timing reports; launches are then not
+                              graph-replayed); 0 (default): no per-launch events */
 };
 
 typedef struct pam_grid {
@@ -180,8 +186,10 @@ int pam_misfit(pam_ctx *ctx, int nshots, const float *dobs, const float *dcal, f
  * whose gradient (pam_gradient) is preconditioned and normalised. */
 int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const float *dseis, float *g_vp, int layout);
 
-/* Elapsed GPU milliseconds of the last whole-run call's step loop, and the
- * average duration of its dominant stencil kernel (both from HIP events). */
+/* Elapsed GPU milliseconds of the last whole-run call's step loop and, with
+ * PAM_OPT_PROFILE 1, the summed duration and the number of its step-kernel
+ * launches (HIP events around each); with PAM_OPT_PROFILE 0 kernel_ms is the
+ * loop time and kernel_launches the number of steps. */
 int pam_last_timing(const pam_ctx *ctx, double *loop_ms, double *kernel_ms, long long *kernel_launches);
 
 #ifdef __cplusplus

@@ -102,7 +102,15 @@ struct pam_ctx {
     std::vector<hipEvent_t> kev;
     double loop_ms = 0.0, kernel_ms = 0.0;
     long long launches = 0;
-    int profile = 1;
+    int profile = 0;
+    // instantiated step-loop graphs, keyed by what their launches bake in;
+    // `epoch` changes whenever the model, wavelet, geometry or an option does
+    std::map<std::string, hipGraphExec_t> graphs;
+    unsigned epoch = 0;
+    // graph-replayed forward runs gather into this fixed buffer ([max_shots][nt_data][nrec])
+    // and copy it out, so that one graph serves every batch of a run
+    float *stage = nullptr;
+    long long stage_bytes = 0;
     std::string err;
 };
 
@@ -271,13 +279,14 @@ extern "C" int pam_destroy(pam_ctx *ctx) {
                      ctx->fs.U1,  ctx->fs.psix, ctx->fs.psiz,  ctx->fs.zetax, ctx->fs.zetaz, ctx->tmp, ctx->save_scale,
                      ctx->aux.U0, ctx->aux.U1, ctx->aux.psix, ctx->aux.psiz, ctx->aux.zetax, ctx->aux.zetaz, ctx->ckpt,
                      ctx->acc_ilum, ctx->maxv, ctx->ilum_tmp, ctx->th_pad, ctx->acc_multi,
-                     ctx->sbd, ctx->ex_save, ctx->ex_work, ctx->ex_gacc};
+                     ctx->sbd, ctx->stage, ctx->ex_save, ctx->ex_work, ctx->ex_gacc};
     for (float *b : bufs) dfree(b);
     dfree(ctx->save);
     dfree(ctx->d_acc);
     dfree(ctx->smax);
     int *ibufs[] = {ctx->d_rz, ctx->d_rx, ctx->d_rrow, ctx->d_rhead, ctx->d_rnext, ctx->d_ssz, ctx->d_ssx};
     for (int *b : ibufs) dfree(b);
+    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
     for (auto e : ctx->kev) (void)hipEventDestroy(e);
     if (ctx->e0) (void)hipEventDestroy(ctx->e0);
     if (ctx->e1) (void)hipEventDestroy(ctx->e1);
@@ -290,6 +299,7 @@ extern "C" const char *pam_last_error(const pam_ctx *ctx) { return ctx ? ctx->er
 
 extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
     if (!ctx) return PAM_E_ARG;
+    ++ctx->epoch;
     switch (key) {
         case PAM_OPT_BATCH:
             if (v < 0 || v > ctx->max_shots) return fail(ctx, PAM_E_ARG, "batch must be in [0, max_shots]");
@@ -322,6 +332,9 @@ extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
             if (v < 0 || v > 1) return fail(ctx, PAM_E_ARG, "adjoint range must be 0 or 1");
             ctx->adj_range = (int)v;
             return PAM_OK;
+        case PAM_OPT_PROFILE:
+            ctx->profile = v ? 1 : 0;
+            return PAM_OK;
         default:
             return fail(ctx, PAM_E_ARG, "unknown option %d", key);
     }
@@ -338,6 +351,7 @@ extern "C" int pam_get_option(const pam_ctx *ctx, int key, long long *value) {
         case PAM_OPT_RESET_ADJ_CPML: *value = ctx->reset_adj_cpml; return PAM_OK;
         case PAM_OPT_NORMALIZE: *value = ctx->normalize; return PAM_OK;
         case PAM_OPT_ADJ_RANGE: *value = ctx->adj_range; return PAM_OK;
+        case PAM_OPT_PROFILE: *value = ctx->profile; return PAM_OK;
         default: return PAM_E_ARG;
     }
 }
@@ -345,6 +359,7 @@ extern "C" int pam_get_option(const pam_ctx *ctx, int key, long long *value) {
 extern "C" int pam_set_model(pam_ctx *ctx, const float *vp, const float *eps, const float *delta,
                              const float *theta, int layout) {
     if (!ctx) return PAM_E_ARG;
+    ++ctx->epoch;
     if (!vp) return fail(ctx, PAM_E_ARG, "vp is NULL");
     if (ctx->phys >= 1 && (!eps || !delta)) return fail(ctx, PAM_E_ARG, "VTI/TTI need eps and delta");
     if (ctx->phys == 2 && !theta) return fail(ctx, PAM_E_ARG, "TTI needs theta");
@@ -376,6 +391,7 @@ extern "C" int pam_set_model(pam_ctx *ctx, const float *vp, const float *eps, co
 
 extern "C" int pam_set_wavelet(pam_ctx *ctx, const float *src) {
     if (!ctx) return PAM_E_ARG;
+    ++ctx->epoch;
     if (!src) return fail(ctx, PAM_E_ARG, "wavelet is NULL");
     HCK(hipMemcpyAsync(ctx->wav, src, (size_t)ctx->g.nt * sizeof(float), hipMemcpyDeviceToDevice, ctx->st));
     std::vector<float> h(ctx->g.nt);
@@ -390,6 +406,7 @@ extern "C" int pam_set_wavelet(pam_ctx *ctx, const float *src) {
 extern "C" int pam_set_geometry(pam_ctx *ctx, const int *sz, const int *sx, int nshot, const int *rz, const int *rx,
                                 int nrec) {
     if (!ctx) return PAM_E_ARG;
+    ++ctx->epoch;
     if (nshot < 0 || nrec < 0) return fail(ctx, PAM_E_ARG, "negative counts");
     if ((nshot && (!sz || !sx)) || (nrec && (!rz || !rx))) return fail(ctx, PAM_E_ARG, "NULL geometry array");
     // sources and receivers must lie where the stencil updates the field: the
@@ -716,6 +733,18 @@ extern "C" int pam_get_field(pam_ctx *ctx, int which, float *out) {
 }
 
 // ------------------------------------------------------------ whole runs
+// (re)allocates a scratch buffer to hold at least `need` bytes
+static int ensure_scratch(pam_ctx *ctx, void **p, long long *have, long long need) {
+    if (*have >= need && *p) return PAM_OK;
+    dfree(*p);
+    *p = nullptr;
+    *have = 0;
+    int r = dalloc(ctx, p, (size_t)need);
+    if (r) return r;
+    *have = need;
+    return PAM_OK;
+}
+
 static int batch_size(pam_ctx *ctx, int nshots) {
     const int b = ctx->batch > 0 ? ctx->batch : auto_batch(ctx);
     return std::max(1, std::min(b, std::min(nshots, ctx->max_shots)));
@@ -739,30 +768,90 @@ static int finish_timing(pam_ctx *ctx, long long nk) {
     return PAM_OK;
 }
 
+// Runs `body` (host code that enqueues the launches of a step loop on the
+// context's stream) through an instantiated hipGraph cached under `key`:
+// captured on first use, replayed afterwards.  The launches must be
+// capturable (kernels and device copies: no memset nodes, no events) and
+// `key` must name everything they bake in that can change between calls;
+// `replay_fix` applies the host-side effects of `body` when it is skipped.
+template <class Body, class Fix>
+static int run_graph(pam_ctx *ctx, const std::string &key, Body &&body, Fix &&replay_fix) {
+    auto it = ctx->graphs.find(key);
+    if (it != ctx->graphs.end()) {
+        replay_fix();
+        HCK(hipGraphLaunch(it->second, ctx->st));
+        return PAM_OK;
+    }
+    if (ctx->graphs.size() >= 16) {  // bound the cache: drop every entry
+        for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+        ctx->graphs.clear();
+    }
+    HCK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+    const int r = body();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->st, &g);
+    if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return fail(ctx, PAM_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail(ctx, PAM_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    ctx->graphs[key] = ex;
+    HCK(hipGraphLaunch(ex, ctx->st));
+    return PAM_OK;
+}
+
 // forward loop over nt steps for shots already primed in U0 (cur) / U1 (prv);
-// extra flags / args are applied to every step
+// extra flags / args are applied to every step.  With PAM_OPT_PROFILE each
+// step kernel is bracketed by HIP events; otherwise, with PAM_OPT_GRAPH, the
+// loop is replayed as a captured graph.
 static int run_forward_loop(pam_ctx *ctx, StepArgs &a, int tile, bool timed) {
-    float *cur = ctx->fs.U0, *prv = ctx->fs.U1;
     const int nt = ctx->g.nt;
-    if (timed && ctx->profile) {
+    const bool events = timed && ctx->profile;
+    if (events) {
         int r = ensure_events(ctx, 2 * (size_t)nt);
         if (r) return r;
     }
     if (timed) HCK(hipEventRecord(ctx->e0, ctx->st));
     const int base_flags = a.flags;
-    for (int k = 0; k < nt; ++k) {
-        a.flags = base_flags;
-        if (!(k >= ctx->g.itlag)) a.flags &= ~F_GATHER;
-        int r = fwd_step(ctx, a, tile, cur, prv, k, (timed && ctx->profile) ? &ctx->kev[2 * k] : nullptr);
-        if (r) return r;
-        std::swap(cur, prv);
-    }
-    if (timed) {
-        HCK(hipEventRecord(ctx->e1, ctx->st));
-        int r = finish_timing(ctx, (timed && ctx->profile) ? nt : 0);
-        if (r) return r;
+    auto body = [&]() -> int {
+        float *cur = ctx->fs.U0, *prv = ctx->fs.U1;
+        for (int k = 0; k < nt; ++k) {
+            a.flags = base_flags;
+            if (!(k >= ctx->g.itlag)) a.flags &= ~F_GATHER;
+            int r = fwd_step(ctx, a, tile, cur, prv, k, events ? &ctx->kev[2 * k] : nullptr);
+            if (r) return r;
+            std::swap(cur, prv);
+        }
+        return PAM_OK;
+    };
+    int r;
+    if (ctx->graph && !events) {
+        char key[256];
+        std::snprintf(key, sizeof key, "fwd|%u|%d|%d|%d|%p|%p", ctx->epoch, a.nshots, tile, base_flags,
+                      (void *)a.seis, (const void *)a.data);
+        r = run_graph(ctx, key, body, [&]() {
+            if (ctx->abc == PAM_CPML) ctx->fs.psi_flip ^= (nt & 1);
+        });
+    } else {
+        r = body();
     }
     a.flags = base_flags;
+    if (r) return r;
+    if (timed) {
+        HCK(hipEventRecord(ctx->e1, ctx->st));
+        r = finish_timing(ctx, events ? nt : 0);
+        if (r) return r;
+        if (!events) {
+            float ms = 0.f;
+            HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
+            ctx->kernel_ms += ms;
+            ctx->launches += nt;
+        }
+    }
     return PAM_OK;
 }
 
@@ -780,21 +869,30 @@ extern "C" int pam_forward(pam_ctx *ctx, int shot0, int nshots, float *seis) {
     const int tile = choose_tile(ctx, B, true);
     ctx->last_tile = tile;
     ctx->last_batch = B;
+    const bool staged = ctx->graph && !ctx->profile;
+    if (staged) {
+        r = ensure_scratch(ctx, (void **)&ctx->stage, &ctx->stage_bytes, (long long)B * nd * 4);
+        if (r) return r;
+    }
     for (int b0 = 0; b0 < nshots; b0 += B) {
         const int nb = std::min(B, nshots - b0);
         r = zero_shot_state(ctx, nb, true, true);
         if (r) return r;
         r = upload_sources(ctx, shot0 + b0, nb);
         if (r) return r;
-        HCK(hipMemsetAsync(seis + (long long)b0 * nd, 0, (size_t)nb * nd * sizeof(float), ctx->st));
+        float *dst = staged ? ctx->stage : seis + (long long)b0 * nd;
+        HCK(hipMemsetAsync(dst, 0, (size_t)nb * nd * sizeof(float), ctx->st));
         KCK(launch_inject_point(ctx->fs.U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, nb, ctx->st));
         StepArgs a;
         base_step_args(ctx, a, nb);
         a.flags |= F_SRC | F_GATHER;
-        a.seis = seis + (long long)b0 * nd;
+        a.seis = dst;
         a.dstride = nd;
         r = run_forward_loop(ctx, a, tile, true);
         if (r) return r;
+        if (staged)
+            HCK(hipMemcpyAsync(seis + (long long)b0 * nd, ctx->stage, (size_t)nb * nd * sizeof(float),
+                               hipMemcpyDeviceToDevice, ctx->st));
     }
     HCK(hipStreamSynchronize(ctx->st));
     return PAM_OK;
@@ -809,16 +907,6 @@ extern "C" int pam_last_timing(const pam_ctx *ctx, double *loop_ms, double *kern
 }
 
 // ------------------------------------------------------------ adjoint passes
-static int ensure_scratch(pam_ctx *ctx, void **p, long long *have, long long need) {
-    if (*have >= need && *p) return PAM_OK;
-    dfree(*p);
-    *p = nullptr;
-    *have = 0;
-    int r = dalloc(ctx, p, (size_t)need);
-    if (r) return r;
-    *have = need;
-    return PAM_OK;
-}
 
 static int alloc_fieldset(pam_ctx *ctx, FieldSet &f) {
     const size_t fb = (size_t)ctx->plane * ctx->max_shots * sizeof(float);
