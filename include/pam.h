@@ -53,7 +53,15 @@ enum pam_storage {
        float16 / bfloat16 scaled by each step's max |rim value| */
     PAM_STORE_BOUNDARY_F32 = 4,
     PAM_STORE_BOUNDARY_F16 = 5,
-    PAM_STORE_BOUNDARY_BF16 = 6
+    PAM_STORE_BOUNDARY_BF16 = 6,
+    /* binomial (revolve) checkpointing with PAM_OPT_SNAPSHOTS slots of the whole
+       state (wavefields and CPML memory): the forward pass is recomputed from
+       the slots in the order of Griewank's treeverse, no step more than r times
+       for the smallest r with C(s + r, s) >= the adjoint steps; slots in float32
+       (the recomputed fields are bit-identical to the first pass) or float16
+       scaled per field and shot.  No counterpart in the reference. */
+    PAM_STORE_REVOLVE_F32 = 7,
+    PAM_STORE_REVOLVE_F16 = 8
 };
 
 /* keys of pam_set_option */
@@ -79,9 +87,10 @@ enum pam_option {
     PAM_OPT_ADJ_RANGE = 8, /* adjoint steps: 0 (default) t = nt-2 .. 1 as src/Migration.py:1039;
                               1 t = nt-1 .. 0 as cpp/Migration.cu:383-390 (imaging only: the
                               gradient needs S_{t+1} and S_{t-1}) */
-    PAM_OPT_PROFILE = 9    /* 1: HIP events around every step kernel of pam_forward (the
+    PAM_OPT_PROFILE = 9,   /* 1: HIP events around every step kernel of pam_forward (the
                               kernel time pam_last_timing reports; launches are then not
                               graph-replayed); 0 (default): no per-launch events */
+    PAM_OPT_SNAPSHOTS = 10 /* checkpoint slots of PAM_STORE_REVOLVE_* (default 50, >= 1) */
 };
 
 typedef struct pam_grid {
@@ -189,7 +198,9 @@ int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const float *dseis,
 /* Elapsed GPU milliseconds of the last whole-run call's step loop and, with
  * PAM_OPT_PROFILE 1, the summed duration and the number of its step-kernel
  * launches (HIP events around each); with PAM_OPT_PROFILE 0 kernel_ms is the
- * loop time and kernel_launches the number of steps. */
+ * loop time and kernel_launches the number of steps.  After pam_rtm /
+ * pam_gradient, kernel_launches counts the time steps run (forward,
+ * recomputed and adjoint) over all shots' batches. */
 int pam_last_timing(const pam_ctx *ctx, double *loop_ms, double *kernel_ms, long long *kernel_launches);
 
 #ifdef __cplusplus

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -70,12 +71,15 @@ struct pam_ctx {
     int last_batch = 0;              // shots per launch of the last whole run
     // options
     int batch = 0, tile = -1, graph = 1, storage = PAM_STORE_ALL_F32, ckpt_step = 0;
-    int reset_adj_cpml = 1, normalize = 1, adj_range = 0;
+    int reset_adj_cpml = 1, normalize = 1, adj_range = 0, snapshots = 50;
+    long long steps_run = 0;  // time steps launched by the current adjoint run
     // checkpoints of the main set ([segment][field][shot][plane]) and the
     // psi_flip at each
     float *ckpt = nullptr;
     long long ckpt_bytes = 0;
     std::vector<int> ckpt_flip;
+    unsigned int *rvbits = nullptr;  // float16 slots: scale bits [slot][field][shot]
+    long long rvbits_bytes = 0;
     unsigned int *smax = nullptr;  // per-snapshot max |S_k| (compressed storage)
     long long smax_bytes = 0;
     // adjoint scratch
@@ -284,6 +288,7 @@ extern "C" int pam_destroy(pam_ctx *ctx) {
     dfree(ctx->save);
     dfree(ctx->d_acc);
     dfree(ctx->smax);
+    dfree(ctx->rvbits);
     int *ibufs[] = {ctx->d_rz, ctx->d_rx, ctx->d_rrow, ctx->d_rhead, ctx->d_rnext, ctx->d_ssz, ctx->d_ssx};
     for (int *b : ibufs) dfree(b);
     for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -314,7 +319,7 @@ extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
             ctx->graph = v ? 1 : 0;
             return PAM_OK;
         case PAM_OPT_STORAGE:
-            if (v < PAM_STORE_ALL_F32 || v > PAM_STORE_BOUNDARY_BF16) return fail(ctx, PAM_E_ARG, "unknown storage");
+            if (v < PAM_STORE_ALL_F32 || v > PAM_STORE_REVOLVE_F16) return fail(ctx, PAM_E_ARG, "unknown storage");
             ctx->storage = (int)v;
             return PAM_OK;
         case PAM_OPT_CKPT_STEP:
@@ -335,6 +340,10 @@ extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
         case PAM_OPT_PROFILE:
             ctx->profile = v ? 1 : 0;
             return PAM_OK;
+        case PAM_OPT_SNAPSHOTS:
+            if (v < 1 || v > 1000000) return fail(ctx, PAM_E_ARG, "snapshots must be in [1, 1e6]");
+            ctx->snapshots = (int)v;
+            return PAM_OK;
         default:
             return fail(ctx, PAM_E_ARG, "unknown option %d", key);
     }
@@ -352,6 +361,7 @@ extern "C" int pam_get_option(const pam_ctx *ctx, int key, long long *value) {
         case PAM_OPT_NORMALIZE: *value = ctx->normalize; return PAM_OK;
         case PAM_OPT_ADJ_RANGE: *value = ctx->adj_range; return PAM_OK;
         case PAM_OPT_PROFILE: *value = ctx->profile; return PAM_OK;
+        case PAM_OPT_SNAPSHOTS: *value = ctx->snapshots; return PAM_OK;
         default: return PAM_E_ARG;
     }
 }
@@ -589,6 +599,7 @@ static int fwd_step_fs(pam_ctx *ctx, FieldSet &f, StepArgs &a, int tile, float *
     if (ev) HCK(hipEventRecord(ev[0], ctx->st));
     KCK(launch_step(ctx->phys, ctx->abc == PAM_CPML, tile, a, ctx->st));
     if (ev) HCK(hipEventRecord(ev[1], ctx->st));
+    ++ctx->steps_run;
     if (ctx->abc == PAM_CPML) f.psi_flip ^= 1;
     return PAM_OK;
 }
@@ -964,6 +975,26 @@ static int restore_ckpt(pam_ctx *ctx, int j, int nb) {
     return PAM_OK;
 }
 
+// Binomial checkpointing (Griewank's treeverse): beta(s, r) = C(s + r, s) is
+// the longest chain that s slots reverse with no step advanced more than r
+// times.
+static double revolve_beta(int s, int r) {
+    double b = 1.0;
+    for (int i = 1; i <= r; ++i) b = b * (s + i) / i;
+    return b;
+}
+// First checkpoint of an n-step range (n >= 2) with s >= 1 free slots: for
+// the smallest r with beta(s, r) >= n, the offset m <= beta(s, r - 1) (the left
+// part is then reversed with r - 1 more sweeps) with n - m <= beta(s - 1, r)
+// (the right part with one slot fewer), which beta(s, r) = beta(s, r - 1) +
+// beta(s - 1, r) guarantees.
+static int revolve_split(int n, int s) {
+    int r = 1;
+    while (revolve_beta(s, r) < n) ++r;
+    const double m = std::min(revolve_beta(s, r - 1), (double)(n - 1));
+    return std::max(1, (int)m);
+}
+
 // RTM (imaging) or FWI gradient (F_GRADM) over shots.  The source field is
 // kept by one of the strategies of enum pam_storage:
 //  - every step in float32 (src/Migration.py:996-1037);
@@ -978,7 +1009,11 @@ static int restore_ckpt(pam_ctx *ctx, int j, int nb) {
 //    backwards in time on the second field set one step ahead of the adjoint
 //    pass (src/Migration.py:1266-1410; the reference's time alignment is
 //    re-derived: S_{k-1} = v^2 dt^2 L(S_k) + 2 S_k - S_{k+1} + wav[k+1] at the
-//    source, rims written back, zero on the absorbing layer).
+//    source, rims written back, zero on the absorbing layer);
+//  - binomial checkpointing: the whole state in a fixed number of slots
+//    (float32 or float16), the forward pass recomputed on the second field set
+//    in treeverse order; each adjoint step's S_t (and D_t) comes from one
+//    recomputed forward step.
 static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, bool grad, float *image, float *ilum,
                        float *g_eps = nullptr, float *g_del = nullptr, float *g_th = nullptr) {
     int r = ready(ctx);
@@ -994,7 +1029,10 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     const int storage = ctx->storage;
     const bool ckpt = storage == PAM_STORE_CHECKPOINT;
     const bool half = storage == PAM_STORE_ALL_F16 || storage == PAM_STORE_ALL_BF16;
-    const bool sb = storage >= PAM_STORE_BOUNDARY_F32;
+    const bool sb = storage >= PAM_STORE_BOUNDARY_F32 && storage <= PAM_STORE_BOUNDARY_BF16;
+    const bool rv = storage == PAM_STORE_REVOLVE_F32 || storage == PAM_STORE_REVOLVE_F16;
+    const bool rv_half = storage == PAM_STORE_REVOLVE_F16;
+    const bool cpml = ctx->abc == PAM_CPML;
     const int sb_kind = sb ? storage - PAM_STORE_BOUNDARY_F32 : 0;  // 0 float32, 1 float16, 2 bfloat16
     if (sb && (g.nz < 9 || g.nx < 9 || g.n_abc < 4))
         return fail(ctx, PAM_E_ARG, "boundary saving needs nz, nx >= 9 and n_abc >= 4");
@@ -1006,6 +1044,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     // buffer (gradient) or S_t from the rebuilt padded field (imaging)
     const int skind = sb ? (grad ? SV_F32_INT : SV_F32_PAD)
                          : linear_adj ? SV_F32_PAD
+                         : rv ? SV_F32_INT
                                       : (storage == PAM_STORE_ALL_F16
                                              ? SV_F16_INT
                                              : (storage == PAM_STORE_ALL_BF16 ? SV_BF16_INT : SV_F32_INT));
@@ -1017,13 +1056,14 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         return fail(ctx, PAM_E_ARG, "the FWI gradient needs the adjoint range nt-2 .. 1 (PAM_OPT_ADJ_RANGE 0)");
     const int t_hi = ctx->adj_range ? nt - 1 : nt - 2, t_lo = ctx->adj_range ? 0 : 1;
     const int nseg = ckpt ? t_hi / C + 1 : 1;  // segments of backward steps
-    const int nbuf = ckpt ? C + 2 : nt;            // snapshots held at once
+    const int nbuf = rv ? 1 : ckpt ? C + 2 : nt;   // snapshots held at once
+    const int nslot = rv ? std::max(1, std::min(ctx->snapshots, t_hi - t_lo + 1)) : 0;  // revolve slots
     const bool multi = grad && (g_eps || g_del || g_th);
     const int nmulti = (ctx->phys == 2) ? 3 : 2;
     // the gradient needs S only through S_{t+1} - 2 S_t + S_{t-1}: store that
     // difference (formed by the forward step in float32, the same arithmetic the
     // adjoint step would do) unless the partials or the linearised adjoint need S
-    const bool d2t = grad && (sb || (!multi && !linear_adj));
+    const bool d2t = grad && (sb || rv || (!multi && !linear_adj));
     // choose the batch so that the snapshots (and checkpoints) fit in free memory
     size_t freeb = 0, totalb = 0;
     HCK(hipMemGetInfo(&freeb, &totalb));
@@ -1031,11 +1071,13 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     const long long per_shot_bytes = per_step * nbuf * elem + 2 * nint * 4 + (linear_adj ? 5 * ctx->plane * 4 : 0) +
                                      (multi ? nmulti * nint * 4 : 0) + (half ? (long long)nt * 4 : 0) +
                                      (ckpt ? (long long)nseg * ckpt_fields(ctx) * ctx->plane * 4 : 0) +
-                                     (sb ? (long long)nt * 4 + nint * 4 : 0);
+                                     (sb ? (long long)nt * 4 + nint * 4 : 0) +
+                                     (rv ? (long long)nslot * ckpt_fields(ctx) * ctx->plane * (rv_half ? 2 : 4) +
+                                               nint * 4 : 0);
     const long long have = (long long)ctx->save_bytes + ctx->acc_bytes + ctx->tmp_bytes + ctx->acc_multi_bytes +
-                           ctx->ckpt_bytes + ctx->smax_bytes + ctx->sbd_bytes;
+                           ctx->ckpt_bytes + ctx->smax_bytes + ctx->sbd_bytes + ctx->rvbits_bytes;
     long long budget = (long long)((freeb + have) * 0.85);
-    if ((ckpt || sb) && !ctx->aux.U0)
+    if ((ckpt || sb || rv) && !ctx->aux.U0)
         budget -= (long long)ctx->plane * ctx->max_shots * 4 * (ctx->abc == PAM_CPML ? 8 : 2);
     while (B > 1 && (long long)B * per_shot_bytes > budget) --B;
     if ((long long)B * per_shot_bytes > budget)
@@ -1056,6 +1098,22 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     if (sb) {
         r = ensure_scratch(ctx, (void **)&ctx->sbd, &ctx->sbd_bytes, (long long)B * nint * 4);
         if (r) return r;
+        r = alloc_fieldset(ctx, ctx->aux);
+        if (r) return r;
+    }
+    if (rv) {
+        r = ensure_scratch(ctx, (void **)&ctx->ckpt, &ctx->ckpt_bytes,
+                           (long long)nslot * ckpt_fields(ctx) * ctx->plane * B * (rv_half ? 2 : 4));
+        if (r) return r;
+        if (grad) {
+            r = ensure_scratch(ctx, (void **)&ctx->sbd, &ctx->sbd_bytes, (long long)B * nint * 4);
+            if (r) return r;
+        }
+        if (rv_half) {
+            r = ensure_scratch(ctx, (void **)&ctx->rvbits, &ctx->rvbits_bytes,
+                               (long long)nslot * ckpt_fields(ctx) * B * 4);
+            if (r) return r;
+        }
         r = alloc_fieldset(ctx, ctx->aux);
         if (r) return r;
     }
@@ -1086,7 +1144,8 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     ctx->launches = 0;
     ctx->next_k = -1;
     int tile0 = choose_tile(ctx, B, true);
-    if ((half || sb) && tile_shape(tile0).V > 0) tile0 = kDefaultTile;  // k_step has no fused max / D_t
+    ctx->steps_run = 0;
+    if ((half || sb || rv) && tile_shape(tile0).V > 0) tile0 = kDefaultTile;  // k_step has no fused max / D_t
     ctx->last_tile = tile0;
     ctx->last_batch = B;
     const long long pint = (long long)g.n_abc * ctx->P + g.n_abc;  // interior origin in a padded field
@@ -1124,7 +1183,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         a.ilum = ctx->acc_ilum;
         a.image = ctx->acc_img;
         a.acc_sstride = nint;
-        {
+        if (!rv) {
             float *cur = ctx->fs.U0, *prv = ctx->fs.U1;
             // boundary saving: rims of S_k into step k of the strip buffer
             auto sb_save = [&](const float *U, int k) -> int {
@@ -1221,6 +1280,10 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
             b.save = grad ? (void *)ctx->sbd : nullptr;
             b.save_kstride = b.save_sstride = grad ? nint : ctx->plane;
         }
+        if (rv && grad) {
+            b.save = ctx->sbd;
+            b.save_kstride = b.save_sstride = nint;
+        }
         b.save_scale = reinterpret_cast<const float *>(ctx->smax);
         b.image = ctx->acc_img;
         b.acc_sstride = nint;
@@ -1265,62 +1328,216 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
             c.gradm = grad ? 1 : 0;
             c.pkstride = per_step;
         }
-        for (int seg = nseg - 1; seg >= 0; --seg) {
-            int hi = t0, lo = t_lo;
-            if (ckpt) {
-                lo = std::max(t_lo, seg * C);
-                hi = std::min(t0, (seg + 1) * C - 1);
-                if (lo > hi) continue;
-                // recompute the source field of steps k0 .. hi+1 from checkpoint seg
-                k0 = std::max(0, seg * C - 1);
-                const int k1 = std::min(nt - 1, hi + 1);
-                r = restore_ckpt(ctx, seg, nb);
-                if (r) return r;
-                StepArgs f = a;
-                f.flags = (a.flags & (F_CERJAN | F_SRC | F_D2T)) | F_SAVE;
-                f.save_k0 = k0;
-                float *fc = ctx->aux.U0, *fp = ctx->aux.U1;
-                for (int k = k0; k <= k1; ++k) {
-                    r = fwd_step_fs(ctx, ctx->aux, f, tile0, fc, fp, k, nullptr);
-                    if (r) return r;
-                    std::swap(fc, fp);
-                }
-            }
-            b.save_k0 = k0;
-            for (int t = hi; t >= lo; --t) {
+        // the adjoint step at t, with S_t / D_t in place (snapshot slot t - k0,
+        // the rebuilt field, or the D buffer)
+        auto adj_at = [&](int t) -> int {
+            if (!linear_adj) {
+                int rr = fwd_step(ctx, b, tile0, cur, prv, t, nullptr);
+                if (rr) return rr;
+            } else {
+                c.cur = cur;
+                c.prv = prv;
+                c.k = t;
+                c.P_ = (const float *)ctx->save + (long long)(t - k0) * per_step;
                 if (sb) {
-                    // rebuild S_t (imaging: into rb) or S_{t-1} and D_t (gradient: S_t in ra)
-                    while (lev > t) {
-                        r = rev_step(false);
-                        if (r) return r;
-                    }
-                    if (grad) {
-                        r = rev_step(true);
-                        if (r) return r;
-                    } else {
-                        b.save = rb;
-                    }
-                    b.save_k0 = t;
+                    c.P_ = grad ? ra : rb;
+                    c.psstride = ctx->plane;
+                    c.D_ = grad ? ctx->sbd : nullptr;
+                    c.dsstride = nint;
                 }
-                if (!linear_adj) {
-                    r = fwd_step(ctx, b, tile0, cur, prv, t, nullptr);
+                if (rv) {
+                    c.D_ = grad ? ctx->sbd : nullptr;
+                    c.dsstride = nint;
+                }
+                KCK(launch_adj_products(ctx->phys, c, ctx->st));
+                KCK(launch_adj_update(ctx->phys, c, ctx->st));
+                ++ctx->steps_run;
+            }
+            if (multi) KCK(partials(t, cur));
+            std::swap(cur, prv);
+            return PAM_OK;
+        };
+        if (rv) {
+            // ---- binomial checkpointing on the second field set (the working set)
+            FieldSet &w = ctx->aux;
+            const int nf = ckpt_fields(ctx);
+            const long long fl = ctx->plane * nb;  // floats per field of a slot
+            const int es = rv_half ? 2 : 4;
+            float *wc = w.U0, *wp = w.U1;
+            int ws = 0;         // step whose state the working set holds
+            bool first = true;  // the first descent: illuminate every step once
+            std::vector<int> slot_of(nt + 1, -1), flip_of(nslot, 0), pool;
+            for (int i = nslot - 1; i >= 0; --i) pool.push_back(i);
+            r = zero_fields(ctx, w, nb, true, true);
+            if (r) return r;
+            KCK(launch_inject_point(w.U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, nb, ctx->st));
+            StepArgs f = a;
+            const int fbase = (a.flags & F_CERJAN) | F_SRC;
+            f.save_kind = skind;
+            f.save = ctx->save;
+            f.save_kstride = f.save_sstride = per_step;
+            f.dsave = ctx->sbd;
+            f.dsave_sstride = nint;
+            auto wstep = [&](int k, int extra) -> int {
+                f.flags = fbase | extra | (first ? F_ILUM : 0);
+                f.save_k0 = k;
+                int rr = fwd_step_fs(ctx, w, f, tile0, wc, wp, k, nullptr);
+                if (rr) return rr;
+                std::swap(wc, wp);
+                ws = k + 1;
+                return PAM_OK;
+            };
+            // the state of the working set: both wavefields, and for CPML the
+            // previous step's psi and zeta
+            auto fields = [&](float **pf) {
+                pf[0] = wc;
+                pf[1] = wp;
+                if (cpml) {
+                    const long long in = w.psi_flip ? ctx->psi_half : 0;
+                    pf[2] = w.psix + in;
+                    pf[3] = w.psiz + in;
+                    pf[4] = w.zetax;
+                    pf[5] = w.zetaz;
+                }
+            };
+            auto store = [&](int k) -> int {
+                if (pool.empty()) return fail(ctx, PAM_E_STATE, "revolve: no free checkpoint slot");
+                const int sl = pool.back();
+                pool.pop_back();
+                slot_of[k] = sl;
+                flip_of[sl] = w.psi_flip;
+                float *pf[6];
+                fields(pf);
+                for (int i = 0; i < nf; ++i) {
+                    char *dst = (char *)ctx->ckpt + ((long long)sl * nf + i) * fl * es;
+                    if (rv_half)
+                        KCK(launch_pack_half(pf[i], ctx->plane, ctx->plane, nb, dst, ctx->plane,
+                                             ctx->rvbits + ((long long)sl * nf + i) * B, ctx->st));
+                    else
+                        HCK(hipMemcpyAsync(dst, pf[i], (size_t)fl * 4, hipMemcpyDeviceToDevice, ctx->st));
+                }
+                return PAM_OK;
+            };
+            auto load = [&](int k) -> int {
+                const int sl = slot_of[k];
+                if (sl < 0) return fail(ctx, PAM_E_STATE, "revolve: step %d has no checkpoint", k);
+                wc = w.U0;
+                wp = w.U1;
+                w.psi_flip = flip_of[sl];
+                float *pf[6];
+                fields(pf);
+                for (int i = 0; i < nf; ++i) {
+                    const char *src = (const char *)ctx->ckpt + ((long long)sl * nf + i) * fl * es;
+                    if (rv_half)
+                        KCK(launch_unpack_half(src, ctx->plane, ctx->plane, nb,
+                                               ctx->rvbits + ((long long)sl * nf + i) * B, pf[i], ctx->plane,
+                                               ctx->st));
+                    else
+                        HCK(hipMemcpyAsync(pf[i], src, (size_t)fl * 4, hipMemcpyDeviceToDevice, ctx->st));
+                }
+                ws = k;
+                return PAM_OK;
+            };
+            auto advance = [&](int from, int to) -> int {
+                if (ws != from) {
+                    int rr = load(from);
+                    if (rr) return rr;
+                }
+                for (int k = from; k < to; ++k) {
+                    int rr = wstep(k, 0);
+                    if (rr) return rr;
+                }
+                return PAM_OK;
+            };
+            // the adjoint step t: one forward step from state t saves S_t (and D_t)
+            auto leaf = [&](int t) -> int {
+                int rr = advance(t, t);
+                if (rr) return rr;
+                rr = wstep(t, F_SAVE | (grad ? F_DSAVE : 0));
+                if (rr) return rr;
+                if (first) {  // the steps past the adjoint range, for the illumination
+                    for (int k = t + 1; k < nt; ++k) {
+                        rr = wstep(k, 0);
+                        if (rr) return rr;
+                    }
+                    first = false;
+                }
+                k0 = t;
+                b.save_k0 = t;
+                return adj_at(t);
+            };
+            // adjoint steps hi-1 .. lo with a slot at lo and s more free slots
+            std::function<int(int, int, int)> rev = [&](int lo, int hi, int s) -> int {
+                const int n = hi - lo;
+                int rr;
+                if (n == 1) return leaf(lo);
+                if (s == 0) {
+                    for (int t = hi - 1; t >= lo; --t) {
+                        rr = advance(lo, t);
+                        if (rr) return rr;
+                        rr = leaf(t);
+                        if (rr) return rr;
+                    }
+                    return PAM_OK;
+                }
+                const int m = lo + revolve_split(n, s);
+                rr = advance(lo, m);
+                if (rr) return rr;
+                rr = store(m);
+                if (rr) return rr;
+                rr = rev(m, hi, s - 1);
+                if (rr) return rr;
+                pool.push_back(slot_of[m]);
+                slot_of[m] = -1;
+                return rev(lo, m, s);
+            };
+            r = advance(0, t_lo);
+            if (r) return r;
+            r = store(t_lo);
+            if (r) return r;
+            r = rev(t_lo, t_hi + 1, nslot - 1);
+            if (r) return r;
+        } else {
+            for (int seg = nseg - 1; seg >= 0; --seg) {
+                int hi = t0, lo = t_lo;
+                if (ckpt) {
+                    lo = std::max(t_lo, seg * C);
+                    hi = std::min(t0, (seg + 1) * C - 1);
+                    if (lo > hi) continue;
+                    // recompute the source field of steps k0 .. hi+1 from checkpoint seg
+                    k0 = std::max(0, seg * C - 1);
+                    const int k1 = std::min(nt - 1, hi + 1);
+                    r = restore_ckpt(ctx, seg, nb);
                     if (r) return r;
-                } else {
-                    c.cur = cur;
-                    c.prv = prv;
-                    c.k = t;
-                    c.P_ = (const float *)ctx->save + (long long)(t - k0) * per_step;
-                    if (sb) {
-                        c.P_ = grad ? ra : rb;
-                        c.psstride = ctx->plane;
-                        c.D_ = grad ? ctx->sbd : nullptr;
-                        c.dsstride = nint;
+                    StepArgs f = a;
+                    f.flags = (a.flags & (F_CERJAN | F_SRC | F_D2T)) | F_SAVE;
+                    f.save_k0 = k0;
+                    float *fc = ctx->aux.U0, *fp = ctx->aux.U1;
+                    for (int k = k0; k <= k1; ++k) {
+                        r = fwd_step_fs(ctx, ctx->aux, f, tile0, fc, fp, k, nullptr);
+                        if (r) return r;
+                        std::swap(fc, fp);
                     }
-                    KCK(launch_adj_products(ctx->phys, c, ctx->st));
-                    KCK(launch_adj_update(ctx->phys, c, ctx->st));
                 }
-                if (multi) KCK(partials(t, cur));
-                std::swap(cur, prv);
+                b.save_k0 = k0;
+                for (int t = hi; t >= lo; --t) {
+                    if (sb) {
+                        // rebuild S_t (imaging: into rb) or S_{t-1} and D_t (gradient: S_t in ra)
+                        while (lev > t) {
+                            r = rev_step(false);
+                            if (r) return r;
+                        }
+                        if (grad) {
+                            r = rev_step(true);
+                            if (r) return r;
+                        } else {
+                            b.save = rb;
+                        }
+                        b.save_k0 = t;
+                    }
+                    r = adj_at(t);
+                    if (r) return r;
+                }
             }
         }
         KCK(launch_reduce_shots(ctx->acc_img, nint, nb, (int)nint, image, 1, ctx->st));
@@ -1344,6 +1561,8 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     float ms = 0.f;
     HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
     ctx->loop_ms = ms;
+    ctx->kernel_ms = ms;
+    ctx->launches = ctx->steps_run;
     HCK(hipStreamSynchronize(ctx->st));
     return PAM_OK;
 }

@@ -23,6 +23,8 @@ enum : int {
     F_D2T_OP = 2048,    // with F_D2T | F_SAVE: D_k is the operator term v^2 dt^2 L(c_k) of the update
                         // instead of out - 2 c + prev (equal in exact arithmetic; free of the
                         // cancellation between time levels when the levels carry rounding noise)
+    F_DSAVE = 4096,     // D_k = S_{k+1} - 2 S_k + S_{k-1} of the interior into dsave (float32), besides
+                        // any F_SAVE snapshot of the input field
 };
 #define MODE_HAS_CERJAN(f) (((f)&F_CERJAN) != 0)
 
@@ -71,6 +73,8 @@ struct StepArgs {
     const float *save_scale;   // [nshots][nt] max |S_k| over the interior (compressed kinds)
     unsigned int *smax;        // F_SMAX target, [nshots][nt] float bits (atomicMax)
     const float *save_prev, *save_next;  // (unused; kept for ABI stability)
+    float *dsave;              // F_DSAVE target [nshots][nz][nx], dsave_sstride apart
+    long long dsave_sstride;
     // interior accumulators, per shot [nshots][nz][nx]
     float *ilum, *image;
     long long acc_sstride;
@@ -173,6 +177,13 @@ int launch_sb_restore(float *U, long long sstride, int P, int N, int nz, int nx,
 // dst = src on the interior, 0 on the absorbing layer (padded fields)
 int launch_copy_interior(const float *src, float *dst, long long sstride, int nzp, int P, int N, int nz, int nx,
                          int nshots, hipStream_t st);
+
+// float16 copy of n floats per shot (shots sstride / qsstride apart), scaled
+// by each shot's max |x| (float bits into bits[shot]); and back
+int launch_pack_half(const float *x, long long sstride, long long n, int nshots, void *q, long long qsstride,
+                     unsigned int *bits, hipStream_t st);
+int launch_unpack_half(const void *q, long long qsstride, long long n, int nshots, const unsigned int *bits, float *x,
+                       long long sstride, hipStream_t st);
 
 // Exact discrete adjoint of the acoustic forward map (pam_forward_adjoint).
 // Fields are padded [nzp][P] per shot, shots `sstride` floats apart; the saved

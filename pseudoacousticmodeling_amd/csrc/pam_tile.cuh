@@ -351,7 +351,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 }
             }
         }
-        if ((flags & F_SMAX) || ((flags & F_D2T) && (flags & F_SAVE))) {
+        if ((flags & F_SMAX) || ((flags & F_D2T) && (flags & F_SAVE)) || (flags & F_DSAVE)) {
             const bool zint = z >= N && z < nzp - N;
             const long long ib = (long long)(z - N) * a.nx - N;
 #pragma unroll
@@ -360,6 +360,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 // interior cells are undamped: D_k = S_{k+1} - 2 S_k + S_{k-1}
                 const float d = (flags & F_D2T_OP) ? opv[j] : out[j] - 2.0f * c0[j] + pv[r][j];
                 if ((flags & F_D2T) && (flags & F_SAVE)) snap_put(a, s, a.k, ib + col + j, 0, d);
+                if (flags & F_DSAVE) a.dsave[(long long)s * a.dsave_sstride + ib + col + j] = d;
                 tmax = fmaxf(tmax, fabsf((flags & F_D2T) ? d : out[j]));
             }
         }

@@ -1,0 +1,96 @@
+"""Binomial (revolve) checkpointing of the source field (PAM_STORE_REVOLVE_*):
+with float32 slots the recomputed forward pass is the first pass bit for
+bit, so the RTM image, FWI gradient and partials must equal the
+save-every-step result exactly, for any number of slots (one slot falls back
+to recomputing from the start of the range); the time steps run must respect
+the treeverse bound (no step recomputed more than r times, r the smallest
+with C(s + r, s) >= the adjoint steps); float16 slots (scaled per field and
+shot) must stay within the tolerance stated below of the float32 result."""
+from math import comb
+
+import numpy as np
+import pytest
+
+from oracle.oracle import rel_l2
+from pseudoacousticmodeling_amd import _lib
+from tests.problems import make_problem, make_engine
+from tests.test_gpu_adjoint import _data, _oracle_adjoint
+
+pytestmark = pytest.mark.gpu
+
+# float16 slots: the restored state carries 2^-12 relative rounding (of each
+# field's max), which the step's spatial operator amplifies in D_t (gradient);
+# the image reads S_t itself
+TOL_F16_IMAGE = 2e-3
+TOL_F16_GRAD = 5e-2
+
+
+def _run(eng, data, grad, multi=False):
+    if grad:
+        out = eng.gradient(data, multiparameter=multi)
+        return {k: v.cpu().numpy() for k, v in out.items() if v is not None}
+    img, il = eng.rtm(data)
+    return {"image": img.cpu().numpy(), "ilum": il.cpu().numpy()}
+
+
+def _reps(n, s):
+    r = 0
+    while comb(s + r, s) < n:
+        r += 1
+    return r
+
+
+@pytest.mark.parametrize("phys,abc", [(0, 0), (0, 1), (1, 0), (1, 1), (2, 1)])
+@pytest.mark.parametrize("grad", [False, True])
+def test_revolve_is_bit_identical_to_save_all(phys, abc, grad):
+    print(f"case revolve phys={phys} abc={abc} grad={grad}", flush=True)
+    nt = 200
+    pb = make_problem(phys, abc, nshots=2, seed=600 + 2 * phys + abc, nt=nt)
+    data = _data(pb, 2, seed=60)
+    eng = make_engine(pb, max_shots=2)
+    multi = grad and phys >= 1
+    ref = _run(eng, data, grad, multi)
+    eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_REVOLVE_F32)
+    n_adj = nt - 2
+    for slots in (5, 50):
+        eng.set_option(_lib.PAM_OPT_SNAPSHOTS, slots)
+        got = _run(eng, data, grad, multi)
+        steps = eng.timing()["kernel_launches"]
+        r = _reps(n_adj, slots - 1)
+        print(f"  slots {slots}: {steps} steps for 2 shots in one batch (nt={nt}, r={r})", flush=True)
+        # first pass nt steps, recomputation at most r - 1 more per adjoint step... bounded by r
+        # sweeps over the range, plus one forward step and one adjoint step per adjoint step
+        assert steps <= nt + r * n_adj + 2 * n_adj
+        for k in ref:
+            assert np.array_equal(got[k], ref[k]), f"{k} differs ({slots} slots)"
+
+
+def test_revolve_single_slot_recomputes_from_the_start():
+    pb = make_problem(0, 1, nshots=1, seed=620, nt=60)
+    data = _data(pb, 1, seed=62)
+    eng = make_engine(pb)
+    ref = _run(eng, data, True)
+    eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_REVOLVE_F32)
+    eng.set_option(_lib.PAM_OPT_SNAPSHOTS, 1)
+    got = _run(eng, data, True)
+    assert np.array_equal(got["g_m"], ref["g_m"])
+
+
+@pytest.mark.parametrize("phys,abc", [(0, 1), (1, 1)])
+@pytest.mark.parametrize("grad", [False, True])
+def test_revolve_float16_slots(phys, abc, grad):
+    print(f"case revolve f16 phys={phys} abc={abc} grad={grad}", flush=True)
+    pb = make_problem(phys, abc, nshots=2, seed=630 + 2 * phys + abc, nt=240)
+    data = _data(pb, 2, seed=63)
+    eng = make_engine(pb, max_shots=2)
+    key = "g_m" if grad else "image"
+    ref = _run(eng, data, grad)[key]
+    o64 = _oracle_adjoint(pb, data, fwi=grad, kind="f64")["image"]
+    eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_REVOLVE_F16)
+    eng.set_option(_lib.PAM_OPT_SNAPSHOTS, 8)
+    got = _run(eng, data, grad)[key]
+    e32 = rel_l2(got, ref)
+    print(f"  float16 slots: rel_l2 vs float32 storage {e32:.3e}, vs FP64 oracle {rel_l2(got, o64):.3e} "
+          f"(float32 storage vs oracle {rel_l2(ref, o64):.3e})", flush=True)
+    assert np.isfinite(got).all()
+    assert e32 <= (TOL_F16_GRAD if grad else TOL_F16_IMAGE)
