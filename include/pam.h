@@ -203,6 +203,27 @@ int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const float *dseis,
  * recomputed and adjoint) over all shots' batches. */
 int pam_last_timing(const pam_ctx *ctx, double *loop_ms, double *kernel_ms, long long *kernel_launches);
 
+/* Groups of grids (several models of different sizes, each with its own
+ * shots), advanced together in one kernel launch per time step: the batched,
+ * uneven-grid counterpart of running the reference's forward driver
+ * (cpp/Modeling.cu:433-479, src/Modeling2D.py:423-467) once per model.  The
+ * members share physics, absorbing boundary and nt; each keeps its own grid,
+ * model, wavelet and geometry.  A group call primes every member on its own
+ * stream, runs the steps on member 0's stream and returns when they are done.
+ * Errors are reported through pam_last_error(ctxs[0]). */
+typedef struct pam_group pam_group;
+int pam_group_create(pam_ctx *const *ctxs, int n, pam_group **out);
+int pam_group_destroy(pam_group *grp);
+/* Forward modeling of shots [0, nshots[i]) of every member i (nshots[i] <=
+ * its max_shots) into seis[i] (device [nshots[i]][nt_data_i][nrec_i]); the
+ * step loop is graph-replayed when member 0 has PAM_OPT_GRAPH set; the z-march
+ * variant is member 0's PAM_OPT_TILE when that names one, else timed on the
+ * first call for the shot layout.  loop_ms (may be NULL) receives the step
+ * loop's GPU time. */
+int pam_group_forward(pam_group *grp, const int *nshots, float *const *seis, double *loop_ms);
+/* the step-kernel variant of the last pam_group_forward (-1 before) */
+int pam_group_get_tile(const pam_group *grp);
+
 #ifdef __cplusplus
 }
 #endif

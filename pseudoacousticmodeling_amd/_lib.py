@@ -32,7 +32,7 @@ EXPORTS = [
     "pam_tile_count",
     "pam_set_model", "pam_set_wavelet", "pam_set_geometry", "pam_reset", "pam_forward_step",
     "pam_get_field", "pam_forward", "pam_rtm", "pam_gradient", "pam_misfit", "pam_forward_adjoint",
-    "pam_last_timing",
+    "pam_last_timing", "pam_group_create", "pam_group_destroy", "pam_group_forward", "pam_group_get_tile",
 ]
 
 
@@ -83,6 +83,10 @@ def load():
     lib.pam_misfit.argtypes = [vp, I, P, P, P, ctypes.POINTER(D)]
     lib.pam_forward_adjoint.argtypes = [vp, I, I, P, P, I]
     lib.pam_last_timing.argtypes = [vp, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(LL)]
+    lib.pam_group_create.argtypes = [ctypes.POINTER(vp), I, ctypes.POINTER(vp)]
+    lib.pam_group_destroy.argtypes = [vp]
+    lib.pam_group_forward.argtypes = [vp, ctypes.POINTER(I), ctypes.POINTER(vp), ctypes.POINTER(D)]
+    lib.pam_group_get_tile.argtypes = [vp]
     _lib = lib
     return lib
 

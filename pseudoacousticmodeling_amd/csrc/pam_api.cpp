@@ -909,6 +909,180 @@ extern "C" int pam_forward(pam_ctx *ctx, int shot0, int nshots, float *seis) {
     return PAM_OK;
 }
 
+// ------------------------------------------------------------ groups of grids
+// Several contexts (models of different sizes, each with its own shots)
+// advanced together: one z-march launch per time step covers every shot of
+// every grid, blocks assigned to grids through a prefix table.
+struct pam_group {
+    std::vector<pam_ctx *> ctx;
+    StepArgs *d_list = nullptr;
+    int *d_boff = nullptr;
+    std::map<std::string, int> tuned;  // z-march variant per shot layout
+    int last_tile = -1;
+};
+
+extern "C" int pam_group_create(pam_ctx *const *ctxs, int n, pam_group **out) {
+    if (!ctxs || n < 1 || !out) return PAM_E_ARG;
+    pam_ctx *c0 = ctxs[0];
+    if (!c0) return PAM_E_ARG;
+    for (int i = 0; i < n; ++i) {
+        const pam_ctx *c = ctxs[i];
+        if (!c) return fail(c0, PAM_E_ARG, "group member %d is NULL", i);
+        if (c->phys != c0->phys || c->abc != c0->abc)
+            return fail(c0, PAM_E_ARG, "group members must share physics and absorbing boundary (member %d)", i);
+        if (c->g.nt != c0->g.nt) return fail(c0, PAM_E_ARG, "group members must share nt (member %d)", i);
+    }
+    pam_group *g = new pam_group();
+    g->ctx.assign(ctxs, ctxs + n);
+    if (hipMalloc((void **)&g->d_list, sizeof(StepArgs) * n) != hipSuccess ||
+        hipMalloc((void **)&g->d_boff, sizeof(int) * (n + 1)) != hipSuccess) {
+        if (g->d_list) (void)hipFree(g->d_list);
+        delete g;
+        return fail(c0, PAM_E_NOMEM, "group argument tables");
+    }
+    *out = g;
+    return PAM_OK;
+}
+
+extern "C" int pam_group_destroy(pam_group *g) {
+    if (!g) return PAM_OK;
+    if (!g->ctx.empty() && g->ctx[0]->st) (void)hipStreamSynchronize(g->ctx[0]->st);
+    dfree(g->d_list);
+    dfree(g->d_boff);
+    delete g;
+    return PAM_OK;
+}
+
+extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *seis, double *loop_ms) {
+    if (!g || !nshots || !seis) return PAM_E_ARG;
+    pam_ctx *c0 = g->ctx[0];
+    pam_ctx *const ctx = c0;  // errors of the group go to member 0
+    const int n = (int)g->ctx.size();
+    const int nt = c0->g.nt;
+    std::vector<StepArgs> list(n);
+    std::vector<int> boff(n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        pam_ctx *c = g->ctx[i];
+        int r = ready(c);
+        if (r) return fail(c0, r, "group member %d: %s", i, c->err.c_str());
+        if (nshots[i] < 1 || nshots[i] > c->max_shots || nshots[i] > c->nshot)
+            return fail(c0, PAM_E_ARG, "group member %d: nshots %d outside [1, min(max_shots, shots)]", i, nshots[i]);
+        if (!seis[i]) return fail(c0, PAM_E_ARG, "group member %d: seis is NULL", i);
+    }
+    // the z-march variant: member 0's PAM_OPT_TILE when it names one, else timed
+    std::string layout;
+    for (int i = 0; i < n; ++i) layout += std::to_string(nshots[i]) + ",";
+    int tile = (c0->tile >= 0 && march_chunks(c0->tile) > 0) ? c0->tile : -1;
+    auto build = [&](int t) -> int {
+        const int ch = march_chunks(t);
+        for (int i = 0; i < n; ++i) {
+            pam_ctx *c = g->ctx[i];
+            const int nb = nshots[i];
+            const long long nd = (long long)(c->g.nt - c->g.itlag) * c->nrec;
+            StepArgs &a = list[i];
+            base_step_args(c, a, nb);
+            a.flags |= F_SRC | F_GATHER;
+            a.seis = seis[i];
+            a.dstride = nd;
+            a.cur = c->fs.U0;
+            a.prv = c->fs.U1;
+            if (c->abc == PAM_CPML) {
+                a.psix_in = c->fs.psix;
+                a.psiz_in = c->fs.psiz;
+                a.psix = c->fs.psix + c->psi_half;
+                a.psiz = c->fs.psiz + c->psi_half;
+            }
+            a.nxt = (a.nxp + 63) / 64;
+            a.nzt = (a.nzp - 8 + ch * 16 - 1) / (ch * 16);
+            boff[i + 1] = boff[i] + a.nxt * a.nzt * nb;
+        }
+        HCK(hipMemcpyAsync(g->d_list, list.data(), sizeof(StepArgs) * n, hipMemcpyHostToDevice, c0->st));
+        HCK(hipMemcpyAsync(g->d_boff, boff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c0->st));
+        HCK(hipStreamSynchronize(c0->st));  // the host tables are reused
+        return PAM_OK;
+    };
+    // prime every member's shots: cleared fields and memory, sources, S_0
+    auto prime = [&]() -> int {
+        for (int i = 0; i < n; ++i) {
+            pam_ctx *c = g->ctx[i];
+            int r = zero_shot_state(c, nshots[i], true, true);
+            if (r) return r;
+            r = upload_sources(c, 0, nshots[i]);
+            if (r) return r;
+            KCK(launch_inject_point(c->fs.U0, c->plane, c->d_ssz, c->d_ssx, c->P, c->wav, 0, nshots[i], c->st));
+        }
+        // the group launches on member 0's stream
+        for (int i = 0; i < n; ++i) HCK(hipStreamSynchronize(g->ctx[i]->st));
+        return PAM_OK;
+    };
+    if (tile < 0) {
+        auto it = g->tuned.find(layout);
+        if (it != g->tuned.end()) {
+            tile = it->second;
+        } else {  // time 6 steps of each z-march variant on cleared fields
+            double best = 1e300;
+            tile = kDefaultTile;
+            for (int t = 0; t < tile_count(); ++t) {
+                if (march_chunks(t) == 0) continue;
+                int r = prime();
+                if (r) return r;
+                r = build(t);
+                if (r) return r;
+                HCK(hipEventRecord(c0->e0, c0->st));
+                for (int k = 0; k < 6; ++k)
+                    KCK(launch_step_multi(c0->phys, c0->abc == PAM_CPML, t, g->d_list, g->d_boff, n, boff[n], k,
+                                          c0->st));
+                HCK(hipEventRecord(c0->e1, c0->st));
+                HCK(hipEventSynchronize(c0->e1));
+                float ms = 0.f;
+                HCK(hipEventElapsedTime(&ms, c0->e0, c0->e1));
+                if (ms < best) {
+                    best = ms;
+                    tile = t;
+                }
+            }
+            g->tuned[layout] = tile;
+        }
+    }
+    g->last_tile = tile;
+    int r = prime();
+    if (r) return r;
+    for (int i = 0; i < n; ++i) {
+        pam_ctx *c = g->ctx[i];
+        const long long nd = (long long)(c->g.nt - c->g.itlag) * c->nrec;
+        HCK(hipMemsetAsync(seis[i], 0, (size_t)nshots[i] * nd * sizeof(float), c0->st));
+    }
+    r = build(tile);
+    if (r) return r;
+    HCK(hipEventRecord(c0->e0, c0->st));
+    auto body = [&]() -> int {
+        for (int k = 0; k < nt; ++k)
+            KCK(launch_step_multi(c0->phys, c0->abc == PAM_CPML, tile, g->d_list, g->d_boff, n, boff[n], k, c0->st));
+        return PAM_OK;
+    };
+    if (c0->graph) {
+        std::string key = "grp|" + std::to_string(tile) + "|";
+        for (int i = 0; i < n; ++i) {
+            char buf[96];
+            std::snprintf(buf, sizeof buf, "%p:%u:%d:%p;", (void *)g->ctx[i], g->ctx[i]->epoch, nshots[i],
+                          (void *)seis[i]);
+            key += buf;
+        }
+        r = run_graph(c0, key, body, []() {});
+    } else {
+        r = body();
+    }
+    if (r) return r;
+    HCK(hipEventRecord(c0->e1, c0->st));
+    HCK(hipEventSynchronize(c0->e1));
+    float ms = 0.f;
+    HCK(hipEventElapsedTime(&ms, c0->e0, c0->e1));
+    if (loop_ms) *loop_ms = ms;
+    return PAM_OK;
+}
+
+extern "C" int pam_group_get_tile(const pam_group *g) { return g ? g->last_tile : -1; }
+
 extern "C" int pam_last_timing(const pam_ctx *ctx, double *loop_ms, double *kernel_ms, long long *kernel_launches) {
     if (!ctx) return PAM_E_ARG;
     if (loop_ms) *loop_ms = ctx->loop_ms;

@@ -127,6 +127,13 @@ bool tile_fuses_psi(int idx);
 // launchers (return hipError_t as int)
 int launch_step(int phys, bool cpml, int tile, const StepArgs &a, hipStream_t st);
 int launch_psi(const PsiArgs &a, hipStream_t st);
+// One time step k of a group of grids in one launch (z-march variants only):
+// list[g] (device) holds grid g's arguments as of step 0, with nxt / nzt set for
+// the variant; grid g owns blocks [boff[g], boff[g+1]) of nblocks = boff[ngrid].
+int launch_step_multi(int phys, bool cpml, int tile, const StepArgs *list, const int *boff, int ngrid, int nblocks,
+                      int k, hipStream_t st);
+// row chunks of 16 per block of a z-march variant (0 for the other families)
+int march_chunks(int tile);
 int launch_adj_products(int phys, const AdjArgs &a, hipStream_t st);
 int launch_adj_update(int phys, const AdjArgs &a, hipStream_t st);
 

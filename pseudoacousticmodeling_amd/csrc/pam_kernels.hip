@@ -436,6 +436,25 @@ int launch_step_vti_cpml(int, const StepArgs &, hipStream_t);
 int launch_step_tti_cerjan(int, const StepArgs &, hipStream_t);
 int launch_step_tti_cpml(int, const StepArgs &, hipStream_t);
 
+int launch_multi_acoustic_cerjan(int, const StepArgs *, const int *, int, int, int, hipStream_t);
+int launch_multi_acoustic_cpml(int, const StepArgs *, const int *, int, int, int, hipStream_t);
+int launch_multi_vti_cerjan(int, const StepArgs *, const int *, int, int, int, hipStream_t);
+int launch_multi_vti_cpml(int, const StepArgs *, const int *, int, int, int, hipStream_t);
+int launch_multi_tti_cerjan(int, const StepArgs *, const int *, int, int, int, hipStream_t);
+int launch_multi_tti_cpml(int, const StepArgs *, const int *, int, int, int, hipStream_t);
+int launch_step_multi(int phys, bool cpml, int tile, const StepArgs *list, const int *boff, int ngrid, int nblocks,
+                      int k, hipStream_t st) {
+    if (phys == 0)
+        return cpml ? launch_multi_acoustic_cpml(tile, list, boff, ngrid, nblocks, k, st)
+                    : launch_multi_acoustic_cerjan(tile, list, boff, ngrid, nblocks, k, st);
+    if (phys == 1)
+        return cpml ? launch_multi_vti_cpml(tile, list, boff, ngrid, nblocks, k, st)
+                    : launch_multi_vti_cerjan(tile, list, boff, ngrid, nblocks, k, st);
+    return cpml ? launch_multi_tti_cpml(tile, list, boff, ngrid, nblocks, k, st)
+                : launch_multi_tti_cerjan(tile, list, boff, ngrid, nblocks, k, st);
+}
+int march_chunks(int tile) { return tile_shape(tile).V < 0 ? tile_shape(tile).TZ : 0; }
+
 int launch_step(int phys, bool cpml, int tile, const StepArgs &a, hipStream_t st) {
     if (phys == 0) return cpml ? launch_step_acoustic_cpml(tile, a, st) : launch_step_acoustic_cerjan(tile, a, st);
     if (phys == 1) return cpml ? launch_step_vti_cpml(tile, a, st) : launch_step_vti_cerjan(tile, a, st);

@@ -71,9 +71,10 @@ __device__ __forceinline__ void store_live4(float *p, const float *v, int c, int
     }
 }
 
+// One block's work: block b (in [0, nxt * nzt * nshots)) of the launch
+// described by a.
 template <int PHYS, bool CPML, int CHUNKS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((PHYS == 2) ? 1 : 3))) void k_march(
-    const StepArgs a) {
+__device__ __forceinline__ void march_body(const StepArgs &a, const int b) {
     constexpr int TX = 64, TC = 16;
     constexpr int HX = CPML ? 8 : 4;       // x-halo of the staged rows
     constexpr int LA = CPML ? 8 : 4;       // rows staged above / below a chunk
@@ -88,13 +89,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((PHYS == 2)
     __shared__ float4 pzr[CPML ? kPsiRing * G : 1];         // new psi_z rows, own columns
     __shared__ float4 pxl[CPML ? TC * (G + 2) : 1];         // new psi_x of the chunk, columns x0-4 .. x0+TX+3
 
-    const int nseg = a.nzt;  // row segments
-    const int nb = a.nxt * nseg * a.nshots;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    if (b >= nb) return;  // block-uniform
     const int strip = b / a.nshots;
     const int s = b - strip * a.nshots;
-    const int seg = strip / a.nxt;
+    const int seg = strip / a.nxt;  // row segment
     const int txi = strip - seg * a.nxt;
     const int x0 = txi * TX;
     const int P = a.P, nzp = a.nzp, nxp = a.nxp, N = a.N;
@@ -548,6 +545,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((PHYS == 2)
     if ((flags & F_SMAX) && a.k + 1 < a.nt) block_max_to(a.smax + (long long)s * a.nt + a.k + 1, tmax);
 #undef SLOT
 #undef PSLOT
+}
+
+template <int PHYS, bool CPML, int CHUNKS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((PHYS == 2) ? 1 : 3))) void k_march(
+    const StepArgs a) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    if (b >= a.nxt * a.nzt * a.nshots) return;  // block-uniform
+    march_body<PHYS, CPML, CHUNKS>(a, b);
+}
+
+// Several grids (a group of models, each with its own shots) advanced one
+// time step in one launch: block b belongs to grid g with boff[g] <= b <
+// boff[g + 1], whose arguments are list[g] as of step 0 (its two wavefields
+// and psi halves trade places on odd steps); k is the step of every grid.
+template <int PHYS, bool CPML, int CHUNKS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((PHYS == 2) ? 1 : 3))) void k_march_multi(
+    const StepArgs *__restrict__ list, const int *__restrict__ boff, int ngrid, int k) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    if (b >= boff[ngrid]) return;  // block-uniform
+    int g = 0;
+    while (g + 1 < ngrid && b >= boff[g + 1]) ++g;
+    StepArgs a = list[g];
+    a.k = k;
+    if (k < a.itlag) a.flags &= ~F_GATHER;
+    if (k & 1) {
+        float *u0 = const_cast<float *>(a.cur);
+        a.cur = a.prv;
+        a.prv = u0;
+        float *p0 = const_cast<float *>(a.psix_in), *q0 = const_cast<float *>(a.psiz_in);
+        a.psix_in = a.psix;
+        a.psiz_in = a.psiz;
+        a.psix = p0;
+        a.psiz = q0;
+    }
+    march_body<PHYS, CPML, CHUNKS>(a, b - boff[g]);
+}
+
+template <int PHYS, bool CPML, int CHUNKS>
+static int launch_march_multi_t(const StepArgs *list, const int *boff, int ngrid, int nblocks, int k, hipStream_t st) {
+    const int blocks = (nblocks + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_march_multi<PHYS, CPML, CHUNKS>), dim3((unsigned)blocks), dim3(256), 0, st, list, boff,
+                       ngrid, k);
+    return (int)hipGetLastError();
 }
 
 template <int PHYS, bool CPML, int CHUNKS>

@@ -27,4 +27,16 @@ static int launch_step_p(int tile, const StepArgs &a, hipStream_t st) {
     }
 }
 
+// the group launch exists for the z-march variants only
+template <int PHYS, bool CPML>
+static int launch_multi_p(int tile, const StepArgs *list, const int *boff, int ngrid, int nblocks, int k,
+                          hipStream_t st) {
+    switch (tile) {
+        case 4: return launch_march_multi_t<PHYS, CPML, 7>(list, boff, ngrid, nblocks, k, st);
+        case 5: return launch_march_multi_t<PHYS, CPML, 14>(list, boff, ngrid, nblocks, k, st);
+        case 6: return launch_march_multi_t<PHYS, CPML, 28>(list, boff, ngrid, nblocks, k, st);
+        default: return (int)hipErrorInvalidValue;
+    }
+}
+
 }  // namespace pam

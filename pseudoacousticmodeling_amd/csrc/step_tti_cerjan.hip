@@ -5,4 +5,8 @@ namespace pam {
 int launch_step_tti_cerjan(int tile, const StepArgs &a, hipStream_t st) {
     return launch_step_p<2, false>(tile, a, st);
 }
+int launch_multi_tti_cerjan(int tile, const StepArgs *list, const int *boff, int ngrid, int nblocks, int k,
+                        hipStream_t st) {
+    return launch_multi_p<2, false>(tile, list, boff, ngrid, nblocks, k, st);
+}
 }  // namespace pam

@@ -168,3 +168,47 @@ class Engine:
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
         self._ck(self.lib.pam_last_timing(self.ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return dict(loop_ms=a.value, kernel_ms=b.value, kernel_launches=c.value)
+
+
+class Group:
+    """Engines of different grids (the same physics, absorbing boundary and
+    nt) advanced together, one launch per time step (pam_group_*)."""
+
+    def __init__(self, engines):
+        self.lib = L.load()
+        self.engines = list(engines)
+        n = len(self.engines)
+        arr = (ctypes.c_void_p * n)(*[e.ctx.value for e in self.engines])
+        grp = ctypes.c_void_p()
+        rc = self.lib.pam_group_create(arr, n, ctypes.byref(grp))
+        L.check(self.engines[0].ctx, rc)
+        self.grp = grp
+        self.loop_ms = 0.0
+
+    def forward(self, nshots, outs=None):
+        """Seismograms of shots [0, nshots[i]) of every engine i, as a list of
+        device tensors [nshots[i]][nt_data][nrec]."""
+        n = len(self.engines)
+        if outs is None:
+            outs = [torch.empty((int(k), e.nt_data, e.nrec), dtype=torch.float32, device=e.device)
+                    for e, k in zip(self.engines, nshots)]
+        ns = (ctypes.c_int * n)(*[int(k) for k in nshots])
+        ptrs = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outs])
+        ms = ctypes.c_double()
+        L.check(self.engines[0].ctx, self.lib.pam_group_forward(self.grp, ns, ptrs, ctypes.byref(ms)))
+        self.loop_ms = ms.value
+        return outs
+
+    def tile(self):
+        return self.lib.pam_group_get_tile(self.grp)
+
+    def close(self):
+        if getattr(self, "grp", None):
+            self.lib.pam_group_destroy(self.grp)
+            self.grp = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
