@@ -67,6 +67,46 @@ def make_engine(w, max_shots):
     return eng
 
 
+def group_leg(shots_per_model):
+    """Sixteen VTI + CPML models of different sizes (201..601 x 801..2401,
+    every pair once), ``shots_per_model`` shots each, nt = 3001 + itlag: all
+    shots of all models advanced in one launch per time step
+    (pam_group_forward), against the same models run one after another
+    (pam_forward each)."""
+    import torch
+
+    from pseudoacousticmodeling_amd import workloads
+    from pseudoacousticmodeling_amd.engine import Group
+
+    ws = workloads.uneven_vti_group(shots_per_model)
+    engs = [make_engine(w, shots_per_model) for w in ws]
+    ns = [shots_per_model] * len(engs)
+    grp = Group(engs)
+    outs = grp.forward(ns)  # tunes, captures
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grp.forward(ns, outs)
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - t0
+    finite = all(bool(torch.isfinite(o).all().item()) for o in outs)
+    for e, o in zip(engs, outs):
+        e.forward(0, shots_per_model, out=o)  # tunes, captures
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e, o in zip(engs, outs):
+        e.forward(0, shots_per_model, out=o)
+    torch.cuda.synchronize()
+    tl = time.perf_counter() - t0
+    upd = sum(workloads.updates(w, shots_per_model) for w in ws)
+    tile = grp.tile()
+    grp.close()
+    for e in engs:
+        e.close()
+    return {"unit": "Gpt-updates/s", "value": upd / tg / 1e9, "per_model_loop": upd / tl / 1e9,
+            "speedup": tl / tg, "seconds": tg, "models": len(ws), "shots_per_model": shots_per_model,
+            "nt": ws[0]["nt"], "variant": tile, "finite": finite}
+
+
 def cpu_baseline(w, seconds):
     """Time the oracle's OpenMP build (float32 storage, float64 arithmetic,
     separate psi / zeta / update passes as the reference's Numba kernels run
@@ -150,6 +190,10 @@ def main(argv=None):
     ap.add_argument("--probe", action="store_true", help="run only the device bandwidth probe")
     ap.add_argument("--no-fwi", action="store_true", help="skip the FWI-gradient legs")
     ap.add_argument("--batch", type=int, default=0, help="shots advanced per launch (0: all)")
+    ap.add_argument("--no-group", action="store_true", help="skip the uneven-grid group leg")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the steps one by one instead of replaying captured graphs")
+    ap.add_argument("--group-shots", type=int, default=4, help="shots per model in the uneven-grid group leg")
     ap.add_argument("--fwi-plain-shots", type=int, default=8,
                     help="shots timed in the per-shot gradient loop (seconds per shot are reported)")
     args = ap.parse_args(argv)
@@ -181,6 +225,8 @@ def main(argv=None):
         eng.set_option(_lib.PAM_OPT_TILE, args.tile)
     if args.batch > 0:
         eng.set_option(_lib.PAM_OPT_BATCH, args.batch)
+    if args.no_graph:
+        eng.set_option(_lib.PAM_OPT_GRAPH, 0)
     seis = torch.empty((S, w["nt"] - w["itlag"], len(w["rx"])), dtype=torch.float32, device="cuda:0")
     for _ in range(args.warmup):
         eng.forward(0, S, out=seis)
@@ -262,6 +308,12 @@ def main(argv=None):
     del seis
     torch.cuda.empty_cache()
 
+    group = None
+    if not args.no_group:
+        group = group_leg(args.group_shots)
+        print(f"bench: uneven-grid group {group['value']:.1f} Gpt-updates/s in one launch per step, "
+              f"{group['per_model_loop']:.1f} model by model", file=sys.stderr, flush=True)
+
     cpu = None
     if args.cpu_seconds > 0:
         print("bench: cpu baseline ...", file=sys.stderr, flush=True)
@@ -288,6 +340,7 @@ def main(argv=None):
                      "bytes_per_launch": bytes_launch, "avg_launch_us": avg_ms * 1e3},
         "plain_loop": plain,
         "fwi_gradient": fwi,
+        "uneven_group": group,
         "cpu_baseline": cpu,
         "finite": finite,
     }

@@ -78,6 +78,38 @@ def acoustic_cerjan_1():
                 rz=rz, rx=rx, cpml_d0=0.0, cpml_f_pico=0.0, cerjan_factor=0.0045)
 
 
+def uneven_vti_group(shots_per_model=4, nt_data=3001, seed=SEED0 + 5):
+    """Sixteen VTI models with a 50-cell CPML, nz cycling over
+    {201, 301, 401, 601} and nx over {801, 1201, 1601, 2401} (every pair once),
+    dx=dz=10 m, dt=1 ms, 3001 recorded samples, Ricker 10 Hz peak
+    (fcut = 30 Hz); vp a smooth seeded random field rescaled to 1500-4500 m/s,
+    epsilon smoothed uniform in [0, 0.25], delta in [0, min(eps, 0.1)];
+    ``shots_per_model`` surface shots spread over each model; receivers at
+    every grid column, z = 10 m.  Returns a list of workload dicts."""
+    rng = np.random.default_rng(seed)
+    h, dt, fcut, N = 10.0, 1e-3, 30.0, 50
+    tlag = 2.0 * np.sqrt(np.pi) / fcut
+    itlag = int(round(tlag / dt))
+    nt = itlag + nt_data
+    src = prep.ricker_cpp(nt, dt, fcut, np.float32(tlag), h, h)
+    d0, fp = prep.dampening_const(N, fcut)
+    out = []
+    for i, nz in enumerate((201, 301, 401, 601)):
+        for j, nx in enumerate((801, 1201, 1601, 2401)):
+            vp = _rescale(_gauss(rng.standard_normal((nz, nx)), 20.0), 1500.0, 4500.0).astype(np.float32)
+            eps = _rescale(_gauss(rng.uniform(0.0, 0.25, (nz, nx)), 8.0), 0.0, 0.25).astype(np.float32)
+            dfrac = _rescale(_gauss(rng.uniform(0.0, 1.0, (nz, nx)), 8.0), 0.0, 1.0)
+            delta = (dfrac * np.minimum(eps, 0.1)).astype(np.float32)
+            xs = (np.arange(shots_per_model) + 0.5) * (nx - 1) * h / shots_per_model
+            out.append(dict(name=f"uneven_vti_{nz}x{nx}", physics=1, abc=1, nz=nz, nx=nx, N=N, dz=h, dx=h,
+                            dt=dt, nt=nt, itlag=itlag, fcut=fcut, vp=vp, eps=eps, delta=delta, theta=None,
+                            src=src, sz=prep.grid_index(np.full(shots_per_model, 10.0), h, N),
+                            sx=prep.grid_index(xs, h, N), rx=prep.grid_index(np.arange(nx) * h, h, N),
+                            rz=prep.grid_index(np.full(nx, 10.0), h, N), cpml_d0=d0, cpml_f_pico=fp,
+                            cerjan_factor=0.0))
+    return out
+
+
 def updates(w, nshots=None):
     """Padded cells x time steps x shots."""
     ns = len(w["sx"]) if nshots is None else nshots
