@@ -27,6 +27,9 @@ enum : int {
                         // any F_SAVE snapshot of the input field
 };
 #define MODE_HAS_CERJAN(f) (((f)&F_CERJAN) != 0)
+// the flags a plain forward step uses: kernels instantiated for them alone
+// (no snapshot, imaging or gradient paths) are launched when nothing else is set
+constexpr int F_FWD_MASK = F_CERJAN | F_SRC | F_GATHER;
 
 // snapshot storage kinds
 enum : int { SV_NONE = 0, SV_F32_INT = 1, SV_F32_PAD = 2, SV_F16_INT = 3, SV_BF16_INT = 4 };

@@ -24,34 +24,57 @@ namespace pam {
 
 // ---------------------------------------------------------------- CPML psi
 // psi = a psi + b dU on the strips, from the current field (src/CudaKernels.py:92-163).
-// Threads enumerate x-strip cells (rows [4,nzp-4) x 2(N-4) columns) followed by
-// z-strip cells (2(N-4) rows x columns [4,nxp-4)).
-__global__ __launch_bounds__(256) void k_psi(const PsiArgs a) {
+// Two kinds of thread, split at a block boundary (block-uniform role):
+//  - x-strip cells, one per thread (rows [4, nzp-4), columns [4, N) and
+//    [nxp-N, nxp-4)): the x-neighbours share the cell's cache lines;
+//  - z-strip columns: a thread marches down a segment of up to kPsiSeg rows of
+//    one z-strip at one column with a 9-row register queue of U, so each row
+//    of U is loaded once per segment rather than once per neighbour.
+constexpr int kPsiSeg = 24;
+
+__global__ __launch_bounds__(256) void k_psi(const PsiArgs a, long long nx_threads, int nsegz) {
     const int s = blockIdx.y;
     const long long so = (long long)s * a.sstride;
     const int wN = a.N - 4;
-    const long long nxs = (long long)(a.nzp - 8) * 2 * wN;
-    const long long nzs = (long long)2 * wN * (a.nxp - 8);
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const float *U = a.cur + so;
-    if (t < nxs) {
+    const long long P = a.P;
+    if (t < nx_threads) {
+        const long long nxs = (long long)(a.nzp - 8) * 2 * wN;
+        if (t >= nxs) return;
         const int row = 4 + (int)(t / (2 * wN));
         const int cc = (int)(t % (2 * wN));
         const int ix = (cc < wN) ? 4 + cc : a.nxp - a.N + (cc - wN);
-        const long long i = (long long)row * a.P + ix;
+        const long long i = (long long)row * P + ix;
         const float d1 = KA1 * (U[i + 1] - U[i - 1]) + KA2 * (U[i + 2] - U[i - 2]) + KA3 * (U[i + 3] - U[i - 3]) +
                          KA4 * (U[i + 4] - U[i - 4]);
         a.psix[so + i] = a.cax[i] * a.psix_in[so + i] + a.cbx[i] * (d1 * a.idx);
-    } else if (t < nxs + nzs) {
-        const long long u = t - nxs;
-        const int rr = (int)(u / (a.nxp - 8));
-        const int ix = 4 + (int)(u % (a.nxp - 8));
-        const int iz = (rr < wN) ? 4 + rr : a.nzp - a.N + (rr - wN);
-        const long long i = (long long)iz * a.P + ix;
-        const long long P = a.P;
-        const float d1 = KA1 * (U[i + P] - U[i - P]) + KA2 * (U[i + 2 * P] - U[i - 2 * P]) +
-                         KA3 * (U[i + 3 * P] - U[i - 3 * P]) + KA4 * (U[i + 4 * P] - U[i - 4 * P]);
+        return;
+    }
+    // z-strip column segment: (side, segment, column)
+    const long long u = t - nx_threads;
+    const int ncol = a.nxp - 8;
+    const long long per_side = (long long)nsegz * ncol;
+    if (u >= 2 * per_side) return;
+    const int side = (int)(u / per_side);
+    const long long v = u - side * per_side;
+    const int seg = (int)(v / ncol);
+    const int ix = 4 + (int)(v - (long long)seg * ncol);
+    const int z0 = (side == 0 ? 4 : a.nzp - a.N) + seg * kPsiSeg;
+    const int z1 = min(z0 + kPsiSeg, side == 0 ? a.N : a.nzp - 4);  // one past the last row
+    float q[9];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) q[h] = U[(long long)(z0 - 4 + h) * P + ix];
+#pragma unroll
+    for (int r = 0; r < kPsiSeg; ++r) {
+        const int z = z0 + r;
+        if (z >= z1) break;  // thread-uniform bound of the segment
+        const long long i = (long long)z * P + ix;
+        q[8] = U[i + 4 * P];
+        const float d1 = KA1 * (q[5] - q[3]) + KA2 * (q[6] - q[2]) + KA3 * (q[7] - q[1]) + KA4 * (q[8] - q[0]);
         a.psiz[so + i] = a.caz[i] * a.psiz_in[so + i] + a.cbz[i] * (d1 * a.idz);
+#pragma unroll
+        for (int h = 0; h < 8; ++h) q[h] = q[h + 1];
     }
 }
 
@@ -462,9 +485,12 @@ int launch_step(int phys, bool cpml, int tile, const StepArgs &a, hipStream_t st
 }
 
 int launch_psi(const PsiArgs &a, hipStream_t st) {
-    const long long n = (long long)(a.nzp - 8) * 2 * (a.N - 4) + (long long)2 * (a.N - 4) * (a.nxp - 8);
-    if (n <= 0) return 0;
-    hipLaunchKernelGGL(k_psi, dim3((unsigned)((n + 255) / 256), a.nshots), dim3(256), 0, st, a);
+    if (a.N <= 4) return 0;
+    const long long nxs = (long long)(a.nzp - 8) * 2 * (a.N - 4);
+    const long long nx_threads = (nxs + 255) / 256 * 256;  // the z-columns start on a block boundary
+    const int nsegz = (a.N - 4 + kPsiSeg - 1) / kPsiSeg;
+    const long long n = nx_threads + 2LL * nsegz * (a.nxp - 8);
+    hipLaunchKernelGGL(k_psi, dim3((unsigned)((n + 255) / 256), a.nshots), dim3(256), 0, st, a, nx_threads, nsegz);
     return (int)hipGetLastError();
 }
 

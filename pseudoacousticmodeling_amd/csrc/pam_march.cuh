@@ -71,10 +71,20 @@ __device__ __forceinline__ void store_live4(float *p, const float *v, int c, int
     }
 }
 
+// The arguments of one step that change from step to step (held apart from
+// StepArgs so that a group launch can read the rest in place).
+struct MarchStep {
+    int k, flags;
+    const float *cur;
+    float *prv;
+    const float *pxi, *pzi;  // psi of the previous step
+    float *pxo, *pzo;        // new psi
+};
+
 // One block's work: block b (in [0, nxt * nzt * nshots)) of the launch
-// described by a.
-template <int PHYS, bool CPML, int CHUNKS>
-__device__ __forceinline__ void march_body(const StepArgs &a, const int b) {
+// described by a and the step arguments m.
+template <int PHYS, bool CPML, int CHUNKS, bool ADJ>
+__device__ __forceinline__ void march_body(const StepArgs &a, const int b, const MarchStep m) {
     constexpr int TX = 64, TC = 16;
     constexpr int HX = CPML ? 8 : 4;       // x-halo of the staged rows
     constexpr int LA = CPML ? 8 : 4;       // rows staged above / below a chunk
@@ -100,10 +110,10 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b) {
     if (ze > nzp - 4) ze = nzp - 4;
     const int nchunk = (ze - zb + TC - 1) / TC;
     const long long so = (long long)s * a.sstride;
-    const float *__restrict__ cur = a.cur + so;
-    float *__restrict__ prv = a.prv + so;
+    const float *__restrict__ cur = m.cur + so;
+    float *__restrict__ prv = m.prv + so;
     const int tid = threadIdx.x;
-    const int flags = a.flags;
+    const int flags = ADJ ? m.flags : (m.flags & F_FWD_MASK);  // forward-only instances drop the rest
     const int cx = tid & 15, rg = tid >> 4;
     const int col = x0 + 4 * cx;
     const bool xst = CPML && (x0 < N || x0 + TX > nxp - N);  // tile meets an x-strip (block-uniform)
@@ -149,7 +159,7 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b) {
             for (int j = 0; j < 4; ++j) {
                 if (!xstrip(c0 + j)) continue;
                 nv[j] = ax[j] * pin[j] + bx[j] * (d1[j] * a.idx);
-                if (g >= 1 && g <= G && z < ze) a.psix[so + (long long)z * P + c0 + j] = nv[j];
+                if (g >= 1 && g <= G && z < ze) m.pxo[so + (long long)z * P + c0 + j] = nv[j];
             }
         }
         pxl[r * (G + 2) + g] = make_float4(nv[0], nv[1], nv[2], nv[3]);
@@ -185,12 +195,12 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b) {
             if (zstrip(r)) {
                 const long long ro = (long long)r * P + x0 + 4 * c;
                 float pin[4], az[4], bz[4], nv[4];
-                vld<4>(pin, a.psiz_in + so + ro);
+                vld<4>(pin, m.pzi + so + ro);
                 vld<4>(az, a.caz + ro);
                 vld<4>(bz, a.cbz + ro);
                 psiz_new(nv, pin, az, bz, r, c);
                 o = make_float4(nv[0], nv[1], nv[2], nv[3]);
-                if (r >= zb) store_live4(a.psiz + so + ro, nv, x0 + 4 * c, nxp);
+                if (r >= zb) store_live4(m.pzo + so + ro, nv, x0 + 4 * c, nxp);
             }
             pzr[PSLOT(r) + c] = o;
         }
@@ -230,13 +240,13 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b) {
                     zbz[j] = zxo[j] = 0.f;
             if (zchunk && zstrip(zr)) {
                 const long long ro = (long long)zr * P + col;
-                vld<4>(pzin, a.psiz_in + so + ro);
+                vld<4>(pzin, m.pzi + so + ro);
                 vld<4>(paz, a.caz + ro);
                 vld<4>(pbz, a.cbz + ro);
             }
             if (xst && z >= 4 && z < nzp - 4) {
                 const long long ro = (long long)z * P + col;
-                vld<4>(pxin, a.psix_in + so + ro);
+                vld<4>(pxin, m.pxi + so + ro);
                 vld<4>(pax, a.cax + ro);
                 vld<4>(pbx, a.cbx + ro);
                 vld<4>(zxo, a.zetax + so + ro);
@@ -245,7 +255,7 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b) {
                 const int hz = zc + hr, hc = x0 - 4 + 4 * hg;
                 if (hz >= 4 && hz < nzp - 4 && hc >= 0 && hc < P) {
                     const long long ro = (long long)hz * P + hc;
-                    vld<4>(hxin, a.psix_in + so + ro);
+                    vld<4>(hxin, m.pxi + so + ro);
                     vld<4>(hax, a.cax + ro);
                     vld<4>(hbx, a.cbx + ro);
                 }
@@ -293,7 +303,7 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b) {
                 float nv[4];
                 psiz_new(nv, pzin, paz, pbz, zr, cx);
                 o = make_float4(nv[0], nv[1], nv[2], nv[3]);
-                if (zr < ze) store_live4(a.psiz + so + (long long)zr * P + col, nv, col, nxp);
+                if (zr < ze) store_live4(m.pzo + so + (long long)zr * P + col, nv, col, nxp);
             }
             pzr[PSLOT(zr) + cx] = o;
             sync_needed = sync_needed || zchunk;
@@ -359,7 +369,7 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b) {
             // receiver gather of the input field (damped as the reference records it)
             if ((flags & F_GATHER) && a.rrow[z] >= 0) {
                 const int slot = a.rrow[z];
-                const int it = a.k - a.itlag;
+                const int it = m.k - a.itlag;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     int rr = a.rhead[(long long)slot * P + col + j];
@@ -381,9 +391,9 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b) {
                     const long long ii = ib + c;
                     if ((flags & F_SAVE) && !(flags & F_D2T)) {
                         if (a.save_kind == SV_F32_PAD) {
-                            if (xlive[j]) snap_put(a, s, a.k, 0, ro + j, (c0[j] * fx[j]) * fz);
+                            if (xlive[j]) snap_put(a, s, m.k, 0, ro + j, (c0[j] * fx[j]) * fz);
                         } else if (cin) {
-                            snap_put(a, s, a.k, ii, 0, c0[j]);
+                            snap_put(a, s, m.k, ii, 0, c0[j]);
                         }
                     }
                     if (!cin) continue;
@@ -392,16 +402,16 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b) {
                         *acc += c0[j] * c0[j];
                     }
                     if (flags & F_IMAGE) {
-                        const float S = snap_get(a, s, a.k, ii, ro + j);
+                        const float S = snap_get(a, s, m.k, ii, ro + j);
                         float *im = a.image + (long long)s * a.acc_sstride + ii;
                         *im += S * c0[j];
                     }
                     if (flags & F_GRADM) {
-                        const float Sp = (flags & F_D2T) ? 0.0f : snap_get(a, s, a.k + 1, ii, ro + j);
-                        const float S0 = (flags & F_D2T) ? 0.0f : snap_get(a, s, a.k, ii, ro + j);
-                        const float Sm = (flags & F_D2T) ? 0.0f : snap_get(a, s, a.k - 1, ii, ro + j);
+                        const float Sp = (flags & F_D2T) ? 0.0f : snap_get(a, s, m.k + 1, ii, ro + j);
+                        const float S0 = (flags & F_D2T) ? 0.0f : snap_get(a, s, m.k, ii, ro + j);
+                        const float Sm = (flags & F_D2T) ? 0.0f : snap_get(a, s, m.k - 1, ii, ro + j);
                         const float d2t =
-                            ((flags & F_D2T) ? snap_get(a, s, a.k, ii, ro + j) : (Sp - 2.0f * S0 + Sm)) * a.idt2;
+                            ((flags & F_D2T) ? snap_get(a, s, m.k, ii, ro + j) : (Sp - 2.0f * S0 + Sm)) * a.idt2;
                         float *im = a.image + (long long)s * a.acc_sstride + ii;
                         *im += d2t * c0[j];
                     }
@@ -492,15 +502,15 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b) {
                 }
                 out[j] = (u * fx[j]) * fz;
             }
-            if ((flags & F_SRC) && z == a.ssz[s] && a.k + 1 < a.nt) {
+            if ((flags & F_SRC) && z == a.ssz[s] && m.k + 1 < a.nt) {
                 const int sx = a.ssx[s];
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    if (col + j == sx) out[j] += a.wav[a.k + 1];
+                    if (col + j == sx) out[j] += a.wav[m.k + 1];
             }
-            if ((flags & F_RECINJ) && a.rrow[z] >= 0 && a.k - 1 - a.itlag >= 0) {
+            if ((flags & F_RECINJ) && a.rrow[z] >= 0 && m.k - 1 - a.itlag >= 0) {
                 const int slot = a.rrow[z];
-                const int tn = a.k - 1 - a.itlag;
+                const int tn = m.k - 1 - a.itlag;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     int rr = a.rhead[(long long)slot * P + col + j];
@@ -518,7 +528,7 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b) {
                     if (!(zint && col + j >= N && col + j < nxp - N)) continue;
                     // interior cells are undamped: D_k = S_{k+1} - 2 S_k + S_{k-1}
                     const float d = (flags & F_D2T_OP) ? opv[j] : out[j] - 2.0f * c0[j] + mc.pv[j];
-                    if ((flags & F_D2T) && (flags & F_SAVE)) snap_put(a, s, a.k, ib + col + j, 0, d);
+                    if ((flags & F_D2T) && (flags & F_SAVE)) snap_put(a, s, m.k, ib + col + j, 0, d);
                     if (flags & F_DSAVE) a.dsave[(long long)s * a.dsave_sstride + ib + col + j] = d;
                     tmax = fmaxf(tmax, fabsf((flags & F_D2T) ? d : out[j]));
                 }
@@ -542,17 +552,18 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b) {
         mc = mn;
         __syncthreads();
     }
-    if ((flags & F_SMAX) && a.k + 1 < a.nt) block_max_to(a.smax + (long long)s * a.nt + a.k + 1, tmax);
+    if ((flags & F_SMAX) && m.k + 1 < a.nt) block_max_to(a.smax + (long long)s * a.nt + m.k + 1, tmax);
 #undef SLOT
 #undef PSLOT
 }
 
-template <int PHYS, bool CPML, int CHUNKS>
+template <int PHYS, bool CPML, int CHUNKS, bool ADJ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((PHYS == 2) ? 1 : 3))) void k_march(
     const StepArgs a) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= a.nxt * a.nzt * a.nshots) return;  // block-uniform
-    march_body<PHYS, CPML, CHUNKS>(a, b);
+    march_body<PHYS, CPML, CHUNKS, ADJ>(a, b,
+                                        MarchStep{a.k, a.flags, a.cur, a.prv, a.psix_in, a.psiz_in, a.psix, a.psiz});
 }
 
 // Several grids (a group of models, each with its own shots) advanced one
@@ -566,20 +577,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((PHYS == 2)
     if (b >= boff[ngrid]) return;  // block-uniform
     int g = 0;
     while (g + 1 < ngrid && b >= boff[g + 1]) ++g;
-    StepArgs a = list[g];
-    a.k = k;
-    if (k < a.itlag) a.flags &= ~F_GATHER;
+    const StepArgs &a = list[g];
+    MarchStep m{k, a.flags, a.cur, a.prv, a.psix_in, a.psiz_in, a.psix, a.psiz};
+    if (k < a.itlag) m.flags &= ~F_GATHER;
     if (k & 1) {
-        float *u0 = const_cast<float *>(a.cur);
-        a.cur = a.prv;
-        a.prv = u0;
-        float *p0 = const_cast<float *>(a.psix_in), *q0 = const_cast<float *>(a.psiz_in);
-        a.psix_in = a.psix;
-        a.psiz_in = a.psiz;
-        a.psix = p0;
-        a.psiz = q0;
+        m.cur = a.prv;
+        m.prv = const_cast<float *>(a.cur);
+        m.pxi = a.psix;
+        m.pzi = a.psiz;
+        m.pxo = const_cast<float *>(a.psix_in);
+        m.pzo = const_cast<float *>(a.psiz_in);
     }
-    march_body<PHYS, CPML, CHUNKS>(a, b - boff[g]);
+    march_body<PHYS, CPML, CHUNKS, false>(a, b - boff[g], m);
 }
 
 template <int PHYS, bool CPML, int CHUNKS>
@@ -597,7 +606,10 @@ static int launch_march_t(const StepArgs &a0, hipStream_t st) {
     a.nzt = (a.nzp - 8 + CHUNKS * 16 - 1) / (CHUNKS * 16);  // row segments
     long long blocks = (long long)a.nxt * a.nzt * a.nshots;
     blocks = (blocks + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_march<PHYS, CPML, CHUNKS>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+    if ((a.flags & ~F_FWD_MASK) != 0)
+        hipLaunchKernelGGL((k_march<PHYS, CPML, CHUNKS, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_march<PHYS, CPML, CHUNKS, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
     return (int)hipGetLastError();
 }
 

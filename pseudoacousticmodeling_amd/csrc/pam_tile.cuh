@@ -26,8 +26,10 @@
 
 namespace pam {
 
-template <int PHYS, bool CPML, int TZ, int WPE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_tile(const StepArgs a) {
+// forward-only instances of the 16-row tile fit 5 waves per SIMD
+template <int PHYS, bool CPML, int TZ, int WPE, bool ADJ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ == 16 && PHYS < 2) ? 5 : WPE))) void k_tile(
+    const StepArgs a) {
     constexpr int TX = 64;
     constexpr int R = TZ / 16;        // rows per thread
     constexpr int ROWS = TZ + 8;      // staged rows (4-row halo each side)
@@ -49,7 +51,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const float *__restrict__ cur = a.cur + so;
     float *__restrict__ prv = a.prv + so;
     const int tid = threadIdx.x;
-    const int flags = a.flags;
+    const int flags = ADJ ? a.flags : (a.flags & F_FWD_MASK);  // forward-only instances drop the rest
     const int cx = tid & 15, rg = tid >> 4;
     const int col = x0 + 4 * cx;   // first of this thread's 4 columns
     const int zr0 = z0 + rg * R;   // first of this thread's R rows
@@ -377,12 +379,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
 
 template <int PHYS, bool CPML, int TZ, int WPE>
 static int launch_tile_t(const StepArgs &a0, hipStream_t st) {
+    const bool adj = (a0.flags & ~F_FWD_MASK) != 0;
     StepArgs a = a0;
     a.nxt = (a.nxp + 63) / 64;
     a.nzt = (a.nzp + TZ - 1) / TZ;
     long long blocks = (long long)a.nxt * a.nzt * a.nshots;
     blocks = (blocks + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_tile<PHYS, CPML, TZ, WPE>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+    if (adj)
+        hipLaunchKernelGGL((k_tile<PHYS, CPML, TZ, WPE, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_tile<PHYS, CPML, TZ, WPE, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
     return (int)hipGetLastError();
 }
 
