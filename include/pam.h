@@ -120,6 +120,8 @@ int pam_create(const pam_grid *grid, int physics, int abc, const pam_abc_params 
                void *hip_stream, int max_shots, pam_ctx **out);
 /* Replaces Modeling::freeMemory (cpp/Modeling.cu:13-38). */
 int pam_destroy(pam_ctx *ctx);
+/* the message of the context's last failure (NULL: of the calling thread's
+ * last failed context-free call) */
 const char *pam_last_error(const pam_ctx *ctx);
 int pam_set_option(pam_ctx *ctx, int key, long long value);
 /* Current value of an option; for PAM_OPT_TILE / PAM_OPT_BATCH left on auto,
@@ -202,6 +204,34 @@ int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const float *dseis,
  * pam_gradient, kernel_launches counts the time steps run (forward,
  * recomputed and adjoint) over all shots' batches. */
 int pam_last_timing(const pam_ctx *ctx, double *loop_ms, double *kernel_ms, long long *kernel_launches);
+
+/* Preprocessing of the migration / inversion drivers, on the GPU.  These take
+ * no context: device arrays, sizes and a HIP stream (NULL: the null stream);
+ * they return when the work is done.  pam_last_error(NULL) explains a
+ * failure of the calling thread's last such call. */
+enum pam_prep_mode {
+    PAM_PREP_PY = 0, /* the Python driver's rule */
+    PAM_PREP_CPP = 1 /* the C++ driver's rule */
+};
+/* Model smoothing before migration (in / out device [nz][nx], velocity or,
+ * with parameter != 0, an anisotropy parameter smoothed as is; water_mask
+ * device [nz][nx] bytes or NULL; sigma in cells).  PAM_PREP_PY replaces
+ * smooth_model (src/utils.py:76-129): the slowness convolved with the
+ * normalised ceil(6 sigma + 1)-point 2-D Gaussian (float64 accumulation) on the
+ * cells a whole kernel fits, the border replicated from the nearest such cell,
+ * each column restored above its first cell outside the mask (PAM_E_ARG when
+ * the kernel does not fit in the model).  PAM_PREP_CPP replaces the C++
+ * driver's smoothing (cpp/Migration.cu:147-236): separable float32 passes
+ * along x then z with clamped indices, every masked cell restored. */
+int pam_smooth_model(const float *in, float *out, const unsigned char *water_mask, int nz, int nx, double sigma,
+                     int parameter, int mode, void *hip_stream);
+/* Linear top mute of shot gathers, replacing Mute (src/utils.py:30-49,
+ * float64; cpp/Migration.cu:86-115, float32): seis device
+ * [nshots][nt][nrec], in place; host coordinates in metres, rec_x / rec_z
+ * [nrec] and src_x / src_z [nshots]; zero before t1 = offset / v0 + shift, a
+ * linear ramp to t2 = t1 + window, unchanged after. */
+int pam_mute(float *seis, int nshots, int nt, int nrec, const double *rec_x, const double *rec_z, const double *src_x,
+             const double *src_z, double dt, double shift, double window, double v0, int mode, void *hip_stream);
 
 /* Groups of grids (several models of different sizes, each with its own
  * shots), advanced together in one kernel launch per time step: the batched,

@@ -22,6 +22,7 @@ PAM_STORE_REVOLVE_F32, PAM_STORE_REVOLVE_F16 = 7, 8
 PAM_OPT_BATCH, PAM_OPT_TILE, PAM_OPT_GRAPH, PAM_OPT_STORAGE = 1, 2, 3, 4
 PAM_OPT_CKPT_STEP, PAM_OPT_RESET_ADJ_CPML, PAM_OPT_NORMALIZE, PAM_OPT_ADJ_RANGE = 5, 6, 7, 8
 PAM_OPT_PROFILE, PAM_OPT_SNAPSHOTS = 9, 10
+PAM_PREP_PY, PAM_PREP_CPP = 0, 1
 
 ERRORS = {-1: "PAM_E_ARG", -2: "PAM_E_HIP", -3: "PAM_E_STATE", -4: "PAM_E_NOMEM",
           -5: "PAM_E_UNSUPPORTED", -6: "PAM_E_NONFINITE"}
@@ -33,6 +34,7 @@ EXPORTS = [
     "pam_set_model", "pam_set_wavelet", "pam_set_geometry", "pam_reset", "pam_forward_step",
     "pam_get_field", "pam_forward", "pam_rtm", "pam_gradient", "pam_misfit", "pam_forward_adjoint",
     "pam_last_timing", "pam_group_create", "pam_group_destroy", "pam_group_forward", "pam_group_get_tile",
+    "pam_smooth_model", "pam_mute",
 ]
 
 
@@ -87,6 +89,9 @@ def load():
     lib.pam_group_destroy.argtypes = [vp]
     lib.pam_group_forward.argtypes = [vp, ctypes.POINTER(I), ctypes.POINTER(vp), ctypes.POINTER(D)]
     lib.pam_group_get_tile.argtypes = [vp]
+    lib.pam_smooth_model.argtypes = [P, P, P, I, I, D, I, I, vp]
+    DP = ctypes.POINTER(D)
+    lib.pam_mute.argtypes = [P, I, I, I, DP, DP, DP, DP, D, D, D, D, I, vp]
     _lib = lib
     return lib
 

@@ -293,20 +293,6 @@ inline std::vector<float> ricker(const Survey &s) {
     return w;
 }
 
-// linear top mute of one gather [nt_data][Nrec] (cpp/Migration.cu:86-115)
-inline void mute(const Survey &s, float *seis, int shot) {
-    for (int r = 0; r < s.Nrec; ++r) {
-        const float dz = s.rec_z[r] - s.shot_z[shot], dx = s.rec_x[r] - s.shot_x[shot];
-        const float t1 = std::sqrt(dz * dz + dx * dx) / s.v0 + s.shift, t2 = t1 + s.window;
-        for (int it = 0; it < s.nt_data; ++it) {
-            const float t = it * s.dt;
-            float &v = seis[(size_t)it * s.Nrec + r];
-            if (t < t1) v = 0.0f;
-            else if (t < t2) v *= (t - t1) / (t2 - t1);
-        }
-    }
-}
-
 // water mask |f - min f| < 1e-3 (cpp/Migration.cu:131-145)
 inline std::vector<char> water_mask(const std::vector<float> &f) {
     float mn = f[0];
@@ -314,41 +300,6 @@ inline std::vector<char> water_mask(const std::vector<float> &f) {
     std::vector<char> m(f.size());
     for (size_t i = 0; i < f.size(); ++i) m[i] = std::fabs(f[i] - mn) < 1e-3f;
     return m;
-}
-
-// separable Gaussian smoothing with clamped borders of slowness (or of a
-// parameter), masked cells restored (cpp/Migration.cu:147-236)
-inline void smooth(const Survey &s, std::vector<float> &f, const std::vector<char> &mask, bool parameter) {
-    const int nx = s.nx, nz = s.nz, n = nx * nz;
-    int ks = (int)std::ceil(6.0f * s.sigma + 1.0f);
-    if (ks % 2 == 0) ks++;
-    const int half = ks / 2;
-    std::vector<float> k(ks);
-    float total = 0.0f;
-    for (int i = 0; i < ks; ++i) {
-        const int x = i - half;
-        k[i] = std::exp(-(x * x) / (2.0f * s.sigma * s.sigma));
-        total += k[i];
-    }
-    for (int i = 0; i < ks; ++i) k[i] /= total;
-    std::vector<float> old(n), tmp(n), out(n);
-    for (int i = 0; i < n; ++i) old[i] = parameter ? f[i] : 1.0f / f[i];
-    for (int z = 0; z < nz; ++z)
-        for (int x = 0; x < nx; ++x) {
-            float v = 0.0f;
-            for (int i = -half; i <= half; ++i) v += k[i + half] * old[z * nx + std::min(std::max(x + i, 0), nx - 1)];
-            tmp[z * nx + x] = v;
-        }
-    for (int z = 0; z < nz; ++z)
-        for (int x = 0; x < nx; ++x) {
-            float v = 0.0f;
-            for (int i = -half; i <= half; ++i) v += k[i + half] * tmp[std::min(std::max(z + i, 0), nz - 1) * nx + x];
-            out[z * nx + x] = v;
-        }
-    for (int i = 0; i < n; ++i) {
-        const float v = mask[i] ? old[i] : out[i];
-        f[i] = parameter ? v : 1.0f / v;
-    }
 }
 
 // --------------------------------------------------------------- device helpers
@@ -368,6 +319,35 @@ inline float *to_device(const std::vector<float> &h) {
     CLI_HIP(hipMalloc(&d, std::max<size_t>(h.size(), 1) * sizeof(float)));
     CLI_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     return d;
+}
+
+// separable Gaussian smoothing of slowness (or of a parameter), clamped
+// borders, masked cells restored: the C++ driver's rule (cpp/Migration.cu:147-236)
+// run on the GPU by pam_smooth_model
+inline void smooth(const Survey &s, std::vector<float> &f, const std::vector<char> &mask, bool parameter) {
+    const size_t n = f.size();
+    float *d_in = to_device(f), *d_out = nullptr;
+    unsigned char *d_mask = nullptr;
+    const std::vector<unsigned char> m(mask.begin(), mask.end());
+    CLI_HIP(hipMalloc(&d_out, n * sizeof(float)));
+    CLI_HIP(hipMalloc(&d_mask, n));
+    CLI_HIP(hipMemcpy(d_mask, m.data(), n, hipMemcpyHostToDevice));
+    if (pam_smooth_model(d_in, d_out, d_mask, s.nz, s.nx, s.sigma, parameter ? 1 : 0, PAM_PREP_CPP, nullptr) != PAM_OK)
+        throw std::runtime_error(std::string("pam_smooth_model: ") + pam_last_error(nullptr));
+    CLI_HIP(hipMemcpy(f.data(), d_out, n * sizeof(float), hipMemcpyDeviceToHost));
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    (void)hipFree(d_mask);
+}
+
+// linear top mute of every shot's gather, device [Nshot][nt_data][Nrec] in
+// place: the C++ driver's rule (cpp/Migration.cu:86-115) run on the GPU by pam_mute
+inline void mute_all(const Survey &s, float *d_data) {
+    const std::vector<double> rx(s.rec_x.begin(), s.rec_x.end()), rz(s.rec_z.begin(), s.rec_z.end());
+    const std::vector<double> sx(s.shot_x.begin(), s.shot_x.end()), sz(s.shot_z.begin(), s.shot_z.end());
+    if (pam_mute(d_data, s.Nshot, s.nt_data, s.Nrec, rx.data(), rz.data(), sx.data(), sz.data(), s.dt, s.shift,
+                 s.window, s.v0, PAM_PREP_CPP, nullptr) != PAM_OK)
+        throw std::runtime_error(std::string("pam_mute: ") + pam_last_error(nullptr));
 }
 
 // context with the survey's grid, physics, boundary, wavelet and geometry

@@ -4,7 +4,9 @@
 // data (cpp/Migration.cu:86-128), adjoint steps t = nt-1 .. 0, image / ilum
 // where ilum > 1e-12 (cpp/Migration.cu:470-484).  "migration": "onthefly"
 // keeps every source-field step in HBM, "checkpoint" keeps uniform
-// checkpoints every "step" steps in HBM.  Usage: run_migration [Parameters.json]
+// checkpoints every "step" steps in HBM, "SB" keeps the interior rims of every
+// step and rebuilds the source field backwards.  Smoothing and mute run on the
+// GPU.  Usage: run_migration [Parameters.json]
 // (`--check` after the path prints the parsed survey only).
 // Writes image_{approx}_Nx{nx}_Nz{nz}.bin, raw float32 [nz][nx], into imageFolder.
 #include <chrono>
@@ -65,10 +67,10 @@ int main(int argc, char **argv) {
                                                      "_Nt" + std::to_string(s.nt_data) + "_Nrec" +
                                                      std::to_string(s.Nrec) + "_fcut" + s.fcut_text() + ".bin",
                                                  nd);
-            cli::mute(s, g.data(), shot);
             std::copy(g.begin(), g.end(), data.begin() + shot * nd);
         }
         float *d_data = cli::to_device(data), *d_img = nullptr, *d_ilum = nullptr;
+        cli::mute_all(s, d_data);
         CLI_HIP(hipMalloc(&d_img, nm * sizeof(float)));
         CLI_HIP(hipMalloc(&d_ilum, nm * sizeof(float)));
         CLI_PAM(ctx, pam_rtm(ctx, 0, s.Nshot, d_data, d_img, d_ilum));

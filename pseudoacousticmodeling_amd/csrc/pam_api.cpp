@@ -300,7 +300,10 @@ extern "C" int pam_destroy(pam_ctx *ctx) {
     return PAM_OK;
 }
 
-extern "C" const char *pam_last_error(const pam_ctx *ctx) { return ctx ? ctx->err.c_str() : "NULL context"; }
+// the last failure of a context-free call on this thread
+static thread_local std::string g_free_err;
+
+extern "C" const char *pam_last_error(const pam_ctx *ctx) { return ctx ? ctx->err.c_str() : g_free_err.c_str(); }
 
 extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
     if (!ctx) return PAM_E_ARG;
@@ -906,6 +909,149 @@ extern "C" int pam_forward(pam_ctx *ctx, int shot0, int nshots, float *seis) {
                                hipMemcpyDeviceToDevice, ctx->st));
     }
     HCK(hipStreamSynchronize(ctx->st));
+    return PAM_OK;
+}
+
+// ------------------------------------------------------------ preprocessing
+static int free_fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static int free_fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_free_err = buf;
+    return code;
+}
+#define FCK(call)                                                                                          \
+    do {                                                                                                   \
+        hipError_t _e = (hipError_t)(call);                                                                \
+        if (_e != hipSuccess) return free_fail(PAM_E_HIP, "%s: %s", #call, hipGetErrorString(_e));         \
+    } while (0)
+
+// stream-ordered scratch released when the call returns
+struct FreeScratch {
+    hipStream_t st;
+    std::vector<void *> p;
+    explicit FreeScratch(hipStream_t s) : st(s) {}
+    void *get(size_t bytes) {
+        void *q = nullptr;
+        if (hipMallocAsync(&q, bytes ? bytes : 4, st) != hipSuccess) return nullptr;
+        p.push_back(q);
+        return q;
+    }
+    ~FreeScratch() {
+        for (void *q : p) (void)hipFreeAsync(q, st);
+        (void)hipStreamSynchronize(st);
+    }
+};
+
+extern "C" int pam_smooth_model(const float *in, float *out, const unsigned char *mask, int nz, int nx, double sigma,
+                                int parameter, int mode, void *hip_stream) {
+    if (!in || !out || nz < 1 || nx < 1 || !(sigma > 0.0) || (mode != PAM_PREP_PY && mode != PAM_PREP_CPP))
+        return free_fail(PAM_E_ARG, "pam_smooth_model: invalid arguments");
+    hipStream_t st = (hipStream_t)hip_stream;
+    const long long n = (long long)nz * nx;
+    FreeScratch sc(st);
+    float *s_old = (float *)sc.get(n * 4), *tmp = (float *)sc.get(n * 4);
+    if (!s_old || !tmp) return free_fail(PAM_E_NOMEM, "pam_smooth_model: scratch");
+    FCK(launch_slowness(in, s_old, n, parameter, st));
+    if (mode == PAM_PREP_PY) {
+        // the normalised 2-D Gaussian of src/utils.py:51-74
+        int ks = (int)std::ceil(6.0 * sigma + 1.0);
+        if (ks % 2 == 0) ++ks;
+        const int h = ks / 2;
+        if (nz <= 2 * h || nx <= 2 * h)
+            return free_fail(PAM_E_ARG, "pam_smooth_model: a %dx%d kernel does not fit a %dx%d model", ks, ks, nz, nx);
+        std::vector<double> val((size_t)ks * ks);
+        double vsum = 0.0;
+        for (int i = 0; i < ks; ++i)
+            for (int j = 0; j < ks; ++j) {
+                const double x = i - h, z = j - h;
+                val[(size_t)i * ks + j] = (1.0 / (2.0 * M_PI * sigma * sigma)) * std::exp(-(x * x + z * z) / (2.0 * sigma * sigma));
+                vsum += val[(size_t)i * ks + j];
+            }
+        std::vector<float> w((size_t)ks * ks);
+        double wsum = 0.0;
+        for (size_t i = 0; i < w.size(); ++i) {
+            w[i] = (float)val[i] / (float)vsum;
+            wsum += (double)w[i];
+        }
+        float *dw = (float *)sc.get(w.size() * 4);
+        int *first = (int *)sc.get((size_t)nx * 4);
+        if (!dw || !first) return free_fail(PAM_E_NOMEM, "pam_smooth_model: scratch");
+        FCK(hipMemcpyAsync(dw, w.data(), w.size() * 4, hipMemcpyHostToDevice, st));
+        FCK(launch_first_dry(mask, nz, nx, first, st));
+        FCK(launch_smooth_conv2d(s_old, nz, nx, dw, ks, wsum, tmp, st));
+        FCK(launch_smooth_finish_py(s_old, tmp, first, nz, nx, h, parameter, out, st));
+        FCK(hipStreamSynchronize(st));  // the host weights are released
+    } else {
+        // the normalised 1-D Gaussian of cpp/Migration.cu:147-169, float32
+        int ks = (int)std::ceil(6.0f * (float)sigma + 1.0f);
+        if (ks % 2 == 0) ++ks;
+        const int h = ks / 2;
+        const float sg = (float)sigma;
+        std::vector<float> w(ks);
+        float total = 0.0f;
+        for (int i = 0; i < ks; ++i) {
+            const float x = (float)(i - h);
+            const float num = -(x * x);
+            const float den = 2.0f * sg * sg;
+            w[i] = std::exp(num / den);
+            total = total + w[i];
+        }
+        for (int i = 0; i < ks; ++i) w[i] = w[i] / total;
+        float *dw = (float *)sc.get((size_t)ks * 4), *sm = (float *)sc.get(n * 4);
+        if (!dw || !sm) return free_fail(PAM_E_NOMEM, "pam_smooth_model: scratch");
+        FCK(hipMemcpyAsync(dw, w.data(), (size_t)ks * 4, hipMemcpyHostToDevice, st));
+        FCK(launch_smooth_sep(s_old, nz, nx, dw, ks, 0, tmp, st));
+        FCK(launch_smooth_sep(tmp, nz, nx, dw, ks, 1, sm, st));
+        FCK(launch_smooth_finish_cpp(s_old, sm, mask, n, parameter, out, st));
+        FCK(hipStreamSynchronize(st));
+    }
+    return PAM_OK;
+}
+
+extern "C" int pam_mute(float *seis, int nshots, int nt, int nrec, const double *rec_x, const double *rec_z,
+                        const double *src_x, const double *src_z, double dt, double shift, double window, double v0,
+                        int mode, void *hip_stream) {
+    if (!seis || nshots < 1 || nt < 1 || nrec < 1 || !rec_x || !rec_z || !src_x || !src_z || !(v0 > 0.0) ||
+        (mode != PAM_PREP_PY && mode != PAM_PREP_CPP))
+        return free_fail(PAM_E_ARG, "pam_mute: invalid arguments");
+    hipStream_t st = (hipStream_t)hip_stream;
+    const size_t m = (size_t)nshots * nrec;
+    std::vector<double> t64;
+    std::vector<float> t32;
+    if (mode == PAM_PREP_PY) {
+        t64.resize(2 * m);
+        for (int s = 0; s < nshots; ++s)
+            for (int r = 0; r < nrec; ++r) {
+                const double dz = rec_z[r] - src_z[s], dx = rec_x[r] - src_x[s];
+                const double t1 = std::sqrt(dz * dz + dx * dx) / v0 + shift;
+                t64[2 * ((size_t)s * nrec + r)] = t1;
+                t64[2 * ((size_t)s * nrec + r) + 1] = t1 + window;
+            }
+    } else {
+        t32.resize(2 * m);
+        for (int s = 0; s < nshots; ++s)
+            for (int r = 0; r < nrec; ++r) {
+                const float dz = (float)rec_z[r] - (float)src_z[s], dx = (float)rec_x[r] - (float)src_x[s];
+                const float dz2 = dz * dz, dx2 = dx * dx;
+                const float dist = std::sqrt(dz2 + dx2);
+                const float q = dist / (float)v0;
+                const float t1 = q + (float)shift;
+                t32[2 * ((size_t)s * nrec + r)] = t1;
+                t32[2 * ((size_t)s * nrec + r) + 1] = t1 + (float)window;
+            }
+    }
+    FreeScratch sc(st);
+    const size_t bytes = mode == PAM_PREP_PY ? t64.size() * 8 : t32.size() * 4;
+    void *d = sc.get(bytes);
+    if (!d) return free_fail(PAM_E_NOMEM, "pam_mute: scratch");
+    FCK(hipMemcpyAsync(d, mode == PAM_PREP_PY ? (const void *)t64.data() : (const void *)t32.data(), bytes,
+                       hipMemcpyHostToDevice, st));
+    FCK(launch_mute(seis, nshots, nt, nrec, d, dt, mode == PAM_PREP_CPP, st));
+    FCK(hipStreamSynchronize(st));
     return PAM_OK;
 }
 

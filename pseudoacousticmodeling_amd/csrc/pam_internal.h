@@ -195,6 +195,19 @@ int launch_pack_half(const float *x, long long sstride, long long n, int nshots,
 int launch_unpack_half(const void *q, long long qsstride, long long n, int nshots, const unsigned int *bits, float *x,
                        long long sstride, hipStream_t st);
 
+// preprocessing (pam_prep.hip)
+int launch_slowness(const float *f, float *s, long long n, int parameter, hipStream_t st);
+int launch_first_dry(const unsigned char *mask, int nz, int nx, int *first, hipStream_t st);
+int launch_smooth_conv2d(const float *s, int nz, int nx, const float *w, int ks, double total, float *conv,
+                         hipStream_t st);
+int launch_smooth_finish_py(const float *s_old, const float *conv, const int *first, int nz, int nx, int h,
+                            int parameter, float *out, hipStream_t st);
+int launch_smooth_sep(const float *in, int nz, int nx, const float *w, int ks, int dir, float *out, hipStream_t st);
+int launch_smooth_finish_cpp(const float *s_old, const float *sm, const unsigned char *mask, long long n,
+                             int parameter, float *out, hipStream_t st);
+// t12: [nshots][nrec][2] (t1, t2), float64 (cpp = 0) or float32 (cpp = 1)
+int launch_mute(float *seis, int nshots, int nt, int nrec, const void *t12, double dt, int cpp, hipStream_t st);
+
 // Exact discrete adjoint of the acoustic forward map (pam_forward_adjoint).
 // Fields are padded [nzp][P] per shot, shots `sstride` floats apart; the saved
 // forward state of step k (C_k = current field after injection, and for CPML

@@ -212,3 +212,41 @@ class Group:
             self.close()
         except Exception:
             pass
+
+
+def smooth_model(f, sigma, water_mask=None, parameter=False, mode=L.PAM_PREP_PY, device="cuda:0"):
+    """Model smoothing before migration on the GPU (pam_smooth_model): the
+    Python driver's rule (src/utils.py:76-129) by default, the C++ driver's
+    (cpp/Migration.cu:147-236) with ``mode=PAM_PREP_CPP``.  ``f`` is a
+    velocity (or, with ``parameter``, an anisotropy parameter) [nz][nx];
+    returns the smoothed float32 array on the host."""
+    lib = L.load()
+    dev = torch.device(device)
+    x = _f32(f, dev)
+    nz, nx = x.shape
+    out = torch.empty_like(x)
+    m = None
+    if water_mask is not None:
+        m = torch.as_tensor(np.ascontiguousarray(water_mask, dtype=np.uint8)).to(dev)
+    with torch.cuda.device(dev):
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        rc = lib.pam_smooth_model(_ptr(x), _ptr(out), _ptr(m), int(nz), int(nx), float(sigma), int(bool(parameter)),
+                                  int(mode), ctypes.c_void_p(stream))
+    L.check(None, rc)
+    return out.cpu().numpy()
+
+
+def mute(seis, rec_x, rec_z, src_x, src_z, dt, shift, window, v0, mode=L.PAM_PREP_PY):
+    """Linear top mute of device gathers ``seis`` [nshots][nt][nrec] in place
+    (pam_mute; src/utils.py:30-49 by default, cpp/Migration.cu:86-115 with
+    ``mode=PAM_PREP_CPP``); coordinates in metres, one source per shot."""
+    lib = L.load()
+    ns, nt, nr = seis.shape
+    D = ctypes.c_double
+    arr = lambda v: (D * len(v))(*[float(t) for t in v])  # noqa: E731
+    with torch.cuda.device(seis.device):
+        stream = torch.cuda.current_stream(seis.device).cuda_stream
+        rc = lib.pam_mute(_ptr(seis), int(ns), int(nt), int(nr), arr(rec_x), arr(rec_z), arr(src_x), arr(src_z),
+                          float(dt), float(shift), float(window), float(v0), int(mode), ctypes.c_void_p(stream))
+    L.check(None, rc)
+    return seis

@@ -23,7 +23,7 @@ import time
 
 import numpy as np
 
-from . import prep
+
 
 
 def _dot(a, b):
@@ -139,21 +139,23 @@ class fwi:
 
     # -- the loop (src/Inversion.py:290-411) -----------------------------------
     def solveFullWaveformInversionMonoparameter(self):
+        from .engine import smooth_model  # the smoothing runs on the GPU
+
         start = time.time()
         p = self.pmt
         mask = np.abs(self.wf.vp - np.min(self.wf.vp)) < 1e-3
-        self.vp0 = prep.smooth_model(self.wf.vp, p.sigma, mask)
+        self.vp0 = smooth_model(self.wf.vp, p.sigma, mask)
         self.vp0.astype(np.float32).tofile(f"{p.modelFolder}fwi_vp_smooth_{p.approximation}_Nx{p.nx}_Nz{p.nz}.bin")
         m = (1.0 / (self.vp0.astype(np.float64) ** 2)).astype(np.float32)
         epsilon = delta = theta = None
         if p.approximation in ("VTI", "TTI"):
-            epsilon = prep.smooth_model(self.wf.epsilon, p.sigma,
-                                        np.abs(self.wf.epsilon - np.min(self.wf.epsilon)) < 1e-3, True)
-            delta = prep.smooth_model(self.wf.delta, p.sigma, np.abs(self.wf.delta - np.min(self.wf.delta)) < 1e-3,
-                                      True)
+            epsilon = smooth_model(self.wf.epsilon, p.sigma,
+                                   np.abs(self.wf.epsilon - np.min(self.wf.epsilon)) < 1e-3, True)
+            delta = smooth_model(self.wf.delta, p.sigma, np.abs(self.wf.delta - np.min(self.wf.delta)) < 1e-3,
+                                 True)
         if p.approximation == "TTI":
-            theta = prep.smooth_model(self.wf.theta, p.sigma, np.abs(self.wf.theta - np.min(self.wf.theta)) < 1e-3,
-                                      True)
+            theta = smooth_model(self.wf.theta, p.sigma, np.abs(self.wf.theta - np.min(self.wf.theta)) < 1e-3,
+                                 True)
         self.history = []
         m_min = 1.0 / (p.vmax * p.vmax)
         m_max = 1.0 / (p.vmin * p.vmin)

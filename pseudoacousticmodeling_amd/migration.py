@@ -26,7 +26,7 @@ import time
 
 import numpy as np
 
-from . import _lib, prep
+from . import _lib
 
 STORAGE = {"float32": _lib.PAM_STORE_ALL_F32, "float16": _lib.PAM_STORE_ALL_F16,
            "bfloat16": _lib.PAM_STORE_ALL_BF16}
@@ -69,14 +69,16 @@ class migration:
             dl = getattr(self, "delta", getattr(wf, "delta", None))
             th = getattr(self, "theta", getattr(wf, "theta", None))
             return vp, eps, dl, th
+        from .engine import smooth_model  # the smoothing runs on the GPU
+
         water = np.abs(wf.vp - np.min(wf.vp)) < 1e-3
-        self.vp = prep.smooth_model(wf.vp, p.sigma, water)
+        self.vp = smooth_model(wf.vp, p.sigma, water)
         eps = dl = th = None
         if p.approximation in ("VTI", "TTI"):
-            eps = prep.smooth_model(wf.epsilon, p.sigma, np.abs(wf.epsilon - np.min(wf.epsilon)) < 1e-3, True)
-            dl = prep.smooth_model(wf.delta, p.sigma, np.abs(wf.delta - np.min(wf.delta)) < 1e-3, True)
+            eps = smooth_model(wf.epsilon, p.sigma, np.abs(wf.epsilon - np.min(wf.epsilon)) < 1e-3, True)
+            dl = smooth_model(wf.delta, p.sigma, np.abs(wf.delta - np.min(wf.delta)) < 1e-3, True)
         if p.approximation == "TTI":
-            th = prep.smooth_model(wf.theta, p.sigma, np.abs(wf.theta - np.min(wf.theta)) < 1e-3, True)
+            th = smooth_model(wf.theta, p.sigma, np.abs(wf.theta - np.min(wf.theta)) < 1e-3, True)
         self.epsilon, self.delta, self.theta = eps, dl, th
         return self.vp, eps, dl, th
 
@@ -94,16 +96,21 @@ class migration:
                                       "(use 'onthefly', 'checkpoint' or 'SB')")
 
     def data(self):
-        """All shots' adjoint sources on the host: the muted gathers (RTM) or
-        the residuals (FWI), [Nshot][nt_data][Nrec]."""
+        """All shots' adjoint sources on the GPU: the recorded gathers muted
+        there (RTM, src/utils.py:30-49) or the residuals (FWI),
+        [Nshot][nt_data][Nrec]."""
+        import torch
+
+        from .engine import mute
+
         p = self.pmt
         out = np.zeros((p.Nshot, p.nt_data, p.Nrec), dtype=np.float32)
         for s in range(p.Nshot):
-            d = self.loadSeismogram(s)
-            if not p.fwi:
-                d = prep.mute(d, p.rec_x, p.rec_z, p.shot_x[s], p.shot_z[s], p.dt, p.shift, p.window, p.v0)
-            out[s] = d
-        return out
+            out[s] = self.loadSeismogram(s)
+        d = torch.from_numpy(out).to(self.wf._eng.device if getattr(self.wf, "_eng", None) else "cuda:0")
+        if not p.fwi:
+            mute(d, p.rec_x, p.rec_z, p.shot_x[: p.Nshot], p.shot_z[: p.Nshot], p.dt, p.shift, p.window, p.v0)
+        return d
 
     def SolveBackwardWaveEquation(self, data=None):
         """RTM image (fwi false) or FWI gradient (fwi true) of every shot.

@@ -33,10 +33,22 @@ def test_library_exports_every_declared_entry_point():
 def test_version_and_null_context_errors():
     lib = _lib.load()
     assert b"gfx950" in lib.pam_version()
-    assert lib.pam_last_error(None) == b"NULL context"
     assert lib.pam_destroy(None) == 0
     assert lib.pam_forward(None, 0, 1, None) == -1
     assert lib.pam_set_option(None, 1, 1) == -1
+
+
+def test_context_free_calls_report_through_the_null_context():
+    """The preprocessing entry points take no context; their argument checks
+    run before any device call, and pam_last_error(NULL) explains them."""
+    lib = _lib.load()
+    assert lib.pam_smooth_model(None, None, None, 10, 10, 1.0, 0, 0, None) == -1
+    assert b"pam_smooth_model" in lib.pam_last_error(None)
+    D = ctypes.c_double
+    x = (D * 2)(0.0, 1.0)
+    assert lib.pam_mute(None, 1, 10, 2, x, x, x, x, 0.001, 0.0, 0.01, 0.0, 0, None) == -1
+    assert b"pam_mute" in lib.pam_last_error(None)
+    assert lib.pam_smooth_model(None, None, None, 10, 10, 1.0, 0, 7, None) == -1
 
 
 def test_derived_sizes_round_and_trunc():
