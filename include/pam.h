@@ -61,7 +61,14 @@ enum pam_storage {
        (the recomputed fields are bit-identical to the first pass) or float16
        scaled per field and shot.  No counterpart in the reference. */
     PAM_STORE_REVOLVE_F32 = 7,
-    PAM_STORE_REVOLVE_F16 = 8
+    PAM_STORE_REVOLVE_F16 = 8,
+    /* random boundary (src/Migration.py:1413-1561): the context's model is a
+       random-boundary model (its absorbing layer randomised by the caller, e.g.
+       prep.random_boundary); the forward pass runs undamped through it and the
+       backward pass rebuilds the source field by reversing the undamped steps
+       over the whole grid (no storage); the adjoint field keeps the absorbing
+       boundary */
+    PAM_STORE_RANDOM_BOUNDARY = 9
 };
 
 /* keys of pam_set_option */

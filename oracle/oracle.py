@@ -550,6 +550,45 @@ class _BoundaryRebuild:
         return self.hi, d
 
 
+class _RandomRebuild:
+    """Random-boundary source field (src/Migration.py:1413-1561): the forward
+    pass ran undamped on the whole grid, so S is rebuilt backwards in time by
+    S_{k-1} = v^2 dt^2 L(S_k) + 2 S_k - S_{k+1} (+ wav[k+1] at the source)
+    over the whole grid, from S_nt and S_{nt-1}; nothing is stored.  D_k is
+    the operator term v^2 dt^2 L(S_k) (+ wav[k+1] at an interior source)."""
+
+    def __init__(self, orc, phys, g: Grid, model, S, U_nt, src, sz, sx):
+        self.orc, self.phys, self.g, self.model = orc, phys, g, model
+        self.src, self.sz, self.sx = src, sz, sx
+        self.sl = (slice(g.N, g.N + g.nz), slice(g.N, g.N + g.nx))
+        self.hi = np.array(U_nt, copy=True)
+        self.lo = np.array(S[g.nt - 1], copy=True)
+        self.lev = g.nt - 1
+
+    def _step(self, need_d=False):
+        g, sl, lev = self.g, self.sl, self.lev
+        vp, eps, dl, th = self.model
+        Uf, Uc = self.hi, self.lo
+        d = self.orc.operator(self.phys, Uc, g, vp, eps, dl, th)[sl].copy() if need_d else None
+        self.orc.update(self.phys, Uf, Uc, g, vp, eps, dl, th)
+        if lev + 1 < g.nt:
+            Uf[self.sz, self.sx] += self.src[lev + 1]
+            iz, ix = self.sz - g.N, self.sx - g.N
+            if need_d and 0 <= iz < g.nz and 0 <= ix < g.nx:
+                d[iz, ix] += self.src[lev + 1]
+        self.hi, self.lo = Uc, Uf
+        self.lev = lev - 1
+        return d
+
+    def get(self, t, need_d):
+        while self.lev > t:
+            self._step()
+        if not need_d:
+            return self.lo, None
+        d = self._step(True)
+        return self.hi, d
+
+
 def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, data, A=None, cpml=None,
                  fwi=False, multiparameter=False, model_interior=None, reset_cpml=True, t_range="py",
                  normalize=1, storage="all", rim_kind="f32"):
@@ -571,13 +610,16 @@ def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, dat
 
     ``storage="boundary"`` takes the source field from the boundary-saving
     rebuild (src/Migration.py:1266-1410; ``_BoundaryRebuild``) with rims kept
-    as ``rim_kind``; ilum still comes from the forward pass."""
+    as ``rim_kind``; ilum still comes from the forward pass.
+    ``storage="random"`` runs the forward pass undamped (``model`` being a
+    random-boundary model) and takes the source field from the whole-grid
+    rebuild (``_RandomRebuild``); the adjoint keeps ``abc``."""
     if t_range not in ("py", "cpp") or normalize not in (1, 2):
         raise ValueError("t_range 'py'|'cpp', normalize 1|2")
     if t_range == "cpp" and fwi:
         raise ValueError("the C++ backward range applies to imaging only")
-    if storage not in ("all", "boundary"):
-        raise ValueError("storage 'all'|'boundary'")
+    if storage not in ("all", "boundary", "random"):
+        raise ValueError("storage 'all'|'boundary'|'random'")
     N = g.N
     vp, eps, dl, th = [orc.arr(m) if m is not None else None for m in model]
     image = np.zeros((g.nz, g.nx), dtype=np.float64)
@@ -590,17 +632,25 @@ def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, dat
     A = orc.arr(A)
     sl = (slice(N, N + g.nz), slice(N, N + g.nx))
     for s, (sz, sx) in enumerate(shots):
-        fw = orc.forward_shot(phys, abc, g, (vp, eps, dl, th), src, sz, sx, rz, rx, A=A, cpml=cpml, save=True)
+        if storage == "random":  # undamped: unit Cerjan factors, no CPML
+            fw = orc.forward_shot(phys, 0, g, (vp, eps, dl, th), src, sz, sx, rz, rx, A=orc.arr(np.ones(max(N, 1))),
+                                  cpml=None, save=True)
+        else:
+            fw = orc.forward_shot(phys, abc, g, (vp, eps, dl, th), src, sz, sx, rz, rx, A=A, cpml=cpml, save=True)
         S = fw["save"]
         ilum_p = np.zeros((g.nz, g.nx), dtype=orc.dtype)
         for k in range(g.nt):
             ilum_p += S[k][sl] * S[k][sl]
         if storage == "all":
             prov = _SaveAll(S, g)
+        elif storage == "random":
+            prov = _RandomRebuild(orc, phys, g, (vp, eps, dl, th), S, fw["current"], orc.arr(src), sz, sx)
         else:
             prov = _BoundaryRebuild(orc, phys, g, (vp, eps, dl, th), S, fw["current"], orc.arr(src), sz, sx,
                                     rim_kind)
         cp = fw["cp"]
+        if storage == "random" and abc == 1:  # the adjoint's own CPML state (the forward had none)
+            cp = orc.make_state(g, 1, vp, *cpml)["cp"]
         if cp is not None and reset_cpml:
             for c in cp[4:]:
                 c[...] = 0

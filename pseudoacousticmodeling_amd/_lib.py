@@ -19,6 +19,7 @@ PAM_INTERIOR, PAM_PADDED = 0, 1
 PAM_STORE_ALL_F32, PAM_STORE_ALL_F16, PAM_STORE_ALL_BF16, PAM_STORE_CHECKPOINT = 0, 1, 2, 3
 PAM_STORE_BOUNDARY_F32, PAM_STORE_BOUNDARY_F16, PAM_STORE_BOUNDARY_BF16 = 4, 5, 6
 PAM_STORE_REVOLVE_F32, PAM_STORE_REVOLVE_F16 = 7, 8
+PAM_STORE_RANDOM_BOUNDARY = 9
 PAM_OPT_BATCH, PAM_OPT_TILE, PAM_OPT_GRAPH, PAM_OPT_STORAGE = 1, 2, 3, 4
 PAM_OPT_CKPT_STEP, PAM_OPT_RESET_ADJ_CPML, PAM_OPT_NORMALIZE, PAM_OPT_ADJ_RANGE = 5, 6, 7, 8
 PAM_OPT_PROFILE, PAM_OPT_SNAPSHOTS = 9, 10

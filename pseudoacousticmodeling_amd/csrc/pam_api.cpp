@@ -322,7 +322,7 @@ extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
             ctx->graph = v ? 1 : 0;
             return PAM_OK;
         case PAM_OPT_STORAGE:
-            if (v < PAM_STORE_ALL_F32 || v > PAM_STORE_REVOLVE_F16) return fail(ctx, PAM_E_ARG, "unknown storage");
+            if (v < PAM_STORE_ALL_F32 || v > PAM_STORE_RANDOM_BOUNDARY) return fail(ctx, PAM_E_ARG, "unknown storage");
             ctx->storage = (int)v;
             return PAM_OK;
         case PAM_OPT_CKPT_STEP:
@@ -1352,6 +1352,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     const bool sb = storage >= PAM_STORE_BOUNDARY_F32 && storage <= PAM_STORE_BOUNDARY_BF16;
     const bool rv = storage == PAM_STORE_REVOLVE_F32 || storage == PAM_STORE_REVOLVE_F16;
     const bool rv_half = storage == PAM_STORE_REVOLVE_F16;
+    const bool rbc = storage == PAM_STORE_RANDOM_BOUNDARY;
     const bool cpml = ctx->abc == PAM_CPML;
     const int sb_kind = sb ? storage - PAM_STORE_BOUNDARY_F32 : 0;  // 0 float32, 1 float16, 2 bfloat16
     if (sb && (g.nz < 9 || g.nx < 9 || g.n_abc < 4))
@@ -1362,28 +1363,28 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                     "adjoint (Cerjan) reads float32 padded snapshots: use PAM_STORE_ALL_F32 or PAM_STORE_CHECKPOINT");
     // boundary saving: the adjoint step reads D_t from a one-step interior
     // buffer (gradient) or S_t from the rebuilt padded field (imaging)
-    const int skind = sb ? (grad ? SV_F32_INT : SV_F32_PAD)
+    const int skind = (sb || rbc) ? (grad ? SV_F32_INT : SV_F32_PAD)
                          : linear_adj ? SV_F32_PAD
                          : rv ? SV_F32_INT
                                       : (storage == PAM_STORE_ALL_F16
                                              ? SV_F16_INT
                                              : (storage == PAM_STORE_ALL_BF16 ? SV_BF16_INT : SV_F32_INT));
     const int elem = (half || sb_kind) ? 2 : 4;
-    const long long per_step = sb ? sb_width(g.nz, g.nx) : (skind == SV_F32_PAD) ? ctx->plane : nint;  // elements per step
+    const long long per_step = rbc ? 1 : sb ? sb_width(g.nz, g.nx) : (skind == SV_F32_PAD) ? ctx->plane : nint;  // per step
     const int C = ckpt ? (ctx->ckpt_step > 0 ? ctx->ckpt_step : (int)std::ceil(std::sqrt((double)nt))) : nt;
     // backward steps t in [t_lo, t_hi]: [1, nt-2] (Python reference) or [0, nt-1] (C++)
     if (grad && ctx->adj_range == 1)
         return fail(ctx, PAM_E_ARG, "the FWI gradient needs the adjoint range nt-2 .. 1 (PAM_OPT_ADJ_RANGE 0)");
     const int t_hi = ctx->adj_range ? nt - 1 : nt - 2, t_lo = ctx->adj_range ? 0 : 1;
     const int nseg = ckpt ? t_hi / C + 1 : 1;  // segments of backward steps
-    const int nbuf = rv ? 1 : ckpt ? C + 2 : nt;   // snapshots held at once
+    const int nbuf = (rv || rbc) ? 1 : ckpt ? C + 2 : nt;  // snapshots held at once
     const int nslot = rv ? std::max(1, std::min(ctx->snapshots, t_hi - t_lo + 1)) : 0;  // revolve slots
     const bool multi = grad && (g_eps || g_del || g_th);
     const int nmulti = (ctx->phys == 2) ? 3 : 2;
     // the gradient needs S only through S_{t+1} - 2 S_t + S_{t-1}: store that
     // difference (formed by the forward step in float32, the same arithmetic the
     // adjoint step would do) unless the partials or the linearised adjoint need S
-    const bool d2t = grad && (sb || rv || (!multi && !linear_adj));
+    const bool d2t = grad && (sb || rv || rbc || (!multi && !linear_adj));
     // choose the batch so that the snapshots (and checkpoints) fit in free memory
     size_t freeb = 0, totalb = 0;
     HCK(hipMemGetInfo(&freeb, &totalb));
@@ -1393,11 +1394,12 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                                      (ckpt ? (long long)nseg * ckpt_fields(ctx) * ctx->plane * 4 : 0) +
                                      (sb ? (long long)nt * 4 + nint * 4 : 0) +
                                      (rv ? (long long)nslot * ckpt_fields(ctx) * ctx->plane * (rv_half ? 2 : 4) +
-                                               nint * 4 : 0);
+                                               nint * 4 : 0) +
+                                     (rbc ? nint * 4 : 0);
     const long long have = (long long)ctx->save_bytes + ctx->acc_bytes + ctx->tmp_bytes + ctx->acc_multi_bytes +
                            ctx->ckpt_bytes + ctx->smax_bytes + ctx->sbd_bytes + ctx->rvbits_bytes;
     long long budget = (long long)((freeb + have) * 0.85);
-    if ((ckpt || sb || rv) && !ctx->aux.U0)
+    if ((ckpt || sb || rv || rbc) && !ctx->aux.U0)
         budget -= (long long)ctx->plane * ctx->max_shots * 4 * (ctx->abc == PAM_CPML ? 8 : 2);
     while (B > 1 && (long long)B * per_shot_bytes > budget) --B;
     if ((long long)B * per_shot_bytes > budget)
@@ -1415,7 +1417,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         r = ensure_scratch(ctx, (void **)&ctx->smax, &ctx->smax_bytes, (long long)B * nt * 4);
         if (r) return r;
     }
-    if (sb) {
+    if (sb || rbc) {
         r = ensure_scratch(ctx, (void **)&ctx->sbd, &ctx->sbd_bytes, (long long)B * nint * 4);
         if (r) return r;
         r = alloc_fieldset(ctx, ctx->aux);
@@ -1465,7 +1467,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     ctx->next_k = -1;
     int tile0 = choose_tile(ctx, B, true);
     ctx->steps_run = 0;
-    if ((half || sb || rv) && tile_shape(tile0).V > 0) tile0 = kDefaultTile;  // k_step has no fused max / D_t
+    if ((half || sb || rv || rbc) && tile_shape(tile0).V > 0) tile0 = kDefaultTile;  // k_step has no fused max / D_t
     ctx->last_tile = tile0;
     ctx->last_batch = B;
     const long long pint = (long long)g.n_abc * ctx->P + g.n_abc;  // interior origin in a padded field
@@ -1492,7 +1494,8 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         }
         StepArgs a;
         base_step_args(ctx, a, nb);
-        a.flags |= F_SRC | F_ILUM | ((ckpt || sb) ? 0 : F_SAVE) | (half ? F_SMAX : 0) | ((d2t && !sb) ? F_D2T : 0);
+        a.flags |= F_SRC | F_ILUM | ((ckpt || sb || rbc) ? 0 : F_SAVE) | (half ? F_SMAX : 0) |
+                   ((d2t && !sb && !rbc) ? F_D2T : 0);
         a.save_kind = skind;
         a.save = ctx->save;
         a.save_k0 = 0;
@@ -1522,13 +1525,29 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                     if (r) return r;
                     ++j;
                 }
-                r = fwd_step(ctx, a, tile0, cur, prv, k, nullptr);
-                if (r) return r;
+                if (rbc) {
+                    // undamped through the random layer: no Cerjan factors, no CPML terms
+                    StepArgs u = a;
+                    u.flags &= ~F_CERJAN;
+                    u.cur = cur;
+                    u.prv = prv;
+                    u.k = k;
+                    KCK(launch_step(ctx->phys, false, tile0, u, ctx->st));
+                    ++ctx->steps_run;
+                } else {
+                    r = fwd_step(ctx, a, tile0, cur, prv, k, nullptr);
+                    if (r) return r;
+                }
                 if (sb && k + 1 < nt) {
                     r = sb_save(prv, k + 1);
                     if (r) return r;
                 }
                 std::swap(cur, prv);
+            }
+            // the random-boundary rebuild starts from S_nt and S_{nt-1} (whole grid)
+            if (rbc) {
+                HCK(hipMemcpyAsync(ctx->aux.U0, cur, (size_t)nb * ctx->plane * 4, hipMemcpyDeviceToDevice, ctx->st));
+                HCK(hipMemcpyAsync(ctx->aux.U1, prv, (size_t)nb * ctx->plane * 4, hipMemcpyDeviceToDevice, ctx->st));
             }
             // the rebuild starts from S_nt and S_{nt-1} (interiors; zero on the absorbing layer)
             if (sb) {
@@ -1567,20 +1586,28 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
             f.prv = ra;
             f.k = lev;
             KCK(launch_step(ctx->phys, false, tile0, f, ctx->st));
-            KCK(launch_sb_restore(ra, ctx->plane, ctx->P, g.n_abc, g.nz, g.nx, nb, sb_kind, ctx->save, per_step * nt,
-                                  reinterpret_cast<const float *>(ctx->smax), nt, lev - 1,
-                                  with_d ? ctx->sbd : nullptr, nint, ctx->d_ssz, ctx->d_ssx, ctx->wav, lev + 1,
-                                  ctx->st));
+            if (rbc) {
+                if (lev + 1 < nt)
+                    KCK(launch_rbc_source(ra, ctx->plane, ctx->P, with_d ? ctx->sbd : nullptr, nint, g.n_abc, g.nz, g.nx,
+                                          ctx->d_ssz, ctx->d_ssx, ctx->wav, lev + 1, nb, ctx->st));
+            } else {
+                KCK(launch_sb_restore(ra, ctx->plane, ctx->P, g.n_abc, g.nz, g.nx, nb, sb_kind, ctx->save,
+                                      per_step * nt, reinterpret_cast<const float *>(ctx->smax), nt, lev - 1,
+                                      with_d ? ctx->sbd : nullptr, nint, ctx->d_ssz, ctx->d_ssx, ctx->wav, lev + 1,
+                                      ctx->st));
+            }
             std::swap(ra, rb);
             --lev;
             return PAM_OK;
         };
         // eps / delta / theta partials of step t (src/Migration.py:1053-1064)
         auto partials = [&](int t, const float *lam) -> int {
-            const float *S = sb ? ra + pint
-                                : (const float *)ctx->save + (long long)(t - k0) * per_step + (skind == SV_F32_PAD ? pint : 0);
-            const int sP = (sb || skind == SV_F32_PAD) ? ctx->P : g.nx;
-            const long long ss = sb ? ctx->plane : per_step * nbuf;
+            const bool rebuilt = sb || rbc;
+            const float *S = rebuilt ? ra + pint
+                                     : (const float *)ctx->save + (long long)(t - k0) * per_step +
+                                           (skind == SV_F32_PAD ? pint : 0);
+            const int sP = (rebuilt || skind == SV_F32_PAD) ? ctx->P : g.nx;
+            const long long ss = rebuilt ? ctx->plane : per_step * nbuf;
             float *acc = ctx->acc_multi;
             return launch_grad_multi(ctx->phys, S, sP, ss, lam + pint, ctx->P, ctx->plane, nb, g.nz, g.nx,
                                      g.dx, g.dz, ctx->eps + pint, ctx->del + pint,
@@ -1596,7 +1623,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         b.save = ctx->save;
         b.save_kstride = per_step;
         b.save_sstride = per_step * nbuf;
-        if (sb) {
+        if (sb || rbc) {
             b.save = grad ? (void *)ctx->sbd : nullptr;
             b.save_kstride = b.save_sstride = grad ? nint : ctx->plane;
         }
@@ -1659,7 +1686,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                 c.prv = prv;
                 c.k = t;
                 c.P_ = (const float *)ctx->save + (long long)(t - k0) * per_step;
-                if (sb) {
+                if (sb || rbc) {
                     c.P_ = grad ? ra : rb;
                     c.psstride = ctx->plane;
                     c.D_ = grad ? ctx->sbd : nullptr;
@@ -1841,7 +1868,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                 }
                 b.save_k0 = k0;
                 for (int t = hi; t >= lo; --t) {
-                    if (sb) {
+                    if (sb || rbc) {
                         // rebuild S_t (imaging: into rb) or S_{t-1} and D_t (gradient: S_t in ra)
                         while (lev > t) {
                             r = rev_step(false);

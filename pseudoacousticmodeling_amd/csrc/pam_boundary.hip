@@ -150,6 +150,26 @@ __global__ __launch_bounds__(256) void k_sb_restore(float *U, long long sstride,
     }
 }
 
+// Random-boundary rebuild: wav[wk] added at each shot's source cell of the
+// padded field U and, when D (interior [nz][nx] per shot) is given and the
+// cell is interior, to D as well (wk in [0, nt))
+__global__ void k_rbc_source(float *U, long long sstride, int P, float *D, long long dsstride, int N, int nz, int nx,
+                             const int *ssz, const int *ssx, const float *wav, int wk, int nshots) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nshots) return;
+    const int z = ssz[s], x = ssx[s];
+    U[(long long)s * sstride + (long long)z * P + x] += wav[wk];
+    const int iz = z - N, ix = x - N;
+    if (D && iz >= 0 && iz < nz && ix >= 0 && ix < nx) D[(long long)s * dsstride + (long long)iz * nx + ix] += wav[wk];
+}
+
+int launch_rbc_source(float *U, long long sstride, int P, float *D, long long dsstride, int N, int nz, int nx,
+                      const int *ssz, const int *ssx, const float *wav, int wk, int nshots, hipStream_t st) {
+    hipLaunchKernelGGL(k_rbc_source, dim3((nshots + 63) / 64), dim3(64), 0, st, U, sstride, P, D, dsstride, N, nz, nx,
+                       ssz, ssx, wav, wk, nshots);
+    return (int)hipGetLastError();
+}
+
 // dst = src on the interior, 0 elsewhere (padded [nzp][P] per shot)
 __global__ void k_copy_interior(const float *src, float *dst, long long sstride, int P, int N, int nz, int nx) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;

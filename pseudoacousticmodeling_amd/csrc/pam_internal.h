@@ -184,6 +184,10 @@ int launch_sb_save(const float *U, long long sstride, int P, int N, int nz, int 
 int launch_sb_restore(float *U, long long sstride, int P, int N, int nz, int nx, int nshots, int kind,
                       const void *strips, long long strip_sstride, const float *scale, int nt, int L, float *D,
                       long long dsstride, const int *ssz, const int *ssx, const float *wav, int wk, hipStream_t st);
+// random-boundary rebuild: wav[wk] at each shot's source cell of U and, when
+// interior, of D (interior [nz][nx] per shot; NULL: none)
+int launch_rbc_source(float *U, long long sstride, int P, float *D, long long dsstride, int N, int nz, int nx,
+                      const int *ssz, const int *ssx, const float *wav, int wk, int nshots, hipStream_t st);
 // dst = src on the interior, 0 on the absorbing layer (padded fields)
 int launch_copy_interior(const float *src, float *dst, long long sstride, int nzp, int P, int N, int nz, int nx,
                          int nshots, hipStream_t st);

@@ -17,8 +17,10 @@ step in HBM (float32, or float16 / bfloat16 with "storage"), "checkpoint"
 keeps uniform checkpoints every "step" steps in HBM and recomputes segments,
 "SB" keeps the 4-cell rims inside the interior edge of every step (float32,
 or float16 / bfloat16 with "storage") and rebuilds the source field backwards
-in time (src/Migration.py:1266-1410).  "RBC" (random boundaries) is not
-provided.
+in time (src/Migration.py:1266-1410).  "RBC" randomises the absorbing layer of
+the velocity model anew for every shot ("dvel", "ratio"), runs the source
+undamped through it and rebuilds the source field by reversing those steps
+(src/Migration.py:382-458, 1413-1561).
 """
 from __future__ import annotations
 
@@ -26,7 +28,7 @@ import time
 
 import numpy as np
 
-from . import _lib
+from . import _lib, prep
 
 STORAGE = {"float32": _lib.PAM_STORE_ALL_F32, "float16": _lib.PAM_STORE_ALL_F16,
            "bfloat16": _lib.PAM_STORE_ALL_BF16}
@@ -91,9 +93,40 @@ class migration:
             eng.set_option(_lib.PAM_OPT_CKPT_STEP, int(p.step) if p.step else 0)
         elif p.migration == "SB":
             eng.set_option(_lib.PAM_OPT_STORAGE, SB_STORAGE[getattr(p, "storage", "float32")])
+        elif p.migration == "RBC":
+            eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_RANDOM_BOUNDARY)
         else:
             raise NotImplementedError(f"migration strategy {p.migration!r} is not provided "
-                                      "(use 'onthefly', 'checkpoint' or 'SB')")
+                                      "(use 'onthefly', 'checkpoint', 'SB' or 'RBC')")
+
+    def _random_boundary_runs(self, eng, d, models, rng=None):
+        """RBC (src/Migration.py:1413-1561): shot by shot, a new random
+        absorbing layer (prep.random_boundary) on the expanded velocity, the
+        image (or gradient) and illumination summed over the shots, then
+        divided by ilum + 1e-5 max(ilum)."""
+        import torch
+
+        p = self.pmt
+        N = p.N_abc
+        vp, eps, dl, th = models
+        vp_exp = prep.expand_model(vp, N)
+        rest = [prep.expand_model(m, N) if m is not None else None for m in (eps, dl, th)]
+        rng = np.random.default_rng() if rng is None else rng
+        eng.set_option(_lib.PAM_OPT_NORMALIZE, 0)
+        acc = il = None
+        for s in range(p.Nshot):
+            vr = prep.random_boundary(vp_exp, N, p.dx, p.dz, p.dvel, p.ratio, rng)
+            eng.set_model(vr, *rest, layout=_lib.PAM_PADDED)
+            if p.fwi:
+                out = eng.gradient(d[s:s + 1], shot0=s, nshots=1)
+                img, ilum = out["g_m"], out["ilum"]
+            else:
+                img, ilum = eng.rtm(d[s:s + 1], shot0=s, nshots=1)
+            acc = img.clone() if acc is None else acc + img
+            il = ilum.clone() if il is None else il + ilum
+        eng.set_option(_lib.PAM_OPT_NORMALIZE, 1)
+        out = acc / (il + 1e-5 * torch.max(il))
+        return out, il
 
     def data(self):
         """All shots' adjoint sources on the GPU: the recorded gathers muted
@@ -122,7 +155,15 @@ class migration:
         eng = self.wf.upload_models(vp, eps, dl, th)
         self._configure(eng)
         d = self.data() if data is None else data
-        if p.fwi:
+        if p.migration == "RBC":
+            import torch
+
+            img, ilum = self._random_boundary_runs(eng, torch.as_tensor(d).to(eng.device), (vp, eps, dl, th))
+            self.migrated_image = img.cpu().numpy()
+            self.ilum = ilum.cpu().numpy()
+            f = (f"{p.gradientsFolder}gradient_{p.approximation}_Nx{p.nx}_Nz{p.nz}.bin" if p.fwi else
+                 f"{p.imageFolder}migrated_image_{p.approximation}_Nx{p.nx}_Nz{p.nz}.bin")
+        elif p.fwi:
             out = eng.gradient(d, multiparameter=bool(p.multiparameter and p.approximation in ("VTI", "TTI")))
             self.migrated_image = out["g_m"].cpu().numpy()
             self.ilum = out["ilum"].cpu().numpy()

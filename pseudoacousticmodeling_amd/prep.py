@@ -181,3 +181,106 @@ def smooth_model(f, sigma, water_mask, parameter=False):
     rows = np.arange(nz)[:, None] < first_dry[None, :]
     s[rows] = s_old[rows]
     return s if parameter else (np.float32(1.0) / s).astype(np.float32)
+
+
+def poisson_disk(x_max, z_max, radius, rng, k=30):
+    """Bridson Poisson-disk samples in [0, x_max) x [0, z_max), no two closer
+    than ``radius`` (src/Migration.py:333-380, with a seeded generator)."""
+    cell = radius / np.sqrt(2.0)
+    gw, gh = int(x_max / cell) + 1, int(z_max / cell) + 1
+    grid = -np.ones((gw, gh), dtype=np.int64)
+    pts = []
+
+    def ok(p):
+        gx, gz = int(p[0] / cell), int(p[1] / cell)
+        for i in range(max(gx - 2, 0), min(gx + 3, gw)):
+            for j in range(max(gz - 2, 0), min(gz + 3, gh)):
+                n = grid[i, j]
+                if n >= 0 and (pts[n][0] - p[0]) ** 2 + (pts[n][1] - p[1]) ** 2 < radius * radius:
+                    return False
+        return True
+
+    p0 = (rng.uniform(0, x_max), rng.uniform(0, z_max))
+    pts.append(p0)
+    grid[int(p0[0] / cell), int(p0[1] / cell)] = 0
+    active = [0]
+    while active:
+        a = int(rng.integers(len(active)))
+        p = pts[active[a]]
+        found = False
+        for _ in range(k):
+            r = rng.uniform(radius, 2 * radius)
+            ang = rng.uniform(0, 2 * np.pi)
+            q = (p[0] + r * np.cos(ang), p[1] + r * np.sin(ang))
+            if 0 <= q[0] < x_max and 0 <= q[1] < z_max and ok(q):
+                pts.append(q)
+                grid[int(q[0] / cell), int(q[1] / cell)] = len(pts) - 1
+                active.append(len(pts) - 1)
+                found = True
+        if not found:
+            active.pop(a)
+    return np.array(pts)
+
+
+def random_boundary(vp_exp, N, dx, dz, dvel, ratio, rng):
+    """Random-boundary velocity model of src/Migration.py:382-458 on a padded
+    [nzp][nxp] model: every absorbing-layer cell drawn from a normal
+    distribution around the velocity at the interior edge, its spread growing
+    linearly (f1d) towards the outer edge and the draw limited to
+    [dvel, v + dvel] (get_randomvalue, src/Migration.py:327-331); the model
+    clipped to [vmin - dvel, vmax + dvel]; then Gaussian blobs (radius
+    U(0.1, 1) ratio, amplitude +-U(0.5, 1) dvel) centred on Poisson-disk points
+    in the outer half of the layer added over the grid, and clipped again.
+    ``rng`` is a numpy Generator: the reference draws from the global
+    generators, so realisations are comparable only statistically."""
+    vp = np.array(vp_exp, dtype=np.float64)
+    nzp, nxp = vp.shape
+    nz, nx = nzp - 2 * N, nxp - 2 * N
+    vmax, vmin = vp.max(), vp.min()
+    f1d = np.linspace(0.0, 1.0, N)
+
+    def rv(v, f):
+        p = rng.normal(v, dvel * f)
+        return np.where(p < dvel, dvel, np.where(p > dvel + v, dvel + v, p))
+
+    ramp = f1d[::-1]  # spread at distance j+1 .. N from the interior: f1d[N-j-1] for column j
+    # left / right columns of the interior rows
+    vl = vp[N:N + nz, N][:, None]
+    vr = vp[N:N + nz, nxp - N - 1][:, None]
+    vp[N:N + nz, :N] = rv(np.repeat(vl, N, 1), np.repeat(ramp[None, :], nz, 0))
+    vp[N:N + nz, nxp - N:] = rv(np.repeat(vr, N, 1), np.repeat(ramp[None, ::-1], nz, 0))
+    # top / bottom rows of the interior columns
+    vt = vp[N, N:N + nx][None, :]
+    vb = vp[nzp - N - 1, N:N + nx][None, :]
+    vp[:N, N:N + nx] = rv(np.repeat(vt, N, 0), np.repeat(ramp[:, None], nx, 1))
+    vp[nzp - N:, N:N + nx] = rv(np.repeat(vb, N, 0), np.repeat(ramp[::-1, None], nx, 1))
+    # corners: spread by the distance to the interior corner (the larger of the two)
+    ii, jj = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    dist = np.maximum(N - 1 - ii, N - 1 - jj)  # 0 next to the interior
+    fc = f1d[dist]
+    for (zs, xs, vc) in ((slice(0, N), slice(0, N), vp[N, N]), (slice(0, N), slice(nxp - N, nxp), vp[N, nxp - N - 1]),
+                         (slice(nzp - N, nzp), slice(0, N), vp[nzp - N - 1, N]),
+                         (slice(nzp - N, nzp), slice(nxp - N, nxp), vp[nzp - N - 1, nxp - N - 1])):
+        f = fc
+        if zs.start != 0:
+            f = f[::-1, :]
+        if xs.start != 0:
+            f = f[:, ::-1]
+        vp[zs, xs] = rv(np.full((N, N), vc), f)
+    vp = np.clip(vp, vmin - dvel, vmax + dvel)
+    # blobs at Poisson-disk points in the outer half of the layer
+    L_abc, D_abc = (nxp - 1) * dx, (nzp - 1) * dz
+    pts = poisson_disk(L_abc, D_abc, ratio, rng)
+    nab_x, nab_z = N * dx, N * dz
+    L, D = (nx - 1) * dx, (nz - 1) * dz
+    keep = ((pts[:, 0] < 0.5 * nab_x) | (pts[:, 0] > L + nab_x + 0.5 * nab_x) |
+            (pts[:, 1] < 0.5 * nab_z) | (pts[:, 1] > D + nab_z + 0.5 * nab_z))
+    x = np.arange(nxp)[None, :] * dx
+    z = np.arange(nzp)[:, None] * dz
+    for xc, zc in pts[keep]:
+        r = rng.uniform(0.1 * ratio, ratio)
+        A = rng.uniform(0.5 * dvel, dvel)
+        sgn = rng.choice([-1.0, 1.0])
+        vp = vp + sgn * A * np.exp(-0.5 * (((x - xc) / r) ** 2 + ((z - zc) / r) ** 2))
+    vp = np.clip(vp, vmin - dvel, vmax + dvel)
+    return vp.astype(np.float32)
