@@ -470,7 +470,7 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b, const
             float out[4], opv[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float pe = (mc.pv[j] * fx[j]) * fz;
+                const bool want_op = (flags & F_D2T_OP) != 0;
                 float u;
                 if (PHYS == 0) {
                     float sum = pxx[j] + pzz[j];
@@ -480,25 +480,13 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b, const
                         if (xs[j]) sum += zx[j];
                         if (zs) sum += zz[j];
                     }
-                    u = mc.v2[j] * sum + 2.0f * c0[j] - pe;
-                    if (flags & F_D2T_OP) opv[j] = mc.v2[j] * sum;
+                    u = lf_acoustic(c0[j], mc.pv[j], fx[j], fz, mc.v2[j], sum, want_op, opv[j]);
                 } else if (PHYS == 1) {
-                    const float Sd = sd_term(mc.e[j], mc.dl[j], PX[j], PZ[j]);
-                    u = 2.0f * c0[j] - pe + mc.v2[j] * ((1.0f + 2.0f * mc.e[j]) + Sd) * PXX[j] +
-                        mc.v2[j] * (1.0f + Sd) * PZZ[j];
-                    if (flags & F_D2T_OP)
-                        opv[j] = mc.v2[j] * ((1.0f + 2.0f * mc.e[j]) + Sd) * PXX[j] + mc.v2[j] * (1.0f + Sd) * PZZ[j];
+                    u = lf_vti(c0[j], mc.pv[j], fx[j], fz, mc.v2[j], mc.e[j], mc.dl[j], PX[j], PZ[j], PXX[j], PZZ[j],
+                               want_op, opv[j]);
                 } else {
-                    const float xi = PX[j] * mc.cs[j] - PZ[j] * mc.sn[j];
-                    const float eta = PX[j] * mc.sn[j] + PZ[j] * mc.cs[j];
-                    const float Sd = sd_term(mc.e[j], mc.dl[j], xi, eta);
-                    const float Ac = (1.0f + 2.0f * mc.e[j]) * (mc.cs[j] * mc.cs[j]) + (mc.sn[j] * mc.sn[j]) + Sd;
-                    const float Bc = (1.0f + 2.0f * mc.e[j]) * (mc.sn[j] * mc.sn[j]) + (mc.cs[j] * mc.cs[j]) + Sd;
-                    u = 2.0f * c0[j] - pe + mc.v2[j] * Ac * PXX[j] + mc.v2[j] * Bc * PZZ[j] -
-                        2.0f * mc.e[j] * mc.v2[j] * (2.0f * mc.sn[j] * mc.cs[j]) * pxz[j];
-                    if (flags & F_D2T_OP)
-                        opv[j] = mc.v2[j] * Ac * PXX[j] + mc.v2[j] * Bc * PZZ[j] -
-                                 2.0f * mc.e[j] * mc.v2[j] * (2.0f * mc.sn[j] * mc.cs[j]) * pxz[j];
+                    u = lf_tti(c0[j], mc.pv[j], fx[j], fz, mc.v2[j], mc.e[j], mc.dl[j], mc.cs[j], mc.sn[j], PX[j],
+                               PZ[j], PXX[j], PZZ[j], pxz[j], want_op, opv[j]);
                 }
                 out[j] = (u * fx[j]) * fz;
             }

@@ -83,6 +83,45 @@ __device__ __forceinline__ float sd_term(float e, float dl, float px, float pz) 
     return (fabsf(den) > 1e-12f) ? num * __builtin_amdgcn_rcpf(den) : 0.0f;
 }
 
+// The leapfrog update of one cell before the Cerjan factors,
+// U_{k+1} = 2 U_k - (U_{k-1} fx fz) + v^2 dt^2 L(U_k)  (c0 = U_k, pv = U_{k-1}
+// as stored, damped once more as the reference holds it; v2 = v^2 dt^2), and,
+// when `want_op`, the operator term v^2 dt^2 L(U_k) alone into `op`.  Every
+// fused multiply-add is written out: left to the compiler, a product fuses
+// into the next addition only when nothing else uses it, so an instance that
+// also stores the operator term would round the update differently from the
+// forward-only instance of the same kernel.
+__device__ __forceinline__ float lf_base(float c0, float pv, float fx, float fz) {
+    return __builtin_fmaf(-(pv * fx), fz, 2.0f * c0);
+}
+// acoustic: sum = U_xx + U_zz (with the CPML corrections)
+__device__ __forceinline__ float lf_acoustic(float c0, float pv, float fx, float fz, float v2, float sum, bool want_op,
+                                             float &op) {
+    if (want_op) op = v2 * sum;
+    return __builtin_fmaf(v2, sum, lf_base(c0, pv, fx, fz));
+}
+// VTI (src/CudaKernels.py:361): (1 + 2 eps + Sd) U_xx + (1 + Sd) U_zz
+__device__ __forceinline__ float lf_vti(float c0, float pv, float fx, float fz, float v2, float e, float dl, float PX,
+                                        float PZ, float PXX, float PZZ, bool want_op, float &op) {
+    const float Sd = sd_term(e, dl, PX, PZ);
+    const float ax = v2 * ((1.0f + 2.0f * e) + Sd), az = v2 * (1.0f + Sd);
+    if (want_op) op = __builtin_fmaf(ax, PXX, az * PZZ);
+    return __builtin_fmaf(az, PZZ, __builtin_fmaf(ax, PXX, lf_base(c0, pv, fx, fz)));
+}
+// TTI: the VTI operator in the frame rotated by theta (cs, sn = cos, sin)
+__device__ __forceinline__ float lf_tti(float c0, float pv, float fx, float fz, float v2, float e, float dl, float cs,
+                                        float sn, float PX, float PZ, float PXX, float PZZ, float pxz, bool want_op,
+                                        float &op) {
+    const float xi = PX * cs - PZ * sn;
+    const float eta = PX * sn + PZ * cs;
+    const float Sd = sd_term(e, dl, xi, eta);
+    const float Ac = (1.0f + 2.0f * e) * (cs * cs) + (sn * sn) + Sd;
+    const float Bc = (1.0f + 2.0f * e) * (sn * sn) + (cs * cs) + Sd;
+    const float tx = v2 * Ac, tz = v2 * Bc, txz = (2.0f * e * v2) * (2.0f * sn * cs);
+    if (want_op) op = __builtin_fmaf(-txz, pxz, __builtin_fmaf(tz, PZZ, tx * PXX));
+    return __builtin_fmaf(-txz, pxz, __builtin_fmaf(tz, PZZ, __builtin_fmaf(tx, PXX, lf_base(c0, pv, fx, fz))));
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
 // works on a contiguous chunk of (tile, shot) space and its L2 sees the halo
 // and model reuse.  nblocks is a multiple of 8.
@@ -412,8 +451,7 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs a) {
         }
 #pragma unroll
         for (int j = 0; j < V; ++j) {
-            const float pe = (pv[j] * fx[j]) * fz;  // U_{k-1} as the reference holds it (damped twice)
-            float u;
+            float u, unused = 0.0f;
             if (PHYS == 0) {
                 float sum = pxx[j] + pzz[j];
                 if (CPML) {
@@ -422,18 +460,12 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs a) {
                     if (xs[j]) sum += zx[j];
                     if (zs) sum += zz[j];
                 }
-                u = v2[j] * sum + 2.0f * c0[j] - pe;
+                u = lf_acoustic(c0[j], pv[j], fx[j], fz, v2[j], sum, false, unused);
             } else if (PHYS == 1) {
-                const float Sd = sd_term(e[j], dl[j], PX[j], PZ[j]);
-                u = 2.0f * c0[j] - pe + v2[j] * ((1.0f + 2.0f * e[j]) + Sd) * PXX[j] + v2[j] * (1.0f + Sd) * PZZ[j];
+                u = lf_vti(c0[j], pv[j], fx[j], fz, v2[j], e[j], dl[j], PX[j], PZ[j], PXX[j], PZZ[j], false, unused);
             } else {
-                const float xi = PX[j] * cs[j] - PZ[j] * sn[j];
-                const float eta = PX[j] * sn[j] + PZ[j] * cs[j];
-                const float Sd = sd_term(e[j], dl[j], xi, eta);
-                const float Ac = (1.0f + 2.0f * e[j]) * (cs[j] * cs[j]) + (sn[j] * sn[j]) + Sd;
-                const float Bc = (1.0f + 2.0f * e[j]) * (sn[j] * sn[j]) + (cs[j] * cs[j]) + Sd;
-                u = 2.0f * c0[j] - pe + v2[j] * Ac * PXX[j] + v2[j] * Bc * PZZ[j] -
-                    2.0f * e[j] * v2[j] * (2.0f * sn[j] * cs[j]) * pxz[j];
+                u = lf_tti(c0[j], pv[j], fx[j], fz, v2[j], e[j], dl[j], cs[j], sn[j], PX[j], PZ[j], PXX[j], PZZ[j],
+                           pxz[j], false, unused);
             }
             out[j] = (u * fx[j]) * fz;
         }

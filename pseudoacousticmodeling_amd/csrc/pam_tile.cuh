@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
         float out[4], opv[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float pe = (pv[r][j] * fx[j]) * fz;  // U_{k-1} as the reference holds it (damped twice)
+            const bool want_op = (flags & F_D2T_OP) != 0;
             float u;
             if (PHYS == 0) {
                 float sum = pxx[j] + pzz[j];
@@ -311,25 +311,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
                     if (xs[j]) sum += zx[j];
                     if (zs) sum += zz[j];
                 }
-                u = v2[r][j] * sum + 2.0f * c0[j] - pe;
-                if (flags & F_D2T_OP) opv[j] = v2[r][j] * sum;
+                u = lf_acoustic(c0[j], pv[r][j], fx[j], fz, v2[r][j], sum, want_op, opv[j]);
             } else if (PHYS == 1) {
-                const float Sd = sd_term(e[r][j], dl[r][j], PX[j], PZ[j]);
-                u = 2.0f * c0[j] - pe + v2[r][j] * ((1.0f + 2.0f * e[r][j]) + Sd) * PXX[j] +
-                    v2[r][j] * (1.0f + Sd) * PZZ[j];
-                if (flags & F_D2T_OP)
-                    opv[j] = v2[r][j] * ((1.0f + 2.0f * e[r][j]) + Sd) * PXX[j] + v2[r][j] * (1.0f + Sd) * PZZ[j];
+                u = lf_vti(c0[j], pv[r][j], fx[j], fz, v2[r][j], e[r][j], dl[r][j], PX[j], PZ[j], PXX[j], PZZ[j],
+                           want_op, opv[j]);
             } else {
-                const float xi = PX[j] * cs[r][j] - PZ[j] * sn[r][j];
-                const float eta = PX[j] * sn[r][j] + PZ[j] * cs[r][j];
-                const float Sd = sd_term(e[r][j], dl[r][j], xi, eta);
-                const float Ac = (1.0f + 2.0f * e[r][j]) * (cs[r][j] * cs[r][j]) + (sn[r][j] * sn[r][j]) + Sd;
-                const float Bc = (1.0f + 2.0f * e[r][j]) * (sn[r][j] * sn[r][j]) + (cs[r][j] * cs[r][j]) + Sd;
-                u = 2.0f * c0[j] - pe + v2[r][j] * Ac * PXX[j] + v2[r][j] * Bc * PZZ[j] -
-                    2.0f * e[r][j] * v2[r][j] * (2.0f * sn[r][j] * cs[r][j]) * pxz[j];
-                if (flags & F_D2T_OP)
-                    opv[j] = v2[r][j] * Ac * PXX[j] + v2[r][j] * Bc * PZZ[j] -
-                             2.0f * e[r][j] * v2[r][j] * (2.0f * sn[r][j] * cs[r][j]) * pxz[j];
+                u = lf_tti(c0[j], pv[r][j], fx[j], fz, v2[r][j], e[r][j], dl[r][j], cs[r][j], sn[r][j], PX[j], PZ[j],
+                           PXX[j], PZZ[j], pxz[j], want_op, opv[j]);
             }
             out[j] = (u * fx[j]) * fz;
         }
