@@ -20,7 +20,9 @@
 //
 // Blocks are ordered tile-major with the shots of a tile adjacent, so the
 // model coefficients of a tile are fetched from HBM once and reused from L2
-// by the other shots; the XCD remap gives each XCD a contiguous range.
+// by the other shots; tiles run in bands of four tile columns so that the
+// halo rows a tile shares with its z-neighbour are reused from L2 too; the
+// XCD remap gives each XCD a contiguous range.
 #pragma once
 #include "pam_step.cuh"
 
@@ -43,8 +45,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
     if (b >= nb) return;  // block-uniform
     const int tile = b / a.nshots;
     const int s = b - tile * a.nshots;
-    const int tzi = tile / a.nxt;
-    const int txi = tile - tzi * a.nxt;
+    // tiles in bands of kBand tile columns, x fastest within a band, then z:
+    // the z-neighbour of a tile (8 shared halo rows) comes kBand tiles later,
+    // while those rows are still in L2, and the x-neighbour right after it
+    constexpr int kBand = 4;
+    const int band = tile / (kBand * a.nzt);
+    const int rem = tile - band * (kBand * a.nzt);
+    const int wb = min(kBand, a.nxt - band * kBand);
+    const int tzi = (wb == kBand) ? rem / kBand : rem / wb;
+    const int txi = band * kBand + (rem - tzi * wb);
     const int x0 = txi * TX, z0 = tzi * TZ;
     const int P = a.P, nzp = a.nzp, nxp = a.nxp, N = a.N;
     const long long so = (long long)s * a.sstride;
