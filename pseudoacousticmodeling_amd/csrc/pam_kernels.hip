@@ -30,7 +30,7 @@ namespace pam {
 //  - z-strip columns: a thread marches down a segment of up to kPsiSeg rows of
 //    one z-strip at one column with a 9-row register queue of U, so each row
 //    of U is loaded once per segment rather than once per neighbour.
-constexpr int kPsiSeg = 24;
+constexpr int kPsiSeg = 12;
 
 __global__ __launch_bounds__(256) void k_psi(const PsiArgs a, long long nx_threads, int nsegz) {
     const int s = blockIdx.y;

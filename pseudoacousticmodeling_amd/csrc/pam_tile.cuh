@@ -20,7 +20,7 @@
 //
 // Blocks are ordered tile-major with the shots of a tile adjacent, so the
 // model coefficients of a tile are fetched from HBM once and reused from L2
-// by the other shots; tiles run in bands of four tile columns so that the
+// by the other shots; tiles run in bands of eight tile columns so that the
 // halo rows a tile shares with its z-neighbour are reused from L2 too; the
 // XCD remap gives each XCD a contiguous range.
 #pragma once
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
     // tiles in bands of kBand tile columns, x fastest within a band, then z:
     // the z-neighbour of a tile (8 shared halo rows) comes kBand tiles later,
     // while those rows are still in L2, and the x-neighbour right after it
-    constexpr int kBand = 4;
+    constexpr int kBand = 8;
     const int band = tile / (kBand * a.nzt);
     const int rem = tile - band * (kBand * a.nzt);
     const int wb = min(kBand, a.nxt - band * kBand);
