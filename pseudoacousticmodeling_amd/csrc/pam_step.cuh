@@ -6,6 +6,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 #include <stdint.h>
+#include <type_traits>
 
 #include "pam_internal.h"
 
@@ -21,6 +22,15 @@ namespace pam {
 #define KA2 (-1.0f / 5.0f)
 #define KA3 (4.0f / 105.0f)
 #define KA4 (-1.0f / 280.0f)
+
+// base + idx elements with the byte offset formed in 32 bits, so that the
+// access can use a wave-uniform 64-bit base and a 32-bit lane offset (the
+// offset of one shot's field stays far below 4 GiB)
+template <typename T>
+__device__ __forceinline__ T *at(T *base, unsigned idx) {
+    typedef typename std::conditional<std::is_const<T>::value, const char, char>::type B;
+    return reinterpret_cast<T *>(reinterpret_cast<B *>(base) + idx * (unsigned)sizeof(T));
+}
 
 template <int V> struct VecOf;
 template <> struct VecOf<2> { typedef float2 T; };

@@ -59,6 +59,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
     const long long so = (long long)s * a.sstride;
     const float *__restrict__ cur = a.cur + so;
     float *__restrict__ prv = a.prv + so;
+    const float *__restrict__ psix = a.psix + so;
+    const float *__restrict__ psiz = a.psiz + so;
+    float *__restrict__ zetax = a.zetax + so;
+    float *__restrict__ zetaz = a.zetaz + so;
     const int tid = threadIdx.x;
     const int flags = ADJ ? a.flags : (a.flags & F_FWD_MASK);  // forward-only instances drop the rest
     const int cx = tid & 15, rg = tid >> 4;
@@ -74,26 +78,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
         const int gz = z0 - 4 + r, gc = x0 - 4 + 4 * q;
         st[u] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (i < ROWS * Q && gz >= 0 && gz < nzp && gc >= 0 && gc < P)
-            st[u] = *reinterpret_cast<const float4 *>(cur + (long long)gz * P + gc);
+            st[u] = *reinterpret_cast<const float4 *>(at(cur, (unsigned)(gz * P + gc)));
     }
     float pv[R][4], v2[R][4], e[R][4], dl[R][4], cs[R][4], sn[R][4];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int z = zr0 + r;
         const bool zin = z < nzp;
-        const long long ro = (long long)(zin ? z : 0) * P + col;
+        const unsigned ro = (unsigned)((zin ? z : 0) * P + col);
 #pragma unroll
         for (int j = 0; j < 4; ++j) pv[r][j] = v2[r][j] = e[r][j] = dl[r][j] = cs[r][j] = sn[r][j] = 0.f;
         if (zin) {
-            vld<4>(pv[r], prv + ro);
-            vld<4>(v2[r], a.v2 + ro);
+            vld<4>(pv[r], at(prv, ro));
+            vld<4>(v2[r], at(a.v2, ro));
             if (PHYS >= 1) {
-                vld<4>(e[r], a.eps + ro);
-                vld<4>(dl[r], a.del + ro);
+                vld<4>(e[r], at(a.eps, ro));
+                vld<4>(dl[r], at(a.del, ro));
             }
             if (PHYS == 2) {
-                vld<4>(cs[r], a.cs + ro);
-                vld<4>(sn[r], a.sn + ro);
+                vld<4>(cs[r], at(a.cs, ro));
+                vld<4>(sn[r], at(a.sn, ro));
             }
         }
     }
@@ -145,7 +149,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
         const int z = zr0 + r;
         if (z >= nzp) break;  // rows beyond the grid (last tile row only)
         const bool zlive = (z >= 4) && (z < nzp - 4);
-        const long long ro = (long long)z * P + col;
+        const unsigned ro = (unsigned)(z * P + col);  // 32-bit lane offsets: SGPR-base addressing
         const float fz = (!CPML && (flags & F_CERJAN)) ? a.cerz[z] : 1.0f;
         const int lr = rg * R + r + 4;  // staged row of z
 
@@ -262,15 +266,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
         }
         if (CPML && tile_xstrip) {
             float pw[12], d2p[4], d1p[4], zo[4], ax[4], bx[4];
-            const float *prow = a.psix + so + (long long)z * P;
+            const float *prow = psix + (unsigned)(z * P);
             vldg<4>(pw + 4, prow, col, P);
             if (col >= 4) vld<4>(pw, prow + col - 4);
             else pw[0] = pw[1] = pw[2] = pw[3] = 0.f;
             vldg<4>(pw + 8, prow, col + 4, P);
             xderivs<4>(pw, d2p, d1p);
-            vld<4>(zo, a.zetax + so + ro);
-            vld<4>(ax, a.cax + ro);
-            vld<4>(bx, a.cbx + ro);
+            vld<4>(zo, at(zetax, ro));
+            vld<4>(ax, at(a.cax, ro));
+            vld<4>(bx, at(a.cbx, ro));
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (!xs[j]) continue;
@@ -279,7 +283,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
                 dpx[j] = dps;
                 PX[j] = px[j] + dps;
                 PXX[j] = pxx[j] + dps + zx[j];
-                a.zetax[so + ro + j] = zx[j];
+                *at(zetax, ro + j) = zx[j];
             }
         }
         if (CPML && zs) {
@@ -287,12 +291,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
 #pragma unroll
             for (int h = 0; h < 9; ++h) {
                 if (h == 4) continue;
-                vld<4>(ps[h], a.psiz + so + (long long)(z - 4 + h) * P + col);
+                vld<4>(ps[h], at(psiz, (unsigned)((z - 4 + h) * P + col)));
             }
             float zo[4], az[4], bz[4];
-            vld<4>(zo, a.zetaz + so + ro);
-            vld<4>(az, a.caz + ro);
-            vld<4>(bz, a.cbz + ro);
+            vld<4>(zo, at(zetaz, ro));
+            vld<4>(az, at(a.caz, ro));
+            vld<4>(bz, at(a.cbz, ro));
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const float dps = (KA1 * (ps[5][j] - ps[3][j]) + KA2 * (ps[6][j] - ps[2][j]) +
@@ -302,7 +306,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
                 dpz[j] = dps;
                 PZ[j] = pz[j] + dps;
                 PZZ[j] = pzz[j] + dps + zz[j];
-                if (xlive[j]) a.zetaz[so + ro + j] = zz[j];
+                if (xlive[j]) *at(zetaz, ro + j) = zz[j];
             }
         }
 
@@ -364,11 +368,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
             }
         }
         if (xlive[0] && xlive[3]) {
-            vst<4>(prv + ro, out);
+            vst<4>(at(prv, ro), out);
         } else {
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                if (xlive[j]) prv[ro + j] = out[j];
+                if (xlive[j]) *at(prv, ro + j) = out[j];
         }
     }
     if ((flags & F_SMAX) && a.k + 1 < a.nt) block_max_to(a.smax + (long long)s * a.nt + a.k + 1, tmax);
