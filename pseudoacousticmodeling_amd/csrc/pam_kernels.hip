@@ -37,18 +37,23 @@ __global__ __launch_bounds__(256) void k_psi(const PsiArgs a, long long nx_threa
     const long long so = (long long)s * a.sstride;
     const int wN = a.N - 4;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const float *U = a.cur + so;
-    const long long P = a.P;
+    // per-shot bases; lane offsets are 32-bit (at())
+    const float *__restrict__ U = a.cur + so;
+    const float *__restrict__ psix_in = a.psix_in + so;
+    const float *__restrict__ psiz_in = a.psiz_in + so;
+    float *__restrict__ psix = a.psix + so;
+    float *__restrict__ psiz = a.psiz + so;
+    const unsigned P = (unsigned)a.P;
     if (t < nx_threads) {
         const long long nxs = (long long)(a.nzp - 8) * 2 * wN;
         if (t >= nxs) return;
         const int row = 4 + (int)(t / (2 * wN));
         const int cc = (int)(t % (2 * wN));
         const int ix = (cc < wN) ? 4 + cc : a.nxp - a.N + (cc - wN);
-        const long long i = (long long)row * P + ix;
-        const float d1 = KA1 * (U[i + 1] - U[i - 1]) + KA2 * (U[i + 2] - U[i - 2]) + KA3 * (U[i + 3] - U[i - 3]) +
-                         KA4 * (U[i + 4] - U[i - 4]);
-        a.psix[so + i] = a.cax[i] * a.psix_in[so + i] + a.cbx[i] * (d1 * a.idx);
+        const unsigned i = (unsigned)row * P + (unsigned)ix;
+        const float *c = at(U, i);
+        const float d1 = KA1 * (c[1] - c[-1]) + KA2 * (c[2] - c[-2]) + KA3 * (c[3] - c[-3]) + KA4 * (c[4] - c[-4]);
+        *at(psix, i) = *at(a.cax, i) * *at(psix_in, i) + *at(a.cbx, i) * (d1 * a.idx);
         return;
     }
     // z-strip column segment: (side, segment, column)
@@ -64,15 +69,15 @@ __global__ __launch_bounds__(256) void k_psi(const PsiArgs a, long long nx_threa
     const int z1 = min(z0 + kPsiSeg, side == 0 ? a.N : a.nzp - 4);  // one past the last row
     float q[9];
 #pragma unroll
-    for (int h = 0; h < 8; ++h) q[h] = U[(long long)(z0 - 4 + h) * P + ix];
+    for (int h = 0; h < 8; ++h) q[h] = *at(U, (unsigned)(z0 - 4 + h) * P + (unsigned)ix);
 #pragma unroll
     for (int r = 0; r < kPsiSeg; ++r) {
         const int z = z0 + r;
         if (z >= z1) break;  // thread-uniform bound of the segment
-        const long long i = (long long)z * P + ix;
-        q[8] = U[i + 4 * P];
+        const unsigned i = (unsigned)z * P + (unsigned)ix;
+        q[8] = *at(U, i + 4 * P);
         const float d1 = KA1 * (q[5] - q[3]) + KA2 * (q[6] - q[2]) + KA3 * (q[7] - q[1]) + KA4 * (q[8] - q[0]);
-        a.psiz[so + i] = a.caz[i] * a.psiz_in[so + i] + a.cbz[i] * (d1 * a.idz);
+        *at(psiz, i) = *at(a.caz, i) * *at(psiz_in, i) + *at(a.cbz, i) * (d1 * a.idz);
 #pragma unroll
         for (int h = 0; h < 8; ++h) q[h] = q[h + 1];
     }

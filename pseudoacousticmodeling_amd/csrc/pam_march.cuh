@@ -38,16 +38,16 @@ __device__ __forceinline__ void march_load_cell(MarchCell<PHYS> &m, const StepAr
 #pragma unroll
     for (int j = 0; j < 4; ++j) m.pv[j] = m.v2[j] = m.e[j] = m.dl[j] = m.cs[j] = m.sn[j] = 0.f;
     if (z < a.nzp) {
-        const long long ro = (long long)z * a.P + col;
-        vld<4>(m.pv, prv + ro);
-        vld<4>(m.v2, a.v2 + ro);
+        const unsigned ro = (unsigned)(z * a.P + col);
+        vld<4>(m.pv, at(prv, ro));
+        vld<4>(m.v2, at(a.v2, ro));
         if (PHYS >= 1) {
-            vld<4>(m.e, a.eps + ro);
-            vld<4>(m.dl, a.del + ro);
+            vld<4>(m.e, at(a.eps, ro));
+            vld<4>(m.dl, at(a.del, ro));
         }
         if (PHYS == 2) {
-            vld<4>(m.cs, a.cs + ro);
-            vld<4>(m.sn, a.sn + ro);
+            vld<4>(m.cs, at(a.cs, ro));
+            vld<4>(m.sn, at(a.sn, ro));
         }
     }
 }
@@ -112,6 +112,13 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b, const
     const long long so = (long long)s * a.sstride;
     const float *__restrict__ cur = m.cur + so;
     float *__restrict__ prv = m.prv + so;
+    // per-shot bases of the CPML fields; lane offsets are 32-bit (at())
+    const float *__restrict__ pxi = m.pxi + so;
+    const float *__restrict__ pzi = m.pzi + so;
+    float *__restrict__ pxo = m.pxo + so;
+    float *__restrict__ pzo = m.pzo + so;
+    float *__restrict__ zetax = a.zetax + so;
+    float *__restrict__ zetaz = a.zetaz + so;
     const int tid = threadIdx.x;
     const int flags = ADJ ? m.flags : (m.flags & F_FWD_MASK);  // forward-only instances drop the rest
     const int cx = tid & 15, rg = tid >> 4;
@@ -159,7 +166,7 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b, const
             for (int j = 0; j < 4; ++j) {
                 if (!xstrip(c0 + j)) continue;
                 nv[j] = ax[j] * pin[j] + bx[j] * (d1[j] * a.idx);
-                if (g >= 1 && g <= G && z < ze) m.pxo[so + (long long)z * P + c0 + j] = nv[j];
+                if (g >= 1 && g <= G && z < ze) *at(pxo, (unsigned)(z * P + c0 + j)) = nv[j];
             }
         }
         pxl[r * (G + 2) + g] = make_float4(nv[0], nv[1], nv[2], nv[3]);
@@ -175,7 +182,7 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b, const
             const int gz = zb - LA + r, gc = x0 - HX + 4 * q;
             st[u] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (i < (TC + 2 * LA) * Q && gz >= 0 && gz < nzp && gc >= 0 && gc < P && want_q(q))
-                st[u] = *reinterpret_cast<const float4 *>(cur + (long long)gz * P + gc);
+                st[u] = *reinterpret_cast<const float4 *>(at(cur, (unsigned)(gz * P + gc)));
         }
 #pragma unroll
         for (int u = 0; u < NPRO; ++u) {
@@ -193,14 +200,14 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b, const
             const int r = zb - 4 + i / G, c = i - (i / G) * G;
             float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
             if (zstrip(r)) {
-                const long long ro = (long long)r * P + x0 + 4 * c;
+                const unsigned ro = (unsigned)(r * P + x0 + 4 * c);
                 float pin[4], az[4], bz[4], nv[4];
-                vld<4>(pin, m.pzi + so + ro);
-                vld<4>(az, a.caz + ro);
-                vld<4>(bz, a.cbz + ro);
+                vld<4>(pin, at(pzi, ro));
+                vld<4>(az, at(a.caz, ro));
+                vld<4>(bz, at(a.cbz, ro));
                 psiz_new(nv, pin, az, bz, r, c);
                 o = make_float4(nv[0], nv[1], nv[2], nv[3]);
-                if (r >= zb) store_live4(m.pzo + so + ro, nv, x0 + 4 * c, nxp);
+                if (r >= zb) store_live4(at(pzo, ro), nv, x0 + 4 * c, nxp);
             }
             pzr[PSLOT(r) + c] = o;
         }
@@ -239,32 +246,32 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b, const
                 pzin[j] = paz[j] = pbz[j] = pxin[j] = pax[j] = pbx[j] = hxin[j] = hax[j] = hbx[j] = zzo[j] = zaz[j] =
                     zbz[j] = zxo[j] = 0.f;
             if (zchunk && zstrip(zr)) {
-                const long long ro = (long long)zr * P + col;
-                vld<4>(pzin, m.pzi + so + ro);
-                vld<4>(paz, a.caz + ro);
-                vld<4>(pbz, a.cbz + ro);
+                const unsigned ro = (unsigned)(zr * P + col);
+                vld<4>(pzin, at(pzi, ro));
+                vld<4>(paz, at(a.caz, ro));
+                vld<4>(pbz, at(a.cbz, ro));
             }
             if (xst && z >= 4 && z < nzp - 4) {
-                const long long ro = (long long)z * P + col;
-                vld<4>(pxin, m.pxi + so + ro);
-                vld<4>(pax, a.cax + ro);
-                vld<4>(pbx, a.cbx + ro);
-                vld<4>(zxo, a.zetax + so + ro);
+                const unsigned ro = (unsigned)(z * P + col);
+                vld<4>(pxin, at(pxi, ro));
+                vld<4>(pax, at(a.cax, ro));
+                vld<4>(pbx, at(a.cbx, ro));
+                vld<4>(zxo, at(zetax, ro));
             }
             if (xst && tid < 32) {
                 const int hz = zc + hr, hc = x0 - 4 + 4 * hg;
                 if (hz >= 4 && hz < nzp - 4 && hc >= 0 && hc < P) {
-                    const long long ro = (long long)hz * P + hc;
-                    vld<4>(hxin, m.pxi + so + ro);
-                    vld<4>(hax, a.cax + ro);
-                    vld<4>(hbx, a.cbx + ro);
+                    const unsigned ro = (unsigned)(hz * P + hc);
+                    vld<4>(hxin, at(pxi, ro));
+                    vld<4>(hax, at(a.cax, ro));
+                    vld<4>(hbx, at(a.cbx, ro));
                 }
             }
             if (zs) {
-                const long long ro = (long long)z * P + col;
-                vld<4>(zzo, a.zetaz + so + ro);
-                vld<4>(zaz, a.caz + ro);
-                vld<4>(zbz, a.cbz + ro);
+                const unsigned ro = (unsigned)(z * P + col);
+                vld<4>(zzo, at(zetaz, ro));
+                vld<4>(zaz, at(a.caz, ro));
+                vld<4>(zbz, at(a.cbz, ro));
             }
         }
         // ---- prefetch chunk ch+1: U rows zc+TC+LA .. zc+2TC+LA-1, and its cells
@@ -276,7 +283,7 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b, const
             const int gz = zc + TC + LA + r, gc = x0 - HX + 4 * q;
             pf[u] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (more && i < NEW && gz < nzp && gc >= 0 && gc < P && want_q(q))
-                pf[u] = *reinterpret_cast<const float4 *>(cur + (long long)gz * P + gc);
+                pf[u] = *reinterpret_cast<const float4 *>(at(cur, (unsigned)(gz * P + gc)));
         }
         if (more) march_load_cell<PHYS>(mn, a, prv, zc + TC + rg, col);
 
@@ -303,7 +310,7 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b, const
                 float nv[4];
                 psiz_new(nv, pzin, paz, pbz, zr, cx);
                 o = make_float4(nv[0], nv[1], nv[2], nv[3]);
-                if (zr < ze) store_live4(m.pzo + so + (long long)zr * P + col, nv, col, nxp);
+                if (zr < ze) store_live4(at(pzo, (unsigned)(zr * P + col)), nv, col, nxp);
             }
             pzr[PSLOT(zr) + cx] = o;
             sync_needed = sync_needed || zchunk;
@@ -321,7 +328,7 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b, const
 
         // ---- compute row z of chunk ch
         if (z < ze) {
-            const long long ro = (long long)z * P + col;
+            const unsigned ro = (unsigned)(z * P + col);
             const float fz = (!CPML && (flags & F_CERJAN)) ? a.cerz[z] : 1.0f;
             float zw[9][4];
 #pragma unroll
@@ -443,7 +450,7 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b, const
                     dpx[j] = dps;
                     PX[j] = px[j] + dps;
                     PXX[j] = pxx[j] + dps + zx[j];
-                    a.zetax[so + ro + j] = zx[j];
+                    *at(zetax, ro + j) = zx[j];
                 }
             }
             if (CPML && zs) {
@@ -462,7 +469,7 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b, const
                     dpz[j] = dps;
                     PZ[j] = pz[j] + dps;
                     PZZ[j] = pzz[j] + dps + zz[j];
-                    if (xlive[j]) a.zetaz[so + ro + j] = zz[j];
+                    if (xlive[j]) *at(zetaz, ro + j) = zz[j];
                 }
             }
 
@@ -522,11 +529,11 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b, const
                 }
             }
             if (xlive[0] && xlive[3]) {
-                vst<4>(prv + ro, out);
+                vst<4>(at(prv, ro), out);
             } else {
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    if (xlive[j]) prv[ro + j] = out[j];
+                    if (xlive[j]) *at(prv, ro + j) = out[j];
             }
         }
 
