@@ -97,7 +97,13 @@ enum pam_option {
     PAM_OPT_PROFILE = 9,   /* 1: HIP events around every step kernel of pam_forward (the
                               kernel time pam_last_timing reports; launches are then not
                               graph-replayed); 0 (default): no per-launch events */
-    PAM_OPT_SNAPSHOTS = 10 /* checkpoint slots of PAM_STORE_REVOLVE_* (default 50, >= 1) */
+    PAM_OPT_SNAPSHOTS = 10, /* checkpoint slots of PAM_STORE_REVOLVE_* (default 50, >= 1) */
+    PAM_OPT_LANES = 11      /* pam_forward: 2 (default) advances two batches at once on two
+                               streams (the context's and one of its own), each lane
+                               PAM_OPT_BATCH shots per launch, or half the autotuned batch
+                               when that is 0, so that one lane's CPML pre-pass and launch
+                               gaps overlap the other's step kernel; 1: one batch at a time.
+                               Runs with PAM_OPT_PROFILE use one lane. */
 };
 
 typedef struct pam_grid {

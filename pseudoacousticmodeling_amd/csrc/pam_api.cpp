@@ -39,6 +39,9 @@ struct pam_ctx {
     hipStream_t st = nullptr;
     bool own_stream = false;
     int max_shots = 1;
+    // second lane of pam_forward: its stream and the fork / join events
+    hipStream_t st2 = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 
     // model coefficient fields
     float *v2 = nullptr, *eps = nullptr, *del = nullptr, *cs = nullptr, *sn = nullptr;
@@ -71,7 +74,7 @@ struct pam_ctx {
     int last_batch = 0;              // shots per launch of the last whole run
     // options
     int batch = 0, tile = -1, graph = 1, storage = PAM_STORE_ALL_F32, ckpt_step = 0;
-    int reset_adj_cpml = 1, normalize = 1, adj_range = 0, snapshots = 50;
+    int reset_adj_cpml = 1, normalize = 1, adj_range = 0, snapshots = 50, lanes = 2;
     long long steps_run = 0;  // time steps launched by the current adjoint run
     // checkpoints of the main set ([segment][field][shot][plane]) and the
     // psi_flip at each
@@ -295,6 +298,9 @@ extern "C" int pam_destroy(pam_ctx *ctx) {
     for (auto e : ctx->kev) (void)hipEventDestroy(e);
     if (ctx->e0) (void)hipEventDestroy(ctx->e0);
     if (ctx->e1) (void)hipEventDestroy(ctx->e1);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+    if (ctx->st2) (void)hipStreamDestroy(ctx->st2);
     if (ctx->own_stream && ctx->st) (void)hipStreamDestroy(ctx->st);
     delete ctx;
     return PAM_OK;
@@ -347,6 +353,10 @@ extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
             if (v < 1 || v > 1000000) return fail(ctx, PAM_E_ARG, "snapshots must be in [1, 1e6]");
             ctx->snapshots = (int)v;
             return PAM_OK;
+        case PAM_OPT_LANES:
+            if (v < 1 || v > 2) return fail(ctx, PAM_E_ARG, "lanes must be 1 or 2");
+            ctx->lanes = (int)v;
+            return PAM_OK;
         default:
             return fail(ctx, PAM_E_ARG, "unknown option %d", key);
     }
@@ -365,6 +375,7 @@ extern "C" int pam_get_option(const pam_ctx *ctx, int key, long long *value) {
         case PAM_OPT_ADJ_RANGE: *value = ctx->adj_range; return PAM_OK;
         case PAM_OPT_PROFILE: *value = ctx->profile; return PAM_OK;
         case PAM_OPT_SNAPSHOTS: *value = ctx->snapshots; return PAM_OK;
+        case PAM_OPT_LANES: *value = ctx->lanes; return PAM_OK;
         default: return PAM_E_ARG;
     }
 }
@@ -544,20 +555,22 @@ static void base_psi_args(pam_ctx *ctx, PsiArgs &p, int nshots, const float *cur
     p.cbz = ctx->cbz;
 }
 
-static int zero_fields(pam_ctx *ctx, FieldSet &f, int nshots, bool fields, bool cpml) {
+// (on stream st, or the context's when null)
+static int zero_fields(pam_ctx *ctx, FieldSet &f, int nshots, bool fields, bool cpml, hipStream_t st = nullptr) {
+    if (!st) st = ctx->st;
     const size_t fb = (size_t)ctx->plane * nshots * sizeof(float);
     if (fields) {
-        HCK(hipMemsetAsync(f.U0, 0, fb, ctx->st));
-        HCK(hipMemsetAsync(f.U1, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(f.U0, 0, fb, st));
+        HCK(hipMemsetAsync(f.U1, 0, fb, st));
     }
     if (cpml && ctx->abc == PAM_CPML) {
-        HCK(hipMemsetAsync(f.psix, 0, fb, ctx->st));
-        HCK(hipMemsetAsync(f.psiz, 0, fb, ctx->st));
-        HCK(hipMemsetAsync(f.psix + ctx->psi_half, 0, fb, ctx->st));
-        HCK(hipMemsetAsync(f.psiz + ctx->psi_half, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(f.psix, 0, fb, st));
+        HCK(hipMemsetAsync(f.psiz, 0, fb, st));
+        HCK(hipMemsetAsync(f.psix + ctx->psi_half, 0, fb, st));
+        HCK(hipMemsetAsync(f.psiz + ctx->psi_half, 0, fb, st));
         f.psi_flip = 0;
-        HCK(hipMemsetAsync(f.zetax, 0, fb, ctx->st));
-        HCK(hipMemsetAsync(f.zetaz, 0, fb, ctx->st));
+        HCK(hipMemsetAsync(f.zetax, 0, fb, st));
+        HCK(hipMemsetAsync(f.zetaz, 0, fb, st));
     }
     return PAM_OK;
 }
@@ -565,19 +578,23 @@ static int zero_shot_state(pam_ctx *ctx, int nshots, bool fields, bool cpml) {
     return zero_fields(ctx, ctx->fs, nshots, fields, cpml);
 }
 
-static int upload_sources(pam_ctx *ctx, int shot0, int nshots) {
+// source cells of shots shot0 .. shot0+nshots-1 into slots slot0.. of d_ssz / d_ssx
+static int upload_sources(pam_ctx *ctx, int shot0, int nshots, int slot0 = 0, hipStream_t st = nullptr) {
+    if (!st) st = ctx->st;
     for (int s = 0; s < nshots; ++s)
         if (shot0 + s < 0 || shot0 + s >= ctx->nshot)
             return fail(ctx, PAM_E_ARG, "shot %d out of range [0,%d)", shot0 + s, ctx->nshot);
-    HCK(hipMemcpyAsync(ctx->d_ssz, ctx->hsz.data() + shot0, nshots * sizeof(int), hipMemcpyHostToDevice, ctx->st));
-    HCK(hipMemcpyAsync(ctx->d_ssx, ctx->hsx.data() + shot0, nshots * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+    HCK(hipMemcpyAsync(ctx->d_ssz + slot0, ctx->hsz.data() + shot0, nshots * sizeof(int), hipMemcpyHostToDevice, st));
+    HCK(hipMemcpyAsync(ctx->d_ssx + slot0, ctx->hsx.data() + shot0, nshots * sizeof(int), hipMemcpyHostToDevice, st));
     return PAM_OK;
 }
 
 // One forward time step of `nshots` shots on the field set f: psi pre-pass
-// (CPML, unless the kernel variant updates psi itself) then the fused step.
+// (CPML, unless the kernel variant updates psi itself) then the fused step,
+// on stream st (the context's when null).
 static int fwd_step_fs(pam_ctx *ctx, FieldSet &f, StepArgs &a, int tile, float *cur, float *prv, int k,
-                       hipEvent_t *ev) {
+                       hipEvent_t *ev, hipStream_t st = nullptr) {
+    if (!st) st = ctx->st;
     if (ctx->abc == PAM_CPML) {
         const long long in = f.psi_flip ? ctx->psi_half : 0, out = f.psi_flip ? 0 : ctx->psi_half;
         a.psix_in = f.psix + in;
@@ -593,15 +610,15 @@ static int fwd_step_fs(pam_ctx *ctx, FieldSet &f, StepArgs &a, int tile, float *
             p.psiz_in = a.psiz_in;
             p.psix = a.psix;
             p.psiz = a.psiz;
-            KCK(launch_psi(p, ctx->st));
+            KCK(launch_psi(p, st));
         }
     }
     a.cur = cur;
     a.prv = prv;
     a.k = k;
-    if (ev) HCK(hipEventRecord(ev[0], ctx->st));
-    KCK(launch_step(ctx->phys, ctx->abc == PAM_CPML, tile, a, ctx->st));
-    if (ev) HCK(hipEventRecord(ev[1], ctx->st));
+    if (ev) HCK(hipEventRecord(ev[0], st));
+    KCK(launch_step(ctx->phys, ctx->abc == PAM_CPML, tile, a, st));
+    if (ev) HCK(hipEventRecord(ev[1], st));
     ++ctx->steps_run;
     if (ctx->abc == PAM_CPML) f.psi_flip ^= 1;
     return PAM_OK;
@@ -789,21 +806,24 @@ static int finish_timing(pam_ctx *ctx, long long nk) {
 // `key` must name everything they bake in that can change between calls;
 // `replay_fix` applies the host-side effects of `body` when it is skipped.
 template <class Body, class Fix>
-static int run_graph(pam_ctx *ctx, const std::string &key, Body &&body, Fix &&replay_fix) {
+static int run_graph(pam_ctx *ctx, const std::string &key, Body &&body, Fix &&replay_fix, hipStream_t st = nullptr) {
+    if (!st) st = ctx->st;
     auto it = ctx->graphs.find(key);
     if (it != ctx->graphs.end()) {
         replay_fix();
-        HCK(hipGraphLaunch(it->second, ctx->st));
+        HCK(hipGraphLaunch(it->second, st));
         return PAM_OK;
     }
-    if (ctx->graphs.size() >= 16) {  // bound the cache: drop every entry
+    if (ctx->graphs.size() >= 16) {  // bound the cache: drop every entry once none can be running
+        HCK(hipStreamSynchronize(ctx->st));
+        if (ctx->st2) HCK(hipStreamSynchronize(ctx->st2));
         for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
         ctx->graphs.clear();
     }
-    HCK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+    HCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     const int r = body();
     hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(ctx->st, &g);
+    const hipError_t e = hipStreamEndCapture(st, &g);
     if (r) {
         if (g) (void)hipGraphDestroy(g);
         return r;
@@ -814,7 +834,7 @@ static int run_graph(pam_ctx *ctx, const std::string &key, Body &&body, Fix &&re
     (void)hipGraphDestroy(g);
     if (ei != hipSuccess) return fail(ctx, PAM_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
     ctx->graphs[key] = ex;
-    HCK(hipGraphLaunch(ex, ctx->st));
+    HCK(hipGraphLaunch(ex, st));
     return PAM_OK;
 }
 
@@ -822,21 +842,26 @@ static int run_graph(pam_ctx *ctx, const std::string &key, Body &&body, Fix &&re
 // extra flags / args are applied to every step.  With PAM_OPT_PROFILE each
 // step kernel is bracketed by HIP events; otherwise, with PAM_OPT_GRAPH, the
 // loop is replayed as a captured graph.
+static int run_forward_loop_fs(pam_ctx *ctx, FieldSet &f, hipStream_t st, StepArgs &a, int tile, bool timed);
 static int run_forward_loop(pam_ctx *ctx, StepArgs &a, int tile, bool timed) {
+    return run_forward_loop_fs(ctx, ctx->fs, ctx->st, a, tile, timed);
+}
+// the same on field set f and stream st
+static int run_forward_loop_fs(pam_ctx *ctx, FieldSet &f, hipStream_t st, StepArgs &a, int tile, bool timed) {
     const int nt = ctx->g.nt;
     const bool events = timed && ctx->profile;
     if (events) {
         int r = ensure_events(ctx, 2 * (size_t)nt);
         if (r) return r;
     }
-    if (timed) HCK(hipEventRecord(ctx->e0, ctx->st));
+    if (timed) HCK(hipEventRecord(ctx->e0, st));
     const int base_flags = a.flags;
     auto body = [&]() -> int {
-        float *cur = ctx->fs.U0, *prv = ctx->fs.U1;
+        float *cur = f.U0, *prv = f.U1;
         for (int k = 0; k < nt; ++k) {
             a.flags = base_flags;
             if (!(k >= ctx->g.itlag)) a.flags &= ~F_GATHER;
-            int r = fwd_step(ctx, a, tile, cur, prv, k, events ? &ctx->kev[2 * k] : nullptr);
+            int r = fwd_step_fs(ctx, f, a, tile, cur, prv, k, events ? &ctx->kev[2 * k] : nullptr, st);
             if (r) return r;
             std::swap(cur, prv);
         }
@@ -845,18 +870,21 @@ static int run_forward_loop(pam_ctx *ctx, StepArgs &a, int tile, bool timed) {
     int r;
     if (ctx->graph && !events) {
         char key[256];
-        std::snprintf(key, sizeof key, "fwd|%u|%d|%d|%d|%p|%p", ctx->epoch, a.nshots, tile, base_flags,
-                      (void *)a.seis, (const void *)a.data);
-        r = run_graph(ctx, key, body, [&]() {
-            if (ctx->abc == PAM_CPML) ctx->fs.psi_flip ^= (nt & 1);
-        });
+        std::snprintf(key, sizeof key, "fwd|%u|%d|%d|%d|%p|%p|%p|%p", ctx->epoch, a.nshots, tile, base_flags,
+                      (void *)a.seis, (const void *)a.data, (void *)f.U0, (const void *)a.ssz);
+        r = run_graph(
+                ctx, key, body,
+                [&]() {
+                    if (ctx->abc == PAM_CPML) f.psi_flip ^= (nt & 1);
+                },
+                st);
     } else {
         r = body();
     }
     a.flags = base_flags;
     if (r) return r;
     if (timed) {
-        HCK(hipEventRecord(ctx->e1, ctx->st));
+        HCK(hipEventRecord(ctx->e1, st));
         r = finish_timing(ctx, events ? nt : 0);
         if (r) return r;
         if (!events) {
@@ -866,6 +894,80 @@ static int run_forward_loop(pam_ctx *ctx, StepArgs &a, int tile, bool timed) {
             ctx->launches += nt;
         }
     }
+    return PAM_OK;
+}
+
+// pam_forward on two lanes of Bl shots per launch: batches alternate between
+// the context's stream and a second stream, each lane with its own shot slots,
+// source slots, gather staging and cached graph; the second stream starts
+// after the work already queued on the first, and the first waits for it at
+// the end.  One lane's psi pre-pass and launch gaps then overlap the other's
+// step kernel.  Every batch is computed exactly as on one lane.
+static int forward_two_lanes(pam_ctx *ctx, int shot0, int nshots, float *seis, int Bl, long long nd) {
+    if (!ctx->st2) HCK(hipStreamCreate(&ctx->st2));  // default priority, like the context's stream
+    if (!ctx->fork_ev) HCK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    if (!ctx->join_ev) HCK(hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming));
+    const int tile = choose_tile(ctx, Bl, true);
+    ctx->last_tile = tile;
+    ctx->last_batch = Bl;
+    const bool staged = ctx->graph != 0;
+    if (staged) {
+        int r = ensure_scratch(ctx, (void **)&ctx->stage, &ctx->stage_bytes, 2LL * Bl * nd * 4);
+        if (r) return r;
+    }
+    // lane 1's view of the field set: shot slots Bl .. 2 Bl - 1
+    FieldSet lane1 = ctx->fs;
+    const long long off = (long long)Bl * ctx->plane;
+    lane1.U0 += off;
+    lane1.U1 += off;
+    if (ctx->abc == PAM_CPML) {
+        lane1.psix += off;
+        lane1.psiz += off;
+        lane1.zetax += off;
+        lane1.zetaz += off;
+    }
+    FieldSet *fsl[2] = {&ctx->fs, &lane1};
+    hipStream_t stl[2] = {ctx->st, ctx->st2};
+    HCK(hipEventRecord(ctx->e0, ctx->st));
+    HCK(hipEventRecord(ctx->fork_ev, ctx->st));
+    HCK(hipStreamWaitEvent(ctx->st2, ctx->fork_ev, 0));
+    int nbatch = 0;
+    for (int b0 = 0; b0 < nshots; b0 += Bl, ++nbatch) {
+        const int lane = nbatch & 1;
+        const int nb = std::min(Bl, nshots - b0);
+        FieldSet &f = *fsl[lane];
+        hipStream_t st = stl[lane];
+        int r = zero_fields(ctx, f, nb, true, true, st);
+        if (r) return r;
+        r = upload_sources(ctx, shot0 + b0, nb, lane * Bl, st);
+        if (r) return r;
+        float *dst = staged ? ctx->stage + (long long)lane * Bl * nd : seis + (long long)b0 * nd;
+        HCK(hipMemsetAsync(dst, 0, (size_t)nb * nd * sizeof(float), st));
+        KCK(launch_inject_point(f.U0, ctx->plane, ctx->d_ssz + lane * Bl, ctx->d_ssx + lane * Bl, ctx->P, ctx->wav,
+                                0, nb, st));
+        StepArgs a;
+        base_step_args(ctx, a, nb);
+        a.flags |= F_SRC | F_GATHER;
+        a.seis = dst;
+        a.dstride = nd;
+        a.ssz = ctx->d_ssz + lane * Bl;
+        a.ssx = ctx->d_ssx + lane * Bl;
+        r = run_forward_loop_fs(ctx, f, st, a, tile, false);
+        if (r) return r;
+        if (staged)
+            HCK(hipMemcpyAsync(seis + (long long)b0 * nd, dst, (size_t)nb * nd * sizeof(float),
+                               hipMemcpyDeviceToDevice, st));
+    }
+    HCK(hipEventRecord(ctx->join_ev, ctx->st2));
+    HCK(hipStreamWaitEvent(ctx->st, ctx->join_ev, 0));
+    HCK(hipEventRecord(ctx->e1, ctx->st));
+    HCK(hipEventSynchronize(ctx->e1));
+    float ms = 0.f;
+    HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
+    ctx->loop_ms += ms;
+    ctx->kernel_ms += ms;
+    ctx->launches += (long long)nbatch * ctx->g.nt;
+    HCK(hipStreamSynchronize(ctx->st));
     return PAM_OK;
 }
 
@@ -879,7 +981,16 @@ extern "C" int pam_forward(pam_ctx *ctx, int shot0, int nshots, float *seis) {
     ctx->loop_ms = ctx->kernel_ms = 0.0;
     ctx->launches = 0;
     ctx->next_k = -1;
-    const int B = batch_size(ctx, nshots);
+    int B = batch_size(ctx, nshots);
+    if (ctx->lanes > 1 && nshots > 1 && ctx->max_shots > 1) {
+        // two lanes: each half the batch (or PAM_OPT_BATCH), in shot slots
+        // [0, Bl) and [Bl, 2 Bl) of the field set, on the context's stream and
+        // the second one; batch i goes to lane i % 2.  Profiled runs make the
+        // same launches on one lane.
+        const int Bl = std::max(1, std::min(ctx->batch > 0 ? ctx->batch : (B + 1) / 2, ctx->max_shots / 2));
+        if (!ctx->profile) return forward_two_lanes(ctx, shot0, nshots, seis, Bl, nd);
+        B = Bl;
+    }
     const int tile = choose_tile(ctx, B, true);
     ctx->last_tile = tile;
     ctx->last_batch = B;

@@ -59,6 +59,7 @@ def test_graph_recaptured_after_model_change():
 def test_profiled_run_reports_kernel_time():
     pb = make_problem(0, 1, nshots=2, seed=520, nt=150)
     eng = make_engine(pb, max_shots=2)
+    eng.set_option(_lib.PAM_OPT_LANES, 1)  # one batch of both shots
     eng.set_option(_lib.PAM_OPT_PROFILE, 1)
     a = _fwd(eng, 2)
     t = eng.timing()

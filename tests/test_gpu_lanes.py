@@ -1,0 +1,73 @@
+"""Two-lane forward runs (PAM_OPT_LANES 2): batches alternate between the
+context's stream and a second stream, each lane in its own shot slots, source
+slots, gather staging and cached graph.  The seismograms must be bit for bit
+those of the same launches made one batch at a time on one stream, with an odd
+number of batches (the lanes get unequal work), on graph capture and replay,
+with graphs off, and with the batch left to the autotuner."""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, rel_l2
+from pseudoacousticmodeling_amd import _lib
+from tests.problems import make_problem, make_engine, oracle_forward
+
+pytestmark = pytest.mark.gpu
+
+
+def _fwd(eng, nshots):
+    return eng.forward(0, nshots).cpu().numpy()
+
+
+@pytest.mark.parametrize("phys,abc", [(0, 0), (1, 1), (2, 1)])
+def test_two_lanes_equal_one_lane(phys, abc):
+    print(f"case lanes phys={phys} abc={abc}", flush=True)
+    pb = make_problem(phys, abc, nshots=7, seed=560 + 2 * phys + abc, nt=241)  # odd nt
+    eng = make_engine(pb, max_shots=7)
+    eng.set_option(_lib.PAM_OPT_TILE, 0)
+    eng.set_option(_lib.PAM_OPT_BATCH, 3)  # batches 3, 3, 1
+    eng.set_option(_lib.PAM_OPT_LANES, 1)
+    ref = _fwd(eng, 7)
+    eng.set_option(_lib.PAM_OPT_LANES, 2)
+    assert eng.get_option(_lib.PAM_OPT_LANES) == 2
+    for call in range(3):  # capture, then replays
+        got = _fwd(eng, 7)
+        assert np.array_equal(got, ref), f"call {call}"
+    assert eng.get_option(_lib.PAM_OPT_BATCH) == 3
+    eng.set_option(_lib.PAM_OPT_GRAPH, 0)
+    assert np.array_equal(_fwd(eng, 7), ref)
+    orc = oracle_forward(Oracle("f32"), pb, shot=6)["seis"]
+    assert rel_l2(ref[6], orc) <= 1e-4
+
+
+def test_two_lanes_with_the_autotuned_batch():
+    """PAM_OPT_BATCH 0: each lane takes half the autotuned batch; a profiled
+    run (one lane) makes the same launches."""
+    pb = make_problem(1, 1, nshots=6, seed=570, nt=160)
+    eng = make_engine(pb, max_shots=6)
+    got = _fwd(eng, 6)
+    bl = eng.get_option(_lib.PAM_OPT_BATCH)
+    tile = eng.get_option(_lib.PAM_OPT_TILE)
+    assert 1 <= bl <= 3
+    eng.set_option(_lib.PAM_OPT_PROFILE, 1)
+    prof = _fwd(eng, 6)
+    t = eng.timing()
+    assert t["kernel_launches"] == pb["g"].nt * -(-6 // bl)
+    eng.set_option(_lib.PAM_OPT_PROFILE, 0)
+    assert np.array_equal(prof, got)
+    eng.set_option(_lib.PAM_OPT_LANES, 1)
+    eng.set_option(_lib.PAM_OPT_BATCH, bl)
+    eng.set_option(_lib.PAM_OPT_TILE, tile)
+    assert np.array_equal(_fwd(eng, 6), got)
+
+
+def test_lanes_option_is_validated():
+    pb = make_problem(0, 0, nshots=1, seed=580, nt=50)
+    eng = make_engine(pb, max_shots=1)
+    for bad in (0, 3):
+        with pytest.raises(RuntimeError):
+            eng.set_option(_lib.PAM_OPT_LANES, bad)
+    eng.set_option(_lib.PAM_OPT_LANES, 1)
+    assert eng.get_option(_lib.PAM_OPT_LANES) == 1
+    # one shot, or max_shots 1: one lane whatever the option
+    eng.set_option(_lib.PAM_OPT_LANES, 2)
+    assert np.isfinite(_fwd(eng, 1)).all()
