@@ -932,34 +932,42 @@ static int forward_two_lanes(pam_ctx *ctx, int shot0, int nshots, float *seis, i
     HCK(hipEventRecord(ctx->fork_ev, ctx->st));
     HCK(hipStreamWaitEvent(ctx->st2, ctx->fork_ev, 0));
     int nbatch = 0;
-    for (int b0 = 0; b0 < nshots; b0 += Bl, ++nbatch) {
-        const int lane = nbatch & 1;
-        const int nb = std::min(Bl, nshots - b0);
-        FieldSet &f = *fsl[lane];
-        hipStream_t st = stl[lane];
-        int r = zero_fields(ctx, f, nb, true, true, st);
-        if (r) return r;
-        r = upload_sources(ctx, shot0 + b0, nb, lane * Bl, st);
-        if (r) return r;
-        float *dst = staged ? ctx->stage + (long long)lane * Bl * nd : seis + (long long)b0 * nd;
-        HCK(hipMemsetAsync(dst, 0, (size_t)nb * nd * sizeof(float), st));
-        KCK(launch_inject_point(f.U0, ctx->plane, ctx->d_ssz + lane * Bl, ctx->d_ssx + lane * Bl, ctx->P, ctx->wav,
-                                0, nb, st));
-        StepArgs a;
-        base_step_args(ctx, a, nb);
-        a.flags |= F_SRC | F_GATHER;
-        a.seis = dst;
-        a.dstride = nd;
-        a.ssz = ctx->d_ssz + lane * Bl;
-        a.ssx = ctx->d_ssx + lane * Bl;
-        r = run_forward_loop_fs(ctx, f, st, a, tile, false);
-        if (r) return r;
-        if (staged)
-            HCK(hipMemcpyAsync(seis + (long long)b0 * nd, dst, (size_t)nb * nd * sizeof(float),
-                               hipMemcpyDeviceToDevice, st));
-    }
-    HCK(hipEventRecord(ctx->join_ev, ctx->st2));
-    HCK(hipStreamWaitEvent(ctx->st, ctx->join_ev, 0));
+    auto batches = [&]() -> int {
+        for (int b0 = 0; b0 < nshots; b0 += Bl, ++nbatch) {
+            const int lane = nbatch & 1;
+            const int nb = std::min(Bl, nshots - b0);
+            FieldSet &f = *fsl[lane];
+            hipStream_t st = stl[lane];
+            int r = zero_fields(ctx, f, nb, true, true, st);
+            if (r) return r;
+            r = upload_sources(ctx, shot0 + b0, nb, lane * Bl, st);
+            if (r) return r;
+            float *dst = staged ? ctx->stage + (long long)lane * Bl * nd : seis + (long long)b0 * nd;
+            HCK(hipMemsetAsync(dst, 0, (size_t)nb * nd * sizeof(float), st));
+            KCK(launch_inject_point(f.U0, ctx->plane, ctx->d_ssz + lane * Bl, ctx->d_ssx + lane * Bl, ctx->P,
+                                    ctx->wav, 0, nb, st));
+            StepArgs a;
+            base_step_args(ctx, a, nb);
+            a.flags |= F_SRC | F_GATHER;
+            a.seis = dst;
+            a.dstride = nd;
+            a.ssz = ctx->d_ssz + lane * Bl;
+            a.ssx = ctx->d_ssx + lane * Bl;
+            r = run_forward_loop_fs(ctx, f, st, a, tile, false);
+            if (r) return r;
+            if (staged)
+                HCK(hipMemcpyAsync(seis + (long long)b0 * nd, dst, (size_t)nb * nd * sizeof(float),
+                                   hipMemcpyDeviceToDevice, st));
+        }
+        return PAM_OK;
+    };
+    const int rb = batches();
+    // join on every path: the context's stream then orders after all of the second lane's work
+    const hipError_t ej = hipEventRecord(ctx->join_ev, ctx->st2);
+    const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(ctx->st, ctx->join_ev, 0) : ej;
+    if (ew != hipSuccess) (void)hipStreamSynchronize(ctx->st2);
+    if (rb) return rb;
+    if (ew != hipSuccess) return fail(ctx, PAM_E_HIP, "joining the second lane: %s", hipGetErrorString(ew));
     HCK(hipEventRecord(ctx->e1, ctx->st));
     HCK(hipEventSynchronize(ctx->e1));
     float ms = 0.f;
