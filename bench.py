@@ -67,6 +67,42 @@ def make_engine(w, max_shots):
     return eng
 
 
+def configs_leg(nt_cap):
+    """The other single-grid workloads, forward modelling with graph replay
+    and the autotuner's choices: acoustic + Cerjan, one shot, every step
+    (config 1); TTI + CPML, 32 shots, and acoustic + CPML, 32 shots (configs 3
+    and 4) on the first `nt_cap` steps, reported per step as the full runs'
+    rate (labelled reduced)."""
+    import torch
+
+    from pseudoacousticmodeling_amd import _lib, workloads
+
+    out = {}
+    for build, cap in ((workloads.acoustic_cerjan_1, None), (workloads.tti_cpml_32, nt_cap),
+                       (workloads.acoustic_cpml_32, nt_cap)):
+        w = build() if cap is None else build(nt_data=cap)
+        S = len(w["sx"])
+        eng = make_engine(w, S)
+        seis = torch.empty((S, w["nt"] - w["itlag"], len(w["rx"])), dtype=torch.float32, device="cuda:0")
+        eng.forward(0, S, out=seis)  # tunes, captures
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.forward(0, S, out=seis)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        full = build()
+        out[w["name"]] = {"value": workloads.updates(w) / dt / 1e9, "unit": "Gpt-updates/s", "seconds": dt,
+                          "steps_run": w["nt"], "steps_full": full["nt"], "reduced": cap is not None,
+                          "shots": S, "padded": [w["nz"] + 2 * w["N"], w["nx"] + 2 * w["N"]],
+                          "shots_per_launch": eng.get_option(_lib.PAM_OPT_BATCH),
+                          "variant": eng.get_option(_lib.PAM_OPT_TILE),
+                          "finite": bool(torch.isfinite(seis).all().item())}
+        eng.close()
+        del seis
+        torch.cuda.empty_cache()
+    return out
+
+
 def group_leg(shots_per_model):
     """Sixteen VTI + CPML models of different sizes (201..601 x 801..2401,
     every pair once), ``shots_per_model`` shots each, nt = 3001 + itlag: all
@@ -193,6 +229,9 @@ def main(argv=None):
                     help="shots advanced per launch (0: the autotuner's choice, halved per lane)")
     ap.add_argument("--lanes", type=int, default=2, help="batches advanced at once on their own streams (1 or 2)")
     ap.add_argument("--no-group", action="store_true", help="skip the uneven-grid group leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other single-grid workloads")
+    ap.add_argument("--configs-steps", type=int, default=500,
+                    help="recorded samples of the reduced config 3 / 4 runs")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the steps one by one instead of replaying captured graphs")
     ap.add_argument("--group-shots", type=int, default=4, help="shots per model in the uneven-grid group leg")
@@ -322,6 +361,13 @@ def main(argv=None):
         print(f"bench: uneven-grid group {group['value']:.1f} Gpt-updates/s in one launch per step, "
               f"{group['per_model_loop']:.1f} model by model", file=sys.stderr, flush=True)
 
+    others = None
+    if not args.no_configs:
+        others = configs_leg(args.configs_steps)
+        for k, v in others.items():
+            print(f"bench: {k} {v['value']:.1f} Gpt-updates/s ({v['steps_run']} of {v['steps_full']} steps, "
+                  f"{v['shots']} shots)", file=sys.stderr, flush=True)
+
     cpu = None
     if args.cpu_seconds > 0:
         print("bench: cpu baseline ...", file=sys.stderr, flush=True)
@@ -353,6 +399,7 @@ def main(argv=None):
         "plain_loop": plain,
         "fwi_gradient": fwi,
         "uneven_group": group,
+        "other_configs": others,
         "cpu_baseline": cpu,
         "finite": finite,
     }

@@ -78,6 +78,74 @@ def acoustic_cerjan_1():
                 rz=rz, rx=rx, cpml_d0=0.0, cpml_f_pico=0.0, cerjan_factor=0.0045)
 
 
+def tti_cpml_32(nshots=32, nt_data=4001, seed=SEED0 + 3):
+    """Tilted TI with a 50-cell CPML (TTI + CPML extends the reference, whose
+    Python driver stops there): nz=401, nx=2001, dx=dz=10 m, dt=0.8 ms, 4001
+    recorded samples (T = 3.2 s); seven layers under six interfaces dipping
+    0, 4, .. 20 degrees, vp 1500..4000 m/s plus a +800 m/s Gaussian lens of
+    radius 300 m; epsilon 0..0.2 and delta 0..0.08 rising layer by layer; theta
+    the dip of the interface above (radians; the reference reads degrees from
+    disk); 32 surface shots spread over the model, Ricker 15 Hz peak
+    (fcut = 45 Hz); receivers at every grid column, z = 10 m."""
+    rng = np.random.default_rng(seed)
+    nz, nx, N, h, dt, fcut = 401, 2001, 50, 10.0, 0.8e-3, 45.0
+    z = np.arange(nz)[:, None] * h
+    x = np.arange(nx)[None, :] * h
+    dips = np.deg2rad(np.arange(6) * 4.0)
+    z0 = np.linspace(400.0, 3600.0, 6) + rng.uniform(-50.0, 50.0, 6)
+    xc = 0.5 * (nx - 1) * h
+    layer = np.zeros((nz, nx), dtype=np.int64)
+    theta = np.zeros((nz, nx))
+    for i in range(6):
+        below = z >= z0[i] + np.tan(dips[i]) * (x - xc)
+        layer += below
+        theta = np.where(below, dips[i], theta)
+    v_l = np.linspace(1500.0, 4000.0, 7)
+    vp = v_l[layer]
+    lz, lx = 0.55 * (nz - 1) * h, 0.6 * (nx - 1) * h
+    vp = vp + 800.0 * np.exp(-((z - lz) ** 2 + (x - lx) ** 2) / 300.0 ** 2)
+    eps = np.linspace(0.0, 0.2, 7)[layer]
+    delta = np.linspace(0.0, 0.08, 7)[layer]
+    tlag = 2.0 * np.sqrt(np.pi) / fcut
+    itlag = int(round(tlag / dt))
+    nt = itlag + nt_data
+    src = prep.ricker_cpp(nt, dt, fcut, np.float32(tlag), h, h)
+    xs = (np.arange(nshots) + 0.5) * (nx - 1) * h / nshots
+    return dict(name="tti_cpml_32", physics=2, abc=1, nz=nz, nx=nx, N=N, dz=h, dx=h, dt=dt, nt=nt, itlag=itlag,
+                fcut=fcut, vp=vp.astype(np.float32), eps=eps.astype(np.float32), delta=delta.astype(np.float32),
+                theta=theta.astype(np.float32), src=src, sz=prep.grid_index(np.full(nshots, 10.0), h, N),
+                sx=prep.grid_index(xs, h, N), rz=prep.grid_index(np.full(nx, 10.0), h, N),
+                rx=prep.grid_index(np.arange(nx) * h, h, N), cpml_d0=prep.dampening_const(N, fcut)[0],
+                cpml_f_pico=prep.dampening_const(N, fcut)[1], cerjan_factor=0.0)
+
+
+def acoustic_cpml_32(nshots=32, nt_data=3001, seed=SEED0 + 4):
+    """Acoustic with a 50-cell CPML: nz=301, nx=1201, dx=dz=10 m, dt=1 ms, 3001
+    recorded samples; six flat layers 1500..4500 m/s cut by a normal fault (the
+    block right of x = 6 km dropped by 150 m) and a -300 m/s Gaussian anomaly of
+    radius 200 m; 32 surface shots spread over the model, Ricker 8 Hz peak
+    (fcut = 24 Hz, itlag 148 with the C++ rounding); receivers at every grid
+    column, z = 10 m."""
+    nz, nx, N, h, dt, fcut = 301, 1201, 50, 10.0, 1e-3, 24.0
+    z = np.arange(nz)[:, None] * h
+    x = np.arange(nx)[None, :] * h
+    zf = z - np.where(x > 6000.0, 150.0, 0.0)
+    layer = np.clip((zf // 500.0).astype(np.int64), 0, 5)
+    vp = np.linspace(1500.0, 4500.0, 6)[layer]
+    vp = vp - 300.0 * np.exp(-((z - 1800.0) ** 2 + (x - 3000.0) ** 2) / 200.0 ** 2)
+    sz_ = prep.derived_sizes(12000.0, 3000.0, 3.0, h, h, dt, fcut, "round")
+    itlag = sz_["itlag"]
+    nt = itlag + nt_data
+    src = prep.ricker_cpp(nt, dt, fcut, np.float32(sz_["tlag"]), h, h)
+    xs = (np.arange(nshots) + 0.5) * (nx - 1) * h / nshots
+    d0, fp = prep.dampening_const(N, fcut)
+    return dict(name="acoustic_cpml_32", physics=0, abc=1, nz=nz, nx=nx, N=N, dz=h, dx=h, dt=dt, nt=nt,
+                itlag=itlag, fcut=fcut, vp=vp.astype(np.float32), eps=None, delta=None, theta=None, src=src,
+                sz=prep.grid_index(np.full(nshots, 10.0), h, N), sx=prep.grid_index(xs, h, N),
+                rz=prep.grid_index(np.full(nx, 10.0), h, N), rx=prep.grid_index(np.arange(nx) * h, h, N),
+                cpml_d0=d0, cpml_f_pico=fp, cerjan_factor=0.0)
+
+
 def uneven_vti_group(shots_per_model=4, nt_data=3001, seed=SEED0 + 5):
     """Sixteen VTI models with a 50-cell CPML, nz cycling over
     {201, 301, 401, 601} and nx over {801, 1201, 1601, 2401} (every pair once),
