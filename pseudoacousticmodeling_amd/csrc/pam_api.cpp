@@ -616,9 +616,7 @@ static int fwd_step_fs(pam_ctx *ctx, FieldSet &f, StepArgs &a, int tile, float *
     a.cur = cur;
     a.prv = prv;
     a.k = k;
-    if (ev) HCK(hipEventRecord(ev[0], st));
-    KCK(launch_step(ctx->phys, ctx->abc == PAM_CPML, tile, a, st));
-    if (ev) HCK(hipEventRecord(ev[1], st));
+    KCK(launch_step(ctx->phys, ctx->abc == PAM_CPML, tile, a, st, ev));  // ev: the kernel's own start / end
     ++ctx->steps_run;
     if (ctx->abc == PAM_CPML) f.psi_flip ^= 1;
     return PAM_OK;

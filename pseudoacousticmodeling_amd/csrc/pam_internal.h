@@ -128,7 +128,8 @@ TileShape tile_shape(int idx);
 bool tile_fuses_psi(int idx);
 
 // launchers (return hipError_t as int)
-int launch_step(int phys, bool cpml, int tile, const StepArgs &a, hipStream_t st);
+// ev: null, or two events that the launch stamps with the kernel's start and end
+int launch_step(int phys, bool cpml, int tile, const StepArgs &a, hipStream_t st, const hipEvent_t *ev = nullptr);
 int launch_psi(const PsiArgs &a, hipStream_t st);
 // One time step k of a group of grids in one launch (z-march variants only):
 // list[g] (device) holds grid g's arguments as of step 0, with nxt / nzt set for

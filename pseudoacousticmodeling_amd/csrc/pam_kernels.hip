@@ -457,12 +457,12 @@ TileShape tile_shape(int idx) {
 }
 bool tile_fuses_psi(int idx) { return tile_shape(idx).V < 0; }
 
-int launch_step_acoustic_cerjan(int, const StepArgs &, hipStream_t);
-int launch_step_acoustic_cpml(int, const StepArgs &, hipStream_t);
-int launch_step_vti_cerjan(int, const StepArgs &, hipStream_t);
-int launch_step_vti_cpml(int, const StepArgs &, hipStream_t);
-int launch_step_tti_cerjan(int, const StepArgs &, hipStream_t);
-int launch_step_tti_cpml(int, const StepArgs &, hipStream_t);
+int launch_step_acoustic_cerjan(int, const StepArgs &, hipStream_t, const hipEvent_t *);
+int launch_step_acoustic_cpml(int, const StepArgs &, hipStream_t, const hipEvent_t *);
+int launch_step_vti_cerjan(int, const StepArgs &, hipStream_t, const hipEvent_t *);
+int launch_step_vti_cpml(int, const StepArgs &, hipStream_t, const hipEvent_t *);
+int launch_step_tti_cerjan(int, const StepArgs &, hipStream_t, const hipEvent_t *);
+int launch_step_tti_cpml(int, const StepArgs &, hipStream_t, const hipEvent_t *);
 
 int launch_multi_acoustic_cerjan(int, const StepArgs *, const int *, int, int, int, hipStream_t);
 int launch_multi_acoustic_cpml(int, const StepArgs *, const int *, int, int, int, hipStream_t);
@@ -483,10 +483,11 @@ int launch_step_multi(int phys, bool cpml, int tile, const StepArgs *list, const
 }
 int march_chunks(int tile) { return tile_shape(tile).V < 0 ? tile_shape(tile).TZ : 0; }
 
-int launch_step(int phys, bool cpml, int tile, const StepArgs &a, hipStream_t st) {
-    if (phys == 0) return cpml ? launch_step_acoustic_cpml(tile, a, st) : launch_step_acoustic_cerjan(tile, a, st);
-    if (phys == 1) return cpml ? launch_step_vti_cpml(tile, a, st) : launch_step_vti_cerjan(tile, a, st);
-    return cpml ? launch_step_tti_cpml(tile, a, st) : launch_step_tti_cerjan(tile, a, st);
+int launch_step(int phys, bool cpml, int tile, const StepArgs &a, hipStream_t st, const hipEvent_t *ev) {
+    if (phys == 0)
+        return cpml ? launch_step_acoustic_cpml(tile, a, st, ev) : launch_step_acoustic_cerjan(tile, a, st, ev);
+    if (phys == 1) return cpml ? launch_step_vti_cpml(tile, a, st, ev) : launch_step_vti_cerjan(tile, a, st, ev);
+    return cpml ? launch_step_tti_cpml(tile, a, st, ev) : launch_step_tti_cerjan(tile, a, st, ev);
 }
 
 int launch_psi(const PsiArgs &a, hipStream_t st) {

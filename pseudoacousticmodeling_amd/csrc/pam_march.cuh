@@ -595,16 +595,16 @@ static int launch_march_multi_t(const StepArgs *list, const int *boff, int ngrid
 }
 
 template <int PHYS, bool CPML, int CHUNKS>
-static int launch_march_t(const StepArgs &a0, hipStream_t st) {
+static int launch_march_t(const StepArgs &a0, hipStream_t st, const hipEvent_t *ev = nullptr) {
     StepArgs a = a0;
     a.nxt = (a.nxp + 63) / 64;
     a.nzt = (a.nzp - 8 + CHUNKS * 16 - 1) / (CHUNKS * 16);  // row segments
     long long blocks = (long long)a.nxt * a.nzt * a.nshots;
     blocks = (blocks + 7) / 8 * 8;
     if ((a.flags & ~F_FWD_MASK) != 0)
-        hipLaunchKernelGGL((k_march<PHYS, CPML, CHUNKS, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+        launch_timed(k_march<PHYS, CPML, CHUNKS, true>, dim3((unsigned)blocks), st, ev, a);
     else
-        hipLaunchKernelGGL((k_march<PHYS, CPML, CHUNKS, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+        launch_timed(k_march<PHYS, CPML, CHUNKS, false>, dim3((unsigned)blocks), st, ev, a);
     return (int)hipGetLastError();
 }
 

@@ -8,6 +8,8 @@
 #include <stdint.h>
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "pam_internal.h"
 
 namespace pam {
@@ -30,6 +32,18 @@ template <typename T>
 __device__ __forceinline__ T *at(T *base, unsigned idx) {
     typedef typename std::conditional<std::is_const<T>::value, const char, char>::type B;
     return reinterpret_cast<T *>(reinterpret_cast<B *>(base) + idx * (unsigned)sizeof(T));
+}
+
+// Launches kernel(args...) on st; with ev (two events), the launch itself
+// records the kernel's start and end in ev[0] / ev[1] (hipExtLaunchKernelGGL),
+// so that the interval is the kernel's execution without the dispatch gap
+// that events recorded around the launch would add.
+template <typename K, typename... Args>
+static inline void launch_timed(K kernel, dim3 grid, hipStream_t st, const hipEvent_t *ev, Args... args) {
+    if (ev)
+        hipExtLaunchKernelGGL(kernel, grid, dim3(256), 0, st, ev[0], ev[1], 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, dim3(256), 0, st, args...);
 }
 
 template <int V> struct VecOf;
@@ -513,14 +527,14 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs a) {
 }
 
 template <int PHYS, bool CPML, int V, int TZ>
-static int launch_step_t(const StepArgs &a0, hipStream_t st) {
+static int launch_step_t(const StepArgs &a0, hipStream_t st, const hipEvent_t *ev = nullptr) {
     StepArgs a = a0;
     a.nxt = (a.nxp - 8 + 64 * V - 1) / (64 * V);
     a.nzt = (a.nzp - 8 + TZ - 1) / TZ;
     const long long waves = (long long)a.nxt * a.nzt * a.nshots;
     long long blocks = (waves + 3) / 4;
     blocks = (blocks + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_step<PHYS, CPML, V, TZ>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+    launch_timed(k_step<PHYS, CPML, V, TZ>, dim3((unsigned)blocks), st, ev, a);
     return (int)hipGetLastError();
 }
 

@@ -379,7 +379,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
 }
 
 template <int PHYS, bool CPML, int TZ, int WPE>
-static int launch_tile_t(const StepArgs &a0, hipStream_t st) {
+static int launch_tile_t(const StepArgs &a0, hipStream_t st, const hipEvent_t *ev = nullptr) {
     const bool adj = (a0.flags & ~F_FWD_MASK) != 0;
     StepArgs a = a0;
     a.nxt = (a.nxp + 63) / 64;
@@ -387,9 +387,9 @@ static int launch_tile_t(const StepArgs &a0, hipStream_t st) {
     long long blocks = (long long)a.nxt * a.nzt * a.nshots;
     blocks = (blocks + 7) / 8 * 8;
     if (adj)
-        hipLaunchKernelGGL((k_tile<PHYS, CPML, TZ, WPE, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+        launch_timed(k_tile<PHYS, CPML, TZ, WPE, true>, dim3((unsigned)blocks), st, ev, a);
     else
-        hipLaunchKernelGGL((k_tile<PHYS, CPML, TZ, WPE, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+        launch_timed(k_tile<PHYS, CPML, TZ, WPE, false>, dim3((unsigned)blocks), st, ev, a);
     return (int)hipGetLastError();
 }
 

@@ -14,15 +14,15 @@ static constexpr TileShape kTiles[] = {{0, 16}, {0, 32}, {0, 64}, {4, 16}, {-1, 
 static constexpr int kNumTiles = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
 
 template <int PHYS, bool CPML>
-static int launch_step_p(int tile, const StepArgs &a, hipStream_t st) {
+static int launch_step_p(int tile, const StepArgs &a, hipStream_t st, const hipEvent_t *ev) {
     switch (tile) {
-        case 0: return launch_tile_t<PHYS, CPML, 16, 1>(a, st);
-        case 1: return launch_tile_t<PHYS, CPML, 32, 1>(a, st);
-        case 2: return launch_tile_t<PHYS, CPML, 64, 1>(a, st);
-        case 3: return launch_step_t<PHYS, CPML, 4, 16>(a, st);
-        case 4: return launch_march_t<PHYS, CPML, 7>(a, st);
-        case 5: return launch_march_t<PHYS, CPML, 14>(a, st);
-        case 6: return launch_march_t<PHYS, CPML, 28>(a, st);
+        case 0: return launch_tile_t<PHYS, CPML, 16, 1>(a, st, ev);
+        case 1: return launch_tile_t<PHYS, CPML, 32, 1>(a, st, ev);
+        case 2: return launch_tile_t<PHYS, CPML, 64, 1>(a, st, ev);
+        case 3: return launch_step_t<PHYS, CPML, 4, 16>(a, st, ev);
+        case 4: return launch_march_t<PHYS, CPML, 7>(a, st, ev);
+        case 5: return launch_march_t<PHYS, CPML, 14>(a, st, ev);
+        case 6: return launch_march_t<PHYS, CPML, 28>(a, st, ev);
         default: return (int)hipErrorInvalidValue;
     }
 }
