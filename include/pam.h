@@ -259,10 +259,12 @@ int pam_group_create(pam_ctx *const *ctxs, int n, pam_group **out);
 int pam_group_destroy(pam_group *grp);
 /* Forward modeling of shots [0, nshots[i]) of every member i (nshots[i] <=
  * its max_shots) into seis[i] (device [nshots[i]][nt_data_i][nrec_i]); the
- * step loop is graph-replayed when member 0 has PAM_OPT_GRAPH set; the z-march
- * variant is member 0's PAM_OPT_TILE when that names one, else timed on the
- * first call for the shot layout.  loop_ms (may be NULL) receives the step
- * loop's GPU time. */
+ * step loop is graph-replayed when member 0 has PAM_OPT_GRAPH set; the
+ * variant -- 0 (the 16-row tile, with a CPML psi pass over the whole group
+ * before each step) or a z-march (4..6) -- is member 0's PAM_OPT_TILE when that
+ * names one of these, else timed on the first call for the shot layout.
+ * loop_ms (may be NULL) receives the step loop's GPU time (two launches per
+ * step for variant 0 with CPML). */
 int pam_group_forward(pam_group *grp, const int *nshots, float *const *seis, double *loop_ms);
 /* the step-kernel variant of the last pam_group_forward (-1 before) */
 int pam_group_get_tile(const pam_group *grp);

@@ -1179,8 +1179,8 @@ extern "C" int pam_mute(float *seis, int nshots, int nt, int nrec, const double 
 struct pam_group {
     std::vector<pam_ctx *> ctx;
     StepArgs *d_list = nullptr;
-    int *d_boff = nullptr;
-    std::map<std::string, int> tuned;  // z-march variant per shot layout
+    int *d_boff = nullptr, *d_pboff = nullptr;  // step-kernel and psi-pass block prefix tables
+    std::map<std::string, int> tuned;  // step-kernel variant per shot layout
     int last_tile = -1;
 };
 
@@ -1198,8 +1198,10 @@ extern "C" int pam_group_create(pam_ctx *const *ctxs, int n, pam_group **out) {
     pam_group *g = new pam_group();
     g->ctx.assign(ctxs, ctxs + n);
     if (hipMalloc((void **)&g->d_list, sizeof(StepArgs) * n) != hipSuccess ||
-        hipMalloc((void **)&g->d_boff, sizeof(int) * (n + 1)) != hipSuccess) {
+        hipMalloc((void **)&g->d_boff, sizeof(int) * (n + 1)) != hipSuccess ||
+        hipMalloc((void **)&g->d_pboff, sizeof(int) * (n + 1)) != hipSuccess) {
         if (g->d_list) (void)hipFree(g->d_list);
+        if (g->d_boff) (void)hipFree(g->d_boff);
         delete g;
         return fail(c0, PAM_E_NOMEM, "group argument tables");
     }
@@ -1212,6 +1214,7 @@ extern "C" int pam_group_destroy(pam_group *g) {
     if (!g->ctx.empty() && g->ctx[0]->st) (void)hipStreamSynchronize(g->ctx[0]->st);
     dfree(g->d_list);
     dfree(g->d_boff);
+    dfree(g->d_pboff);
     delete g;
     return PAM_OK;
 }
@@ -1223,7 +1226,7 @@ extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *
     const int n = (int)g->ctx.size();
     const int nt = c0->g.nt;
     std::vector<StepArgs> list(n);
-    std::vector<int> boff(n + 1, 0);
+    std::vector<int> boff(n + 1, 0), pboff(n + 1, 0);
     for (int i = 0; i < n; ++i) {
         pam_ctx *c = g->ctx[i];
         int r = ready(c);
@@ -1232,10 +1235,11 @@ extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *
             return fail(c0, PAM_E_ARG, "group member %d: nshots %d outside [1, min(max_shots, shots)]", i, nshots[i]);
         if (!seis[i]) return fail(c0, PAM_E_ARG, "group member %d: seis is NULL", i);
     }
-    // the z-march variant: member 0's PAM_OPT_TILE when it names one, else timed
+    // the variant (the 16-row tile or a z-march): member 0's PAM_OPT_TILE when
+    // it names one of these, else timed
     std::string layout;
     for (int i = 0; i < n; ++i) layout += std::to_string(nshots[i]) + ",";
-    int tile = (c0->tile >= 0 && march_chunks(c0->tile) > 0) ? c0->tile : -1;
+    int tile = (c0->tile >= 0 && tile_has_multi(c0->tile)) ? c0->tile : -1;
     auto build = [&](int t) -> int {
         const int ch = march_chunks(t);
         for (int i = 0; i < n; ++i) {
@@ -1256,11 +1260,14 @@ extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *
                 a.psiz = c->fs.psiz + c->psi_half;
             }
             a.nxt = (a.nxp + 63) / 64;
-            a.nzt = (a.nzp - 8 + ch * 16 - 1) / (ch * 16);
+            a.nzt = (t == 0) ? (a.nzp + 15) / 16 : (a.nzp - 8 + ch * 16 - 1) / (ch * 16);
             boff[i + 1] = boff[i] + a.nxt * a.nzt * nb;
+            const bool psi_pass = t == 0 && c->abc == PAM_CPML;  // the tile reads psi from a pre-pass
+            pboff[i + 1] = pboff[i] + (psi_pass ? psi_blocks_per_shot(c->nzp, c->nxp, c->g.n_abc) * nb : 0);
         }
         HCK(hipMemcpyAsync(g->d_list, list.data(), sizeof(StepArgs) * n, hipMemcpyHostToDevice, c0->st));
         HCK(hipMemcpyAsync(g->d_boff, boff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c0->st));
+        HCK(hipMemcpyAsync(g->d_pboff, pboff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c0->st));
         HCK(hipStreamSynchronize(c0->st));  // the host tables are reused
         return PAM_OK;
     };
@@ -1286,7 +1293,7 @@ extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *
             double best = 1e300;
             tile = kDefaultTile;
             for (int t = 0; t < tile_count(); ++t) {
-                if (march_chunks(t) == 0) continue;
+                if (!tile_has_multi(t)) continue;
                 int r = prime();
                 if (r) return r;
                 r = build(t);
@@ -1294,6 +1301,7 @@ extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *
                 HCK(hipEventRecord(c0->e0, c0->st));
                 for (int k = 0; k < 6; ++k)
                     KCK(launch_step_multi(c0->phys, c0->abc == PAM_CPML, t, g->d_list, g->d_boff, n, boff[n], k,
+                                          g->d_pboff, pboff[n],
                                           c0->st));
                 HCK(hipEventRecord(c0->e1, c0->st));
                 HCK(hipEventSynchronize(c0->e1));
@@ -1320,7 +1328,8 @@ extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *
     HCK(hipEventRecord(c0->e0, c0->st));
     auto body = [&]() -> int {
         for (int k = 0; k < nt; ++k)
-            KCK(launch_step_multi(c0->phys, c0->abc == PAM_CPML, tile, g->d_list, g->d_boff, n, boff[n], k, c0->st));
+            KCK(launch_step_multi(c0->phys, c0->abc == PAM_CPML, tile, g->d_list, g->d_boff, n, boff[n], k,
+                                  g->d_pboff, pboff[n], c0->st));
         return PAM_OK;
     };
     if (c0->graph) {

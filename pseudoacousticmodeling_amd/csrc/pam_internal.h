@@ -129,13 +129,21 @@ bool tile_fuses_psi(int idx);
 
 // launchers (return hipError_t as int)
 // ev: null, or two events that the launch stamps with the kernel's start and end
+// CPML psi pass of a group of grids (block prefix table pboff, see k_psi_multi)
+int launch_psi_multi(const StepArgs *list, const int *pboff, int ngrid, int npblocks, int k, hipStream_t st);
+// blocks of one shot's psi pass (0 without absorbing strips)
+int psi_blocks_per_shot(int nzp, int nxp, int N);
 int launch_step(int phys, bool cpml, int tile, const StepArgs &a, hipStream_t st, const hipEvent_t *ev = nullptr);
 int launch_psi(const PsiArgs &a, hipStream_t st);
-// One time step k of a group of grids in one launch (z-march variants only):
-// list[g] (device) holds grid g's arguments as of step 0, with nxt / nzt set for
-// the variant; grid g owns blocks [boff[g], boff[g+1]) of nblocks = boff[ngrid].
+// One time step k of a group of grids in one launch (variant 0, the 16-row
+// tile, or a z-march variant): list[g] (device) holds grid g's arguments as of
+// step 0, with nxt / nzt set for the variant; grid g owns step-kernel blocks
+// [boff[g], boff[g+1]) of nblocks = boff[ngrid] and, for variant 0 with CPML,
+// psi-pass blocks [pboff[g], pboff[g+1]) of npblocks = pboff[ngrid].
 int launch_step_multi(int phys, bool cpml, int tile, const StepArgs *list, const int *boff, int ngrid, int nblocks,
-                      int k, hipStream_t st);
+                      int k, const int *pboff, int npblocks, hipStream_t st);
+// the step-kernel variants a group launch offers
+bool tile_has_multi(int tile);
 // row chunks of 16 per block of a z-march variant (0 for the other families)
 int march_chunks(int tile);
 int launch_adj_products(int phys, const AdjArgs &a, hipStream_t st);

@@ -32,11 +32,12 @@ namespace pam {
 //    of U is loaded once per segment rather than once per neighbour.
 constexpr int kPsiSeg = 12;
 
-__global__ __launch_bounds__(256) void k_psi(const PsiArgs a, long long nx_threads, int nsegz) {
-    const int s = blockIdx.y;
+// thread t of shot s; x-strip threads first (nx_threads, a whole number of
+// blocks), then nsegz segments of each z-strip column
+__device__ __forceinline__ void psi_body(const PsiArgs &a, const long long t, const int s, const long long nx_threads,
+                                         const int nsegz) {
     const long long so = (long long)s * a.sstride;
     const int wN = a.N - 4;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     // per-shot bases; lane offsets are 32-bit (at())
     const float *__restrict__ U = a.cur + so;
     const float *__restrict__ psix_in = a.psix_in + so;
@@ -81,6 +82,58 @@ __global__ __launch_bounds__(256) void k_psi(const PsiArgs a, long long nx_threa
 #pragma unroll
         for (int h = 0; h < 8; ++h) q[h] = q[h + 1];
     }
+}
+
+// threads of one shot's psi pass: x-strip cells (rounded up to whole blocks)
+// and the z-strip column segments
+static __host__ __device__ inline void psi_threads(int nzp, int nxp, int N, long long *nx_threads, int *nsegz,
+                                                   long long *n) {
+    const long long nxs = (long long)(nzp - 8) * 2 * (N - 4);
+    *nx_threads = (nxs + 255) / 256 * 256;
+    *nsegz = (N - 4 + kPsiSeg - 1) / kPsiSeg;
+    *n = *nx_threads + 2LL * (*nsegz) * (nxp - 8);
+}
+
+__global__ __launch_bounds__(256) void k_psi(const PsiArgs a, long long nx_threads, int nsegz) {
+    psi_body(a, (long long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, nx_threads, nsegz);
+}
+
+// The psi pass of a group of grids in one launch: block b belongs to grid g
+// with pboff[g] <= b < pboff[g + 1] (blocks of shot 0, then shot 1, ...);
+// list[g] holds the grid's arguments as of step 0, its wavefields and psi
+// halves trading places on odd steps as in k_tile_multi.
+__global__ __launch_bounds__(256) void k_psi_multi(const StepArgs *__restrict__ list, const int *__restrict__ pboff,
+                                                   int ngrid, int k) {
+    const int b = blockIdx.x;
+    if (b >= pboff[ngrid]) return;  // block-uniform
+    int g = 0;
+    while (g + 1 < ngrid && b >= pboff[g + 1]) ++g;
+    const StepArgs &sa = list[g];
+    PsiArgs p;
+    p.nzp = sa.nzp;
+    p.nxp = sa.nxp;
+    p.P = sa.P;
+    p.N = sa.N;
+    p.nshots = sa.nshots;
+    p.sstride = sa.sstride;
+    p.idx = sa.idx;
+    p.idz = sa.idz;
+    p.cur = (k & 1) ? sa.prv : sa.cur;
+    p.psix_in = (k & 1) ? sa.psix : sa.psix_in;
+    p.psiz_in = (k & 1) ? sa.psiz : sa.psiz_in;
+    p.psix = const_cast<float *>((k & 1) ? sa.psix_in : sa.psix);
+    p.psiz = const_cast<float *>((k & 1) ? sa.psiz_in : sa.psiz);
+    p.cax = sa.cax;
+    p.cbx = sa.cbx;
+    p.caz = sa.caz;
+    p.cbz = sa.cbz;
+    long long nx_threads, n;
+    int nsegz;
+    psi_threads(p.nzp, p.nxp, p.N, &nx_threads, &nsegz, &n);
+    const int per_shot = (int)((n + 255) / 256);
+    const int bb = b - pboff[g];
+    const int s = bb / per_shot;
+    psi_body(p, (long long)(bb - s * per_shot) * 256 + threadIdx.x, s, nx_threads, nsegz);
 }
 
 // ------------------------------------------- linearised VTI/TTI adjoint (Cerjan)
@@ -464,24 +517,25 @@ int launch_step_vti_cpml(int, const StepArgs &, hipStream_t, const hipEvent_t *)
 int launch_step_tti_cerjan(int, const StepArgs &, hipStream_t, const hipEvent_t *);
 int launch_step_tti_cpml(int, const StepArgs &, hipStream_t, const hipEvent_t *);
 
-int launch_multi_acoustic_cerjan(int, const StepArgs *, const int *, int, int, int, hipStream_t);
-int launch_multi_acoustic_cpml(int, const StepArgs *, const int *, int, int, int, hipStream_t);
-int launch_multi_vti_cerjan(int, const StepArgs *, const int *, int, int, int, hipStream_t);
-int launch_multi_vti_cpml(int, const StepArgs *, const int *, int, int, int, hipStream_t);
-int launch_multi_tti_cerjan(int, const StepArgs *, const int *, int, int, int, hipStream_t);
-int launch_multi_tti_cpml(int, const StepArgs *, const int *, int, int, int, hipStream_t);
+int launch_multi_acoustic_cerjan(int, const StepArgs *, const int *, int, int, int, const int *, int, hipStream_t);
+int launch_multi_acoustic_cpml(int, const StepArgs *, const int *, int, int, int, const int *, int, hipStream_t);
+int launch_multi_vti_cerjan(int, const StepArgs *, const int *, int, int, int, const int *, int, hipStream_t);
+int launch_multi_vti_cpml(int, const StepArgs *, const int *, int, int, int, const int *, int, hipStream_t);
+int launch_multi_tti_cerjan(int, const StepArgs *, const int *, int, int, int, const int *, int, hipStream_t);
+int launch_multi_tti_cpml(int, const StepArgs *, const int *, int, int, int, const int *, int, hipStream_t);
 int launch_step_multi(int phys, bool cpml, int tile, const StepArgs *list, const int *boff, int ngrid, int nblocks,
-                      int k, hipStream_t st) {
+                      int k, const int *pboff, int npblocks, hipStream_t st) {
     if (phys == 0)
-        return cpml ? launch_multi_acoustic_cpml(tile, list, boff, ngrid, nblocks, k, st)
-                    : launch_multi_acoustic_cerjan(tile, list, boff, ngrid, nblocks, k, st);
+        return cpml ? launch_multi_acoustic_cpml(tile, list, boff, ngrid, nblocks, k, pboff, npblocks, st)
+                    : launch_multi_acoustic_cerjan(tile, list, boff, ngrid, nblocks, k, pboff, npblocks, st);
     if (phys == 1)
-        return cpml ? launch_multi_vti_cpml(tile, list, boff, ngrid, nblocks, k, st)
-                    : launch_multi_vti_cerjan(tile, list, boff, ngrid, nblocks, k, st);
-    return cpml ? launch_multi_tti_cpml(tile, list, boff, ngrid, nblocks, k, st)
-                : launch_multi_tti_cerjan(tile, list, boff, ngrid, nblocks, k, st);
+        return cpml ? launch_multi_vti_cpml(tile, list, boff, ngrid, nblocks, k, pboff, npblocks, st)
+                    : launch_multi_vti_cerjan(tile, list, boff, ngrid, nblocks, k, pboff, npblocks, st);
+    return cpml ? launch_multi_tti_cpml(tile, list, boff, ngrid, nblocks, k, pboff, npblocks, st)
+                : launch_multi_tti_cerjan(tile, list, boff, ngrid, nblocks, k, pboff, npblocks, st);
 }
 int march_chunks(int tile) { return tile_shape(tile).V < 0 ? tile_shape(tile).TZ : 0; }
+bool tile_has_multi(int tile) { return tile == 0 || march_chunks(tile) > 0; }
 
 int launch_step(int phys, bool cpml, int tile, const StepArgs &a, hipStream_t st, const hipEvent_t *ev) {
     if (phys == 0)
@@ -492,11 +546,22 @@ int launch_step(int phys, bool cpml, int tile, const StepArgs &a, hipStream_t st
 
 int launch_psi(const PsiArgs &a, hipStream_t st) {
     if (a.N <= 4) return 0;
-    const long long nxs = (long long)(a.nzp - 8) * 2 * (a.N - 4);
-    const long long nx_threads = (nxs + 255) / 256 * 256;  // the z-columns start on a block boundary
-    const int nsegz = (a.N - 4 + kPsiSeg - 1) / kPsiSeg;
-    const long long n = nx_threads + 2LL * nsegz * (a.nxp - 8);
+    long long nx_threads, n;  // the z-columns start on a block boundary
+    int nsegz;
+    psi_threads(a.nzp, a.nxp, a.N, &nx_threads, &nsegz, &n);
     hipLaunchKernelGGL(k_psi, dim3((unsigned)((n + 255) / 256), a.nshots), dim3(256), 0, st, a, nx_threads, nsegz);
+    return (int)hipGetLastError();
+}
+int psi_blocks_per_shot(int nzp, int nxp, int N) {
+    if (N <= 4) return 0;
+    long long nx_threads, n;
+    int nsegz;
+    psi_threads(nzp, nxp, N, &nx_threads, &nsegz, &n);
+    return (int)((n + 255) / 256);
+}
+int launch_psi_multi(const StepArgs *list, const int *pboff, int ngrid, int npblocks, int k, hipStream_t st) {
+    if (npblocks <= 0) return 0;
+    hipLaunchKernelGGL(k_psi_multi, dim3((unsigned)npblocks), dim3(256), 0, st, list, pboff, ngrid, k);
     return (int)hipGetLastError();
 }
 

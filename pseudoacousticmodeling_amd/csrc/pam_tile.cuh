@@ -28,10 +28,20 @@
 
 namespace pam {
 
-// forward-only instances of the 16-row tile fit 5 waves per SIMD
-template <int PHYS, bool CPML, int TZ, int WPE, bool ADJ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ == 16 && PHYS < 2) ? 5 : WPE))) void k_tile(
-    const StepArgs a) {
+// The per-step arguments of one tile pass: step, flags, the two wavefields
+// and the new psi (k_tile takes them from its StepArgs; the group kernel
+// trades the buffers by step parity)
+struct TileStep {
+    int k, flags;
+    const float *cur;
+    float *prv;
+    const float *psix, *psiz;
+};
+
+// One block's work: block b (in [0, nxt * nzt * nshots)) of the launch
+// described by a and the step arguments m.
+template <int PHYS, bool CPML, int TZ, bool ADJ>
+__device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const TileStep m) {
     constexpr int TX = 64;
     constexpr int R = TZ / 16;        // rows per thread
     constexpr int ROWS = TZ + 8;      // staged rows (4-row halo each side)
@@ -40,9 +50,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
     __shared__ float4 su[ROWS * Q];
     __shared__ float4 sd[(PHYS == 2) ? ROWS * (TX / 4) : 1];
 
-    const int nb = a.nxt * a.nzt * a.nshots;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    if (b >= nb) return;  // block-uniform
     const int tile = b / a.nshots;
     const int s = b - tile * a.nshots;
     // tiles in bands of kBand tile columns, x fastest within a band, then z:
@@ -57,14 +64,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
     const int x0 = txi * TX, z0 = tzi * TZ;
     const int P = a.P, nzp = a.nzp, nxp = a.nxp, N = a.N;
     const long long so = (long long)s * a.sstride;
-    const float *__restrict__ cur = a.cur + so;
-    float *__restrict__ prv = a.prv + so;
-    const float *__restrict__ psix = a.psix + so;
-    const float *__restrict__ psiz = a.psiz + so;
+    const float *__restrict__ cur = m.cur + so;
+    float *__restrict__ prv = m.prv + so;
+    const float *__restrict__ psix = m.psix + so;
+    const float *__restrict__ psiz = m.psiz + so;
     float *__restrict__ zetax = a.zetax + so;
     float *__restrict__ zetaz = a.zetaz + so;
     const int tid = threadIdx.x;
-    const int flags = ADJ ? a.flags : (a.flags & F_FWD_MASK);  // forward-only instances drop the rest
+    const int flags = ADJ ? m.flags : (m.flags & F_FWD_MASK);  // forward-only instances drop the rest
     const int cx = tid & 15, rg = tid >> 4;
     const int col = x0 + 4 * cx;   // first of this thread's 4 columns
     const int zr0 = z0 + rg * R;   // first of this thread's R rows
@@ -202,7 +209,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
         // ---- receiver gather of the input field (damped as the reference records it)
         if ((flags & F_GATHER) && a.rrow[z] >= 0) {
             const int slot = a.rrow[z];
-            const int it = a.k - a.itlag;
+            const int it = m.k - a.itlag;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 int rr = a.rhead[(long long)slot * P + col + j];
@@ -225,9 +232,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
                 const long long ii = ib + c;
                 if ((flags & F_SAVE) && !(flags & F_D2T)) {
                     if (a.save_kind == SV_F32_PAD) {
-                        if (xlive[j]) snap_put(a, s, a.k, 0, ro + j, (c0[j] * fx[j]) * fz);
+                        if (xlive[j]) snap_put(a, s, m.k, 0, ro + j, (c0[j] * fx[j]) * fz);
                     } else if (cin) {
-                        snap_put(a, s, a.k, ii, 0, c0[j]);
+                        snap_put(a, s, m.k, ii, 0, c0[j]);
                     }
                 }
                 if (!cin) continue;
@@ -236,16 +243,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
                     *acc += c0[j] * c0[j];
                 }
                 if (flags & F_IMAGE) {
-                    const float S = snap_get(a, s, a.k, ii, ro + j);
+                    const float S = snap_get(a, s, m.k, ii, ro + j);
                     float *im = a.image + (long long)s * a.acc_sstride + ii;
                     *im += S * c0[j];
                 }
                 if (flags & F_GRADM) {
-                    const float Sp = (flags & F_D2T) ? 0.0f : snap_get(a, s, a.k + 1, ii, ro + j);
-                    const float S0 = (flags & F_D2T) ? 0.0f : snap_get(a, s, a.k, ii, ro + j);
-                    const float Sm = (flags & F_D2T) ? 0.0f : snap_get(a, s, a.k - 1, ii, ro + j);
+                    const float Sp = (flags & F_D2T) ? 0.0f : snap_get(a, s, m.k + 1, ii, ro + j);
+                    const float S0 = (flags & F_D2T) ? 0.0f : snap_get(a, s, m.k, ii, ro + j);
+                    const float Sm = (flags & F_D2T) ? 0.0f : snap_get(a, s, m.k - 1, ii, ro + j);
                     const float d2t =
-                            ((flags & F_D2T) ? snap_get(a, s, a.k, ii, ro + j) : (Sp - 2.0f * S0 + Sm)) * a.idt2;
+                            ((flags & F_D2T) ? snap_get(a, s, m.k, ii, ro + j) : (Sp - 2.0f * S0 + Sm)) * a.idt2;
                     float *im = a.image + (long long)s * a.acc_sstride + ii;
                     *im += d2t * c0[j];
                 }
@@ -335,16 +342,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
             out[j] = (u * fx[j]) * fz;
         }
         // writer-side injection of the next step's source
-        if ((flags & F_SRC) && z == a.ssz[s] && a.k + 1 < a.nt) {
+        if ((flags & F_SRC) && z == a.ssz[s] && m.k + 1 < a.nt) {
             const int sx = a.ssx[s];
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                if (col + j == sx) out[j] += a.wav[a.k + 1];
+                if (col + j == sx) out[j] += a.wav[m.k + 1];
         }
         // writer-side injection of the data of the next (earlier) adjoint step
-        if ((flags & F_RECINJ) && a.rrow[z] >= 0 && a.k - 1 - a.itlag >= 0) {
+        if ((flags & F_RECINJ) && a.rrow[z] >= 0 && m.k - 1 - a.itlag >= 0) {
             const int slot = a.rrow[z];
-            const int tn = a.k - 1 - a.itlag;
+            const int tn = m.k - 1 - a.itlag;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 int rr = a.rhead[(long long)slot * P + col + j];
@@ -362,7 +369,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
                 if (!(zint && col + j >= N && col + j < nxp - N)) continue;
                 // interior cells are undamped: D_k = S_{k+1} - 2 S_k + S_{k-1}
                 const float d = (flags & F_D2T_OP) ? opv[j] : out[j] - 2.0f * c0[j] + pv[r][j];
-                if ((flags & F_D2T) && (flags & F_SAVE)) snap_put(a, s, a.k, ib + col + j, 0, d);
+                if ((flags & F_D2T) && (flags & F_SAVE)) snap_put(a, s, m.k, ib + col + j, 0, d);
                 if (flags & F_DSAVE) a.dsave[(long long)s * a.dsave_sstride + ib + col + j] = d;
                 tmax = fmaxf(tmax, fabsf((flags & F_D2T) ? d : out[j]));
             }
@@ -375,7 +382,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
                 if (xlive[j]) *at(prv, ro + j) = out[j];
         }
     }
-    if ((flags & F_SMAX) && a.k + 1 < a.nt) block_max_to(a.smax + (long long)s * a.nt + a.k + 1, tmax);
+    if ((flags & F_SMAX) && m.k + 1 < a.nt) block_max_to(a.smax + (long long)s * a.nt + m.k + 1, tmax);
+}
+
+// forward-only instances of the 16-row tile fit 5 waves per SIMD
+template <int PHYS, bool CPML, int TZ, int WPE, bool ADJ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ == 16 && PHYS < 2) ? 5 : WPE))) void k_tile(
+    const StepArgs a) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    if (b >= a.nxt * a.nzt * a.nshots) return;  // block-uniform
+    tile_body<PHYS, CPML, TZ, ADJ>(a, b, TileStep{a.k, a.flags, a.cur, a.prv, a.psix, a.psiz});
+}
+
+// A group of grids in one launch: block b belongs to grid g with boff[g] <= b <
+// boff[g + 1], whose arguments are list[g] as of step 0 (its two wavefields and
+// psi halves trade places on odd steps); k is the step of every grid.  The new
+// psi comes from k_psi_multi.
+template <int PHYS, bool CPML, int TZ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((TZ == 16 && PHYS < 2) ? 5 : 1))) void k_tile_multi(
+    const StepArgs *__restrict__ list, const int *__restrict__ boff, int ngrid, int k) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    if (b >= boff[ngrid]) return;  // block-uniform
+    int g = 0;
+    while (g + 1 < ngrid && b >= boff[g + 1]) ++g;
+    const StepArgs &a = list[g];
+    TileStep m{k, a.flags, a.cur, a.prv, a.psix, a.psiz};
+    if (k < a.itlag) m.flags &= ~F_GATHER;
+    if (k & 1) {
+        m.cur = a.prv;
+        m.prv = const_cast<float *>(a.cur);
+        m.psix = a.psix_in;
+        m.psiz = a.psiz_in;
+    }
+    tile_body<PHYS, CPML, TZ, false>(a, b - boff[g], m);
+}
+
+template <int PHYS, bool CPML, int TZ>
+static int launch_tile_multi_t(const StepArgs *list, const int *boff, int ngrid, int nblocks, int k,
+                               const int *pboff, int npblocks, hipStream_t st) {
+    if (CPML && npblocks > 0) {
+        const int r = launch_psi_multi(list, pboff, ngrid, npblocks, k, st);
+        if (r) return r;
+    }
+    const int blocks = (nblocks + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_tile_multi<PHYS, CPML, TZ>), dim3((unsigned)blocks), dim3(256), 0, st, list, boff, ngrid, k);
+    return (int)hipGetLastError();
 }
 
 template <int PHYS, bool CPML, int TZ, int WPE>

@@ -27,11 +27,13 @@ static int launch_step_p(int tile, const StepArgs &a, hipStream_t st, const hipE
     }
 }
 
-// the group launch exists for the z-march variants only
+// the group launch exists for the 16-row tile (its psi pass through the table
+// pboff) and the z-march variants
 template <int PHYS, bool CPML>
 static int launch_multi_p(int tile, const StepArgs *list, const int *boff, int ngrid, int nblocks, int k,
-                          hipStream_t st) {
+                          const int *pboff, int npblocks, hipStream_t st) {
     switch (tile) {
+        case 0: return launch_tile_multi_t<PHYS, CPML, 16>(list, boff, ngrid, nblocks, k, pboff, npblocks, st);
         case 4: return launch_march_multi_t<PHYS, CPML, 7>(list, boff, ngrid, nblocks, k, st);
         case 5: return launch_march_multi_t<PHYS, CPML, 14>(list, boff, ngrid, nblocks, k, st);
         case 6: return launch_march_multi_t<PHYS, CPML, 28>(list, boff, ngrid, nblocks, k, st);

@@ -6,7 +6,7 @@ int launch_step_tti_cerjan(int tile, const StepArgs &a, hipStream_t st, const hi
     return launch_step_p<2, false>(tile, a, st, ev);
 }
 int launch_multi_tti_cerjan(int tile, const StepArgs *list, const int *boff, int ngrid, int nblocks, int k,
-                        hipStream_t st) {
-    return launch_multi_p<2, false>(tile, list, boff, ngrid, nblocks, k, st);
+                        const int *pboff, int npblocks, hipStream_t st) {
+    return launch_multi_p<2, false>(tile, list, boff, ngrid, nblocks, k, pboff, npblocks, st);
 }
 }  // namespace pam
