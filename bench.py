@@ -106,9 +106,9 @@ def configs_leg(nt_cap):
 def group_leg(shots_per_model):
     """Sixteen VTI + CPML models of different sizes (201..601 x 801..2401,
     every pair once), ``shots_per_model`` shots each, nt = 3001 + itlag: all
-    shots of all models advanced in one launch per time step
-    (pam_group_forward), against the same models run one after another
-    (pam_forward each)."""
+    shots of all models advanced together (pam_group_forward: cache-sized
+    sub-groups, one launch per time step each, on two streams), against the
+    same models run one after another (pam_forward each)."""
     import torch
 
     from pseudoacousticmodeling_amd import workloads
@@ -358,7 +358,7 @@ def main(argv=None):
     group = None
     if not args.no_group:
         group = group_leg(args.group_shots)
-        print(f"bench: uneven-grid group {group['value']:.1f} Gpt-updates/s in one launch per step, "
+        print(f"bench: uneven-grid group {group['value']:.1f} Gpt-updates/s together, "
               f"{group['per_model_loop']:.1f} model by model", file=sys.stderr, flush=True)
 
     others = None

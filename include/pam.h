@@ -98,12 +98,17 @@ enum pam_option {
                               kernel time pam_last_timing reports; launches are then not
                               graph-replayed); 0 (default): no per-launch events */
     PAM_OPT_SNAPSHOTS = 10, /* checkpoint slots of PAM_STORE_REVOLVE_* (default 50, >= 1) */
-    PAM_OPT_LANES = 11      /* pam_forward: 2 (default) advances two batches at once on two
+    PAM_OPT_LANES = 11,     /* pam_forward: 2 (default) advances two batches at once on two
                                streams (the context's and one of its own), each lane
                                PAM_OPT_BATCH shots per launch, or half the autotuned batch
                                when that is 0, so that one lane's CPML pre-pass and launch
                                gaps overlap the other's step kernel; 1: one batch at a time.
                                Runs with PAM_OPT_PROFILE use one lane. */
+    PAM_OPT_GROUP_BYTES = 12  /* pam_group_forward (read from member 0): wavefield bytes
+                                 (two fields per shot) of one sub-group of consecutive
+                                 members advanced in one launch per step; default 56 MiB,
+                                 so that a launch's working set stays in the 256 MB MALL;
+                                 0: the whole group in one launch per step */
 };
 
 typedef struct pam_grid {
@@ -259,12 +264,15 @@ int pam_group_create(pam_ctx *const *ctxs, int n, pam_group **out);
 int pam_group_destroy(pam_group *grp);
 /* Forward modeling of shots [0, nshots[i]) of every member i (nshots[i] <=
  * its max_shots) into seis[i] (device [nshots[i]][nt_data_i][nrec_i]); the
- * step loop is graph-replayed when member 0 has PAM_OPT_GRAPH set; the
- * variant -- 0 (the 16-row tile, with a CPML psi pass over the whole group
- * before each step) or a z-march (4..6) -- is member 0's PAM_OPT_TILE when that
- * names one of these, else timed on the first call for the shot layout.
- * loop_ms (may be NULL) receives the step loop's GPU time (two launches per
- * step for variant 0 with CPML). */
+ * step loop is graph-replayed when member 0 has PAM_OPT_GRAPH set.  The
+ * members are split into sub-groups of consecutive members within member 0's
+ * PAM_OPT_GROUP_BYTES; each sub-group runs its steps with one launch per step
+ * (two for variant 0 with CPML: the psi pass, then the step), and with
+ * PAM_OPT_LANES 2 sub-groups alternate between member 0's stream and a second
+ * one.  The variant -- 0 (the 16-row tile) or a z-march (4..6) -- is member
+ * 0's PAM_OPT_TILE when that names one of these, else timed on the first call
+ * for the shot layout.  loop_ms (may be NULL) receives the GPU time of the
+ * steps. */
 int pam_group_forward(pam_group *grp, const int *nshots, float *const *seis, double *loop_ms);
 /* the step-kernel variant of the last pam_group_forward (-1 before) */
 int pam_group_get_tile(const pam_group *grp);

@@ -22,7 +22,7 @@ PAM_STORE_REVOLVE_F32, PAM_STORE_REVOLVE_F16 = 7, 8
 PAM_STORE_RANDOM_BOUNDARY = 9
 PAM_OPT_BATCH, PAM_OPT_TILE, PAM_OPT_GRAPH, PAM_OPT_STORAGE = 1, 2, 3, 4
 PAM_OPT_CKPT_STEP, PAM_OPT_RESET_ADJ_CPML, PAM_OPT_NORMALIZE, PAM_OPT_ADJ_RANGE = 5, 6, 7, 8
-PAM_OPT_PROFILE, PAM_OPT_SNAPSHOTS, PAM_OPT_LANES = 9, 10, 11
+PAM_OPT_PROFILE, PAM_OPT_SNAPSHOTS, PAM_OPT_LANES, PAM_OPT_GROUP_BYTES = 9, 10, 11, 12
 PAM_PREP_PY, PAM_PREP_CPP = 0, 1
 
 ERRORS = {-1: "PAM_E_ARG", -2: "PAM_E_HIP", -3: "PAM_E_STATE", -4: "PAM_E_NOMEM",

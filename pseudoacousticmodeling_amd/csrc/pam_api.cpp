@@ -75,6 +75,7 @@ struct pam_ctx {
     // options
     int batch = 0, tile = -1, graph = 1, storage = PAM_STORE_ALL_F32, ckpt_step = 0;
     int reset_adj_cpml = 1, normalize = 1, adj_range = 0, snapshots = 50, lanes = 2;
+    long long group_bytes = 56LL << 20;  // wavefields of one sub-group of a group launch (0: no split)
     long long steps_run = 0;  // time steps launched by the current adjoint run
     // checkpoints of the main set ([segment][field][shot][plane]) and the
     // psi_flip at each
@@ -357,6 +358,10 @@ extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
             if (v < 1 || v > 2) return fail(ctx, PAM_E_ARG, "lanes must be 1 or 2");
             ctx->lanes = (int)v;
             return PAM_OK;
+        case PAM_OPT_GROUP_BYTES:
+            if (v < 0) return fail(ctx, PAM_E_ARG, "group bytes must be >= 0");
+            ctx->group_bytes = v;
+            return PAM_OK;
         default:
             return fail(ctx, PAM_E_ARG, "unknown option %d", key);
     }
@@ -376,6 +381,7 @@ extern "C" int pam_get_option(const pam_ctx *ctx, int key, long long *value) {
         case PAM_OPT_PROFILE: *value = ctx->profile; return PAM_OK;
         case PAM_OPT_SNAPSHOTS: *value = ctx->snapshots; return PAM_OK;
         case PAM_OPT_LANES: *value = ctx->lanes; return PAM_OK;
+        case PAM_OPT_GROUP_BYTES: *value = ctx->group_bytes; return PAM_OK;
         default: return PAM_E_ARG;
     }
 }
@@ -1179,7 +1185,7 @@ extern "C" int pam_mute(float *seis, int nshots, int nt, int nrec, const double 
 struct pam_group {
     std::vector<pam_ctx *> ctx;
     StepArgs *d_list = nullptr;
-    int *d_boff = nullptr, *d_pboff = nullptr;  // step-kernel and psi-pass block prefix tables
+    int *d_boff = nullptr, *d_pboff = nullptr;  // step-kernel and psi-pass block prefix tables (2n: per sub-group)
     std::map<std::string, int> tuned;  // step-kernel variant per shot layout
     int last_tile = -1;
 };
@@ -1198,8 +1204,8 @@ extern "C" int pam_group_create(pam_ctx *const *ctxs, int n, pam_group **out) {
     pam_group *g = new pam_group();
     g->ctx.assign(ctxs, ctxs + n);
     if (hipMalloc((void **)&g->d_list, sizeof(StepArgs) * n) != hipSuccess ||
-        hipMalloc((void **)&g->d_boff, sizeof(int) * (n + 1)) != hipSuccess ||
-        hipMalloc((void **)&g->d_pboff, sizeof(int) * (n + 1)) != hipSuccess) {
+        hipMalloc((void **)&g->d_boff, sizeof(int) * 2 * n) != hipSuccess ||
+        hipMalloc((void **)&g->d_pboff, sizeof(int) * 2 * n) != hipSuccess) {
         if (g->d_list) (void)hipFree(g->d_list);
         if (g->d_boff) (void)hipFree(g->d_boff);
         delete g;
@@ -1225,8 +1231,6 @@ extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *
     pam_ctx *const ctx = c0;  // errors of the group go to member 0
     const int n = (int)g->ctx.size();
     const int nt = c0->g.nt;
-    std::vector<StepArgs> list(n);
-    std::vector<int> boff(n + 1, 0), pboff(n + 1, 0);
     for (int i = 0; i < n; ++i) {
         pam_ctx *c = g->ctx[i];
         int r = ready(c);
@@ -1235,6 +1239,25 @@ extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *
             return fail(c0, PAM_E_ARG, "group member %d: nshots %d outside [1, min(max_shots, shots)]", i, nshots[i]);
         if (!seis[i]) return fail(c0, PAM_E_ARG, "group member %d: seis is NULL", i);
     }
+    // sub-groups of consecutive members whose wavefields (two per shot) stay
+    // within member 0's PAM_OPT_GROUP_BYTES, so that each launch's working set
+    // stays in the MALL as a single grid's batch does (a member larger than
+    // that forms a sub-group of its own); sub-group j holds members
+    // [first[j], first[j+1]) and its block tables start at first[j] + j
+    std::vector<int> first;
+    long long acc = 0;
+    for (int i = 0; i < n; ++i) {
+        const long long bytes = 8LL * g->ctx[i]->plane * nshots[i];
+        if (i == 0 || (c0->group_bytes > 0 && acc + bytes > c0->group_bytes)) {
+            first.push_back(i);
+            acc = 0;
+        }
+        acc += bytes;
+    }
+    const int nsub = (int)first.size();
+    first.push_back(n);
+    std::vector<StepArgs> list(n);
+    std::vector<int> boff(n + nsub, 0), pboff(n + nsub, 0), nblk(nsub, 0), npblk(nsub, 0);
     // the variant (the 16-row tile or a z-march): member 0's PAM_OPT_TILE when
     // it names one of these, else timed
     std::string layout;
@@ -1242,33 +1265,48 @@ extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *
     int tile = (c0->tile >= 0 && tile_has_multi(c0->tile)) ? c0->tile : -1;
     auto build = [&](int t) -> int {
         const int ch = march_chunks(t);
-        for (int i = 0; i < n; ++i) {
-            pam_ctx *c = g->ctx[i];
-            const int nb = nshots[i];
-            const long long nd = (long long)(c->g.nt - c->g.itlag) * c->nrec;
-            StepArgs &a = list[i];
-            base_step_args(c, a, nb);
-            a.flags |= F_SRC | F_GATHER;
-            a.seis = seis[i];
-            a.dstride = nd;
-            a.cur = c->fs.U0;
-            a.prv = c->fs.U1;
-            if (c->abc == PAM_CPML) {
-                a.psix_in = c->fs.psix;
-                a.psiz_in = c->fs.psiz;
-                a.psix = c->fs.psix + c->psi_half;
-                a.psiz = c->fs.psiz + c->psi_half;
+        for (int j = 0; j < nsub; ++j) {
+            const int o = first[j] + j;  // this sub-group's tables
+            boff[o] = pboff[o] = 0;
+            for (int i = first[j]; i < first[j + 1]; ++i) {
+                pam_ctx *c = g->ctx[i];
+                const int nb = nshots[i];
+                const long long nd = (long long)(c->g.nt - c->g.itlag) * c->nrec;
+                StepArgs &a = list[i];
+                base_step_args(c, a, nb);
+                a.flags |= F_SRC | F_GATHER;
+                a.seis = seis[i];
+                a.dstride = nd;
+                a.cur = c->fs.U0;
+                a.prv = c->fs.U1;
+                if (c->abc == PAM_CPML) {
+                    a.psix_in = c->fs.psix;
+                    a.psiz_in = c->fs.psiz;
+                    a.psix = c->fs.psix + c->psi_half;
+                    a.psiz = c->fs.psiz + c->psi_half;
+                }
+                a.nxt = (a.nxp + 63) / 64;
+                a.nzt = (t == 0) ? (a.nzp + 15) / 16 : (a.nzp - 8 + ch * 16 - 1) / (ch * 16);
+                const int l = o + (i - first[j]);
+                boff[l + 1] = boff[l] + a.nxt * a.nzt * nb;
+                const bool psi_pass = t == 0 && c->abc == PAM_CPML;  // the tile reads psi from a pre-pass
+                pboff[l + 1] = pboff[l] + (psi_pass ? psi_blocks_per_shot(c->nzp, c->nxp, c->g.n_abc) * nb : 0);
             }
-            a.nxt = (a.nxp + 63) / 64;
-            a.nzt = (t == 0) ? (a.nzp + 15) / 16 : (a.nzp - 8 + ch * 16 - 1) / (ch * 16);
-            boff[i + 1] = boff[i] + a.nxt * a.nzt * nb;
-            const bool psi_pass = t == 0 && c->abc == PAM_CPML;  // the tile reads psi from a pre-pass
-            pboff[i + 1] = pboff[i] + (psi_pass ? psi_blocks_per_shot(c->nzp, c->nxp, c->g.n_abc) * nb : 0);
+            const int e = o + (first[j + 1] - first[j]);
+            nblk[j] = boff[e];
+            npblk[j] = pboff[e];
         }
         HCK(hipMemcpyAsync(g->d_list, list.data(), sizeof(StepArgs) * n, hipMemcpyHostToDevice, c0->st));
-        HCK(hipMemcpyAsync(g->d_boff, boff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c0->st));
-        HCK(hipMemcpyAsync(g->d_pboff, pboff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c0->st));
+        HCK(hipMemcpyAsync(g->d_boff, boff.data(), sizeof(int) * (n + nsub), hipMemcpyHostToDevice, c0->st));
+        HCK(hipMemcpyAsync(g->d_pboff, pboff.data(), sizeof(int) * (n + nsub), hipMemcpyHostToDevice, c0->st));
         HCK(hipStreamSynchronize(c0->st));  // the host tables are reused
+        return PAM_OK;
+    };
+    // step k of sub-group j on stream st
+    auto launch_sub = [&](int t, int j, int k, hipStream_t st) -> int {
+        const int o = first[j] + j, f = first[j];
+        KCK(launch_step_multi(c0->phys, c0->abc == PAM_CPML, t, g->d_list + f, g->d_boff + o, first[j + 1] - f,
+                              nblk[j], k, g->d_pboff + o, npblk[j], st));
         return PAM_OK;
     };
     // prime every member's shots: cleared fields and memory, sources, S_0
@@ -1281,35 +1319,65 @@ extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *
             if (r) return r;
             KCK(launch_inject_point(c->fs.U0, c->plane, c->d_ssz, c->d_ssx, c->P, c->wav, 0, nshots[i], c->st));
         }
-        // the group launches on member 0's stream
+        // the group launches on member 0's stream (and its second lane)
         for (int i = 0; i < n; ++i) HCK(hipStreamSynchronize(g->ctx[i]->st));
         return PAM_OK;
+    };
+    // sub-group j runs on lane j % 2 (member 0's stream and its second stream)
+    const int lanes = (c0->lanes > 1 && nsub > 1) ? 2 : 1;
+    if (lanes > 1) {
+        if (!c0->st2) HCK(hipStreamCreate(&c0->st2));  // default priority
+        if (!c0->fork_ev) HCK(hipEventCreateWithFlags(&c0->fork_ev, hipEventDisableTiming));
+        if (!c0->join_ev) HCK(hipEventCreateWithFlags(&c0->join_ev, hipEventDisableTiming));
+    }
+    auto lane_stream = [&](int j) { return (j & 1) && lanes > 1 ? c0->st2 : c0->st; };
+    auto fork = [&]() -> int {
+        if (lanes > 1) {
+            HCK(hipEventRecord(c0->fork_ev, c0->st));
+            HCK(hipStreamWaitEvent(c0->st2, c0->fork_ev, 0));
+        }
+        return PAM_OK;
+    };
+    auto join = [&](int r0) -> int {  // on every path
+        if (lanes > 1) {
+            const hipError_t ej = hipEventRecord(c0->join_ev, c0->st2);
+            const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(c0->st, c0->join_ev, 0) : ej;
+            if (ew != hipSuccess) (void)hipStreamSynchronize(c0->st2);
+            if (!r0 && ew != hipSuccess)
+                return fail(c0, PAM_E_HIP, "joining the second lane: %s", hipGetErrorString(ew));
+        }
+        return r0;
     };
     if (tile < 0) {
         auto it = g->tuned.find(layout);
         if (it != g->tuned.end()) {
             tile = it->second;
-        } else {  // time 6 steps of each z-march variant on cleared fields
+        } else {  // time 12 steps of every sub-group with each variant, on the lanes of the
+                  // run, twice (the first pass also warms the kernels and caches): the faster
             double best = 1e300;
             tile = kDefaultTile;
             for (int t = 0; t < tile_count(); ++t) {
                 if (!tile_has_multi(t)) continue;
-                int r = prime();
+                int r = build(t);
                 if (r) return r;
-                r = build(t);
-                if (r) return r;
-                HCK(hipEventRecord(c0->e0, c0->st));
-                for (int k = 0; k < 6; ++k)
-                    KCK(launch_step_multi(c0->phys, c0->abc == PAM_CPML, t, g->d_list, g->d_boff, n, boff[n], k,
-                                          g->d_pboff, pboff[n],
-                                          c0->st));
-                HCK(hipEventRecord(c0->e1, c0->st));
-                HCK(hipEventSynchronize(c0->e1));
-                float ms = 0.f;
-                HCK(hipEventElapsedTime(&ms, c0->e0, c0->e1));
-                if (ms < best) {
-                    best = ms;
-                    tile = t;
+                for (int pass = 0; pass < 2; ++pass) {
+                    r = prime();
+                    if (r) return r;
+                    HCK(hipEventRecord(c0->e0, c0->st));
+                    r = fork();
+                    if (r) return r;
+                    for (int j = 0; j < nsub && !r; ++j)
+                        for (int k = 0; k < 12 && !r; ++k) r = launch_sub(t, j, k, lane_stream(j));
+                    r = join(r);
+                    if (r) return r;
+                    HCK(hipEventRecord(c0->e1, c0->st));
+                    HCK(hipEventSynchronize(c0->e1));
+                    float ms = 0.f;
+                    HCK(hipEventElapsedTime(&ms, c0->e0, c0->e1));
+                    if (ms < best) {
+                        best = ms;
+                        tile = t;
+                    }
                 }
             }
             g->tuned[layout] = tile;
@@ -1325,25 +1393,39 @@ extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *
     }
     r = build(tile);
     if (r) return r;
+    // each sub-group runs its nt steps on its lane, one launch per step (two
+    // with the psi pass)
     HCK(hipEventRecord(c0->e0, c0->st));
-    auto body = [&]() -> int {
-        for (int k = 0; k < nt; ++k)
-            KCK(launch_step_multi(c0->phys, c0->abc == PAM_CPML, tile, g->d_list, g->d_boff, n, boff[n], k,
-                                  g->d_pboff, pboff[n], c0->st));
+    r = fork();
+    if (r) return r;
+    auto subgroups = [&]() -> int {
+        for (int j = 0; j < nsub; ++j) {
+            hipStream_t st = lane_stream(j);
+            auto body = [&]() -> int {
+                for (int k = 0; k < nt; ++k) {
+                    int rr = launch_sub(tile, j, k, st);
+                    if (rr) return rr;
+                }
+                return PAM_OK;
+            };
+            int rr;
+            if (c0->graph) {
+                std::string key = "grp|" + std::to_string(tile) + "|" + std::to_string(j) + "|";
+                for (int i = first[j]; i < first[j + 1]; ++i) {
+                    char buf[96];
+                    std::snprintf(buf, sizeof buf, "%p:%u:%d:%p;", (void *)g->ctx[i], g->ctx[i]->epoch, nshots[i],
+                                  (void *)seis[i]);
+                    key += buf;
+                }
+                rr = run_graph(c0, key, body, []() {}, st);
+            } else {
+                rr = body();
+            }
+            if (rr) return rr;
+        }
         return PAM_OK;
     };
-    if (c0->graph) {
-        std::string key = "grp|" + std::to_string(tile) + "|";
-        for (int i = 0; i < n; ++i) {
-            char buf[96];
-            std::snprintf(buf, sizeof buf, "%p:%u:%d:%p;", (void *)g->ctx[i], g->ctx[i]->epoch, nshots[i],
-                          (void *)seis[i]);
-            key += buf;
-        }
-        r = run_graph(c0, key, body, []() {});
-    } else {
-        r = body();
-    }
+    r = join(subgroups());
     if (r) return r;
     HCK(hipEventRecord(c0->e1, c0->st));
     HCK(hipEventSynchronize(c0->e1));

@@ -61,3 +61,28 @@ def test_group_rejects_mismatched_members():
     b = make_engine(make_problem(0, 1, nshots=1, seed=761, nt=100))
     with pytest.raises(_lib.PamError, match="PAM_E_ARG"):
         Group([a, b])
+
+
+@pytest.mark.parametrize("lanes", [2, 1])
+def test_group_in_sub_groups_equals_each_model_alone(lanes):
+    """A small PAM_OPT_GROUP_BYTES splits the group into sub-groups (one member
+    each here); with two lanes they alternate between two streams."""
+    from pseudoacousticmodeling_amd.engine import Group
+
+    print(f"case group sub-groups lanes={lanes}", flush=True)
+    shots = [2, 1, 3]
+    pbs = [make_problem(1, 1, nz=nz, nx=nx, nshots=k, seed=780 + i, nt=161)
+           for i, ((nz, nx), k) in enumerate(zip(SIZES, shots))]
+    engs = [make_engine(pb, max_shots=k) for pb, k in zip(pbs, shots)]
+    for e in engs:
+        e.set_option(_lib.PAM_OPT_TILE, 0)
+    refs = [e.forward(0, k).cpu().numpy() for e, k in zip(engs, shots)]
+    engs[0].set_option(_lib.PAM_OPT_GROUP_BYTES, 1)
+    engs[0].set_option(_lib.PAM_OPT_LANES, lanes)
+    assert engs[0].get_option(_lib.PAM_OPT_GROUP_BYTES) == 1
+    grp = Group(engs)
+    for call in range(2):
+        outs = [o.cpu().numpy() for o in grp.forward(shots)]
+        for i, (o, r) in enumerate(zip(outs, refs)):
+            assert np.array_equal(o, r), f"member {i} differs (call {call})"
+    grp.close()
