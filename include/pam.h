@@ -99,11 +99,13 @@ enum pam_option {
                               graph-replayed); 0 (default): no per-launch events */
     PAM_OPT_SNAPSHOTS = 10, /* checkpoint slots of PAM_STORE_REVOLVE_* (default 50, >= 1) */
     PAM_OPT_LANES = 11,     /* pam_forward: 2 (default) advances two batches at once on two
-                               streams (the context's and one of its own), each lane
+                               streams the context creates together (default priority),
+                               forked from its stream and joined back into it, each lane
                                PAM_OPT_BATCH shots per launch, or half the autotuned batch
                                when that is 0, so that one lane's CPML pre-pass and launch
-                               gaps overlap the other's step kernel; 1: one batch at a time.
-                               Runs with PAM_OPT_PROFILE use one lane. */
+                               gaps overlap the other's step kernel; 1: one batch at a time
+                               on the context's stream.  Runs with PAM_OPT_PROFILE use one
+                               lane.  Group runs use member 0's lanes for their sub-groups. */
     PAM_OPT_GROUP_BYTES = 12  /* pam_group_forward (read from member 0): wavefield bytes
                                  (two fields per shot) of one sub-group of consecutive
                                  members advanced in one launch per step; default 56 MiB,
@@ -268,8 +270,7 @@ int pam_group_destroy(pam_group *grp);
  * members are split into sub-groups of consecutive members within member 0's
  * PAM_OPT_GROUP_BYTES; each sub-group runs its steps with one launch per step
  * (two for variant 0 with CPML: the psi pass, then the step), and with
- * PAM_OPT_LANES 2 sub-groups alternate between member 0's stream and a second
- * one.  The variant -- 0 (the 16-row tile) or a z-march (4..6) -- is member
+ * PAM_OPT_LANES 2 sub-groups alternate between member 0's two lane streams.  The variant -- 0 (the 16-row tile) or a z-march (4..6) -- is member
  * 0's PAM_OPT_TILE when that names one of these, else timed on the first call
  * for the shot layout.  loop_ms (may be NULL) receives the GPU time of the
  * steps. */

@@ -39,9 +39,11 @@ struct pam_ctx {
     hipStream_t st = nullptr;
     bool own_stream = false;
     int max_shots = 1;
-    // second lane of pam_forward: its stream and the fork / join events
-    hipStream_t st2 = nullptr;
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // the two lanes of pam_forward and group runs: streams created together
+    // (so that they are served by different hardware queues; neither is the
+    // caller's stream), forked from and joined back into st
+    hipStream_t lane_st[2] = {nullptr, nullptr};
+    hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
 
     // model coefficient fields
     float *v2 = nullptr, *eps = nullptr, *del = nullptr, *cs = nullptr, *sn = nullptr;
@@ -300,8 +302,10 @@ extern "C" int pam_destroy(pam_ctx *ctx) {
     if (ctx->e0) (void)hipEventDestroy(ctx->e0);
     if (ctx->e1) (void)hipEventDestroy(ctx->e1);
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
-    if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
-    if (ctx->st2) (void)hipStreamDestroy(ctx->st2);
+    for (int i = 0; i < 2; ++i) {
+        if (ctx->join_ev[i]) (void)hipEventDestroy(ctx->join_ev[i]);
+        if (ctx->lane_st[i]) (void)hipStreamDestroy(ctx->lane_st[i]);
+    }
     if (ctx->own_stream && ctx->st) (void)hipStreamDestroy(ctx->st);
     delete ctx;
     return PAM_OK;
@@ -820,7 +824,8 @@ static int run_graph(pam_ctx *ctx, const std::string &key, Body &&body, Fix &&re
     }
     if (ctx->graphs.size() >= 16) {  // bound the cache: drop every entry once none can be running
         HCK(hipStreamSynchronize(ctx->st));
-        if (ctx->st2) HCK(hipStreamSynchronize(ctx->st2));
+        for (hipStream_t l : ctx->lane_st)
+            if (l) HCK(hipStreamSynchronize(l));
         for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
         ctx->graphs.clear();
     }
@@ -901,16 +906,45 @@ static int run_forward_loop_fs(pam_ctx *ctx, FieldSet &f, hipStream_t st, StepAr
     return PAM_OK;
 }
 
-// pam_forward on two lanes of Bl shots per launch: batches alternate between
-// the context's stream and a second stream, each lane with its own shot slots,
-// source slots, gather staging and cached graph; the second stream starts
-// after the work already queued on the first, and the first waits for it at
-// the end.  One lane's psi pre-pass and launch gaps then overlap the other's
-// step kernel.  Every batch is computed exactly as on one lane.
-static int forward_two_lanes(pam_ctx *ctx, int shot0, int nshots, float *seis, int Bl, long long nd) {
-    if (!ctx->st2) HCK(hipStreamCreate(&ctx->st2));  // default priority, like the context's stream
+// The lane streams (default priority, created together on first use) and
+// their fork from / join into the context's stream.
+static int ensure_lanes(pam_ctx *ctx) {
+    for (int i = 0; i < 2; ++i) {
+        if (!ctx->lane_st[i]) HCK(hipStreamCreate(&ctx->lane_st[i]));
+        if (!ctx->join_ev[i]) HCK(hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming));
+    }
     if (!ctx->fork_ev) HCK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
-    if (!ctx->join_ev) HCK(hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming));
+    return PAM_OK;
+}
+static int lanes_fork(pam_ctx *ctx) {
+    HCK(hipEventRecord(ctx->fork_ev, ctx->st));
+    for (hipStream_t l : ctx->lane_st) HCK(hipStreamWaitEvent(l, ctx->fork_ev, 0));
+    return PAM_OK;
+}
+// the context's stream waits for both lanes, on every path (r0: the result so
+// far); if that cannot be arranged the lanes are drained
+static int lanes_join(pam_ctx *ctx, int r0) {
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+        e = hipEventRecord(ctx->join_ev[i], ctx->lane_st[i]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->st, ctx->join_ev[i], 0);
+    }
+    if (e != hipSuccess)
+        for (hipStream_t l : ctx->lane_st) (void)hipStreamSynchronize(l);
+    if (r0) return r0;
+    if (e != hipSuccess) return fail(ctx, PAM_E_HIP, "joining the lanes: %s", hipGetErrorString(e));
+    return PAM_OK;
+}
+
+// pam_forward on two lanes of Bl shots per launch: batches alternate between
+// the two lane streams, each lane with its own shot slots, source slots,
+// gather staging and cached graph; the lanes start after the work already
+// queued on the context's stream, which waits for both at the end.  One
+// lane's psi pre-pass and launch gaps then overlap the other's step kernel.
+// Every batch is computed exactly as on one lane.
+static int forward_two_lanes(pam_ctx *ctx, int shot0, int nshots, float *seis, int Bl, long long nd) {
+    int rl = ensure_lanes(ctx);
+    if (rl) return rl;
     const int tile = choose_tile(ctx, Bl, true);
     ctx->last_tile = tile;
     ctx->last_batch = Bl;
@@ -931,10 +965,10 @@ static int forward_two_lanes(pam_ctx *ctx, int shot0, int nshots, float *seis, i
         lane1.zetaz += off;
     }
     FieldSet *fsl[2] = {&ctx->fs, &lane1};
-    hipStream_t stl[2] = {ctx->st, ctx->st2};
+    hipStream_t stl[2] = {ctx->lane_st[0], ctx->lane_st[1]};
     HCK(hipEventRecord(ctx->e0, ctx->st));
-    HCK(hipEventRecord(ctx->fork_ev, ctx->st));
-    HCK(hipStreamWaitEvent(ctx->st2, ctx->fork_ev, 0));
+    rl = lanes_fork(ctx);
+    if (rl) return rl;
     int nbatch = 0;
     auto batches = [&]() -> int {
         for (int b0 = 0; b0 < nshots; b0 += Bl, ++nbatch) {
@@ -965,13 +999,8 @@ static int forward_two_lanes(pam_ctx *ctx, int shot0, int nshots, float *seis, i
         }
         return PAM_OK;
     };
-    const int rb = batches();
-    // join on every path: the context's stream then orders after all of the second lane's work
-    const hipError_t ej = hipEventRecord(ctx->join_ev, ctx->st2);
-    const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(ctx->st, ctx->join_ev, 0) : ej;
-    if (ew != hipSuccess) (void)hipStreamSynchronize(ctx->st2);
-    if (rb) return rb;
-    if (ew != hipSuccess) return fail(ctx, PAM_E_HIP, "joining the second lane: %s", hipGetErrorString(ew));
+    rl = lanes_join(ctx, batches());  // the context's stream then orders after all lane work
+    if (rl) return rl;
     HCK(hipEventRecord(ctx->e1, ctx->st));
     HCK(hipEventSynchronize(ctx->e1));
     float ms = 0.f;
@@ -1323,31 +1352,15 @@ extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *
         for (int i = 0; i < n; ++i) HCK(hipStreamSynchronize(g->ctx[i]->st));
         return PAM_OK;
     };
-    // sub-group j runs on lane j % 2 (member 0's stream and its second stream)
+    // sub-group j runs on lane j % 2 of member 0 (one lane: member 0's stream)
     const int lanes = (c0->lanes > 1 && nsub > 1) ? 2 : 1;
     if (lanes > 1) {
-        if (!c0->st2) HCK(hipStreamCreate(&c0->st2));  // default priority
-        if (!c0->fork_ev) HCK(hipEventCreateWithFlags(&c0->fork_ev, hipEventDisableTiming));
-        if (!c0->join_ev) HCK(hipEventCreateWithFlags(&c0->join_ev, hipEventDisableTiming));
+        const int rl = ensure_lanes(c0);
+        if (rl) return rl;
     }
-    auto lane_stream = [&](int j) { return (j & 1) && lanes > 1 ? c0->st2 : c0->st; };
-    auto fork = [&]() -> int {
-        if (lanes > 1) {
-            HCK(hipEventRecord(c0->fork_ev, c0->st));
-            HCK(hipStreamWaitEvent(c0->st2, c0->fork_ev, 0));
-        }
-        return PAM_OK;
-    };
-    auto join = [&](int r0) -> int {  // on every path
-        if (lanes > 1) {
-            const hipError_t ej = hipEventRecord(c0->join_ev, c0->st2);
-            const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(c0->st, c0->join_ev, 0) : ej;
-            if (ew != hipSuccess) (void)hipStreamSynchronize(c0->st2);
-            if (!r0 && ew != hipSuccess)
-                return fail(c0, PAM_E_HIP, "joining the second lane: %s", hipGetErrorString(ew));
-        }
-        return r0;
-    };
+    auto lane_stream = [&](int j) { return lanes > 1 ? c0->lane_st[j & 1] : c0->st; };
+    auto fork = [&]() -> int { return lanes > 1 ? lanes_fork(c0) : PAM_OK; };
+    auto join = [&](int r0) -> int { return lanes > 1 ? lanes_join(c0, r0) : r0; };
     if (tile < 0) {
         auto it = g->tuned.find(layout);
         if (it != g->tuned.end()) {
