@@ -4,19 +4,23 @@
 Workload (one "step" = one pass of the hot path over one batch): forward
 modelling of the 64-shot pseudo-acoustic VTI + 50-cell CPML survey
 (nz=351, nx=1701, dx=dz=12.5 m -> 451 x 1801 padded cells, nt = 98 + 5001
-time steps, 1701 receivers per shot), all 64 shots advanced together in one
-launch per time step, seismograms written to HBM.  Inputs (models, wavelet,
-geometry) are resident on the device before the timed region.
+time steps, 1701 receivers per shot), the shots advanced in batches (the
+autotuner's batch, halved per lane, two batches at once on two streams),
+seismograms written to HBM.  Inputs (models, wavelet, geometry) are resident
+on the device before the timed region.
 
 Prints one JSON line:
   value      Gpt-updates/s = padded cells x time steps x shots / wall time
   roofline   achieved algorithmic HBM GB/s of the dominant kernel (the step
              kernel variant the autotuner picked, at the batch size it picked),
-             from HIP events around every launch of the timed steps, against
-             the 8 TB/s HBM3E peak
+             from HIP events that each launch stamps with the kernel's start
+             and end in a profiled run of the same launches, against the
+             8 TB/s HBM3E peak; plus the aggregate over the timed runs
   plain_loop the same 64 shots run one shot per launch (the per-shot loop)
   fwi_gradient seconds per shot of the FWI gradient of the same survey, all
              64 shots batched vs the per-shot loop
+  uneven_group the 16-model config-5 sample advanced as one group
+  other_configs configs 1, 3 and 4 (the last two on a reduced step count)
   cpu_baseline the CPU restatement of the reference's Numba kernels
              (float32 storage, float64 arithmetic, OpenMP over rows) on a
              bounded sample of the same workload
