@@ -1715,8 +1715,10 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         }
         StepArgs a;
         base_step_args(ctx, a, nb);
+        // (D_t is formed here only when this pass saves it: with checkpoints the
+        // recomputation forms it, so the pass runs the forward-only kernel)
         a.flags |= F_SRC | F_ILUM | ((ckpt || sb || rbc) ? 0 : F_SAVE) | (half ? F_SMAX : 0) |
-                   ((d2t && !sb && !rbc) ? F_D2T : 0);
+                   ((d2t && !sb && !rbc && !ckpt) ? F_D2T : 0);
         a.save_kind = skind;
         a.save = ctx->save;
         a.save_k0 = 0;
@@ -2078,7 +2080,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                     r = restore_ckpt(ctx, seg, nb);
                     if (r) return r;
                     StepArgs f = a;
-                    f.flags = (a.flags & (F_CERJAN | F_SRC | F_D2T)) | F_SAVE;
+                    f.flags = (a.flags & (F_CERJAN | F_SRC)) | F_SAVE | (d2t ? F_D2T : 0);
                     f.save_k0 = k0;
                     float *fc = ctx->aux.U0, *fp = ctx->aux.U1;
                     for (int k = k0; k <= k1; ++k) {
