@@ -25,7 +25,14 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                 # the z-march keeps its row queues in registers only when the
                 # row loop is fully unrolled; the default threshold gives up on
                 # the 4-cells-per-lane variants and spills the queues to scratch
-                "-mllvm", "-pragma-unroll-threshold=500000"]
+                "-mllvm", "-pragma-unroll-threshold=500000",
+                # products fuse into additions only within one expression, as
+                # written: under the default (fast) contraction a product fuses
+                # across statements depending on the basic-block layout of each
+                # instance, so two kernels instantiating the same step body
+                # (a grid alone, a group of grids; the forward-only and adjoint
+                # instances) could round one cell differently
+                "-ffp-contract=on"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", f"-I{os.path.join(ROOT, 'include')}"]
 
 

@@ -111,11 +111,15 @@ __device__ __forceinline__ float sd_term(float e, float dl, float px, float pz) 
 // U_{k+1} = 2 U_k - (U_{k-1} fx fz) + v^2 dt^2 L(U_k)  (c0 = U_k, pv = U_{k-1}
 // as stored, damped once more as the reference holds it; v2 = v^2 dt^2), and,
 // when `want_op`, the operator term v^2 dt^2 L(U_k) alone into `op`.  Every
-// fused multiply-add is written out: left to the compiler, a product fuses
-// into the next addition only when nothing else uses it, so an instance that
-// also stores the operator term would round the update differently from the
-// forward-only instance of the same kernel.
+// fused multiply-add is written out, so the update rounds the same in every
+// instance (forward-only or storing the operator term) whatever contraction
+// the compiler would choose; the build also restricts contraction to single
+// expressions.
 __device__ __forceinline__ float lf_base(float c0, float pv, float fx, float fz) {
+    // factors known to be 1 (CPML, interior tiles): 2 c0 - pv in one rounding
+    // either way (2 c0 is exact), so both forms give the same bits
+    if (__builtin_constant_p(fx) && __builtin_constant_p(fz) && fx == 1.0f && fz == 1.0f)
+        return __builtin_fmaf(2.0f, c0, -pv);
     return __builtin_fmaf(-(pv * fx), fz, 2.0f * c0);
 }
 // acoustic: sum = U_xx + U_zz (with the CPML corrections)

@@ -38,9 +38,38 @@ struct TileStep {
     const float *psix, *psiz;
 };
 
+// Tile of block b: tiles in bands of kBand tile columns, x fastest within a
+// band, then z (the z-neighbour of a tile, which shares 8 halo rows with it,
+// comes kBand tiles later, while those rows are still in L2, and the
+// x-neighbour right after it), the shots of a tile adjacent.
+template <int TZ>
+__device__ __forceinline__ void tile_of(const StepArgs &a, const int b, int &x0, int &z0, int &s) {
+    constexpr int kBand = 8;
+    const int tile = b / a.nshots;
+    s = b - tile * a.nshots;
+    const int band = tile / (kBand * a.nzt);
+    const int rem = tile - band * (kBand * a.nzt);
+    const int wb = min(kBand, a.nxt - band * kBand);
+    const int tzi = (wb == kBand) ? rem / kBand : rem / wb;
+    const int txi = band * kBand + (rem - tzi * wb);
+    x0 = txi * 64;
+    z0 = tzi * TZ;
+}
+
+// true when the 64 x TZ tile at (x0, z0) lies max(N, 4) cells or more inside
+// every edge: no rims, no absorbing layer (Cerjan factors 1, no CPML strips),
+// no rows or columns beyond the grid
+template <int TZ>
+__device__ __forceinline__ bool tile_inner(const StepArgs &a, int x0, int z0) {
+    const int m = max(a.N, 4);
+    return x0 >= m && x0 + 64 <= a.nxp - m && z0 >= m && z0 + TZ <= a.nzp - m;
+}
+
 // One block's work: block b (in [0, nxt * nzt * nshots)) of the launch
-// described by a and the step arguments m.
-template <int PHYS, bool CPML, int TZ, bool ADJ>
+// described by a and the step arguments m.  INNER: the tile satisfies
+// tile_inner, so the per-cell edge tests and the absorbing-layer terms drop
+// out (the values are the same bits: the factors there are exactly 1).
+template <int PHYS, bool CPML, int TZ, bool ADJ, bool INNER = false>
 __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const TileStep m) {
     constexpr int TX = 64;
     constexpr int R = TZ / 16;        // rows per thread
@@ -50,18 +79,8 @@ __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const 
     __shared__ float4 su[ROWS * Q];
     __shared__ float4 sd[(PHYS == 2) ? ROWS * (TX / 4) : 1];
 
-    const int tile = b / a.nshots;
-    const int s = b - tile * a.nshots;
-    // tiles in bands of kBand tile columns, x fastest within a band, then z:
-    // the z-neighbour of a tile (8 shared halo rows) comes kBand tiles later,
-    // while those rows are still in L2, and the x-neighbour right after it
-    constexpr int kBand = 8;
-    const int band = tile / (kBand * a.nzt);
-    const int rem = tile - band * (kBand * a.nzt);
-    const int wb = min(kBand, a.nxt - band * kBand);
-    const int tzi = (wb == kBand) ? rem / kBand : rem / wb;
-    const int txi = band * kBand + (rem - tzi * wb);
-    const int x0 = txi * TX, z0 = tzi * TZ;
+    int x0, z0, s;
+    tile_of<TZ>(a, b, x0, z0, s);
     const int P = a.P, nzp = a.nzp, nxp = a.nxp, N = a.N;
     const long long so = (long long)s * a.sstride;
     const float *__restrict__ cur = m.cur + so;
@@ -84,14 +103,14 @@ __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const 
         const int r = i / Q, q = i - r * Q;
         const int gz = z0 - 4 + r, gc = x0 - 4 + 4 * q;
         st[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i < ROWS * Q && gz >= 0 && gz < nzp && gc >= 0 && gc < P)
+        if (i < ROWS * Q && (INNER || (gz >= 0 && gz < nzp && gc >= 0 && gc < P)))
             st[u] = *reinterpret_cast<const float4 *>(at(cur, (unsigned)(gz * P + gc)));
     }
     float pv[R][4], v2[R][4], e[R][4], dl[R][4], cs[R][4], sn[R][4];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int z = zr0 + r;
-        const bool zin = z < nzp;
+        const bool zin = INNER || z < nzp;
         const unsigned ro = (unsigned)((zin ? z : 0) * P + col);
 #pragma unroll
         for (int j = 0; j < 4; ++j) pv[r][j] = v2[r][j] = e[r][j] = dl[r][j] = cs[r][j] = sn[r][j] = 0.f;
@@ -144,20 +163,20 @@ __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const 
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int c = col + j;
-        xlive[j] = (c >= 4) && (c < nxp - 4);
-        fx[j] = (!CPML && (flags & F_CERJAN) && c < P) ? a.cerx[c] : 1.0f;
-        xs[j] = CPML && ((c >= 4 && c < N) || (c >= nxp - N && c < nxp - 4));
+        xlive[j] = INNER || ((c >= 4) && (c < nxp - 4));
+        fx[j] = (!INNER && !CPML && (flags & F_CERJAN) && c < P) ? a.cerx[c] : 1.0f;
+        xs[j] = !INNER && CPML && ((c >= 4 && c < N) || (c >= nxp - N && c < nxp - 4));
     }
-    const bool tile_xstrip = CPML && (x0 < N || x0 + TX > nxp - N);
+    const bool tile_xstrip = !INNER && CPML && (x0 < N || x0 + TX > nxp - N);
     float tmax = 0.0f;  // F_SMAX: max |new field| over this thread's interior cells
 
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int z = zr0 + r;
-        if (z >= nzp) break;  // rows beyond the grid (last tile row only)
-        const bool zlive = (z >= 4) && (z < nzp - 4);
+        if (!INNER && z >= nzp) break;  // rows beyond the grid (last tile row only)
+        const bool zlive = INNER || ((z >= 4) && (z < nzp - 4));
         const unsigned ro = (unsigned)(z * P + col);  // 32-bit lane offsets: SGPR-base addressing
-        const float fz = (!CPML && (flags & F_CERJAN)) ? a.cerz[z] : 1.0f;
+        const float fz = (!INNER && !CPML && (flags & F_CERJAN)) ? a.cerz[z] : 1.0f;
         const int lr = rg * R + r + 4;  // staged row of z
 
         // x-window of row z
@@ -262,7 +281,7 @@ __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const 
 
         // ---- CPML: zeta with the new psi, corrected derivatives
         float PXX[4], PZZ[4], PX[4], PZ[4], zx[4], zz[4], dpx[4], dpz[4];
-        const bool zs = CPML && ((z >= 4 && z < N) || (z >= nzp - N && z < nzp - 4));
+        const bool zs = !INNER && CPML && ((z >= 4 && z < N) || (z >= nzp - N && z < nzp - 4));
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             PXX[j] = pxx[j];
@@ -374,7 +393,7 @@ __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const 
                 tmax = fmaxf(tmax, fabsf((flags & F_D2T) ? d : out[j]));
             }
         }
-        if (xlive[0] && xlive[3]) {
+        if (INNER || (xlive[0] && xlive[3])) {
             vst<4>(at(prv, ro), out);
         } else {
 #pragma unroll
@@ -391,7 +410,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ
     const StepArgs a) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= a.nxt * a.nzt * a.nshots) return;  // block-uniform
-    tile_body<PHYS, CPML, TZ, ADJ>(a, b, TileStep{a.k, a.flags, a.cur, a.prv, a.psix, a.psiz});
+    const TileStep m{a.k, a.flags, a.cur, a.prv, a.psix, a.psiz};
+    if constexpr (!ADJ) {
+        int x0, z0, s;
+        tile_of<TZ>(a, b, x0, z0, s);
+        if (tile_inner<TZ>(a, x0, z0)) {  // block-uniform
+            tile_body<PHYS, CPML, TZ, ADJ, true>(a, b, m);
+            return;
+        }
+    }
+    tile_body<PHYS, CPML, TZ, ADJ>(a, b, m);
 }
 
 // A group of grids in one launch: block b belongs to grid g with boff[g] <= b <
@@ -414,7 +442,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((TZ == 16 &
         m.psix = a.psix_in;
         m.psiz = a.psiz_in;
     }
-    tile_body<PHYS, CPML, TZ, false>(a, b - boff[g], m);
+    int x0, z0, s;
+    tile_of<TZ>(a, b - boff[g], x0, z0, s);
+    if (tile_inner<TZ>(a, x0, z0))  // block-uniform
+        tile_body<PHYS, CPML, TZ, false, true>(a, b - boff[g], m);
+    else
+        tile_body<PHYS, CPML, TZ, false>(a, b - boff[g], m);
 }
 
 template <int PHYS, bool CPML, int TZ>
