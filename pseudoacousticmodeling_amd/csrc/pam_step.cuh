@@ -191,6 +191,51 @@ __device__ __forceinline__ float snap_get(const StepArgs &a, int s, int k, long 
             return 0.0f;
     }
 }
+// The same with the slot's element offset `off` (wave-uniform, snap_off) and
+// 32-bit lane indices: the access is a uniform base plus a 32-bit lane offset.
+__device__ __forceinline__ long long snap_off(const StepArgs &a, int s, int k) {
+    return (long long)s * a.save_sstride + (long long)(k - a.save_k0) * a.save_kstride;
+}
+__device__ __forceinline__ float snap_get_at(const StepArgs &a, long long off, int s, int k, unsigned ii, unsigned pi) {
+    switch (a.save_kind) {
+        case SV_F32_INT:
+            return *at(reinterpret_cast<const float *>(a.save) + off, ii);
+        case SV_F32_PAD:
+            return *at(reinterpret_cast<const float *>(a.save) + off, pi);
+        case SV_F16_INT:
+            return __half2float(*at(reinterpret_cast<const __half *>(a.save) + off, ii)) *
+                   a.save_scale[(long long)s * a.nt + k];
+        case SV_BF16_INT:
+            return __bfloat162float(*at(reinterpret_cast<const __hip_bfloat16 *>(a.save) + off, ii)) *
+                   a.save_scale[(long long)s * a.nt + k];
+        default:
+            return 0.0f;
+    }
+}
+__device__ __forceinline__ void snap_put_at(const StepArgs &a, long long off, int s, int k, unsigned ii, unsigned pi,
+                                            float v) {
+    float inv = 0.0f;
+    if (a.save_kind == SV_F16_INT || a.save_kind == SV_BF16_INT) {
+        const float m = a.save_scale[(long long)s * a.nt + k];
+        inv = (m > 0.0f) ? 1.0f / m : 0.0f;
+    }
+    switch (a.save_kind) {
+        case SV_F32_INT:
+            *at(reinterpret_cast<float *>(a.save) + off, ii) = v;
+            break;
+        case SV_F32_PAD:
+            *at(reinterpret_cast<float *>(a.save) + off, pi) = v;
+            break;
+        case SV_F16_INT:
+            *at(reinterpret_cast<__half *>(a.save) + off, ii) = __float2half(v * inv);
+            break;
+        case SV_BF16_INT:
+            *at(reinterpret_cast<__hip_bfloat16 *>(a.save) + off, ii) = __float2bfloat16(v * inv);
+            break;
+        default:
+            break;
+    }
+}
 __device__ __forceinline__ void snap_put(const StepArgs &a, int s, int k, long long ii, long long pi, float v) {
     const long long base = (long long)s * a.save_sstride + (long long)(k - a.save_k0) * a.save_kstride;
     float inv = 0.0f;

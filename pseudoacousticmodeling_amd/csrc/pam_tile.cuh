@@ -68,8 +68,10 @@ __device__ __forceinline__ bool tile_inner(const StepArgs &a, int x0, int z0) {
 // One block's work: block b (in [0, nxt * nzt * nshots)) of the launch
 // described by a and the step arguments m.  INNER: the tile satisfies
 // tile_inner, so the per-cell edge tests and the absorbing-layer terms drop
-// out (the values are the same bits: the factors there are exactly 1).
-template <int PHYS, bool CPML, int TZ, bool ADJ, bool INNER = false>
+// out (the values are the same bits: the factors there are exactly 1).  FM:
+// the flags this instance implements (the step's flags are masked by it, so
+// the code of the others drops out).
+template <int PHYS, bool CPML, int TZ, int FM, bool INNER = false>
 __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const TileStep m) {
     constexpr int TX = 64;
     constexpr int R = TZ / 16;        // rows per thread
@@ -90,7 +92,7 @@ __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const 
     float *__restrict__ zetax = a.zetax + so;
     float *__restrict__ zetaz = a.zetaz + so;
     const int tid = threadIdx.x;
-    const int flags = ADJ ? m.flags : (m.flags & F_FWD_MASK);  // forward-only instances drop the rest
+    const int flags = m.flags & FM;
     const int cx = tid & 15, rg = tid >> 4;
     const int col = x0 + 4 * cx;   // first of this thread's 4 columns
     const int zr0 = z0 + rg * R;   // first of this thread's R rows
@@ -243,36 +245,42 @@ __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const 
         // ---- snapshots / illumination / imaging of the input field
         if ((flags & (F_SAVE | F_ILUM | F_IMAGE | F_GRADM)) && zlive) {
             const bool zint = (z >= N) && (z < nzp - N);
-            const long long ib = (long long)(z - N) * a.nx - N;  // interior index = ib + c
+            const int ib = (z - N) * a.nx - N;  // interior index = ib + c (used on interior cells only)
+            const long long acc_s = (long long)s * a.acc_sstride;  // per-shot bases: wave-uniform
+            const long long sk = snap_off(a, s, m.k);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int c = col + j;
                 const bool cin = zint && c >= N && c < nxp - N;
-                const long long ii = ib + c;
+                const unsigned ii = (unsigned)(ib + c);
                 if ((flags & F_SAVE) && !(flags & F_D2T)) {
                     if (a.save_kind == SV_F32_PAD) {
-                        if (xlive[j]) snap_put(a, s, m.k, 0, ro + j, (c0[j] * fx[j]) * fz);
+                        if (xlive[j]) snap_put_at(a, sk, s, m.k, 0u, ro + j, (c0[j] * fx[j]) * fz);
                     } else if (cin) {
-                        snap_put(a, s, m.k, ii, 0, c0[j]);
+                        snap_put_at(a, sk, s, m.k, ii, 0u, c0[j]);
                     }
                 }
                 if (!cin) continue;
                 if (flags & F_ILUM) {
-                    float *acc = a.ilum + (long long)s * a.acc_sstride + ii;
+                    float *acc = at(a.ilum + acc_s, ii);
                     *acc += c0[j] * c0[j];
                 }
                 if (flags & F_IMAGE) {
-                    const float S = snap_get(a, s, m.k, ii, ro + j);
-                    float *im = a.image + (long long)s * a.acc_sstride + ii;
+                    const float S = snap_get_at(a, sk, s, m.k, ii, ro + j);
+                    float *im = at(a.image + acc_s, ii);
                     *im += S * c0[j];
                 }
                 if (flags & F_GRADM) {
-                    const float Sp = (flags & F_D2T) ? 0.0f : snap_get(a, s, m.k + 1, ii, ro + j);
-                    const float S0 = (flags & F_D2T) ? 0.0f : snap_get(a, s, m.k, ii, ro + j);
-                    const float Sm = (flags & F_D2T) ? 0.0f : snap_get(a, s, m.k - 1, ii, ro + j);
-                    const float d2t =
-                            ((flags & F_D2T) ? snap_get(a, s, m.k, ii, ro + j) : (Sp - 2.0f * S0 + Sm)) * a.idt2;
-                    float *im = a.image + (long long)s * a.acc_sstride + ii;
+                    float d2t;
+                    if (flags & F_D2T) {
+                        d2t = snap_get_at(a, sk, s, m.k, ii, ro + j) * a.idt2;
+                    } else {
+                        const float Sp = snap_get_at(a, snap_off(a, s, m.k + 1), s, m.k + 1, ii, ro + j);
+                        const float S0 = snap_get_at(a, sk, s, m.k, ii, ro + j);
+                        const float Sm = snap_get_at(a, snap_off(a, s, m.k - 1), s, m.k - 1, ii, ro + j);
+                        d2t = (Sp - 2.0f * S0 + Sm) * a.idt2;
+                    }
+                    float *im = at(a.image + acc_s, ii);
                     *im += d2t * c0[j];
                 }
             }
@@ -382,14 +390,16 @@ __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const 
         }
         if ((flags & F_SMAX) || ((flags & F_D2T) && (flags & F_SAVE)) || (flags & F_DSAVE)) {
             const bool zint = z >= N && z < nzp - N;
-            const long long ib = (long long)(z - N) * a.nx - N;
+            const int ib = (z - N) * a.nx - N;  // interior index = ib + c (used on interior cells only)
+            const long long sk = snap_off(a, s, m.k);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (!(zint && col + j >= N && col + j < nxp - N)) continue;
                 // interior cells are undamped: D_k = S_{k+1} - 2 S_k + S_{k-1}
                 const float d = (flags & F_D2T_OP) ? opv[j] : out[j] - 2.0f * c0[j] + pv[r][j];
-                if ((flags & F_D2T) && (flags & F_SAVE)) snap_put(a, s, m.k, ib + col + j, 0, d);
-                if (flags & F_DSAVE) a.dsave[(long long)s * a.dsave_sstride + ib + col + j] = d;
+                const unsigned ii = (unsigned)(ib + col + j);
+                if ((flags & F_D2T) && (flags & F_SAVE)) snap_put_at(a, sk, s, m.k, ii, 0u, d);
+                if (flags & F_DSAVE) *at(a.dsave + (long long)s * a.dsave_sstride, ii) = d;
                 tmax = fmaxf(tmax, fabsf((flags & F_D2T) ? d : out[j]));
             }
         }
@@ -404,22 +414,23 @@ __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const 
     if ((flags & F_SMAX) && m.k + 1 < a.nt) block_max_to(a.smax + (long long)s * a.nt + m.k + 1, tmax);
 }
 
-// forward-only instances of the 16-row tile fit 5 waves per SIMD
-template <int PHYS, bool CPML, int TZ, int WPE, bool ADJ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!ADJ && TZ == 16 && PHYS < 2) ? 5 : WPE))) void k_tile(
+// the forward instances of the 16-row tile (without and with illumination) fit
+// 5 waves per SIMD; the saving and adjoint ones would spill there and run at 4
+template <int PHYS, bool CPML, int TZ, int WPE, int FM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(((FM == F_FWD_MASK || FM == F_PASS_ILUM) && TZ == 16 && PHYS < 2) ? 5 : WPE))) void k_tile(
     const StepArgs a) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= a.nxt * a.nzt * a.nshots) return;  // block-uniform
     const TileStep m{a.k, a.flags, a.cur, a.prv, a.psix, a.psiz};
-    if constexpr (!ADJ) {
+    if constexpr (FM != F_ALL) {
         int x0, z0, s;
         tile_of<TZ>(a, b, x0, z0, s);
         if (tile_inner<TZ>(a, x0, z0)) {  // block-uniform
-            tile_body<PHYS, CPML, TZ, ADJ, true>(a, b, m);
+            tile_body<PHYS, CPML, TZ, FM, true>(a, b, m);
             return;
         }
     }
-    tile_body<PHYS, CPML, TZ, ADJ>(a, b, m);
+    tile_body<PHYS, CPML, TZ, FM>(a, b, m);
 }
 
 // A group of grids in one launch: block b belongs to grid g with boff[g] <= b <
@@ -445,9 +456,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((TZ == 16 &
     int x0, z0, s;
     tile_of<TZ>(a, b - boff[g], x0, z0, s);
     if (tile_inner<TZ>(a, x0, z0))  // block-uniform
-        tile_body<PHYS, CPML, TZ, false, true>(a, b - boff[g], m);
+        tile_body<PHYS, CPML, TZ, F_FWD_MASK, true>(a, b - boff[g], m);
     else
-        tile_body<PHYS, CPML, TZ, false>(a, b - boff[g], m);
+        tile_body<PHYS, CPML, TZ, F_FWD_MASK>(a, b - boff[g], m);
 }
 
 template <int PHYS, bool CPML, int TZ>
@@ -464,16 +475,33 @@ static int launch_tile_multi_t(const StepArgs *list, const int *boff, int ngrid,
 
 template <int PHYS, bool CPML, int TZ, int WPE>
 static int launch_tile_t(const StepArgs &a0, hipStream_t st, const hipEvent_t *ev = nullptr) {
-    const bool adj = (a0.flags & ~F_FWD_MASK) != 0;
     StepArgs a = a0;
     a.nxt = (a.nxp + 63) / 64;
     a.nzt = (a.nzp + TZ - 1) / TZ;
     long long blocks = (long long)a.nxt * a.nzt * a.nshots;
     blocks = (blocks + 7) / 8 * 8;
-    if (adj)
-        launch_timed(k_tile<PHYS, CPML, TZ, WPE, true>, dim3((unsigned)blocks), st, ev, a);
-    else
-        launch_timed(k_tile<PHYS, CPML, TZ, WPE, false>, dim3((unsigned)blocks), st, ev, a);
+    const dim3 grid((unsigned)blocks);
+    // the smallest instance whose flag set holds the step's flags
+    const auto within = [&](int fm) { return (a.flags & ~fm) == 0; };
+    if (within(F_FWD_MASK)) {
+        launch_timed(k_tile<PHYS, CPML, TZ, WPE, F_FWD_MASK>, grid, st, ev, a);
+        return (int)hipGetLastError();
+    }
+    if constexpr (TZ == 16) {
+        if (within(F_PASS_ILUM)) {
+            launch_timed(k_tile<PHYS, CPML, TZ, WPE, F_PASS_ILUM>, grid, st, ev, a);
+            return (int)hipGetLastError();
+        }
+        if (within(F_PASS_SAVE)) {
+            launch_timed(k_tile<PHYS, CPML, TZ, WPE, F_PASS_SAVE>, grid, st, ev, a);
+            return (int)hipGetLastError();
+        }
+        if (within(F_PASS_ADJ)) {
+            launch_timed(k_tile<PHYS, CPML, TZ, WPE, F_PASS_ADJ>, grid, st, ev, a);
+            return (int)hipGetLastError();
+        }
+    }
+    launch_timed(k_tile<PHYS, CPML, TZ, WPE, F_ALL>, grid, st, ev, a);
     return (int)hipGetLastError();
 }
 
