@@ -33,10 +33,12 @@ constexpr int F_FWD_MASK = F_CERJAN | F_SRC | F_GATHER;
 // the flag sets of the three passes of an imaging or gradient run with the
 // source field recomputed from checkpoints or boundaries: the forward pass
 // with illumination, the recompute that stores S_k or D_k, and the adjoint
-// pass; the 16-row tile has an instance for each, and one for every flag
+// pass (of a gradient or of an image); the 16-row tile has an instance for
+// each, and one for every flag
 constexpr int F_PASS_ILUM = F_FWD_MASK | F_ILUM;
 constexpr int F_PASS_SAVE = F_FWD_MASK | F_SAVE | F_D2T;
-constexpr int F_PASS_ADJ = F_CERJAN | F_RECINJ | F_IMAGE | F_GRADM | F_D2T;
+constexpr int F_PASS_GRAD = F_CERJAN | F_RECINJ | F_GRADM | F_D2T;
+constexpr int F_PASS_IMAGE = F_CERJAN | F_RECINJ | F_IMAGE;
 constexpr int F_ALL = 0x7fffffff;
 
 // snapshot storage kinds

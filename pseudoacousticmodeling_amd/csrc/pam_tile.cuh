@@ -414,10 +414,11 @@ __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const 
     if ((flags & F_SMAX) && m.k + 1 < a.nt) block_max_to(a.smax + (long long)s * a.nt + m.k + 1, tmax);
 }
 
-// the forward instances of the 16-row tile (without and with illumination) fit
-// 5 waves per SIMD; the saving and adjoint ones would spill there and run at 4
+// the forward instances of the 16-row tile (without and with illumination)
+// and the imaging one fit 5 waves per SIMD; the saving and gradient ones would
+// spill there and run at 4
 template <int PHYS, bool CPML, int TZ, int WPE, int FM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(((FM == F_FWD_MASK || FM == F_PASS_ILUM) && TZ == 16 && PHYS < 2) ? 5 : WPE))) void k_tile(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(((FM == F_FWD_MASK || FM == F_PASS_ILUM || FM == F_PASS_IMAGE) && TZ == 16 && PHYS < 2) ? 5 : WPE))) void k_tile(
     const StepArgs a) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= a.nxt * a.nzt * a.nshots) return;  // block-uniform
@@ -496,8 +497,12 @@ static int launch_tile_t(const StepArgs &a0, hipStream_t st, const hipEvent_t *e
             launch_timed(k_tile<PHYS, CPML, TZ, WPE, F_PASS_SAVE>, grid, st, ev, a);
             return (int)hipGetLastError();
         }
-        if (within(F_PASS_ADJ)) {
-            launch_timed(k_tile<PHYS, CPML, TZ, WPE, F_PASS_ADJ>, grid, st, ev, a);
+        if (within(F_PASS_GRAD)) {
+            launch_timed(k_tile<PHYS, CPML, TZ, WPE, F_PASS_GRAD>, grid, st, ev, a);
+            return (int)hipGetLastError();
+        }
+        if (within(F_PASS_IMAGE)) {
+            launch_timed(k_tile<PHYS, CPML, TZ, WPE, F_PASS_IMAGE>, grid, st, ev, a);
             return (int)hipGetLastError();
         }
     }
