@@ -240,7 +240,8 @@ def main(argv=None):
                     help="launch the steps one by one instead of replaying captured graphs")
     ap.add_argument("--group-shots", type=int, default=4, help="shots per model in the uneven-grid group leg")
     ap.add_argument("--fwi-plain-shots", type=int, default=8,
-                    help="shots timed in the per-shot gradient loop (seconds per shot are reported)")
+                    help="shots timed in the per-shot gradient loop (seconds per shot are reported; 0: skip "
+                         "the loop, e.g. to profile the batched gradient alone)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -341,20 +342,22 @@ def main(argv=None):
         torch.cuda.synchronize()
         gb = time.perf_counter() - t0
         gfinite = bool(torch.isfinite(out_g["g_m"]).all().item())
-        np_ = max(1, min(args.fwi_plain_shots, S))
-        eng.set_option(_lib.PAM_OPT_BATCH, 1)
-        eng.gradient(res[:1])
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        eng.gradient(res[:np_])
-        torch.cuda.synchronize()
-        gp = time.perf_counter() - t0
-        eng.set_option(_lib.PAM_OPT_BATCH, args.batch)
-        fwi = {"unit": "s/shot", "batched": gb / S, "plain": gp / np_, "speedup": (gp / np_) / (gb / S),
-               "shots_batched": S, "shots_plain_loop": np_, "storage": "uniform checkpoints in HBM",
-               "finite": gfinite}
-        print(f"bench: FWI gradient {gb / S * 1e3:.1f} ms/shot batched, {gp / np_ * 1e3:.1f} ms/shot plain",
-              file=sys.stderr, flush=True)
+        np_ = min(args.fwi_plain_shots, S)
+        gp = None
+        if np_ > 0:
+            eng.set_option(_lib.PAM_OPT_BATCH, 1)
+            eng.gradient(res[:1])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            eng.gradient(res[:np_])
+            torch.cuda.synchronize()
+            gp = time.perf_counter() - t0
+            eng.set_option(_lib.PAM_OPT_BATCH, args.batch)
+        fwi = {"unit": "s/shot", "batched": gb / S, "plain": gp / np_ if gp else None,
+               "speedup": (gp / np_) / (gb / S) if gp else None, "shots_batched": S, "shots_plain_loop": np_,
+               "storage": "uniform checkpoints in HBM", "finite": gfinite}
+        plain_txt = f", {gp / np_ * 1e3:.1f} ms/shot plain" if gp else ""
+        print(f"bench: FWI gradient {gb / S * 1e3:.1f} ms/shot batched{plain_txt}", file=sys.stderr, flush=True)
     eng.close()
     del seis
     torch.cuda.empty_cache()
