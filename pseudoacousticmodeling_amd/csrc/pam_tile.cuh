@@ -242,49 +242,6 @@ __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const 
             }
         }
 
-        // ---- snapshots / illumination / imaging of the input field
-        if ((flags & (F_SAVE | F_ILUM | F_IMAGE | F_GRADM)) && zlive) {
-            const bool zint = (z >= N) && (z < nzp - N);
-            const int ib = (z - N) * a.nx - N;  // interior index = ib + c (used on interior cells only)
-            const long long acc_s = (long long)s * a.acc_sstride;  // per-shot bases: wave-uniform
-            const long long sk = snap_off(a, s, m.k);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int c = col + j;
-                const bool cin = zint && c >= N && c < nxp - N;
-                const unsigned ii = (unsigned)(ib + c);
-                if ((flags & F_SAVE) && !(flags & F_D2T)) {
-                    if (a.save_kind == SV_F32_PAD) {
-                        if (xlive[j]) snap_put_at(a, sk, s, m.k, 0u, ro + j, (c0[j] * fx[j]) * fz);
-                    } else if (cin) {
-                        snap_put_at(a, sk, s, m.k, ii, 0u, c0[j]);
-                    }
-                }
-                if (!cin) continue;
-                if (flags & F_ILUM) {
-                    float *acc = at(a.ilum + acc_s, ii);
-                    *acc += c0[j] * c0[j];
-                }
-                if (flags & F_IMAGE) {
-                    const float S = snap_get_at(a, sk, s, m.k, ii, ro + j);
-                    float *im = at(a.image + acc_s, ii);
-                    *im += S * c0[j];
-                }
-                if (flags & F_GRADM) {
-                    float d2t;
-                    if (flags & F_D2T) {
-                        d2t = snap_get_at(a, sk, s, m.k, ii, ro + j) * a.idt2;
-                    } else {
-                        const float Sp = snap_get_at(a, snap_off(a, s, m.k + 1), s, m.k + 1, ii, ro + j);
-                        const float S0 = snap_get_at(a, sk, s, m.k, ii, ro + j);
-                        const float Sm = snap_get_at(a, snap_off(a, s, m.k - 1), s, m.k - 1, ii, ro + j);
-                        d2t = (Sp - 2.0f * S0 + Sm) * a.idt2;
-                    }
-                    float *im = at(a.image + acc_s, ii);
-                    *im += d2t * c0[j];
-                }
-            }
-        }
         if (!zlive) continue;
 
         // ---- CPML: zeta with the new psi, corrected derivatives
@@ -410,15 +367,59 @@ __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const 
             for (int j = 0; j < 4; ++j)
                 if (xlive[j]) *at(prv, ro + j) = out[j];
         }
+        // ---- snapshots / illumination / imaging of the input field (after the update:
+        // fewer values are live here than before it)
+        if ((flags & (F_SAVE | F_ILUM | F_IMAGE | F_GRADM)) && zlive) {
+            const bool zint = (z >= N) && (z < nzp - N);
+            const int ib = (z - N) * a.nx - N;  // interior index = ib + c (used on interior cells only)
+            const long long acc_s = (long long)s * a.acc_sstride;  // per-shot bases: wave-uniform
+            const long long sk = snap_off(a, s, m.k);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int c = col + j;
+                const bool cin = zint && c >= N && c < nxp - N;
+                const unsigned ii = (unsigned)(ib + c);
+                if ((flags & F_SAVE) && !(flags & F_D2T)) {
+                    if (a.save_kind == SV_F32_PAD) {
+                        if (xlive[j]) snap_put_at(a, sk, s, m.k, 0u, ro + j, (c0[j] * fx[j]) * fz);
+                    } else if (cin) {
+                        snap_put_at(a, sk, s, m.k, ii, 0u, c0[j]);
+                    }
+                }
+                if (!cin) continue;
+                if (flags & F_ILUM) {
+                    float *acc = at(a.ilum + acc_s, ii);
+                    *acc += c0[j] * c0[j];
+                }
+                if (flags & F_IMAGE) {
+                    const float S = snap_get_at(a, sk, s, m.k, ii, ro + j);
+                    float *im = at(a.image + acc_s, ii);
+                    *im += S * c0[j];
+                }
+                if (flags & F_GRADM) {
+                    float d2t;
+                    if (flags & F_D2T) {
+                        d2t = snap_get_at(a, sk, s, m.k, ii, ro + j) * a.idt2;
+                    } else {
+                        const float Sp = snap_get_at(a, snap_off(a, s, m.k + 1), s, m.k + 1, ii, ro + j);
+                        const float S0 = snap_get_at(a, sk, s, m.k, ii, ro + j);
+                        const float Sm = snap_get_at(a, snap_off(a, s, m.k - 1), s, m.k - 1, ii, ro + j);
+                        d2t = (Sp - 2.0f * S0 + Sm) * a.idt2;
+                    }
+                    float *im = at(a.image + acc_s, ii);
+                    *im += d2t * c0[j];
+                }
+            }
+        }
     }
     if ((flags & F_SMAX) && m.k + 1 < a.nt) block_max_to(a.smax + (long long)s * a.nt + m.k + 1, tmax);
 }
 
-// the forward instances of the 16-row tile (without and with illumination)
-// and the imaging one fit 5 waves per SIMD; the saving and gradient ones would
-// spill there and run at 4
+// the instances of the 16-row tile for one pass fit 5 waves per SIMD (the
+// gradient one spills 3 VGPRs there and still runs faster than at 4); the
+// instance for every flag runs at 4
 template <int PHYS, bool CPML, int TZ, int WPE, int FM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(((FM == F_FWD_MASK || FM == F_PASS_ILUM || FM == F_PASS_IMAGE) && TZ == 16 && PHYS < 2) ? 5 : WPE))) void k_tile(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(((FM == F_FWD_MASK || FM == F_PASS_ILUM || FM == F_PASS_IMAGE || FM == F_PASS_GRAD || FM == F_PASS_SAVE) && TZ == 16 && PHYS < 2) ? 5 : WPE))) void k_tile(
     const StepArgs a) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= a.nxt * a.nzt * a.nshots) return;  // block-uniform
