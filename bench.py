@@ -345,7 +345,9 @@ def main(argv=None):
         np_ = min(args.fwi_plain_shots, S)
         gp = None
         if np_ > 0:
+            # the per-shot loop: one shot at a time on one stream
             eng.set_option(_lib.PAM_OPT_BATCH, 1)
+            eng.set_option(_lib.PAM_OPT_LANES, 1)
             eng.gradient(res[:1])
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -353,6 +355,7 @@ def main(argv=None):
             torch.cuda.synchronize()
             gp = time.perf_counter() - t0
             eng.set_option(_lib.PAM_OPT_BATCH, args.batch)
+            eng.set_option(_lib.PAM_OPT_LANES, lanes)
         fwi = {"unit": "s/shot", "batched": gb / S, "plain": gp / np_ if gp else None,
                "speedup": (gp / np_) / (gb / S) if gp else None, "shots_batched": S, "shots_plain_loop": np_,
                "storage": "uniform checkpoints in HBM", "finite": gfinite}

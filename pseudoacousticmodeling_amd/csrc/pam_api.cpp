@@ -1536,6 +1536,281 @@ static int revolve_split(int n, int s) {
     return std::max(1, (int)m);
 }
 
+// ---- uniform checkpoints on two lanes
+// The settings of one uniform-checkpoint run that every batch shares.
+struct CkptRun {
+    int shot0 = 0;
+    const float *data = nullptr;
+    long long nd = 0, nint = 0, per_step = 0;
+    bool grad = false, d2t = false;
+    int tile = 0, C = 0, nseg = 0, t_lo = 0, t_hi = 0, nbuf = 0, skind = 0;
+    float *image = nullptr, *ilum = nullptr;
+    std::vector<hipEvent_t> red_ev;   // reduction of batch i done
+    std::vector<char> red_issued;     // batch i's reduction enqueued
+};
+
+// One batch of a uniform-checkpoint run on one lane, issued a time step at a
+// time (ckpt_tick) so that one host thread can keep two lanes' queues filled:
+// the forward pass with illumination saving the checkpoints, then segment by
+// segment (the last first) the recompute storing S_k or D_k and the adjoint
+// steps over it, then the reduction of the batch's accumulators, enqueued
+// after the previous batch's so that the sums are those of one lane.
+struct CkptLane {
+    FieldSet F, A;  // the lane's views of the two field sets
+    float *save = nullptr, *acc_ilum = nullptr, *acc_img = nullptr, *ckpt = nullptr;
+    std::vector<int> flip;  // psi buffer of each checkpoint
+    hipStream_t st = nullptr;
+    int slot0 = 0, Bl = 0;
+    // the batch in flight
+    int batch = -1, b0 = 0, nb = 0, phase = 0;
+    int k = 0, j = 0, seg = 0, t = 0, lo = 0, hi = 0, k0 = 0, k1 = 0;
+    StepArgs a{}, b{}, f{};
+    float *cur = nullptr, *prv = nullptr, *fc = nullptr, *fp = nullptr;
+};
+
+static FieldSet fieldset_view(const pam_ctx *ctx, const FieldSet &f, int slot0) {
+    FieldSet v = f;
+    const long long off = (long long)slot0 * ctx->plane;
+    v.U0 += off;
+    v.U1 += off;
+    if (ctx->abc == PAM_CPML) {
+        v.psix += off;
+        v.psiz += off;
+        v.zetax += off;
+        v.zetaz += off;
+    }
+    return v;
+}
+
+// save_ckpt / restore_ckpt on a lane
+static int ckpt_lane_save(pam_ctx *ctx, CkptLane &L, int j) {
+    const size_t fb = (size_t)ctx->plane * L.nb * sizeof(float);
+    const long long fl = ctx->plane * L.Bl;  // a lane's checkpoint fields hold Bl shots
+    float *c = L.ckpt + (long long)j * ckpt_fields(ctx) * fl;
+    HCK(hipMemcpyAsync(c, L.cur, fb, hipMemcpyDeviceToDevice, L.st));
+    HCK(hipMemcpyAsync(c + fl, L.prv, fb, hipMemcpyDeviceToDevice, L.st));
+    if (ctx->abc == PAM_CPML) {
+        const long long in = L.F.psi_flip ? ctx->psi_half : 0;
+        HCK(hipMemcpyAsync(c + 2 * fl, L.F.psix + in, fb, hipMemcpyDeviceToDevice, L.st));
+        HCK(hipMemcpyAsync(c + 3 * fl, L.F.psiz + in, fb, hipMemcpyDeviceToDevice, L.st));
+        HCK(hipMemcpyAsync(c + 4 * fl, L.F.zetax, fb, hipMemcpyDeviceToDevice, L.st));
+        HCK(hipMemcpyAsync(c + 5 * fl, L.F.zetaz, fb, hipMemcpyDeviceToDevice, L.st));
+    }
+    L.flip[j] = L.F.psi_flip;
+    return PAM_OK;
+}
+static int ckpt_lane_restore(pam_ctx *ctx, CkptLane &L, int j) {
+    const size_t fb = (size_t)ctx->plane * L.nb * sizeof(float);
+    const long long fl = ctx->plane * L.Bl;
+    const float *c = L.ckpt + (long long)j * ckpt_fields(ctx) * fl;
+    HCK(hipMemcpyAsync(L.A.U0, c, fb, hipMemcpyDeviceToDevice, L.st));
+    HCK(hipMemcpyAsync(L.A.U1, c + fl, fb, hipMemcpyDeviceToDevice, L.st));
+    if (ctx->abc == PAM_CPML) {
+        L.A.psi_flip = L.flip[j];
+        const long long in = L.A.psi_flip ? ctx->psi_half : 0;
+        HCK(hipMemcpyAsync(L.A.psix + in, c + 2 * fl, fb, hipMemcpyDeviceToDevice, L.st));
+        HCK(hipMemcpyAsync(L.A.psiz + in, c + 3 * fl, fb, hipMemcpyDeviceToDevice, L.st));
+        HCK(hipMemcpyAsync(L.A.zetax, c + 4 * fl, fb, hipMemcpyDeviceToDevice, L.st));
+        HCK(hipMemcpyAsync(L.A.zetaz, c + 5 * fl, fb, hipMemcpyDeviceToDevice, L.st));
+    }
+    return PAM_OK;
+}
+
+// Issues the next piece of lane L's batch (one time step, or the batch's set-up
+// or reduction); *done once the reduction is enqueued.  The reduction waits
+// (returns without issuing) until the previous batch's is enqueued.
+static int ckpt_tick(pam_ctx *ctx, CkptRun &R, CkptLane &L, bool *done) {
+    const pam_grid &g = ctx->g;
+    const int nt = g.nt;
+    const long long nint = R.nint;
+    *done = false;
+    switch (L.phase) {
+        case 0: {  // forward pass set-up
+            int r = zero_fields(ctx, L.F, L.nb, true, true, L.st);
+            if (r) return r;
+            r = upload_sources(ctx, R.shot0 + L.b0, L.nb, L.slot0, L.st);
+            if (r) return r;
+            HCK(hipMemsetAsync(L.acc_ilum, 0, (size_t)L.nb * nint * 4, L.st));
+            HCK(hipMemsetAsync(L.acc_img, 0, (size_t)L.nb * nint * 4, L.st));
+            KCK(launch_inject_point(L.F.U0, ctx->plane, ctx->d_ssz + L.slot0, ctx->d_ssx + L.slot0, ctx->P, ctx->wav,
+                                    0, L.nb, L.st));
+            base_step_args(ctx, L.a, L.nb);
+            L.a.flags |= F_SRC | F_ILUM;
+            L.a.save_kind = R.skind;
+            L.a.save = L.save;
+            L.a.save_k0 = 0;
+            L.a.save_kstride = R.per_step;
+            L.a.save_sstride = R.per_step * R.nbuf;
+            L.a.ilum = L.acc_ilum;
+            L.a.image = L.acc_img;
+            L.a.acc_sstride = nint;
+            L.a.ssz = ctx->d_ssz + L.slot0;
+            L.a.ssx = ctx->d_ssx + L.slot0;
+            L.cur = L.F.U0;
+            L.prv = L.F.U1;
+            L.k = 0;
+            L.j = 0;
+            L.phase = 1;
+            return PAM_OK;
+        }
+        case 1: {  // forward step k, checkpoint j first when it starts a segment
+            if (L.j < R.nseg && L.k == std::max(0, L.j * R.C - 1)) {
+                int r = ckpt_lane_save(ctx, L, L.j);
+                if (r) return r;
+                ++L.j;
+            }
+            int r = fwd_step_fs(ctx, L.F, L.a, R.tile, L.cur, L.prv, L.k, nullptr, L.st);
+            if (r) return r;
+            std::swap(L.cur, L.prv);
+            if (++L.k < nt) return PAM_OK;
+            // backward pass set-up: the adjoint field reuses the lane's main set
+            r = zero_fields(ctx, L.F, L.nb, true, ctx->reset_adj_cpml != 0, L.st);
+            if (r) return r;
+            if (R.t_hi >= g.itlag)
+                KCK(launch_inject_rec(L.F.U0, ctx->plane, ctx->P, ctx->d_rz, ctx->d_rx, ctx->nrec,
+                                      R.data + (long long)L.b0 * R.nd, R.nd, R.t_hi - g.itlag, g.dx * g.dz, L.nb,
+                                      L.st));
+            L.cur = L.F.U0;
+            L.prv = L.F.U1;
+            base_step_args(ctx, L.b, L.nb);
+            L.b.flags |= F_RECINJ | (R.grad ? F_GRADM : F_IMAGE) | (R.d2t ? F_D2T : 0);
+            L.b.data = R.data + (long long)L.b0 * R.nd;
+            L.b.dstride = R.nd;
+            L.b.save_kind = R.skind;
+            L.b.save = L.save;
+            L.b.save_kstride = R.per_step;
+            L.b.save_sstride = R.per_step * R.nbuf;
+            L.b.image = L.acc_img;
+            L.b.acc_sstride = nint;
+            L.b.ssz = ctx->d_ssz + L.slot0;
+            L.b.ssx = ctx->d_ssx + L.slot0;
+            L.seg = R.nseg - 1;
+            L.phase = 2;
+            return PAM_OK;
+        }
+        case 2: {  // segment start: restore its checkpoint
+            if (L.seg < 0) {
+                L.phase = 5;
+                return PAM_OK;
+            }
+            L.lo = std::max(R.t_lo, L.seg * R.C);
+            L.hi = std::min(R.t_hi, (L.seg + 1) * R.C - 1);
+            if (L.lo > L.hi) {
+                --L.seg;
+                return PAM_OK;
+            }
+            L.k0 = std::max(0, L.seg * R.C - 1);
+            L.k1 = std::min(nt - 1, L.hi + 1);
+            int r = ckpt_lane_restore(ctx, L, L.seg);
+            if (r) return r;
+            L.f = L.a;
+            L.f.flags = (L.a.flags & (F_CERJAN | F_SRC)) | F_SAVE | (R.d2t ? F_D2T : 0);
+            L.f.save_k0 = L.k0;
+            L.fc = L.A.U0;
+            L.fp = L.A.U1;
+            L.k = L.k0;
+            L.phase = 3;
+            return PAM_OK;
+        }
+        case 3: {  // recompute step k of the segment, storing S_k or D_k
+            int r = fwd_step_fs(ctx, L.A, L.f, R.tile, L.fc, L.fp, L.k, nullptr, L.st);
+            if (r) return r;
+            std::swap(L.fc, L.fp);
+            if (++L.k <= L.k1) return PAM_OK;
+            L.b.save_k0 = L.k0;
+            L.t = L.hi;
+            L.phase = 4;
+            return PAM_OK;
+        }
+        case 4: {  // adjoint step t
+            int r = fwd_step_fs(ctx, L.F, L.b, R.tile, L.cur, L.prv, L.t, nullptr, L.st);
+            if (r) return r;
+            std::swap(L.cur, L.prv);
+            if (--L.t < L.lo) {
+                --L.seg;
+                L.phase = 2;
+            }
+            return PAM_OK;
+        }
+        case 5: {  // the batch's sums, after the previous batch's
+            if (L.batch > 0) {
+                if (!R.red_issued[L.batch - 1]) return PAM_OK;
+                HCK(hipStreamWaitEvent(L.st, R.red_ev[L.batch - 1], 0));
+            }
+            KCK(launch_reduce_shots(L.acc_img, nint, L.nb, (int)nint, R.image, 1, L.st));
+            KCK(launch_reduce_shots(L.acc_ilum, nint, L.nb, (int)nint, R.ilum, 1, L.st));
+            HCK(hipEventRecord(R.red_ev[L.batch], L.st));
+            R.red_issued[L.batch] = 1;
+            *done = true;
+            return PAM_OK;
+        }
+        default:
+            return fail(ctx, PAM_E_STATE, "checkpoint lane in an unknown phase");
+    }
+}
+
+// The batches of a uniform-checkpoint run (sizes `sizes`, in shot order) on two
+// lanes of at most Bl shots each, interleaved a time step at a time; the lanes
+// start after the work queued on the context's stream, which waits for both.
+static int ckpt_two_lanes(pam_ctx *ctx, CkptRun &R, const std::vector<int> &sizes, int Bl) {
+    int r = ensure_lanes(ctx);
+    if (r) return r;
+    const int nbat = (int)sizes.size();
+    R.red_ev.assign(nbat, nullptr);
+    R.red_issued.assign(nbat, 0);
+    for (auto &e : R.red_ev) HCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    CkptLane L[2];
+    for (int l = 0; l < 2; ++l) {
+        L[l].slot0 = l * Bl;
+        L[l].Bl = Bl;
+        L[l].F = fieldset_view(ctx, ctx->fs, l * Bl);
+        L[l].A = fieldset_view(ctx, ctx->aux, l * Bl);
+        L[l].save = reinterpret_cast<float *>(ctx->save) + (long long)l * Bl * R.per_step * R.nbuf;
+        L[l].acc_ilum = ctx->acc_ilum + (long long)l * Bl * R.nint;
+        L[l].acc_img = ctx->acc_img + (long long)l * Bl * R.nint;
+        L[l].ckpt = ctx->ckpt + (long long)l * R.nseg * ckpt_fields(ctx) * ctx->plane * Bl;
+        L[l].flip.assign(R.nseg, 0);
+        L[l].st = ctx->lane_st[l];
+    }
+    r = lanes_fork(ctx);
+    if (r) return r;
+    auto body = [&]() -> int {
+        int next = 0, b0 = 0;
+        auto start = [&](CkptLane &X) {
+            X.batch = next;
+            X.b0 = b0;
+            X.nb = sizes[next];
+            X.phase = 0;
+            b0 += sizes[next];
+            ++next;
+        };
+        bool active[2] = {false, false};
+        for (int l = 0; l < 2 && next < nbat; ++l) {
+            start(L[l]);
+            active[l] = true;
+        }
+        while (active[0] || active[1]) {
+            for (int l = 0; l < 2; ++l) {
+                if (!active[l]) continue;
+                bool done = false;
+                int rr = ckpt_tick(ctx, R, L[l], &done);
+                if (rr) return rr;
+                if (!done) continue;
+                if (next < nbat)
+                    start(L[l]);
+                else
+                    active[l] = false;
+            }
+        }
+        return PAM_OK;
+    };
+    r = lanes_join(ctx, body());
+    for (auto &e : R.red_ev)
+        if (e) (void)hipEventDestroy(e);
+    R.red_ev.clear();
+    return r;
+}
+
 // RTM (imaging) or FWI gradient (F_GRADM) over shots.  The source field is
 // kept by one of the strategies of enum pam_storage:
 //  - every step in float32 (src/Migration.py:996-1037);
@@ -1621,6 +1896,16 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         const int nbat = (nshots + B - 1) / B;
         B = (nshots + nbat - 1) / nbat;
     }
+    // uniform checkpoints with the pressure-form adjoint and no partials:
+    // batches of Bl shots alternate between two lanes, as in pam_forward (an
+    // explicit batch is the per-lane one); the buffers hold both lanes' shots
+    bool two = ckpt && !multi && !linear_adj && ctx->lanes == 2 && nshots > 1 && ctx->max_shots >= 2;
+    int Bl = 0;
+    if (two) {
+        Bl = ctx->batch > 0 ? ctx->batch : (B + 1) / 2;
+        Bl = std::max(1, std::min(Bl, std::min(nshots, ctx->max_shots / 2)));
+        B = 2 * Bl;
+    }
     const long long per_shot_bytes = per_step * nbuf * elem + 2 * nint * 4 + (linear_adj ? 5 * ctx->plane * 4 : 0) +
                                      (multi ? nmulti * nint * 4 : 0) + (half ? (long long)nt * 4 : 0) +
                                      (ckpt ? (long long)nseg * ckpt_fields(ctx) * ctx->plane * 4 : 0) +
@@ -1636,6 +1921,11 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     while (B > 1 && (long long)B * per_shot_bytes > budget) --B;
     if ((long long)B * per_shot_bytes > budget)
         return fail(ctx, PAM_E_NOMEM, "one shot's snapshots (%lld bytes) exceed free device memory", per_shot_bytes);
+    if (two) {
+        Bl = std::min(Bl, B / 2);
+        two = Bl >= 1;
+        if (two) B = 2 * Bl;
+    }
     r = ensure_scratch(ctx, &ctx->save, &ctx->save_bytes, (long long)B * per_step * nbuf * elem);
     if (r) return r;
     r = ensure_scratch(ctx, (void **)&ctx->acc_ilum, &ctx->acc_bytes, (long long)B * nint * 4 * 2);
@@ -1697,14 +1987,44 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     ctx->loop_ms = ctx->kernel_ms = 0.0;
     ctx->launches = 0;
     ctx->next_k = -1;
-    int tile0 = choose_tile(ctx, B, true);
+    int tile0 = choose_tile(ctx, two ? Bl : B, true);
     ctx->steps_run = 0;
     if ((half || sb || rv || rbc) && tile_shape(tile0).V > 0) tile0 = kDefaultTile;  // k_step has no fused max / D_t
     ctx->last_tile = tile0;
-    ctx->last_batch = B;
+    ctx->last_batch = two ? Bl : B;
     const long long pint = (long long)g.n_abc * ctx->P + g.n_abc;  // interior origin in a padded field
     HCK(hipEventRecord(ctx->e0, ctx->st));
-    for (int b0 = 0; b0 < nshots; b0 += B) {
+    if (two) {
+        CkptRun R;
+        R.shot0 = shot0;
+        R.data = data;
+        R.nd = nd;
+        R.nint = nint;
+        R.per_step = per_step;
+        R.grad = grad;
+        R.d2t = d2t;
+        R.tile = tile0;
+        R.C = C;
+        R.nseg = nseg;
+        R.t_lo = t_lo;
+        R.t_hi = t_hi;
+        R.nbuf = nbuf;
+        R.skind = skind;
+        R.image = image;
+        R.ilum = ilum_acc;
+        // an explicit batch: Bl, Bl, ..., the rest (the batches one lane would
+        // run); the automatic one: as even as Bl allows
+        std::vector<int> sizes;
+        if (ctx->batch > 0) {
+            for (int b0 = 0; b0 < nshots; b0 += Bl) sizes.push_back(std::min(Bl, nshots - b0));
+        } else {
+            const int nbat = (nshots + Bl - 1) / Bl;
+            for (int i = 0; i < nbat; ++i) sizes.push_back(nshots / nbat + (i < nshots % nbat ? 1 : 0));
+        }
+        r = ckpt_two_lanes(ctx, R, sizes, Bl);
+        if (r) return r;
+    }
+    for (int b0 = 0; !two && b0 < nshots; b0 += B) {
         const int nb = std::min(B, nshots - b0);
         // ---- forward pass: illumination, and snapshots or checkpoints
         r = zero_shot_state(ctx, nb, true, true);

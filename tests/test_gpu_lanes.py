@@ -1,9 +1,11 @@
-"""Two-lane forward runs (PAM_OPT_LANES 2): batches alternate between the
-context's stream and a second stream, each lane in its own shot slots, source
-slots, gather staging and cached graph.  The seismograms must be bit for bit
-those of the same launches made one batch at a time on one stream, with an odd
-number of batches (the lanes get unequal work), on graph capture and replay,
-with graphs off, and with the batch left to the autotuner."""
+"""Two-lane runs (PAM_OPT_LANES 2): batches alternate between two streams the
+context creates, each lane in its own shot slots, source slots, gather staging
+and cached graph.  Forward: the seismograms must be bit for bit those of the
+same launches made one batch at a time on one stream, with an odd number of
+batches (the lanes get unequal work), on graph capture and replay, with graphs
+off, and with the batch left to the autotuner.  RTM / FWI with uniform
+checkpoints: the image or gradient and the illumination must be bit for bit
+those of the same batches on one stream."""
 import numpy as np
 import pytest
 
@@ -71,3 +73,63 @@ def test_lanes_option_is_validated():
     # one shot, or max_shots 1: one lane whatever the option
     eng.set_option(_lib.PAM_OPT_LANES, 2)
     assert np.isfinite(_fwd(eng, 1)).all()
+
+
+def _ckpt_run(eng, data, grad):
+    if grad:
+        out = eng.gradient(data)
+        return out["g_m"].cpu().numpy(), out["ilum"].cpu().numpy()
+    image, ilum = eng.rtm(data)
+    return image.cpu().numpy(), ilum.cpu().numpy()
+
+
+@pytest.mark.parametrize("phys,abc", [(0, 0), (0, 1), (1, 1), (2, 1)])
+@pytest.mark.parametrize("grad", [True, False])
+def test_checkpoint_adjoint_on_two_lanes_equals_one_lane(phys, abc, grad):
+    """Uniform checkpoints (pressure-form adjoint): batches of 2 shots
+    alternating between two lanes (7 shots: batches 2, 2, 2, 1) give the
+    gradient or image and the illumination of the same batches run one after
+    another on one stream, bit for bit -- each batch's sums are added after the
+    previous batch's -- on two calls in a row."""
+    import torch
+
+    from tests.test_gpu_adjoint import _data
+
+    print(f"case checkpoint lanes phys={phys} abc={abc} grad={grad}", flush=True)
+    pb = make_problem(phys, abc, nshots=7, seed=580 + 2 * phys + abc, nt=181)
+    data = torch.from_numpy(_data(pb, 7, seed=58))
+    eng = make_engine(pb, max_shots=7)
+    eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_CHECKPOINT)
+    eng.set_option(_lib.PAM_OPT_CKPT_STEP, 20)
+    eng.set_option(_lib.PAM_OPT_TILE, 0)
+    eng.set_option(_lib.PAM_OPT_BATCH, 2)
+    eng.set_option(_lib.PAM_OPT_LANES, 1)
+    ref = _ckpt_run(eng, data, grad)
+    eng.set_option(_lib.PAM_OPT_LANES, 2)
+    for call in range(2):
+        got = _ckpt_run(eng, data, grad)
+        for name, g, r in zip(("result", "ilum"), got, ref):
+            assert np.array_equal(g, r), f"{name} differs (call {call})"
+    assert np.isfinite(ref[0]).all() and np.abs(ref[0]).max() > 0
+
+
+def test_checkpoint_adjoint_on_two_lanes_with_the_automatic_batch():
+    """PAM_OPT_BATCH 0: the two-lane checkpoint run spreads the shots evenly
+    over batches of at most half the one-lane batch; the result matches the
+    one-lane run to float32 summation order."""
+    import torch
+
+    from tests.test_gpu_adjoint import _data
+
+    pb = make_problem(1, 1, nshots=6, seed=590, nt=150)
+    data = torch.from_numpy(_data(pb, 6, seed=59))
+    eng = make_engine(pb, max_shots=6)
+    eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_CHECKPOINT)
+    eng.set_option(_lib.PAM_OPT_LANES, 2)
+    got = _ckpt_run(eng, data, True)
+    bl = eng.get_option(_lib.PAM_OPT_BATCH)
+    assert 1 <= bl <= 3
+    eng.set_option(_lib.PAM_OPT_LANES, 1)
+    ref = _ckpt_run(eng, data, True)
+    for g, r in zip(got, ref):
+        assert rel_l2(g, r) <= 1e-6
