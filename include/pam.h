@@ -84,7 +84,8 @@ enum pam_option {
                               options and output buffers stay the same) and replayed; 0: the
                               steps are launched one by one */
     PAM_OPT_STORAGE = 4,   /* enum pam_storage for rtm / gradient (default ALL_F32) */
-    PAM_OPT_CKPT_STEP = 5, /* checkpoint interval for PAM_STORE_CHECKPOINT */
+    PAM_OPT_CKPT_STEP = 5, /* checkpoint interval for PAM_STORE_CHECKPOINT (0: 2 ceil(sqrt(nt)) for
+                              pam_rtm / pam_gradient, ceil(sqrt(nt)) for pam_forward_adjoint) */
     PAM_OPT_RESET_ADJ_CPML = 6, /* 1 (default): clear CPML memory before the
                                    backward pass; 0: keep the forward pass's
                                    values as src/Migration.py does */

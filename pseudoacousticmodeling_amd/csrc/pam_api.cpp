@@ -1867,7 +1867,12 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                                              : (storage == PAM_STORE_ALL_BF16 ? SV_BF16_INT : SV_F32_INT));
     const int elem = (half || sb_kind) ? 2 : 4;
     const long long per_step = rbc ? 1 : sb ? sb_width(g.nz, g.nx) : (skind == SV_F32_PAD) ? ctx->plane : nint;  // per step
-    const int C = ckpt ? (ctx->ckpt_step > 0 ? ctx->ckpt_step : (int)std::ceil(std::sqrt((double)nt))) : nt;
+    // default interval 2 ceil(sqrt(nt)): every step is recomputed once whatever
+    // the interval, so a longer one only trades checkpoint copies for segment
+    // buffer (2% faster than ceil(sqrt(nt)) on a 451x1801 VTI grid, nt 5099)
+    const int C = ckpt ? (ctx->ckpt_step > 0 ? ctx->ckpt_step
+                                             : std::min(nt, 2 * (int)std::ceil(std::sqrt((double)nt))))
+                       : nt;
     // backward steps t in [t_lo, t_hi]: [1, nt-2] (Python reference) or [0, nt-1] (C++)
     if (grad && ctx->adj_range == 1)
         return fail(ctx, PAM_E_ARG, "the FWI gradient needs the adjoint range nt-2 .. 1 (PAM_OPT_ADJ_RANGE 0)");
