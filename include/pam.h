@@ -106,7 +106,12 @@ enum pam_option {
                                when that is 0, so that one lane's CPML pre-pass and launch
                                gaps overlap the other's step kernel; 1: one batch at a time
                                on the context's stream.  Runs with PAM_OPT_PROFILE use one
-                               lane.  Group runs use member 0's lanes for their sub-groups. */
+                               lane.  Group runs use member 0's lanes for their sub-groups.
+                               pam_rtm / pam_gradient with PAM_STORE_CHECKPOINT (without
+                               eps / delta / theta partials or the linearised VTI/TTI
+                               Cerjan adjoint) use them too: PAM_OPT_BATCH shots per lane,
+                               or half the one-lane batch; the results equal one lane's
+                               with the same batch bit for bit. */
     PAM_OPT_GROUP_BYTES = 12  /* pam_group_forward (read from member 0): wavefield bytes
                                  (two fields per shot) of one sub-group of consecutive
                                  members advanced in one launch per step; default 56 MiB,
