@@ -2023,7 +2023,9 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         if (ctx->batch > 0) {
             for (int b0 = 0; b0 < nshots; b0 += Bl) sizes.push_back(std::min(Bl, nshots - b0));
         } else {
-            const int nbat = (nshots + Bl - 1) / Bl;
+            // an even number of batches, so that neither lane ends alone
+            int nbat = (nshots + Bl - 1) / Bl;
+            if (nbat > 1 && (nbat & 1) && nbat < nshots) ++nbat;
             for (int i = 0; i < nbat; ++i) sizes.push_back(nshots / nbat + (i < nshots % nbat ? 1 : 0));
         }
         r = ckpt_two_lanes(ctx, R, sizes, Bl);
