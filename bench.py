@@ -19,8 +19,11 @@ Prints one JSON line:
   plain_loop the same 64 shots run one shot per launch (the per-shot loop)
   fwi_gradient seconds per shot of the FWI gradient of the same survey, all
              64 shots batched vs the per-shot loop
-  uneven_group the 16-model config-5 sample advanced as one group
-  other_configs configs 1, 3 and 4 (the last two on a reduced step count)
+  uneven_group sixteen VTI + CPML models of different sizes (201..601 x
+             801..2401), 4 shots each, advanced as one group
+  other_configs acoustic + Cerjan (1 shot, 301 x 501 padded), TTI + CPML (32
+             shots, 501 x 2101) and acoustic + CPML (32 shots, 401 x 1301),
+             the last two on a reduced step count
   cpu_baseline the CPU restatement of the reference's Numba kernels
              (float32 storage, float64 arithmetic, OpenMP over rows) on a
              bounded sample of the same workload
@@ -73,10 +76,9 @@ def make_engine(w, max_shots):
 
 def configs_leg(nt_cap):
     """The other single-grid workloads, forward modelling with graph replay
-    and the autotuner's choices: acoustic + Cerjan, one shot, every step
-    (config 1); TTI + CPML, 32 shots, and acoustic + CPML, 32 shots (configs 3
-    and 4) on the first `nt_cap` steps, reported per step as the full runs'
-    rate (labelled reduced)."""
+    and the autotuner's choices: acoustic + Cerjan, one shot, every step;
+    TTI + CPML, 32 shots, and acoustic + CPML, 32 shots, on the first `nt_cap`
+    steps, reported per step as the full runs' rate (labelled reduced)."""
     import torch
 
     from pseudoacousticmodeling_amd import _lib, workloads
@@ -235,7 +237,7 @@ def main(argv=None):
     ap.add_argument("--no-group", action="store_true", help="skip the uneven-grid group leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the other single-grid workloads")
     ap.add_argument("--configs-steps", type=int, default=500,
-                    help="recorded samples of the reduced config 3 / 4 runs")
+                    help="recorded samples of the reduced TTI + CPML and acoustic + CPML runs")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the steps one by one instead of replaying captured graphs")
     ap.add_argument("--group-shots", type=int, default=4, help="shots per model in the uneven-grid group leg")

@@ -52,13 +52,13 @@ def test_context_free_calls_report_through_the_null_context():
 
 
 def test_derived_sizes_round_and_trunc():
-    # config 4 (fcut = 24, dt = 1 ms): itlag differs between the two drivers
+    # the 32-shot acoustic + CPML survey (fcut = 24, dt = 1 ms): itlag differs between the two drivers
     r = prep.derived_sizes(12000.0, 3000.0, 3.0, 10.0, 10.0, 1e-3, 24.0, "round")
     t = prep.derived_sizes(12000.0, 3000.0, 3.0, 10.0, 10.0, 1e-3, 24.0, "trunc")
     assert (r["itlag"], t["itlag"]) == (148, 147)
     assert r["nx"] == t["nx"] == 1201 and r["nz"] == t["nz"] == 301
     assert r["nt_data"] == t["nt_data"] == 3001
-    # config 1: fcut = 30 -> tlag = 0.11816 s
+    # the one-shot two-layer acoustic survey: fcut = 30 -> tlag = 0.11816 s
     c1 = prep.derived_sizes(4000.0, 2000.0, 2.0, 10.0, 10.0, 1e-3, 30.0, "round")
     assert (c1["itlag"], c1["nt_data"], c1["nx"], c1["nz"], c1["nt"]) == (118, 2001, 401, 201, 2119)
     assert prep.derived_sizes(4000.0, 2000.0, 2.0, 10.0, 10.0, 1e-3, 30.0, "round") == \
