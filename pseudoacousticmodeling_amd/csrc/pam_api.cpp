@@ -62,6 +62,9 @@ struct pam_ctx {
     int *d_rz = nullptr, *d_rx = nullptr, *d_rrow = nullptr, *d_rhead = nullptr, *d_rnext = nullptr;
     int *d_ssz = nullptr, *d_ssx = nullptr;  // [max_shots]
     bool geom_set = false;
+    char errbuf[256] = {0};
+    // set by k_nonfinite when a checked field holds NaN or Inf
+    int *d_bad = nullptr;
     // wavefields and CPML memory: the main set and the recompute set
     FieldSet fs, aux;
     long long psi_half = 0;  // floats between the two halves of psi
@@ -193,6 +196,20 @@ extern "C" int pam_create(const pam_grid *grid, int physics, int abc, const pam_
     if (!(g.dx > 0.f) || !(g.dz > 0.f) || !(g.dt > 0.f)) return bad("dx, dz and dt must be positive");
     if (g.nt < 1 || g.itlag < 0 || g.itlag > g.nt) return bad("need nt >= 1 and 0 <= itlag <= nt");
     if (max_shots < 1) return bad("max_shots must be >= 1");
+    {
+        // the kernels address one shot's field (and one snapshot of it) with a
+        // 32-bit byte offset from a per-shot base: a padded plane must stay
+        // below 4 GiB
+        const long long nzp = (long long)g.nz + 2LL * g.n_abc, nxp = (long long)g.nx + 2LL * g.n_abc;
+        const long long plane_bytes = nzp * ((nxp + 63) / 64 * 64) * 4;
+        if (plane_bytes >= (1LL << 32)) {
+            std::snprintf(ctx->errbuf, sizeof ctx->errbuf,
+                          "padded grid %lldx%lld: one field of %lld bytes reaches 4 GiB (the kernels' 32-bit "
+                          "per-shot offsets); split the model",
+                          nzp, nxp, plane_bytes);
+            return bad(ctx->errbuf);
+        }
+    }
     ctx->g = g;
     ctx->phys = physics;
     ctx->abc = abc;
@@ -244,7 +261,9 @@ extern "C" int pam_create(const pam_grid *grid, int physics, int abc, const pam_
     DALLOC(ctx->d_ssz, (size_t)max_shots * sizeof(int));
     DALLOC(ctx->d_ssx, (size_t)max_shots * sizeof(int));
     DALLOC(ctx->maxv, 64);
-    DALLOC(ctx->d_acc, 64);
+    // misfit: one float64 partial per block of the first reduction pass, then the sum
+    DALLOC(ctx->d_acc, (size_t)(kMisfitParts + 1) * sizeof(double));
+    DALLOC(ctx->d_bad, 64);
     HCK(hipMemsetAsync(ctx->fs.U0, 0, fb, ctx->st));
     HCK(hipMemsetAsync(ctx->fs.U1, 0, fb, ctx->st));
     // Cerjan factors: 1 in the interior, A[i] in the strips (cpp/Modeling.cu:111-120)
@@ -293,6 +312,7 @@ extern "C" int pam_destroy(pam_ctx *ctx) {
     for (float *b : bufs) dfree(b);
     dfree(ctx->save);
     dfree(ctx->d_acc);
+    dfree(ctx->d_bad);
     dfree(ctx->smax);
     dfree(ctx->rvbits);
     int *ibufs[] = {ctx->d_rz, ctx->d_rx, ctx->d_rrow, ctx->d_rhead, ctx->d_rnext, ctx->d_ssz, ctx->d_ssx};
@@ -495,6 +515,30 @@ extern "C" int pam_set_geometry(pam_ctx *ctx, const int *sz, const int *sx, int 
 }
 
 // ------------------------------------------------------------------ helpers
+// Blow-up detection (the reference only warns at set-up, cpp/Modeling.cu:321-326):
+// a run clears the flag, checks the fields it ends with (NaN / Inf persist once
+// an unstable step has produced them), and reports after its last
+// synchronisation.
+static int bad_clear(pam_ctx *ctx) {
+    HCK(hipMemsetAsync(ctx->d_bad, 0, sizeof(int), ctx->st));
+    return PAM_OK;
+}
+static int bad_check(pam_ctx *ctx, const float *x, long long n, hipStream_t st = nullptr) {
+    KCK(launch_nonfinite(x, n, ctx->d_bad, st ? st : ctx->st));
+    return PAM_OK;
+}
+static int bad_report(pam_ctx *ctx, const char *what) {
+    int h = 0;
+    HCK(hipMemcpyAsync(&h, ctx->d_bad, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    HCK(hipStreamSynchronize(ctx->st));
+    if (h)
+        return fail(ctx, PAM_E_NONFINITE,
+                    "%s: the wavefield holds NaN or Inf (an unstable run: dt above the stability limit of the "
+                    "model, or a non-finite input)",
+                    what);
+    return PAM_OK;
+}
+
 static int ready(pam_ctx *ctx) {
     if (!ctx->model_set) return fail(ctx, PAM_E_STATE, "no model set (pam_set_model)");
     if (!ctx->wav_set) return fail(ctx, PAM_E_STATE, "no wavelet set (pam_set_wavelet)");
@@ -813,21 +857,24 @@ static int finish_timing(pam_ctx *ctx, long long nk) {
 // capturable (kernels and device copies: no memset nodes, no events) and
 // `key` must name everything they bake in that can change between calls;
 // `replay_fix` applies the host-side effects of `body` when it is skipped.
+// The graphs go to `cache` (the context's own, or a group's: a group's graphs
+// bake in its device tables and are destroyed with it).
 template <class Body, class Fix>
-static int run_graph(pam_ctx *ctx, const std::string &key, Body &&body, Fix &&replay_fix, hipStream_t st = nullptr) {
+static int run_graph_in(pam_ctx *ctx, std::map<std::string, hipGraphExec_t> &cache, const std::string &key,
+                        Body &&body, Fix &&replay_fix, hipStream_t st = nullptr) {
     if (!st) st = ctx->st;
-    auto it = ctx->graphs.find(key);
-    if (it != ctx->graphs.end()) {
+    auto it = cache.find(key);
+    if (it != cache.end()) {
         replay_fix();
         HCK(hipGraphLaunch(it->second, st));
         return PAM_OK;
     }
-    if (ctx->graphs.size() >= 16) {  // bound the cache: drop every entry once none can be running
+    if (cache.size() >= 16) {  // bound the cache: drop every entry once none can be running
         HCK(hipStreamSynchronize(ctx->st));
         for (hipStream_t l : ctx->lane_st)
             if (l) HCK(hipStreamSynchronize(l));
-        for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
-        ctx->graphs.clear();
+        for (auto &kv : cache) (void)hipGraphExecDestroy(kv.second);
+        cache.clear();
     }
     HCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     const int r = body();
@@ -842,9 +889,13 @@ static int run_graph(pam_ctx *ctx, const std::string &key, Body &&body, Fix &&re
     const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ei != hipSuccess) return fail(ctx, PAM_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
-    ctx->graphs[key] = ex;
+    cache[key] = ex;
     HCK(hipGraphLaunch(ex, st));
     return PAM_OK;
+}
+template <class Body, class Fix>
+static int run_graph(pam_ctx *ctx, const std::string &key, Body &&body, Fix &&replay_fix, hipStream_t st = nullptr) {
+    return run_graph_in(ctx, ctx->graphs, key, body, replay_fix, st);
 }
 
 // forward loop over nt steps for shots already primed in U0 (cur) / U1 (prv);
@@ -967,8 +1018,6 @@ static int forward_two_lanes(pam_ctx *ctx, int shot0, int nshots, float *seis, i
     FieldSet *fsl[2] = {&ctx->fs, &lane1};
     hipStream_t stl[2] = {ctx->lane_st[0], ctx->lane_st[1]};
     HCK(hipEventRecord(ctx->e0, ctx->st));
-    rl = lanes_fork(ctx);
-    if (rl) return rl;
     int nbatch = 0;
     auto batches = [&]() -> int {
         for (int b0 = 0; b0 < nshots; b0 += Bl, ++nbatch) {
@@ -993,12 +1042,20 @@ static int forward_two_lanes(pam_ctx *ctx, int shot0, int nshots, float *seis, i
             a.ssx = ctx->d_ssx + lane * Bl;
             r = run_forward_loop_fs(ctx, f, st, a, tile, false);
             if (r) return r;
+            r = bad_check(ctx, f.U0, (long long)nb * ctx->plane, st);
+            if (r) return r;
+            r = bad_check(ctx, f.U1, (long long)nb * ctx->plane, st);
+            if (r) return r;
             if (staged)
                 HCK(hipMemcpyAsync(seis + (long long)b0 * nd, dst, (size_t)nb * nd * sizeof(float),
                                    hipMemcpyDeviceToDevice, st));
         }
         return PAM_OK;
     };
+    rl = bad_clear(ctx);
+    if (rl) return rl;
+    rl = lanes_fork(ctx);
+    if (rl) return rl;
     rl = lanes_join(ctx, batches());  // the context's stream then orders after all lane work
     if (rl) return rl;
     HCK(hipEventRecord(ctx->e1, ctx->st));
@@ -1008,8 +1065,7 @@ static int forward_two_lanes(pam_ctx *ctx, int shot0, int nshots, float *seis, i
     ctx->loop_ms += ms;
     ctx->kernel_ms += ms;
     ctx->launches += (long long)nbatch * ctx->g.nt;
-    HCK(hipStreamSynchronize(ctx->st));
-    return PAM_OK;
+    return bad_report(ctx, "pam_forward");
 }
 
 extern "C" int pam_forward(pam_ctx *ctx, int shot0, int nshots, float *seis) {
@@ -1040,6 +1096,8 @@ extern "C" int pam_forward(pam_ctx *ctx, int shot0, int nshots, float *seis) {
         r = ensure_scratch(ctx, (void **)&ctx->stage, &ctx->stage_bytes, (long long)B * nd * 4);
         if (r) return r;
     }
+    r = bad_clear(ctx);
+    if (r) return r;
     for (int b0 = 0; b0 < nshots; b0 += B) {
         const int nb = std::min(B, nshots - b0);
         r = zero_shot_state(ctx, nb, true, true);
@@ -1056,12 +1114,15 @@ extern "C" int pam_forward(pam_ctx *ctx, int shot0, int nshots, float *seis) {
         a.dstride = nd;
         r = run_forward_loop(ctx, a, tile, true);
         if (r) return r;
+        r = bad_check(ctx, ctx->fs.U0, (long long)nb * ctx->plane);
+        if (r) return r;
+        r = bad_check(ctx, ctx->fs.U1, (long long)nb * ctx->plane);
+        if (r) return r;
         if (staged)
             HCK(hipMemcpyAsync(seis + (long long)b0 * nd, ctx->stage, (size_t)nb * nd * sizeof(float),
                                hipMemcpyDeviceToDevice, ctx->st));
     }
-    HCK(hipStreamSynchronize(ctx->st));
-    return PAM_OK;
+    return bad_report(ctx, "pam_forward");
 }
 
 // ------------------------------------------------------------ preprocessing
@@ -1217,6 +1278,10 @@ struct pam_group {
     int *d_boff = nullptr, *d_pboff = nullptr;  // step-kernel and psi-pass block prefix tables (2n: per sub-group)
     std::map<std::string, int> tuned;  // step-kernel variant per shot layout
     int last_tile = -1;
+    // the group's step-loop graphs: they read d_list / d_boff / d_pboff, so they
+    // live and die with the group (a new group over the same members must not
+    // replay a graph that reads a destroyed group's tables)
+    std::map<std::string, hipGraphExec_t> graphs;
 };
 
 extern "C" int pam_group_create(pam_ctx *const *ctxs, int n, pam_group **out) {
@@ -1235,8 +1300,9 @@ extern "C" int pam_group_create(pam_ctx *const *ctxs, int n, pam_group **out) {
     if (hipMalloc((void **)&g->d_list, sizeof(StepArgs) * n) != hipSuccess ||
         hipMalloc((void **)&g->d_boff, sizeof(int) * 2 * n) != hipSuccess ||
         hipMalloc((void **)&g->d_pboff, sizeof(int) * 2 * n) != hipSuccess) {
-        if (g->d_list) (void)hipFree(g->d_list);
-        if (g->d_boff) (void)hipFree(g->d_boff);
+        dfree(g->d_list);
+        dfree(g->d_boff);
+        dfree(g->d_pboff);
         delete g;
         return fail(c0, PAM_E_NOMEM, "group argument tables");
     }
@@ -1246,7 +1312,14 @@ extern "C" int pam_group_create(pam_ctx *const *ctxs, int n, pam_group **out) {
 
 extern "C" int pam_group_destroy(pam_group *g) {
     if (!g) return PAM_OK;
-    if (!g->ctx.empty() && g->ctx[0]->st) (void)hipStreamSynchronize(g->ctx[0]->st);
+    if (!g->ctx.empty()) {
+        pam_ctx *c0 = g->ctx[0];
+        if (c0->st) (void)hipStreamSynchronize(c0->st);
+        for (hipStream_t l : c0->lane_st)
+            if (l) (void)hipStreamSynchronize(l);
+    }
+    for (auto &kv : g->graphs) (void)hipGraphExecDestroy(kv.second);
+    g->graphs.clear();
     dfree(g->d_list);
     dfree(g->d_boff);
     dfree(g->d_pboff);
@@ -1430,7 +1503,7 @@ extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *
                                   (void *)seis[i]);
                     key += buf;
                 }
-                rr = run_graph(c0, key, body, []() {}, st);
+                rr = run_graph_in(c0, g->graphs, key, body, []() {}, st);
             } else {
                 rr = body();
             }
@@ -2456,6 +2529,15 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                     KCK(launch_reduce_shots(ctx->acc_multi + (long long)m * B * nint, nint, nb, (int)nint,
                                             out_multi[m], 1, ctx->st));
     }
+    // the image / gradient and illumination sum the source and adjoint fields
+    // of every step: a blow-up of either leaves NaN or Inf in them (checked
+    // before the normalisation, which a zero illumination would make 0 / 0)
+    r = bad_clear(ctx);
+    if (r) return r;
+    r = bad_check(ctx, image, nint);
+    if (r) return r;
+    r = bad_check(ctx, ilum_acc, nint);
+    if (r) return r;
     if (ctx->normalize) {
         KCK(launch_maxabs(ilum_acc, (int)nint, ctx->maxv, ctx->st));
         KCK(launch_normalize(image, ilum_acc, (int)nint, ctx->normalize, ctx->maxv, ctx->st));
@@ -2471,8 +2553,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     ctx->loop_ms = ms;
     ctx->kernel_ms = ms;
     ctx->launches = ctx->steps_run;
-    HCK(hipStreamSynchronize(ctx->st));
-    return PAM_OK;
+    return bad_report(ctx, grad ? "pam_gradient" : "pam_rtm");
 }
 
 extern "C" int pam_rtm(pam_ctx *ctx, int shot0, int nshots, const float *data, float *image, float *ilum) {
@@ -2493,10 +2574,13 @@ extern "C" int pam_misfit(pam_ctx *ctx, int nshots, const float *dobs, const flo
                           double *misfit) {
     if (!ctx) return PAM_E_ARG;
     if (!dobs || !dcal || !misfit) return fail(ctx, PAM_E_ARG, "NULL argument");
+    if (nshots < 1) return fail(ctx, PAM_E_ARG, "nshots must be >= 1");
+    if (!ctx->geom_set) return fail(ctx, PAM_E_STATE, "no geometry set (pam_set_geometry)");
     const long long n = (long long)nshots * (ctx->g.nt - ctx->g.itlag) * ctx->nrec;
-    HCK(hipMemsetAsync(ctx->d_acc, 0, sizeof(double), ctx->st));
-    KCK(launch_residual(dobs, dcal, residual, n, ctx->d_acc, ctx->st));
-    HCK(hipMemcpyAsync(misfit, ctx->d_acc, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    if (n < 1) return fail(ctx, PAM_E_ARG, "no samples (nt_data * nrec is 0)");
+    // two passes in a fixed order: the same inputs give the same bits
+    KCK(launch_residual(dobs, dcal, residual, n, ctx->d_acc, ctx->d_acc + kMisfitParts, ctx->st));
+    HCK(hipMemcpyAsync(misfit, ctx->d_acc + kMisfitParts, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
     HCK(hipStreamSynchronize(ctx->st));
     return PAM_OK;
 }
@@ -2699,10 +2783,14 @@ extern "C" int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const fl
         KCK(launch_pack_padded(ctx->ex_gacc, ctx->nzp, ctx->nxp, ctx->P, g_vp, ctx->st));
     else
         KCK(launch_expand_T(ctx->ex_gacc, ctx->nzp, ctx->nxp, ctx->P, g.n_abc, g_vp, ctx->st));
+    r = bad_clear(ctx);
+    if (r) return r;
+    r = bad_check(ctx, g_vp, layout == PAM_PADDED ? (long long)ctx->nzp * ctx->nxp : (long long)g.nz * g.nx);
+    if (r) return r;
     HCK(hipEventRecord(ctx->e1, ctx->st));
     HCK(hipEventSynchronize(ctx->e1));
     float ms = 0.f;
     HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
     ctx->loop_ms = ms;
-    return PAM_OK;
+    return bad_report(ctx, "pam_forward_adjoint");
 }

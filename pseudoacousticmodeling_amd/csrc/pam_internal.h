@@ -179,7 +179,16 @@ int launch_reduce_shots(const float *parts, long long sstride, int nshots, int n
                         hipStream_t st);
 int launch_normalize(float *image, const float *ilum, int n, int mode, const float *maxv, hipStream_t st);
 int launch_maxabs(const float *x, int n, float *out, hipStream_t st);
-int launch_residual(const float *dobs, const float *dcal, float *res, long long n, double *acc, hipStream_t st);
+// Misfit in two deterministic passes: residual = dobs - dcal and per-block
+// float64 partial sums of residual^2 / 2 into part[0 .. misfit_parts(n)), then
+// one block sums the partials in a fixed order into *out.  The same n gives the
+// same bits on every call; part needs kMisfitParts doubles.
+constexpr int kMisfitParts = 1024;
+int misfit_parts(long long n);
+int launch_residual(const float *dobs, const float *dcal, float *res, long long n, double *part, double *out,
+                    hipStream_t st);
+// *flag = 1 when any of the n floats of x is NaN or Inf (flag is not cleared)
+int launch_nonfinite(const float *x, long long n, int *flag, hipStream_t st);
 // multiparameter partials (src/CudaKernels.py:1174-1379) of nshots shots:
 // S (saved source field at step t) and lam (adjoint field) are given at their
 // interior origin with row pitches sP / lP and shot strides ss / ls; the

@@ -408,22 +408,41 @@ __global__ void k_normalize(float *image, const float *ilum, long long n, int mo
         image[i] = (ilum[i] > 1e-12f) ? image[i] / ilum[i] : 0.0f;
     }
 }
-// residual = dobs - dcal and a float64 sum of residual^2 / 2
-__global__ void k_residual(const float *dobs, const float *dcal, float *res, long long n, double *acc) {
-    __shared__ double red[256];
+// block-wide float64 sum of one value per thread (256 threads), fixed order:
+// a wavefront butterfly, then the four wavefront sums in order
+__device__ __forceinline__ double block_sum256(double v) {
+    __shared__ double wsum[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+}
+// residual = dobs - dcal and, per block, the float64 sum of residual^2 / 2
+// over a grid-stride share into part[blockIdx.x]; every partial is written
+__global__ void k_residual(const float *dobs, const float *dcal, float *res, long long n, double *part) {
     double sum = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         const float r = dobs[i] - dcal[i];
         if (res) res[i] = r;
         sum += 0.5 * (double)r * (double)r;
     }
-    red[threadIdx.x] = sum;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(acc, red[0]);
+    const double b = block_sum256(sum);
+    if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+// *out = the sum of the nparts partials (one block, fixed order)
+__global__ void k_sum_parts(const double *part, int nparts, double *out) {
+    double sum = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) sum += part[i];
+    const double b = block_sum256(sum);
+    if (threadIdx.x == 0) *out = b;
+}
+// *flag = 1 if any value is NaN or Inf
+__global__ void k_nonfinite(const float *x, long long n, int *flag) {
+    bool bad = false;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        bad |= !isfinite(x[i]);
+    if (__any(bad) && (threadIdx.x & 63) == 0) *flag = 1;
 }
 // multiparameter partials on interior arrays, float64 arithmetic
 // (src/CudaKernels.py:1174-1379)
@@ -640,11 +659,24 @@ int launch_normalize(float *image, const float *ilum, int n, int mode, const flo
                        reinterpret_cast<const unsigned int *>(maxv));
     return (int)hipGetLastError();
 }
-int launch_residual(const float *dobs, const float *dcal, float *res, long long n, double *acc, hipStream_t st) {
+int misfit_parts(long long n) {
+    long long blocks = (n + 255) / 256;
+    if (blocks > kMisfitParts) blocks = kMisfitParts;
+    if (blocks < 1) blocks = 1;
+    return (int)blocks;
+}
+int launch_residual(const float *dobs, const float *dcal, float *res, long long n, double *part, double *out,
+                    hipStream_t st) {
+    const int blocks = misfit_parts(n);
+    hipLaunchKernelGGL(k_residual, dim3((unsigned)blocks), dim3(256), 0, st, dobs, dcal, res, n, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, (const double *)part, blocks, out);
+    return (int)hipGetLastError();
+}
+int launch_nonfinite(const float *x, long long n, int *flag, hipStream_t st) {
     long long blocks = (n + 255) / 256;
     if (blocks > 2048) blocks = 2048;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_residual, dim3((unsigned)blocks), dim3(256), 0, st, dobs, dcal, res, n, acc);
+    hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)blocks), dim3(256), 0, st, x, n, flag);
     return (int)hipGetLastError();
 }
 int launch_grad_multi(int phys, const float *S, int sP, long long ss, const float *lam, int lP, long long ls,

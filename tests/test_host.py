@@ -120,3 +120,22 @@ def test_stability_check():
 def test_oracle_builds_load(kind):
     o = orc.Oracle(kind)
     assert hasattr(o.lib, "or_forward_shot")
+
+
+def test_create_rejects_a_plane_of_4_gib():
+    """The kernels address one shot's field with a 32-bit byte offset from a
+    per-shot base: pam_create refuses a padded plane of 4 GiB or more before
+    any device call, with a message that says so (no GPU needed)."""
+    lib = _lib.load()
+    ctx = ctypes.c_void_p()
+    # 32768 x 32768 padded cells x 4 B = 4 GiB exactly
+    g = _lib.PamGrid(32768 - 100, 32768 - 100, 50, 10.0, 10.0, 1e-3, 100, 0)
+    p = _lib.PamAbcParams(0.0, 0.0, 0.0)
+    rc = lib.pam_create(ctypes.byref(g), 0, 0, ctypes.byref(p), None, 1, ctypes.byref(ctx))
+    assert rc == -1
+    assert b"4 GiB" in lib.pam_last_error(ctx)
+    assert lib.pam_destroy(ctx) == 0
+    # one row of pitch less is accepted by the size check (the call then goes on
+    # to allocate, which this test does not reach): the bound is exact
+    nzp, P = 32768, 32768
+    assert nzp * P * 4 == 1 << 32 and (nzp - 1) * P * 4 < 1 << 32
