@@ -210,6 +210,24 @@ int pam_gradient(pam_ctx *ctx, int shot0, int nshots, const float *residual, flo
 int pam_misfit(pam_ctx *ctx, int nshots, const float *dobs, const float *dcal, float *residual,
                double *misfit);
 
+/* pam_misfit without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It takes no
+ * context: dobs / dcal / residual (may be NULL) are device
+ * [nshots][nt_data][nrec], and the misfit sum_shots 1/2 sum (dobs - dcal)^2 goes
+ * to the DEVICE double `misfit`; the work is only enqueued on hip_stream (NULL:
+ * the null stream), on the calling thread's current device, and the call
+ * returns without synchronising.  The result equals pam_misfit's bit for bit
+ * (the same two fixed-order passes).  The per-block partial sums live in a
+ * fixed 8 KiB array per GPU that the library holds until the process ends:
+ * every call on one GPU shares it, so calls that overlap on one GPU (two
+ * streams, two threads) race; order them.  At most
+ * PAM_MISFIT_NOALLOC_MAX_SAMPLES samples per call (PAM_E_ARG beyond);
+ * failures report through pam_last_error(NULL). */
+#define PAM_MISFIT_NOALLOC_MAX_SAMPLES (1LL << 40)
+int pam_misfit_noalloc(int nshots, int nt_data, int nrec, const float *dobs, const float *dcal, float *residual,
+                       double *misfit, void *hip_stream);
+
 /* Exact discrete adjoint of pam_forward (acoustic physics, Cerjan or CPML):
  * given dseis = dJ/dseis [nshots][nt_data][nrec] (device) for the seismograms
  * pam_forward produces for shots shot0 .. shot0+nshots-1, writes dJ/dvp into

@@ -18,6 +18,9 @@
 
 using namespace pam;
 
+// the last failure of a context-free call on this thread (pam_last_error(NULL))
+extern thread_local std::string g_free_err;
+
 // The fields one set of shots is advanced with: the two wavefield buffers and
 // the CPML memory variables ([max_shots][plane] each; psi twice, double-
 // buffered across steps, psi_flip selecting the half that holds the previous
@@ -226,6 +229,13 @@ extern "C" int pam_create(const pam_grid *grid, int physics, int abc, const pam_
         ctx->own_stream = true;
     }
     *out = ctx;
+    // the allocation-free entry points' scratch address on this device, looked
+    // up here once so that their first call does no lookup (a failure is
+    // reported by those calls themselves)
+    {
+        double *sp = nullptr;
+        if (pam_noalloc_scratch(&sp) != PAM_OK) g_free_err.clear();
+    }
     // wavefields and per-shot CPML memory
     const size_t fb = (size_t)ctx->plane * max_shots * sizeof(float);
     DALLOC(ctx->fs.U0, fb);
@@ -332,7 +342,7 @@ extern "C" int pam_destroy(pam_ctx *ctx) {
 }
 
 // the last failure of a context-free call on this thread
-static thread_local std::string g_free_err;
+thread_local std::string g_free_err;
 
 extern "C" const char *pam_last_error(const pam_ctx *ctx) { return ctx ? ctx->err.c_str() : g_free_err.c_str(); }
 

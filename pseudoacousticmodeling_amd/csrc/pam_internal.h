@@ -187,6 +187,9 @@ constexpr int kMisfitParts = 1024;
 int misfit_parts(long long n);
 int launch_residual(const float *dobs, const float *dcal, float *res, long long n, double *part, double *out,
                     hipStream_t st);
+// the device address of the allocation-free entry points' scratch array on the
+// calling thread's current device (looked up once per device)
+int pam_noalloc_scratch(double **out);
 // *flag = 1 when any of the n floats of x is NaN or Inf (flag is not cleared)
 int launch_nonfinite(const float *x, long long n, int *flag, hipStream_t st);
 // multiparameter partials (src/CudaKernels.py:1174-1379) of nshots shots:
