@@ -47,7 +47,9 @@ def test_rtm_matches_oracle(phys, abc):
     ilum = ilum.cpu().numpy()
     e_img = rel_l2(image, ref["image"])
     e_ilum = rel_l2(ilum, ref["ilum"])
-    print(f"  image rel_l2 {e_img:.3e}, ilum rel_l2 {e_ilum:.3e}", flush=True)
+    ref64 = _oracle_adjoint(pb, data, fwi=False, kind="f64")
+    print(f"  image rel_l2 {e_img:.3e}, ilum rel_l2 {e_ilum:.3e}; float64 oracle: image "
+          f"{rel_l2(image, ref64['image']):.3e}, ilum {rel_l2(ilum, ref64['ilum']):.3e}", flush=True)
     assert np.isfinite(image).all() and np.abs(image).max() > 0
     assert e_ilum <= TOL
     assert e_img <= TOL
@@ -63,7 +65,8 @@ def test_gradient_matches_oracle(phys, abc):
     out = eng.gradient(data)
     g = out["g_m"].cpu().numpy()
     err = rel_l2(g, ref["image"])
-    print(f"  gradient rel_l2 {err:.3e}", flush=True)
+    e64 = rel_l2(g, _oracle_adjoint(pb, data, fwi=True, kind="f64")["image"])
+    print(f"  gradient rel_l2 {err:.3e}; float64 oracle {e64:.3e}", flush=True)
     assert np.isfinite(g).all() and np.abs(g).max() > 0
     assert err <= TOL
 
@@ -93,9 +96,10 @@ def test_multiparameter_partials_match_oracle(phys, abc):
     out = eng.gradient(data, multiparameter=True)
     names = [("g_eps", "g_eps"), ("g_delta", "g_delta")] + ([("g_theta", "g_theta")] if phys == 2 else [])
     assert rel_l2(out["g_m"].cpu().numpy(), ref["image"]) <= TOL
+    ref64 = _oracle_adjoint(pb, data, fwi=True, multiparameter=True, kind="f64")
     for gk, rk in names:
         g = out[gk].cpu().numpy()
         err = rel_l2(g, ref[rk])
-        print(f"  {gk} rel_l2 {err:.3e}", flush=True)
+        print(f"  {gk} rel_l2 {err:.3e}; float64 oracle {rel_l2(g, ref64[rk]):.3e}", flush=True)
         assert np.isfinite(g).all() and np.abs(g).max() > 0
         assert err <= TOL

@@ -1,0 +1,242 @@
+"""The benchmark workloads themselves through the C ABI, at their full grid
+sizes (pseudoacousticmodeling_amd/workloads.py):
+
+- acoustic + Cerjan (factor 0.0045), 201x401 (301x501 padded), all 2119 steps,
+  400 receivers: against the float32 and float64 oracles;
+- VTI + CPML, 351x1701 (451x1801 padded), 1701 receivers: 2 shots of 300
+  steps against the oracles; all 64 shots (shortened record) on two lanes
+  equal one shot at a time bit for bit; the FWI gradient on the bench's path
+  (uniform checkpoints, two lanes) within 1e-4 of the float64 oracle -- the
+  accuracy target of the batched gradient;
+- TTI + CPML, 401x2001 (501x2101 padded): forward against the oracles, and RTM
+  with boundary saving, float32 and bfloat16 rims, against the oracle's
+  restatement of the same strategy;
+- acoustic + CPML, 301x1201 (401x1301 padded): the gradient against the
+  oracles;
+- sixteen uneven VTI + CPML grids, 201..601 x 801..2401: the smallest and the
+  largest against the oracle, and all sixteen with 8 shots each (shortened
+  record) as one group equal to each model alone bit for bit.
+
+Records are shortened (itlag + a few hundred steps) where the CPU oracle
+would otherwise take minutes; the full records run in bench.py.  Every test
+prints its error against the float64 oracle."""
+import numpy as np
+import pytest
+
+from oracle.oracle import Grid, Oracle, rel_l2, rtm_onthefly
+from pseudoacousticmodeling_amd import _lib
+from pseudoacousticmodeling_amd import workloads as W
+from tests.problems import make_engine, oracle_forward
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def _pb(w, nshots=None):
+    """A workload as the tests' problem dict (the first `nshots` shots)."""
+    ns = len(w["sx"]) if nshots is None else nshots
+    g = Grid(nz=w["nz"], nx=w["nx"], N=w["N"], dz=w["dz"], dx=w["dx"], dt=w["dt"], nt=w["nt"], itlag=w["itlag"])
+    return dict(g=g, vp=w["vp"], eps=w["eps"], dl=w["delta"], th=w["theta"], src=np.asarray(w["src"], np.float32),
+                sz=np.asarray(w["sz"][:ns], np.int32), sx=np.asarray(w["sx"][:ns], np.int32),
+                rz=np.asarray(w["rz"], np.int32), rx=np.asarray(w["rx"], np.int32), phys=w["physics"],
+                abc=w["abc"], d0=w["cpml_d0"], fp=w["cpml_f_pico"], cf=w["cerjan_factor"])
+
+
+def _engine(pb, max_shots):
+    return make_engine(pb, max_shots=max_shots, cerjan_factor=pb["cf"])
+
+
+def _forward_vs_oracles(pb, seis, shots):
+    e32 = e64 = 0.0
+    for s in shots:
+        r32 = oracle_forward(Oracle("f32"), pb, shot=s, cerjan_factor=pb["cf"])["seis"]
+        r64 = oracle_forward(Oracle("f64"), pb, shot=s, cerjan_factor=pb["cf"])["seis"]
+        e32 = max(e32, rel_l2(seis[s], r32))
+        e64 = max(e64, rel_l2(seis[s], r64))
+    return e32, e64
+
+
+def _adjoint_oracle(pb, data, fwi, kind, **kw):
+    orc = Oracle(kind)
+    g = pb["g"]
+    model = tuple(orc.expand(m, g.N) if m is not None else None for m in (pb["vp"], pb["eps"], pb["dl"], pb["th"]))
+    A = orc.cerjan(g.N, pb["cf"]) if pb["abc"] == 0 else None
+    cpml = (pb["d0"], pb["fp"]) if pb["abc"] == 1 else None
+    shots = list(zip(pb["sz"].tolist(), pb["sx"].tolist()))
+    return rtm_onthefly(orc, pb["phys"], pb["abc"], g, model, pb["src"], shots, pb["rz"], pb["rx"], data, A=A,
+                        cpml=cpml, fwi=fwi, model_interior=(pb["vp"], pb["eps"], pb["dl"], pb["th"]), **kw)
+
+
+def _residual(pb, eng, nshots, scale=1.02):
+    """dobs - dcal with dobs from a model scaled by `scale` (both on the GPU)."""
+    dcal = eng.forward(0, nshots)
+    eng.set_model((pb["vp"] * np.float32(scale)).astype(np.float32), pb["eps"], pb["dl"], pb["th"])
+    dobs = eng.forward(0, nshots)
+    eng.set_model(pb["vp"], pb["eps"], pb["dl"], pb["th"])
+    return (dobs - dcal).cpu().numpy()
+
+
+def test_acoustic_cerjan_1_full_record():
+    w = W.acoustic_cerjan_1()
+    pb = _pb(w)
+    print(f"case acoustic_cerjan_1 padded {pb['g'].nzp}x{pb['g'].nxp} nt={pb['g'].nt} nrec={len(pb['rx'])} "
+          f"cerjan factor {pb['cf']}", flush=True)
+    assert pb["g"].nt == 2119 and len(pb["rx"]) == 400 and (pb["g"].nzp, pb["g"].nxp) == (301, 501)
+    eng = _engine(pb, 1)
+    seis = eng.forward(0, 1).cpu().numpy()
+    assert seis.shape == (1, 2001, 400)
+    e32, e64 = _forward_vs_oracles(pb, seis, [0])
+    print(f"  seismogram rel_l2: float32 oracle {e32:.2e}, float64 oracle {e64:.2e}", flush=True)
+    assert np.isfinite(seis).all() and np.abs(seis).max() > 0
+    assert e32 <= TOL and e64 <= TOL
+    eng.close()
+
+
+def test_vti_cpml_64_full_grid_forward():
+    w = W.vti_cpml_64(nshots=2, nt_data=202)
+    pb = _pb(w)
+    print(f"case vti_cpml_64 padded {pb['g'].nzp}x{pb['g'].nxp} nt={pb['g'].nt} nrec={len(pb['rx'])}", flush=True)
+    eng = _engine(pb, 2)
+    seis = eng.forward(0, 2).cpu().numpy()
+    e32, e64 = _forward_vs_oracles(pb, seis, [0, 1])
+    print(f"  seismogram rel_l2: float32 oracle {e32:.2e}, float64 oracle {e64:.2e}", flush=True)
+    assert np.isfinite(seis).all() and np.abs(seis).max() > 0
+    assert e32 <= TOL and e64 <= TOL
+    eng.close()
+
+
+def test_vti_cpml_64_all_shots_on_two_lanes_equal_one_shot_at_a_time():
+    w = W.vti_cpml_64(nshots=64, nt_data=52)
+    pb = _pb(w)
+    print(f"case vti_cpml_64 all 64 shots nt={pb['g'].nt}", flush=True)
+    eng = _engine(pb, 64)
+    eng.set_option(_lib.PAM_OPT_TILE, 0)  # the bench's variant, for both runs
+    lanes = eng.forward(0, 64).cpu().numpy()
+    eng.set_option(_lib.PAM_OPT_LANES, 1)
+    eng.set_option(_lib.PAM_OPT_BATCH, 1)
+    single = eng.forward(0, 64).cpu().numpy()
+    assert np.array_equal(lanes, single)
+    peak = np.abs(lanes).reshape(64, -1).max(axis=1)
+    assert np.isfinite(lanes).all() and (peak > 0).all()
+    eng.close()
+
+
+def test_vti_cpml_64_gradient_within_1e4_of_float64_oracle():
+    """The bench's gradient path (uniform checkpoints, two lanes) on the full
+    grid against the float64 oracle's on-the-fly gradient."""
+    w = W.vti_cpml_64(nshots=2, nt_data=602)
+    pb = _pb(w)
+    print(f"case vti_cpml_64 gradient nt={pb['g'].nt}, checkpoints, two lanes", flush=True)
+    eng = _engine(pb, 2)
+    res = _residual(pb, eng, 2)
+    eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_CHECKPOINT)
+    eng.set_option(_lib.PAM_OPT_LANES, 2)
+    eng.set_option(_lib.PAM_OPT_BATCH, 1)  # one shot per lane: both lanes busy
+    g = eng.gradient(res)["g_m"].cpu().numpy()
+    ref64 = _adjoint_oracle(pb, res, True, "f64")["image"]
+    ref32 = _adjoint_oracle(pb, res, True, "f32")["image"]
+    e64, e32 = rel_l2(g, ref64), rel_l2(g, ref32)
+    print(f"  gradient rel_l2: float64 oracle {e64:.2e}, float32 oracle {e32:.2e} "
+          f"(float32 oracle vs float64: {rel_l2(ref32, ref64):.2e})", flush=True)
+    assert np.isfinite(g).all() and np.abs(g).max() > 0
+    assert e64 <= TOL and e32 <= TOL
+    eng.close()
+
+
+def test_tti_cpml_32_full_grid_forward():
+    w = W.tti_cpml_32(nshots=1, nt_data=152)
+    pb = _pb(w)
+    print(f"case tti_cpml_32 padded {pb['g'].nzp}x{pb['g'].nxp} nt={pb['g'].nt}", flush=True)
+    eng = _engine(pb, 1)
+    seis = eng.forward(0, 1).cpu().numpy()
+    e32, e64 = _forward_vs_oracles(pb, seis, [0])
+    print(f"  seismogram rel_l2: float32 oracle {e32:.2e}, float64 oracle {e64:.2e}", flush=True)
+    assert np.isfinite(seis).all() and np.abs(seis).max() > 0
+    assert e32 <= TOL and e64 <= TOL
+    eng.close()
+
+
+@pytest.mark.parametrize("rims", ["f32", "bf16"])
+def test_tti_cpml_32_rtm_boundary_saving(rims):
+    """RTM with the source field rebuilt from saved rims in float32 or scaled
+    bfloat16, against the oracle's restatement of the same strategy; the
+    distance of each to the GPU's save-every-step image is the strategy's own
+    error.  A float32 time reversal cannot undo the forward pass's rounding,
+    so two float32 rebuilds that round differently (FMA on the GPU, none in
+    the oracle) each sit some 1e-4 from the exact rebuild in their own
+    direction: the accuracy bar is the distance to the float64 restatement of
+    the strategy (1e-4 with float32 rims), and the GPU must be at least as close
+    to it as the float32 restatement is; the two float32 rebuilds agree within
+    2e-4 (float32 rims) or 5e-3 (bfloat16 rims)."""
+    w = W.tti_cpml_32(nshots=1, nt_data=152)
+    pb = _pb(w)
+    print(f"case tti_cpml_32 rtm boundary rims={rims} nt={pb['g'].nt}", flush=True)
+    eng = _engine(pb, 1)
+    data = _residual(pb, eng, 1, scale=1.05)
+    img_all = eng.rtm(data)[0].cpu().numpy()
+    eng.set_option(_lib.PAM_OPT_STORAGE,
+                   _lib.PAM_STORE_BOUNDARY_F32 if rims == "f32" else _lib.PAM_STORE_BOUNDARY_BF16)
+    img = eng.rtm(data)[0].cpu().numpy()
+    ref = _adjoint_oracle(pb, data, False, "f32", storage="boundary", rim_kind=rims)["image"]
+    ref64 = _adjoint_oracle(pb, data, False, "f64", storage="boundary", rim_kind=rims)["image"]
+    e, e64, eo64 = rel_l2(img, ref), rel_l2(img, ref64), rel_l2(ref, ref64)
+    print(f"  image rel_l2: same-strategy float32 oracle {e:.2e}, float64 {e64:.2e} (float32 oracle vs "
+          f"float64 {eo64:.2e}); vs save-all on the GPU {rel_l2(img, img_all):.2e}", flush=True)
+    assert np.isfinite(img).all() and np.abs(img).max() > 0
+    assert e <= (2e-4 if rims == "f32" else 5e-3)
+    assert e64 <= 2.0 * eo64 + 1e-5
+    if rims == "f32":
+        assert e64 <= TOL
+    eng.close()
+
+
+def test_acoustic_cpml_32_full_grid_gradient():
+    w = W.acoustic_cpml_32(nshots=1, nt_data=152)
+    pb = _pb(w)
+    print(f"case acoustic_cpml_32 gradient padded {pb['g'].nzp}x{pb['g'].nxp} nt={pb['g'].nt}", flush=True)
+    eng = _engine(pb, 1)
+    res = _residual(pb, eng, 1)
+    g = eng.gradient(res)["g_m"].cpu().numpy()
+    ref32 = _adjoint_oracle(pb, res, True, "f32")["image"]
+    ref64 = _adjoint_oracle(pb, res, True, "f64")["image"]
+    e32, e64 = rel_l2(g, ref32), rel_l2(g, ref64)
+    print(f"  gradient rel_l2: float32 oracle {e32:.2e}, float64 oracle {e64:.2e}", flush=True)
+    assert np.isfinite(g).all() and np.abs(g).max() > 0
+    assert e32 <= TOL and e64 <= TOL
+    eng.close()
+
+
+@pytest.mark.parametrize("which", [0, 15])
+def test_uneven_vti_grid_forward(which):
+    w = W.uneven_vti_group(shots_per_model=1, nt_data=182)[which]
+    pb = _pb(w)
+    print(f"case {w['name']} padded {pb['g'].nzp}x{pb['g'].nxp} nt={pb['g'].nt}", flush=True)
+    eng = _engine(pb, 1)
+    seis = eng.forward(0, 1).cpu().numpy()
+    e32, e64 = _forward_vs_oracles(pb, seis, [0])
+    print(f"  seismogram rel_l2: float32 oracle {e32:.2e}, float64 oracle {e64:.2e}", flush=True)
+    assert np.isfinite(seis).all() and np.abs(seis).max() > 0
+    assert e32 <= TOL and e64 <= TOL
+    eng.close()
+
+
+def test_uneven_vti_group_of_sixteen_equals_each_model_alone():
+    from pseudoacousticmodeling_amd.engine import Group
+
+    ws = W.uneven_vti_group(shots_per_model=8, nt_data=32)
+    pbs = [_pb(w) for w in ws]
+    print(f"case uneven group: {len(ws)} models x 8 shots, nt={pbs[0]['g'].nt}", flush=True)
+    engs = [_engine(pb, 8) for pb in pbs]
+    for e in engs:
+        e.set_option(_lib.PAM_OPT_TILE, 0)
+        e.set_option(_lib.PAM_OPT_LANES, 1)
+    alone = [e.forward(0, 8).cpu().numpy() for e in engs]
+    grp = Group(engs)
+    outs = [o.cpu().numpy() for o in grp.forward([8] * len(engs))]
+    for i, (o, a) in enumerate(zip(outs, alone)):
+        assert np.array_equal(o, a), f"model {i} ({ws[i]['name']}) differs"
+        assert np.isfinite(o).all() and (np.abs(o).reshape(8, -1).max(axis=1) > 0).all()
+    grp.close()
+    for e in engs:
+        e.close()
