@@ -246,7 +246,14 @@ void or_update_acoustic(store *Uf, const store *Uc, const store *vp, int nzp, in
 static inline real sd_term(real e, real dl, real px, real pz) {
     real px2 = px * px, pz2 = pz * pz;
     real num = R(-2.0) * (e - dl) * px2 * pz2;
+#ifdef ORACLE_PLANT_SD_DEN_EPS
+    /* a mistake planted on purpose, 2 (1 + eps) for 2 (1 + delta): only the
+       oracle's own pin test builds this variant, to show that its criterion
+       (the exact acoustic qP phase velocity to O(t^3)) catches it */
+    real den = (R(1.0) + R(2.0) * e) * (px2 * px2) + (pz2 * pz2) + R(2.0) * (R(1.0) + e) * px2 * pz2;
+#else
     real den = (R(1.0) + R(2.0) * e) * (px2 * px2) + (pz2 * pz2) + R(2.0) * (R(1.0) + dl) * px2 * pz2;
+#endif
     return (RFABS(den) > DEN_EPS) ? num / den : R(0.0);
 }
 

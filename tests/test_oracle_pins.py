@@ -9,8 +9,11 @@ coefficient, sign, index or transposed operand) would break:
   second-derivative and 4 first-derivative coefficients uniquely);
 * the mixed derivative and the TTI rotation: polynomial exactness, and
   rotational covariance on plane waves (TTI(theta) on k == VTI on R(theta) k);
-* Sd: elliptic exactness (eps = delta -> Sd = 0) and Thomsen's weak-anisotropy
-  phase velocity V^2/v^2 = 1 + 2 eps n_x^4 + 2 delta n_x^2 n_z^2 + O(eps^2);
+* Sd: elliptic exactness (eps = delta -> Sd = 0), Thomsen's weak-anisotropy
+  phase velocity V^2/v^2 = 1 + 2 eps n_x^4 + 2 delta n_x^2 n_z^2 + O(eps^2), and
+  the exact acoustic qP phase velocity to O(t^3) as (eps, delta) = t (eps0,
+  delta0), which fixes the denominator's first-order terms (a planted
+  2 (1 + eps) for 2 (1 + delta) converges only at O(t^2));
 * the leapfrog update: discrete energy conservation with no absorbing layer;
 * physics reductions: VTI(eps=delta=0) == acoustic, TTI(theta=0) == VTI, the
   same with CPML, mirror symmetry;
@@ -130,18 +133,19 @@ def _plane(n, h, kx, kz, phase=0.3):
     return np.cos(ph), np.sin(ph)
 
 
-def _ratio(phys, kx, kz, eps, dl, th=0.0, n=40, h=10.0):
+def _ratio(phys, kx, kz, eps, dl, th=0.0, n=40, h=10.0, lib=None):
     """-P/(k^2 U) for a plane wave: the normalised squared phase velocity."""
+    lib = F64.lib if lib is None else lib
     U, S = _plane(n, h, kx, kz)
     Uf = np.zeros((n, n))
     one = np.ones((n, n))
     E = np.full((n, n), eps)
     Dl = np.full((n, n), dl)
     if phys == 1:
-        F64.lib.or_update_vti(_p(Uf), _p(U), _p(one), _p(E), _p(Dl), _I(n), _I(n), _D(h), _D(h), _D(1.0))
+        lib.or_update_vti(_p(Uf), _p(U), _p(one), _p(E), _p(Dl), _I(n), _I(n), _D(h), _D(h), _D(1.0))
     else:
         T = np.full((n, n), th)
-        F64.lib.or_update_tti(_p(Uf), _p(U), _p(one), _p(E), _p(Dl), _p(T), _I(n), _I(n), _D(h), _D(h), _D(1.0))
+        lib.or_update_tti(_p(Uf), _p(U), _p(one), _p(E), _p(Dl), _p(T), _I(n), _I(n), _D(h), _D(h), _D(1.0))
     P = Uf - 2 * U
     m = np.zeros_like(U, dtype=bool)
     m[IN] = True
@@ -163,6 +167,54 @@ def test_sd_elliptic_and_weak_anisotropy_limits():
         assert np.all(np.abs(R - 1 - first) < 20 * e * e + 1e-9)
         # a wrong sign of the Sd numerator would be first order
         assert abs(2 * (e - d) * nx_ ** 2 * nz_ ** 2) > 20 * e * e
+
+
+def _qp_exact(e, d, nx_, nz_):
+    """Squared qP phase velocity over vp^2 of the acoustic (shear-wave-free)
+    VTI medium: with A = 1 + 2 e nx^2 and Y = 2 (e - d) nx^2 nz^2 for a unit
+    wave normal, V^2/vp^2 = A/2 + sqrt(A^2 - 4 Y)/2 (Alkhalifah's acoustic
+    approximation, the Vs -> 0 limit of Thomsen's exact qP velocity)."""
+    A = 1.0 + 2.0 * e * nx_ ** 2
+    Y = 2.0 * (e - d) * nx_ ** 2 * nz_ ** 2
+    return 0.5 * A + 0.5 * np.sqrt(A * A - 4.0 * Y)
+
+
+def _sd_orders(lib):
+    """log2 of the error ratio between (e, d) = t (0.2, 0.05) and t/2 against the
+    exact qP velocity, per angle: about 3 when the pseudo-acoustic velocity
+    A - Y / (A - Y) (the Sd operator on a plane wave) agrees with it to
+    second order, about 2 for a denominator wrong at first order."""
+    k = 2 * np.pi / 1200.0  # 120 cells per wavelength: the stencil's own error is ~ (k h)^8 ~ 5e-11
+    e0, d0 = 0.2, 0.05
+    orders = []
+    for ang in (20.0, 45.0, 70.0):
+        nx_, nz_ = np.sin(np.deg2rad(ang)), np.cos(np.deg2rad(ang))
+        errs = []
+        for t in (0.2, 0.1, 0.05):
+            R = np.median(_ratio(1, k * nx_, k * nz_, t * e0, t * d0, n=48, lib=lib))
+            errs.append(abs(R - _qp_exact(t * e0, t * d0, nx_, nz_)))
+        orders.append([np.log2(errs[i] / errs[i + 1]) for i in range(2)])
+        # and the error is small in absolute terms: O(t^3) with its known coefficient scale
+        assert errs[-1] < 1e-3
+    return np.array(orders)
+
+
+def test_sd_matches_the_exact_qp_velocity_to_second_order(tmp_path):
+    """The Sd denominator beyond first order in (eps, delta): the oracle's plane-
+    wave velocity converges to the exact acoustic qP velocity at O(t^3) as
+    (eps, delta) = t (eps0, delta0) -> 0; an oracle built with 2 (1 + eps) in
+    place of 2 (1 + delta) in the denominator converges only at O(t^2)."""
+    good = _sd_orders(F64.lib)
+    assert np.all(good > 2.8), good
+    import os
+    import subprocess
+
+    src = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "pam_oracle.c")
+    so = str(tmp_path / "liboracle_planted.so")
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-ffp-contract=off", "-DORACLE_F64",
+                    "-DORACLE_PLANT_SD_DEN_EPS", "-o", so, src, "-lm"], check=True)
+    bad = _sd_orders(ctypes.CDLL(so))
+    assert np.all(bad < 2.2), bad
 
 
 def test_tti_is_vti_in_the_rotated_frame():
