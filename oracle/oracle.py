@@ -479,16 +479,164 @@ def rim_mask(nz, nx):
     return m
 
 
-class _SaveAll:
-    """Source field kept at every step (src/Migration.py:996-1037)."""
+def quantize_with_scale(v, m, kind):
+    """v stored in float16 / bfloat16 as v * (1/m) for a given scale m (float32
+    arithmetic; m <= 0 stores zeros), read back as q * m."""
+    v = np.asarray(v, dtype=np.float32)
+    m = np.float32(m)
+    inv = np.float32(1.0) / m if m > 0 else np.float32(0.0)
+    q = v * inv
+    q = q.astype(np.float16).astype(np.float32) if kind == "f16" else _bf16_round(q)
+    return q * m
 
-    def __init__(self, S, g: Grid):
-        self.S, self.sl = S, (slice(g.N, g.N + g.nz), slice(g.N, g.N + g.nx))
+
+class _SaveAll:
+    """Source field kept at every step (src/Migration.py:996-1037).
+
+    ``kind`` 'f16' / 'bf16' restates the library's 16-bit scaled snapshots
+    (no counterpart in the reference): an image reads S_t stored as S_t / m_t
+    with m_t = max |S_t| over the interior; a gradient reads the second time
+    difference D_t = S_{t+1} - 2 S_t + S_{t-1} (formed in float32 on the
+    interior) stored with the scale of the step before, max |D_{t-1}| (the
+    producing step finds the max only as it writes, so D_t's own max comes one
+    step late; S_{-1} = 0)."""
+
+    def __init__(self, S, g: Grid, kind="f32"):
+        self.S, self.sl, self.kind = S, (slice(g.N, g.N + g.nz), slice(g.N, g.N + g.nx)), kind
+
+    def _d(self, t):
+        S, sl = self.S, self.sl
+        prev = S[t - 1][sl] if t >= 1 else np.zeros_like(S[t][sl])
+        return S[t + 1][sl] - 2.0 * S[t][sl] + prev
 
     def get(self, t, need_d):
         S, sl = self.S, self.sl
-        d = (S[t + 1][sl] - 2.0 * S[t][sl] + S[t - 1][sl]) if need_d else None
-        return S[t], d
+        if self.kind == "f32":
+            return S[t], (self._d(t) if need_d else None)
+        if need_d:
+            scale = np.max(np.abs(self._d(t - 1)))
+            return S[t], quantize_with_scale(self._d(t), scale, self.kind)
+        St = np.array(S[t], copy=True)
+        St[sl] = quantize_scaled(S[t][sl], self.kind)
+        return St, None
+
+
+def _revolve_beta(s, r):
+    b = 1.0
+    for i in range(1, r + 1):
+        b = b * (s + i) / i
+    return b
+
+
+def revolve_split(n, s):
+    """Offset of the first checkpoint of an n-step range with s free slots
+    (Griewank's treeverse): for the smallest r with beta(s, r) >= n,
+    m = min(beta(s, r - 1), n - 1), at least 1; beta(s, r) = C(s + r, s)."""
+    r = 1
+    while _revolve_beta(s, r) < n:
+        r += 1
+    return max(1, int(min(_revolve_beta(s, r - 1), float(n - 1))))
+
+
+class _Revolve:
+    """Binomial (revolve) checkpointing of the source field as the library runs
+    it (no counterpart in the reference): whole states (both wavefields and,
+    with CPML, psi and zeta after the previous step) kept in ``slots`` slots,
+    in float32 or ('f16') each field scaled by its max |value| over the padded
+    grid and rounded to float16; the forward pass recomputed from the slots in
+    treeverse order, each adjoint step's S_t and D_t taken from the one forward
+    step run from state t.  The state is carried as the library carries it:
+    the current field holds the source sample of its step, and the previous
+    field is stored damped once (the Cerjan factors are applied to it again as
+    the next step reads it)."""
+
+    def __init__(self, orc, phys, abc, g: Grid, model, A, cpml, src, sz, sx, t_lo, t_hi, slots, kind):
+        self.orc, self.phys, self.abc, self.g, self.model, self.A = orc, phys, abc, g, model, A
+        self.sl = (slice(g.N, g.N + g.nz), slice(g.N, g.N + g.nx))
+        self.src, self.sz, self.sx, self.kind = src, sz, sx, kind
+        st = orc.make_state(g, abc, model[0], *(cpml or (None, None)))
+        self.cur, self.prv = st["U0"], st["U1"]
+        self.cp = st["cp"]
+        self._inject(self.cur, 0)
+        self.ws = 0
+        self.slots = {}
+        self.S, self.D = {}, {}
+        self.first = True
+        self._prev_int = None
+        nslot = max(1, min(slots, t_hi - t_lo + 1))
+        self._advance(0, t_lo)
+        self._store(t_lo)
+        self._rev(t_lo, t_hi + 1, nslot - 1)
+
+    def _inject(self, U, k):
+        if 0 <= k < self.g.nt:
+            self.orc.lib.or_inject(_ptr(U), _I(self.g.nxp), _I(self.sz), _I(self.sx), _D(float(self.src[k])))
+
+    def _fields(self):
+        return [self.cur, self.prv] + (list(self.cp[4:]) if self.abc == 1 else [])
+
+    def _step(self, k, save=False):
+        o, g = self.orc, self.g
+        sl = self.sl
+        if save:
+            S = np.array(self.cur, copy=True)
+            if self.abc == 0:  # the padded snapshot is the current field damped once more
+                o.cerjan_apply(S, g, self.A)
+            self.S[k] = S
+            prev_int = np.array(self.prv[sl], copy=True)
+        if self.abc == 0:
+            o.cerjan_apply(self.prv, g, self.A)  # the previous field as the step reads it
+            vp, eps, dl, th = self.model
+            o.update(self.phys, self.prv, self.cur, g, vp, eps, dl, th)
+            o.cerjan_apply(self.prv, g, self.A)
+        else:
+            o.forward_step(self.phys, 1, self.prv, self.cur, g, self.model, self.A, self.cp, -1, -1, 0.0)
+        self._inject(self.prv, k + 1)
+        if save:  # D_t on the interior (undamped there), in float32 as the step forms it
+            self.D[k] = self.prv[sl] - 2.0 * self.cur[sl] + prev_int
+        self.cur, self.prv = self.prv, self.cur
+        self.ws = k + 1
+
+    def _advance(self, frm, to):
+        if self.ws != frm:
+            self._load(frm)
+        for k in range(frm, to):
+            self._step(k)
+
+    def _store(self, k):
+        self.slots[k] = [quantize_scaled(f, "f16") if self.kind == "f16" else np.array(f, copy=True)
+                         for f in self._fields()]
+
+    def _load(self, k):
+        for dst, src in zip(self._fields(), self.slots[k]):
+            dst[...] = src
+        self.ws = k
+
+    def _leaf(self, t):
+        self._advance(t, t)
+        self._step(t, save=True)
+        if self.first:  # the first descent runs on to the end of the record (illumination)
+            self.first = False
+            self.ws = self.g.nt
+
+    def _rev(self, lo, hi, s):
+        n = hi - lo
+        if n == 1:
+            return self._leaf(lo)
+        if s == 0:
+            for t in range(hi - 1, lo - 1, -1):
+                self._advance(lo, t)
+                self._leaf(t)
+            return
+        m = lo + revolve_split(n, s)
+        self._advance(lo, m)
+        self._store(m)
+        self._rev(m, hi, s - 1)
+        del self.slots[m]
+        self._rev(lo, m, s)
+
+    def get(self, t, need_d):
+        return self.S[t], (self.D[t] if need_d else None)
 
 
 class _BoundaryRebuild:
@@ -591,7 +739,7 @@ class _RandomRebuild:
 
 def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, data, A=None, cpml=None,
                  fwi=False, multiparameter=False, model_interior=None, reset_cpml=True, t_range="py",
-                 normalize=1, storage="all", rim_kind="f32"):
+                 normalize=1, storage="all", rim_kind="f32", snap_kind="f32", slots=50):
     """On-the-fly RTM / FWI gradient of src/Migration.py:963-1111 (GPU path):
     per shot, a forward pass storing the current field after every step and
     accumulating ilum += S^2 over the interior, then a backward pass
@@ -613,13 +761,17 @@ def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, dat
     as ``rim_kind``; ilum still comes from the forward pass.
     ``storage="random"`` runs the forward pass undamped (``model`` being a
     random-boundary model) and takes the source field from the whole-grid
-    rebuild (``_RandomRebuild``); the adjoint keeps ``abc``."""
+    rebuild (``_RandomRebuild``); the adjoint keeps ``abc``.
+    ``snap_kind`` 'f16' / 'bf16' keeps the save-all snapshots in 16 bits as
+    the library does (``_SaveAll``); ``storage="revolve"`` recomputes the
+    source field from ``slots`` binomial checkpoints kept in ``snap_kind``
+    'f32' or 'f16' (``_Revolve``)."""
     if t_range not in ("py", "cpp") or normalize not in (1, 2):
         raise ValueError("t_range 'py'|'cpp', normalize 1|2")
     if t_range == "cpp" and fwi:
         raise ValueError("the C++ backward range applies to imaging only")
-    if storage not in ("all", "boundary", "random"):
-        raise ValueError("storage 'all'|'boundary'|'random'")
+    if storage not in ("all", "boundary", "random", "revolve"):
+        raise ValueError("storage 'all'|'boundary'|'random'|'revolve'")
     N = g.N
     vp, eps, dl, th = [orc.arr(m) if m is not None else None for m in model]
     image = np.zeros((g.nz, g.nx), dtype=np.float64)
@@ -641,8 +793,12 @@ def rtm_onthefly(orc: Oracle, phys, abc, g: Grid, model, src, shots, rz, rx, dat
         ilum_p = np.zeros((g.nz, g.nx), dtype=orc.dtype)
         for k in range(g.nt):
             ilum_p += S[k][sl] * S[k][sl]
+        t_hi, t_lo = (g.nt - 2, 1) if t_range == "py" else (g.nt - 1, 0)
         if storage == "all":
-            prov = _SaveAll(S, g)
+            prov = _SaveAll(S, g, snap_kind)
+        elif storage == "revolve":
+            prov = _Revolve(orc, phys, abc, g, (vp, eps, dl, th), A, cpml, orc.arr(src), sz, sx, t_lo, t_hi,
+                            slots, snap_kind)
         elif storage == "random":
             prov = _RandomRebuild(orc, phys, g, (vp, eps, dl, th), S, fw["current"], orc.arr(src), sz, sx)
         else:

@@ -1940,6 +1940,11 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         return fail(ctx, PAM_E_UNSUPPORTED,
                     "float16 / bfloat16 snapshots feed the pressure-form adjoint only; the linearised VTI/TTI "
                     "adjoint (Cerjan) reads float32 padded snapshots: use PAM_STORE_ALL_F32 or PAM_STORE_CHECKPOINT");
+    if (half && grad && (g_eps || g_del || g_th))
+        return fail(ctx, PAM_E_UNSUPPORTED,
+                    "the eps / delta / theta partials read float32 snapshots of S_t: use PAM_STORE_ALL_F32, "
+                    "PAM_STORE_CHECKPOINT or a revolve / boundary strategy with them, not float16 / bfloat16 "
+                    "save-all snapshots");
     // boundary saving: the adjoint step reads D_t from a one-step interior
     // buffer (gradient) or S_t from the rebuilt padded field (imaging)
     const int skind = (sb || rbc) ? (grad ? SV_F32_INT : SV_F32_PAD)

@@ -23,7 +23,7 @@ def _data(pb, nshots, seed):
     return np.stack([oracle_forward(orc, pert, shot=s)["seis"] for s in range(nshots)]).astype(np.float32)
 
 
-def _oracle_adjoint(pb, data, fwi, multiparameter=False, kind="f32"):
+def _oracle_adjoint(pb, data, fwi, multiparameter=False, kind="f32", **storage):
     orc = Oracle(kind)
     g = pb["g"]
     model = tuple(orc.expand(m, g.N) if m is not None else None for m in (pb["vp"], pb["eps"], pb["dl"], pb["th"]))
@@ -32,7 +32,7 @@ def _oracle_adjoint(pb, data, fwi, multiparameter=False, kind="f32"):
     shots = list(zip(pb["sz"].tolist(), pb["sx"].tolist()))
     interior = (pb["vp"], pb["eps"], pb["dl"], pb["th"])
     return rtm_onthefly(orc, pb["phys"], pb["abc"], g, model, pb["src"], shots, pb["rz"], pb["rx"], data, A=A,
-                        cpml=cpml, fwi=fwi, multiparameter=multiparameter, model_interior=interior)
+                        cpml=cpml, fwi=fwi, multiparameter=multiparameter, model_interior=interior, **storage)
 
 
 @pytest.mark.parametrize("phys,abc", [(p, a) for p in (0, 1, 2) for a in (0, 1)])

@@ -15,6 +15,7 @@ from oracle.oracle import rel_l2
 from pseudoacousticmodeling_amd import _lib
 from tests.problems import make_problem, make_engine
 from tests.test_gpu_adjoint import _data, _oracle_adjoint
+from tests.test_gpu_storage import _check_same_strategy
 
 pytestmark = pytest.mark.gpu
 
@@ -23,6 +24,11 @@ pytestmark = pytest.mark.gpu
 # the image reads S_t itself
 TOL_F16_IMAGE = 2e-3
 TOL_F16_GRAD = 5e-2
+# against the oracle's restatement of the same schedule and slots
+# (oracle/oracle.py _Revolve), with the criterion of the 16-bit save-all
+# snapshots (tests/test_gpu_storage.py): images within 1e-4; gradients, whose
+# D_t applies the spatial operator to each restored state's rounding, with own
+# errors within 30% of each other and a mutual distance under 3/4 of them
 
 
 def _run(eng, data, grad, multi=False):
@@ -90,7 +96,13 @@ def test_revolve_float16_slots(phys, abc, grad):
     eng.set_option(_lib.PAM_OPT_SNAPSHOTS, 8)
     got = _run(eng, data, grad)[key]
     e32 = rel_l2(got, ref)
-    print(f"  float16 slots: rel_l2 vs float32 storage {e32:.3e}, vs FP64 oracle {rel_l2(got, o64):.3e} "
-          f"(float32 storage vs oracle {rel_l2(ref, o64):.3e})", flush=True)
+    same = _oracle_adjoint(pb, data, fwi=grad, kind="f32", storage="revolve", slots=8, snap_kind="f16")["image"]
+    o32 = _oracle_adjoint(pb, data, fwi=grad, kind="f32")["image"]
+    es = rel_l2(got, same)
+    own = rel_l2(same, o32)
+    print(f"  float16 slots: rel_l2 vs the same-strategy float32 oracle {es:.3e}; vs float32 storage {e32:.3e} "
+          f"(oracle: {own:.3e}), vs FP64 oracle {rel_l2(got, o64):.3e} (float32 storage vs oracle "
+          f"{rel_l2(ref, o64):.3e})", flush=True)
     assert np.isfinite(got).all()
+    _check_same_strategy(grad, es, e32, own)
     assert e32 <= (TOL_F16_GRAD if grad else TOL_F16_IMAGE)

@@ -2,8 +2,10 @@
 gradient and partials): uniform checkpoints in HBM must reproduce the
 save-every-step result bit for bit; float16 / bfloat16 snapshots (scaled per
 snapshot by their max over the interior; the gradient stores the second time
-difference formed in float32) must stay within a stated tolerance of the
-float32 result and of the FP64 oracle."""
+difference formed in float32, scaled by the previous step's max) must match
+the oracle's restatement of the same storage (oracle/oracle.py _SaveAll)
+within 1e-4, and their distance to the float32 result -- the storage's own
+error -- stays within the stated tolerance."""
 import numpy as np
 import pytest
 
@@ -18,6 +20,22 @@ pytestmark = pytest.mark.gpu
 TOL_F16 = 2e-3
 # bfloat16 snapshots: 8-bit significand, rounding 2^-9 relative per sample
 TOL_BF16 = 1.5e-2
+# against the oracle's restatement of the same 16-bit storage: images within
+# 1e-4.  A gradient reads the second time difference D_t, whose cancellation
+# turns the ~1e-6 float32 differences between GPU and oracle into flipped
+# 16-bit roundings at a few percent of the samples, so two correct
+# implementations of the same storage differ by a fraction of that storage's
+# own error: their own errors (16-bit vs float32 storage) agree within 30%,
+# and they are closer to each other than 3/4 of that error.
+TOL_SAME = 1e-4
+
+
+def _check_same_strategy(grad, es, own_gpu, own_oracle):
+    if not grad:
+        assert es <= TOL_SAME
+    else:
+        assert abs(own_gpu / own_oracle - 1.0) <= 0.3
+        assert es <= 0.75 * own_oracle
 
 
 def _run(eng, data, grad, multi=False):
@@ -55,14 +73,20 @@ def test_half_precision_snapshots(phys, abc, grad):
     key = "g_m" if grad else "image"
     ref32 = _run(eng, data, grad)[key]
     o64 = _oracle_adjoint(pb, data, fwi=grad, kind="f64")["image"]
-    for kind, tol in ((_lib.PAM_STORE_ALL_F16, TOL_F16), (_lib.PAM_STORE_ALL_BF16, TOL_BF16)):
+    o32 = _oracle_adjoint(pb, data, fwi=grad, kind="f32")["image"]
+    for kind, tol, name in ((_lib.PAM_STORE_ALL_F16, TOL_F16, "f16"), (_lib.PAM_STORE_ALL_BF16, TOL_BF16, "bf16")):
         eng.set_option(_lib.PAM_OPT_STORAGE, kind)
         got = _run(eng, data, grad)[key]
         e32 = rel_l2(got, ref32)
         e64 = rel_l2(got, o64)
-        print(f"  storage {kind}: rel_l2 vs float32 storage {e32:.3e}, vs oracle {e64:.3e} "
-              f"(float32 storage vs oracle {rel_l2(ref32, o64):.3e})", flush=True)
+        same = _oracle_adjoint(pb, data, fwi=grad, kind="f32", snap_kind=name)["image"]
+        es = rel_l2(got, same)
+        own = rel_l2(same, o32)  # the restated storage's own error
+        print(f"  storage {name}: rel_l2 vs the same-strategy float32 oracle {es:.3e}; vs float32 storage "
+              f"{e32:.3e} (oracle: {own:.3e}), vs FP64 oracle {e64:.3e} (float32 storage vs FP64 oracle "
+              f"{rel_l2(ref32, o64):.3e})", flush=True)
         assert np.isfinite(got).all()
+        _check_same_strategy(grad, es, e32, own)
         assert e32 <= tol
 
 
@@ -73,3 +97,16 @@ def test_half_precision_rejected_for_the_linearised_adjoint():
     eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_ALL_F16)
     with pytest.raises(_lib.PamError, match="PAM_E_UNSUPPORTED"):
         eng.rtm(data)
+
+
+def test_half_precision_rejected_with_partials():
+    """The eps / delta partials read float32 snapshots of S_t: 16-bit save-all
+    snapshots with partials are refused before any launch."""
+    pb = make_problem(1, 1, nshots=1, seed=141, nt=120)
+    data = _data(pb, 1, seed=15)
+    eng = make_engine(pb)
+    for kind in (_lib.PAM_STORE_ALL_F16, _lib.PAM_STORE_ALL_BF16):
+        eng.set_option(_lib.PAM_OPT_STORAGE, kind)
+        with pytest.raises(_lib.PamError, match="PAM_E_UNSUPPORTED"):
+            eng.gradient(data, multiparameter=True)
+        eng.gradient(data)  # the plain gradient still runs
