@@ -44,7 +44,9 @@ enum pam_layout { PAM_INTERIOR = 0, PAM_PADDED = 1 };
 enum pam_storage {
     PAM_STORE_ALL_F32 = 0,  /* every step saved, float32 */
     PAM_STORE_ALL_F16 = 1,  /* every step saved, float16 with a per-snapshot scale */
-    PAM_STORE_ALL_BF16 = 2, /* every step saved, bfloat16 with a per-snapshot scale */
+    PAM_STORE_ALL_BF16 = 2, /* every step saved, bfloat16 with a per-snapshot scale (F16 / BF16:
+                               not with the linearised VTI/TTI Cerjan adjoint nor with eps / delta /
+                               theta partials, which read float32 snapshots: PAM_E_UNSUPPORTED) */
     PAM_STORE_CHECKPOINT = 3, /* uniform checkpoints, segments recomputed (exact) */
     /* boundary saving (src/Migration.py:1266-1410): the 4-cell rims just inside
        the interior edge of every step's source field are kept, and the backward
@@ -180,8 +182,34 @@ int pam_set_geometry(pam_ctx *ctx, const int *sz, const int *sx, int nshot, cons
 int pam_reset(pam_ctx *ctx, int shot);
 int pam_forward_step(pam_ctx *ctx, int k, float *seis /* device [nt_data][nrec] or NULL */);
 /* Copies a padded [nzp][nxp] field of slot 0 to a device buffer.  which: 0 the
- * current field c_k (after the injection of step k), 1 the previous field. */
+ * current field c_k (after the injection of step k, k the next forward step), 1
+ * the previous field; 2 the adjoint field lambda_t of the next adjoint step t
+ * (after the injection of data row t), 3 the adjoint field before it. */
 int pam_get_field(pam_ctx *ctx, int which, float *out);
+
+/* Per-step adjoint interface, mirroring Migration::backward_step(k, P)
+ * (cpp/Migration.cu:238-251, cpp/Migration.cuh:56) and
+ * migration.backward_step[GPU](k, P) (src/Migration.py:237-286) with the
+ * imaging of the drivers' loops (cpp/Migration.cu:383-390: image += P lambda,
+ * ilum += P^2 over the interior): one shot in slot 0 of the context's second
+ * field set, so that it runs beside the per-step forward interface.
+ * pam_adjoint_reset binds the shot's data (device [nt_data][nrec], the muted
+ * record or the residual, read until the next reset) and clears the adjoint
+ * field, and the CPML memory unless PAM_OPT_RESET_ADJ_CPML is 0 (then the
+ * memory the per-step forward run left is taken over, as src/Migration.py
+ * does).  pam_adjoint_step(t) runs one adjoint step: data row t injected at
+ * the receivers (divided by dx dz) when t >= itlag, the adjoint update -- the
+ * forward operator (acoustic, and VTI/TTI with CPML) or the linearised VTI/TTI
+ * adjoint reading S_t (Cerjan) --, the absorbing boundary, and, with image /
+ * ilum (device [nz][nx], may be NULL), image += S_t lambda_t and ilum += S_t^2
+ * on the interior.  S_t is the source field of step t as pam_get_field(ctx, 0)
+ * returns it before forward step t (device padded [nzp][nxp]); it may be NULL
+ * when nothing reads it.  Steps are issued in decreasing order, t, t-1, ... 0,
+ * from any first t.  A whole-run pam_rtm / pam_gradient / pam_forward_adjoint
+ * ends the per-step adjoint (PAM_E_STATE until the next reset).  One shot
+ * driven this way gives pam_rtm's image (PAM_OPT_NORMALIZE 0) bit for bit. */
+int pam_adjoint_reset(pam_ctx *ctx, int shot, const float *data);
+int pam_adjoint_step(pam_ctx *ctx, int t, const float *S_t, float *image, float *ilum);
 
 /* Whole-run forward modeling, replacing Modeling::solveWaveEquation
  * (cpp/Modeling.cu:433-479) and wavefield.solveWaveEquationGPU

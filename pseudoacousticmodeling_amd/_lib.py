@@ -35,7 +35,7 @@ EXPORTS = [
     "pam_set_model", "pam_set_wavelet", "pam_set_geometry", "pam_reset", "pam_forward_step",
     "pam_get_field", "pam_forward", "pam_rtm", "pam_gradient", "pam_misfit", "pam_forward_adjoint",
     "pam_last_timing", "pam_group_create", "pam_group_destroy", "pam_group_forward", "pam_group_get_tile",
-    "pam_smooth_model", "pam_mute", "pam_misfit_noalloc",
+    "pam_smooth_model", "pam_mute", "pam_misfit_noalloc", "pam_adjoint_reset", "pam_adjoint_step",
 ]
 
 
@@ -86,6 +86,8 @@ def load():
     lib.pam_misfit.argtypes = [vp, I, P, P, P, ctypes.POINTER(D)]
     lib.pam_forward_adjoint.argtypes = [vp, I, I, P, P, I]
     lib.pam_misfit_noalloc.argtypes = [I, I, I, P, P, P, P, vp]
+    lib.pam_adjoint_reset.argtypes = [vp, I, P]
+    lib.pam_adjoint_step.argtypes = [vp, I, P, P, P]
     lib.pam_last_timing.argtypes = [vp, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(LL)]
     lib.pam_group_create.argtypes = [ctypes.POINTER(vp), I, ctypes.POINTER(vp)]
     lib.pam_group_destroy.argtypes = [vp]

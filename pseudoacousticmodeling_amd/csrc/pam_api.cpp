@@ -74,6 +74,15 @@ struct pam_ctx {
     // per-step interface state (slot 0)
     float *cur = nullptr, *prv = nullptr;
     int next_k = -1;
+    // per-step adjoint state (slot 0 of the second field set): the shot's data,
+    // the adjoint fields, the next step (-2: none in progress, -1: the first
+    // step may be any t), and the source field of the step in the kernels'
+    // pitch (damped once more for the linearised adjoint)
+    const float *adj_data = nullptr;
+    float *adj_cur = nullptr, *adj_prv = nullptr;
+    int adj_next_t = -2;
+    float *adj_S = nullptr;
+    long long adj_S_bytes = 0;
     // step-kernel variant chosen by the autotuner, per batch size
     std::map<int, int> tuned;
     std::map<int, double> tuned_ms;  // its time per shot-step
@@ -318,7 +327,7 @@ extern "C" int pam_destroy(pam_ctx *ctx) {
                      ctx->fs.U1,  ctx->fs.psix, ctx->fs.psiz,  ctx->fs.zetax, ctx->fs.zetaz, ctx->tmp, ctx->save_scale,
                      ctx->aux.U0, ctx->aux.U1, ctx->aux.psix, ctx->aux.psiz, ctx->aux.zetax, ctx->aux.zetaz, ctx->ckpt,
                      ctx->acc_ilum, ctx->maxv, ctx->ilum_tmp, ctx->th_pad, ctx->acc_multi,
-                     ctx->sbd, ctx->stage, ctx->ex_save, ctx->ex_work, ctx->ex_gacc};
+                     ctx->sbd, ctx->stage, ctx->ex_save, ctx->ex_work, ctx->ex_gacc, ctx->adj_S};
     for (float *b : bufs) dfree(b);
     dfree(ctx->save);
     dfree(ctx->d_acc);
@@ -818,10 +827,144 @@ extern "C" int pam_forward_step(pam_ctx *ctx, int k, float *seis) {
 
 extern "C" int pam_get_field(pam_ctx *ctx, int which, float *out) {
     if (!ctx || !out) return PAM_E_ARG;
-    if (ctx->next_k < 0) return fail(ctx, PAM_E_STATE, "no shot in progress");
-    const float *src = which == 0 ? ctx->cur : ctx->prv;
+    if (which < 0 || which > 3) return fail(ctx, PAM_E_ARG, "which must be 0 .. 3");
+    if (which < 2 && ctx->next_k < 0) return fail(ctx, PAM_E_STATE, "no shot in progress");
+    if (which >= 2 && ctx->adj_next_t < -1) return fail(ctx, PAM_E_STATE, "no adjoint shot in progress");
+    const float *src = which == 0 ? ctx->cur : which == 1 ? ctx->prv : which == 2 ? ctx->adj_cur : ctx->adj_prv;
     KCK(launch_pack_padded(src, ctx->nzp, ctx->nxp, ctx->P, out, ctx->st));
     HCK(hipStreamSynchronize(ctx->st));
+    return PAM_OK;
+}
+
+// ------------------------------------------------------------ per-step adjoint
+static int ensure_scratch(pam_ctx *ctx, void **p, long long *have, long long need);
+static int alloc_fieldset(pam_ctx *ctx, FieldSet &f);
+
+extern "C" int pam_adjoint_reset(pam_ctx *ctx, int shot, const float *data) {
+    if (!ctx) return PAM_E_ARG;
+    int r = ready(ctx);
+    if (r) return r;
+    if (shot < 0 || shot >= ctx->nshot) return fail(ctx, PAM_E_ARG, "shot %d out of range", shot);
+    if (!data) return fail(ctx, PAM_E_ARG, "data is NULL");
+    r = alloc_fieldset(ctx, ctx->aux);
+    if (r) return r;
+    const bool linear_adj = ctx->phys >= 1 && ctx->abc == PAM_CERJAN;
+    r = ensure_scratch(ctx, (void **)&ctx->adj_S, &ctx->adj_S_bytes, ctx->plane * 4);
+    if (r) return r;
+    if (linear_adj) {
+        r = ensure_scratch(ctx, (void **)&ctx->tmp, &ctx->tmp_bytes, 5 * ctx->plane * 4);
+        if (r) return r;
+    }
+    // the adjoint field cleared; the CPML memory cleared, or with
+    // PAM_OPT_RESET_ADJ_CPML 0 taken over from the forward per-step state as
+    // the reference's driver leaves it (src/Migration.py:1000-1039)
+    const bool keep = ctx->abc == PAM_CPML && !ctx->reset_adj_cpml;
+    r = zero_fields(ctx, ctx->aux, 1, true, !keep);
+    if (r) return r;
+    if (keep) {
+        if (ctx->next_k < 0) return fail(ctx, PAM_E_STATE, "PAM_OPT_RESET_ADJ_CPML 0 needs a forward per-step run");
+        const size_t fb = (size_t)ctx->plane * 4;
+        for (long long h : {0LL, ctx->psi_half}) {
+            HCK(hipMemcpyAsync(ctx->aux.psix + h, ctx->fs.psix + h, fb, hipMemcpyDeviceToDevice, ctx->st));
+            HCK(hipMemcpyAsync(ctx->aux.psiz + h, ctx->fs.psiz + h, fb, hipMemcpyDeviceToDevice, ctx->st));
+        }
+        HCK(hipMemcpyAsync(ctx->aux.zetax, ctx->fs.zetax, fb, hipMemcpyDeviceToDevice, ctx->st));
+        HCK(hipMemcpyAsync(ctx->aux.zetaz, ctx->fs.zetaz, fb, hipMemcpyDeviceToDevice, ctx->st));
+        ctx->aux.psi_flip = ctx->fs.psi_flip;
+    }
+    ctx->adj_data = data;
+    ctx->adj_cur = ctx->aux.U0;
+    ctx->adj_prv = ctx->aux.U1;
+    ctx->adj_next_t = -1;
+    HCK(hipStreamSynchronize(ctx->st));
+    return PAM_OK;
+}
+
+extern "C" int pam_adjoint_step(pam_ctx *ctx, int t, const float *S_t, float *image, float *ilum) {
+    if (!ctx) return PAM_E_ARG;
+    if (ctx->adj_next_t < -1) return fail(ctx, PAM_E_STATE, "pam_adjoint_reset must precede pam_adjoint_step");
+    const pam_grid &g = ctx->g;
+    if (t < 0 || t >= g.nt) return fail(ctx, PAM_E_ARG, "t=%d outside [0, nt=%d)", t, g.nt);
+    if (ctx->adj_next_t >= 0 && t != ctx->adj_next_t)
+        return fail(ctx, PAM_E_STATE, "adjoint steps must be issued in decreasing order: expected t=%d", ctx->adj_next_t);
+    const bool linear_adj = ctx->phys >= 1 && ctx->abc == PAM_CERJAN;
+    if (!S_t && (image || ilum || linear_adj))
+        return fail(ctx, PAM_E_ARG, "S_t is NULL (imaging and the linearised VTI/TTI adjoint read it)");
+    const long long nd = (long long)(g.nt - g.itlag) * ctx->nrec;
+    const long long nint = (long long)g.nz * g.nx;
+    // the first step: data row t into the adjoint field; each step then adds row
+    // t-1 to the field it writes (writer-side, as the whole-run adjoint pass does)
+    if (ctx->adj_next_t < 0 && t >= g.itlag)
+        KCK(launch_inject_rec(ctx->adj_cur, ctx->plane, ctx->P, ctx->d_rz, ctx->d_rx, ctx->nrec, ctx->adj_data, nd,
+                              t - g.itlag, g.dx * g.dz, 1, ctx->st));
+    if (S_t) {
+        KCK(launch_copy_padded(S_t, ctx->nzp, ctx->nxp, ctx->adj_S, ctx->P, ctx->st));
+        // the linearised adjoint reads the source field as the forward step
+        // saves it: damped once more (interior cells are unchanged)
+        if (linear_adj) KCK(launch_cerjan_scale(ctx->adj_S, ctx->nzp, ctx->nxp, ctx->P, ctx->cerx, ctx->cerz, ctx->st));
+        if (ilum) KCK(launch_ilum_acc(ctx->adj_S, ctx->P, g.n_abc, g.nz, g.nx, ilum, ctx->st));
+    }
+    if (!linear_adj) {
+        StepArgs b;
+        base_step_args(ctx, b, 1);
+        b.flags |= F_RECINJ | (image ? F_IMAGE : 0);
+        b.data = ctx->adj_data;
+        b.dstride = nd;
+        b.save_kind = SV_F32_PAD;
+        b.save = ctx->adj_S;
+        b.save_k0 = t;
+        b.save_kstride = b.save_sstride = ctx->plane;
+        b.image = image;
+        b.acc_sstride = nint;
+        const int r = fwd_step_fs(ctx, ctx->aux, b, choose_tile(ctx, 1, false), ctx->adj_cur, ctx->adj_prv, t, nullptr);
+        if (r) return r;
+    } else {
+        AdjArgs c;
+        std::memset(&c, 0, sizeof(c));
+        c.nzp = ctx->nzp;
+        c.nxp = ctx->nxp;
+        c.P = ctx->P;
+        c.N = g.n_abc;
+        c.nz = g.nz;
+        c.nx = g.nx;
+        c.nshots = 1;
+        c.sstride = ctx->plane;
+        c.idx2 = 1.0f / (g.dx * g.dx);
+        c.idz2 = 1.0f / (g.dz * g.dz);
+        c.idx = 1.0f / g.dx;
+        c.idz = 1.0f / g.dz;
+        c.idxdz = 1.0f / (g.dz * g.dx);
+        c.idt2 = 1.0f / (g.dt * g.dt);
+        c.dxdz = g.dx * g.dz;
+        c.k = t;
+        c.nt = g.nt;
+        c.itlag = g.itlag;
+        c.nrec = ctx->nrec;
+        c.cur = ctx->adj_cur;
+        c.prv = ctx->adj_prv;
+        c.P_ = ctx->adj_S;
+        c.psstride = c.pkstride = ctx->plane;
+        c.v2 = ctx->v2;
+        c.eps = ctx->eps;
+        c.del = ctx->del;
+        c.cs = ctx->cs;
+        c.sn = ctx->sn;
+        c.cerx = ctx->cerx;
+        c.cerz = ctx->cerz;
+        c.tmp = ctx->tmp;
+        c.tsstride = 5 * ctx->plane;
+        c.rrow = ctx->d_rrow;
+        c.rhead = ctx->d_rhead;
+        c.rnext = ctx->d_rnext;
+        c.data = ctx->adj_data;
+        c.dstride = nd;
+        c.image = image;
+        c.acc_sstride = nint;
+        KCK(launch_adj_products(ctx->phys, c, ctx->st));
+        KCK(launch_adj_update(ctx->phys, c, ctx->st));
+    }
+    std::swap(ctx->adj_cur, ctx->adj_prv);
+    ctx->adj_next_t = t - 1;
     return PAM_OK;
 }
 
@@ -1917,6 +2060,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                        float *g_eps = nullptr, float *g_del = nullptr, float *g_th = nullptr) {
     int r = ready(ctx);
     if (r) return r;
+    ctx->adj_next_t = -2;  // the second field set is reused
     if (nshots < 1) return fail(ctx, PAM_E_ARG, "nshots must be >= 1");
     if (!data || !image) return fail(ctx, PAM_E_ARG, "data and image must not be NULL");
     const pam_grid &g = ctx->g;
@@ -2626,6 +2770,7 @@ extern "C" int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const fl
     int r = ready(ctx);
     if (r) return r;
     if (ctx->phys != PAM_ACOUSTIC) return fail(ctx, PAM_E_UNSUPPORTED, "the exact adjoint covers acoustic physics");
+    ctx->adj_next_t = -2;  // the second field set is reused
     if (nshots < 1) return fail(ctx, PAM_E_ARG, "nshots must be >= 1");
     if (!dseis || !g_vp) return fail(ctx, PAM_E_ARG, "dseis and g_vp must not be NULL");
     if (layout != PAM_INTERIOR && layout != PAM_PADDED) return fail(ctx, PAM_E_ARG, "layout must be 0 or 1");

@@ -190,6 +190,10 @@ int launch_residual(const float *dobs, const float *dcal, float *res, long long 
 // the device address of the allocation-free entry points' scratch array on the
 // calling thread's current device (looked up once per device)
 int pam_noalloc_scratch(double **out);
+// U = (U * cerx[x]) * cerz[z] over a padded field of pitch P
+int launch_cerjan_scale(float *U, int nzp, int nxp, int P, const float *cerx, const float *cerz, hipStream_t st);
+// ilum [nz][nx] += S^2 on the interior of the padded field S (pitch P, absorbing width N)
+int launch_ilum_acc(const float *S, int P, int N, int nz, int nx, float *ilum, hipStream_t st);
 // *flag = 1 when any of the n floats of x is NaN or Inf (flag is not cleared)
 int launch_nonfinite(const float *x, long long n, int *flag, hipStream_t st);
 // multiparameter partials (src/CudaKernels.py:1174-1379) of nshots shots:

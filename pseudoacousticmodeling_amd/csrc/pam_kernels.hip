@@ -437,6 +437,24 @@ __global__ void k_sum_parts(const double *part, int nparts, double *out) {
     const double b = block_sum256(sum);
     if (threadIdx.x == 0) *out = b;
 }
+// U = (U * cerx[x]) * cerz[z] over a padded field (the product and its order
+// as the step kernels damp a field they save)
+__global__ void k_cerjan_scale(float *U, int nzp, int nxp, int P, const float *cerx, const float *cerz) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (x >= nxp) return;
+    float *u = U + (long long)z * P + x;
+    *u = (*u * cerx[x]) * cerz[z];
+}
+// ilum[iz][ix] += S^2 on the interior of a padded field of pitch P
+__global__ void k_ilum_acc(const float *S, int P, int N, int nz, int nx, float *ilum) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iz = blockIdx.y;
+    if (ix >= nx) return;
+    const float v = S[(long long)(iz + N) * P + ix + N];
+    float *acc = ilum + (long long)iz * nx + ix;
+    *acc += v * v;
+}
 // *flag = 1 if any value is NaN or Inf
 __global__ void k_nonfinite(const float *x, long long n, int *flag) {
     bool bad = false;
@@ -670,6 +688,14 @@ int launch_residual(const float *dobs, const float *dcal, float *res, long long 
     const int blocks = misfit_parts(n);
     hipLaunchKernelGGL(k_residual, dim3((unsigned)blocks), dim3(256), 0, st, dobs, dcal, res, n, part);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, (const double *)part, blocks, out);
+    return (int)hipGetLastError();
+}
+int launch_cerjan_scale(float *U, int nzp, int nxp, int P, const float *cerx, const float *cerz, hipStream_t st) {
+    hipLaunchKernelGGL(k_cerjan_scale, dim3((nxp + 255) / 256, nzp), dim3(256), 0, st, U, nzp, nxp, P, cerx, cerz);
+    return (int)hipGetLastError();
+}
+int launch_ilum_acc(const float *S, int P, int N, int nz, int nx, float *ilum, hipStream_t st) {
+    hipLaunchKernelGGL(k_ilum_acc, dim3((nx + 255) / 256, nz), dim3(256), 0, st, S, P, N, nz, nx, ilum);
     return (int)hipGetLastError();
 }
 int launch_nonfinite(const float *x, long long n, int *flag, hipStream_t st) {

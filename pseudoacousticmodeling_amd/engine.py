@@ -115,6 +115,17 @@ class Engine:
         self._ck(self.lib.pam_get_field(self.ctx, int(which), _ptr(out)))
         return out
 
+    def adjoint_reset(self, shot, data):
+        """Per-step adjoint of one shot (Migration::backward_step): ``data`` is
+        its [nt_data][nrec] record (kept alive until the next reset)."""
+        self._adj_data = _f32(data, self.device)
+        self._ck(self.lib.pam_adjoint_reset(self.ctx, int(shot), _ptr(self._adj_data)))
+
+    def adjoint_step(self, t, S_t=None, image=None, ilum=None):
+        """One adjoint step t: image += S_t lambda_t, ilum += S_t^2 (device
+        [nz][nx] tensors, accumulated in place when given)."""
+        self._ck(self.lib.pam_adjoint_step(self.ctx, int(t), _ptr(S_t), _ptr(image), _ptr(ilum)))
+
     # -- whole runs ------------------------------------------------------------
     def forward(self, shot0=0, nshots=None, out=None):
         nshots = self.nshot - shot0 if nshots is None else nshots

@@ -145,6 +145,30 @@ class migration:
             mute(d, p.rec_x, p.rec_z, p.shot_x[: p.Nshot], p.shot_z[: p.Nshot], p.dt, p.shift, p.window, p.v0)
         return d
 
+    # -- per-step adjoint (src/Migration.py:237-286) ---------------------------
+    def reset_field(self, shot=0, data=None):
+        """Clears the adjoint field of ``shot`` and binds its record ``data``
+        ([nt_data][Nrec], host or device; the muted gather for RTM, the
+        residual for FWI), with zeroed device image / illumination
+        accumulators (``image_dev``, ``ilum_dev``) for backward_stepGPU."""
+        import torch
+
+        eng = self.wf._eng
+        d = self.loadSeismogram(shot) if data is None else data
+        eng.adjoint_reset(shot, d)
+        self.image_dev = torch.zeros((eng.nz, eng.nx), dtype=torch.float32, device=eng.device)
+        self.ilum_dev = torch.zeros_like(self.image_dev)
+
+    def backward_stepGPU(self, k, P=None):
+        """One adjoint step k (pam_adjoint_step): data row k injected at the
+        receivers, the adjoint update (the linearised VTI/TTI adjoint reads P,
+        the padded source field S_k on the device), the absorbing boundary,
+        and with P the imaging image += P lambda_k, ilum += P^2."""
+        self.wf._eng.adjoint_step(k, P, self.image_dev if P is not None else None,
+                                  self.ilum_dev if P is not None else None)
+
+    backward_step = backward_stepGPU
+
     def SolveBackwardWaveEquation(self, data=None):
         """RTM image (fwi false) or FWI gradient (fwi true) of every shot.
         ``data`` (host or device [Nshot][nt_data][Nrec]) replaces the file
