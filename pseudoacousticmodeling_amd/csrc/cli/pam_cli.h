@@ -19,6 +19,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <limits>
 #include <vector>
 
 #include "pam.h"
@@ -291,6 +292,93 @@ inline std::vector<float> ricker(const Survey &s) {
         w[n] = (1.0f - 2.0f * arg) * std::exp(-arg) * scale;
     }
     return w;
+}
+
+// Dispersion and stability check of the C++ driver (cpp/Modeling.cu:184-327),
+// float32: acoustic lambda_min / 4.28 and dx_lim / (4 vmax); VTI / TTI the dt
+// limit sqrt(2) / (sqrt(S) max(vp sqrt(A))) with S the 8th-order stencil sum
+// and A the squared slowness factor at the Nyquist wavenumbers (Sd gated at
+// |den| <= 1e-12), and the qP phase velocity over 181 angles in [0, pi/2] (VTI)
+// or 361 in [0, pi] relative to the tilt (TTI, theta in radians).  Prints the
+// driver's report; the return value says whether dx, dz and dt pass.
+struct Stability {
+    float vmin = 0, vmax = 0, dx_lim = 0, dt_lim = 0;
+    bool ok = false;
+};
+inline float sd_unit(float e, float d, float s2, float c2) {
+    const float num = -2.0f * (e - d) * s2 * c2;
+    const float den = (1.0f + 2.0f * e) * (s2 * s2) + (c2 * c2) + 2.0f * (1.0f + d) * s2 * c2;
+    return std::fabs(den) > 1e-12f ? num / den : 0.0f;
+}
+inline Stability check_dispersion_stability(const Survey &s, const std::vector<float> &vp,
+                                            const std::vector<float> &eps, const std::vector<float> &del,
+                                            const std::vector<float> &th) {
+    Stability r;
+    float vmin = vp[0], vmax = vp[0];
+    for (float v : vp) {
+        vmin = std::min(vmin, v);
+        vmax = std::max(vmax, v);
+    }
+    const bool aniso = s.approximation == "VTI" || s.approximation == "TTI";
+    float emax = 0.0f, dmax = 0.0f;
+    if (!aniso) {
+        r.dx_lim = vmin / s.fcut / 4.28f;
+        r.dt_lim = r.dx_lim / (4.0f * vmax);
+    } else {
+        const float c[4] = {8.0f / 5.0f, -1.0f / 5.0f, 8.0f / 315.0f, -1.0f / 560.0f};
+        float S = 0.0f;
+        for (int m = 1; m <= 4; ++m) S += c[m - 1] * (m % 2 ? 2.0f : 0.0f);
+        const float idx2 = (1.0f / s.dx) * (1.0f / s.dx), idz2 = (1.0f / s.dz) * (1.0f / s.dz);
+        float smax = 0.0f;
+        for (size_t i = 0; i < vp.size(); ++i) {
+            emax = std::max(emax, eps[i]);
+            dmax = std::max(dmax, del[i]);
+            const float Sk = sd_unit(eps[i], del[i], idx2, idz2);
+            const float A = ((1.0f + 2.0f * eps[i]) + Sk) * idx2 + (1.0f + Sk) * idz2;
+            smax = std::max(smax, vp[i] * std::sqrt(A));
+        }
+        r.dt_lim = std::sqrt(2.0f) / (std::sqrt(S) * smax);
+        const bool tti = s.approximation == "TTI";
+        const int nang = tti ? 361 : 181;
+        const float amax = tti ? kPi : 0.5f * kPi;
+        vmin = std::numeric_limits<float>::infinity();
+        vmax = 0.0f;
+        for (int ia = 0; ia < nang; ++ia) {
+            const float phi = amax * (float)ia / (float)(nang - 1);
+            for (size_t i = 0; i < vp.size(); ++i) {
+                const float b = tti ? phi - th[i] : phi;
+                const float sn = std::sin(b), cs = std::cos(b);
+                const float s2 = sn * sn, c2 = cs * cs;
+                const float v = vp[i] * std::sqrt((1.0f + 2.0f * eps[i]) * s2 + c2 + sd_unit(eps[i], del[i], s2, c2));
+                vmin = std::min(vmin, v);
+                vmax = std::max(vmax, v);
+            }
+        }
+        r.dx_lim = vmin / s.fcut / 4.28f;
+    }
+    r.vmin = vmin;
+    r.vmax = vmax;
+    r.ok = s.dx <= r.dx_lim && s.dz <= r.dx_lim && s.dt <= r.dt_lim;
+    std::cout << std::setprecision(9);
+    if (aniso) {
+        std::cout << "info: Maximum epsilon: " << emax << "\n";
+        std::cout << "info: Maximum delta: " << dmax << "\n";
+    }
+    std::cout << "info: Dispersion and stability check\n"
+              << "info: Minimum velocity: " << vmin << " m/s\n"
+              << "info: Maximum velocity: " << vmax << " m/s\n"
+              << "info: Maximum frequency: " << s.fcut << " Hz\n"
+              << "info: Current dx: " << s.dx << " m\n"
+              << "info: Current dz: " << s.dz << " m\n"
+              << "info: Current dt: " << s.dt << " s\n"
+              << "info: Critical dx for dispersion: " << r.dx_lim << " m\n"
+              << "info: Critical dz for dispersion: " << r.dx_lim << " m\n"
+              << "info: Critical dt for stability: " << r.dt_lim << " s\n"
+              << (r.ok ? "info: Dispersion and stability conditions satisfied."
+                       : "WARNING: Dispersion or stability conditions not satisfied.")
+              << std::endl;
+    std::cout << std::setprecision(6);
+    return r;
 }
 
 // water mask |f - min f| < 1e-3 (cpp/Migration.cu:131-145)

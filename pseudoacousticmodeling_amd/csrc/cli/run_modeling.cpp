@@ -60,8 +60,10 @@ int main(int argc, char **argv) {
             for (int i = 0; i < s.Nrec; ++i) std::cout << "rec " << s.rz[i] << " " << s.rx[i] << "\n";
             return 0;
         }
-        std::cout << "info: Solving " << s.approximation << " wave equation, " << s.Nshot << " shots, " << s.nx
-                  << "x" << s.nz << ", nt=" << s.nt << std::endl;
+        const bool stability_only = argc > 2 && std::string(argv[2]) == "--stability";
+        if (!stability_only)
+            std::cout << "info: Solving " << s.approximation << " wave equation, " << s.Nshot << " shots, " << s.nx
+                      << "x" << s.nz << ", nt=" << s.nt << std::endl;
         const size_t nm = (size_t)s.nx * s.nz;
         std::vector<float> vp = cli::read_bin(s.vpFile, nm), eps, del, th;
         if (s.approximation != "acoustic") {
@@ -72,6 +74,9 @@ int main(int argc, char **argv) {
             th = cli::read_bin(s.thetaFile, nm);
             for (float &v : th) v = v * cli::kPi / 180.0f;  // degrees on disk (cpp/Modeling.cu:364-367)
         }
+        // the reference checks as it sets the model (cpp/Modeling.cu:371); --stability: the check alone
+        cli::check_dispersion_stability(s, vp, eps, del, th);
+        if (stability_only) return 0;
         pam_ctx *ctx = cli::make_context(s);
         float *d_vp = cli::to_device(vp);
         float *d_eps = eps.empty() ? nullptr : cli::to_device(eps);

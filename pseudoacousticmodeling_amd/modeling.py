@@ -89,15 +89,30 @@ class wavefield:
         return prep.cerjan_vector(self.pmt.N_abc, getattr(self.pmt, "cerjan_factor", 0.0))
 
     def checkDispersionAndStability(self):
-        """Acoustic dispersion / stability limits (src/Modeling2D.py:202-223);
-        for VTI / TTI the horizontal velocity vp sqrt(1+2 eps) bounds dt."""
+        """Dispersion / stability limits (src/Modeling2D.py:202-256): the
+        acoustic rule, or for VTI / TTI the qP phase-velocity scan and the
+        8th-order dt limit with the Sd term (prep.stability_anisotropic)."""
         p = self.pmt
-        vmax = float(np.max(self.vp))
         if p.approximation in ("VTI", "TTI"):
-            vmax = float(np.max(self.vp * np.sqrt(1.0 + 2.0 * self.epsilon)))
-        s = prep.stability_acoustic(np.array([np.min(self.vp), vmax]), p.fcut, p.dx, p.dz, p.dt)
-        print(f"info: dx_lim = {s['dx_lim']:.3f} m, dt_lim = {s['dt_lim'] * 1e3:.4f} ms")
-        if not s["ok"]:
+            s = prep.stability_anisotropic(self.vp, self.epsilon, self.delta, p.fcut, p.dx, p.dz, p.dt,
+                                           theta=self.theta if p.approximation == "TTI" else None)
+            print("info: Dispersion and stability check")
+            print(f"info: Minimum qP velocity: {s['vqp_min']:.2f} m/s")
+            print(f"info: Maximum qP velocity: {s['vqp_max']:.2f} m/s")
+            print(f"info: Maximum epsilon: {s['eps_max']:.4f}")
+            print(f"info: Maximum delta: {s['delta_max']:.4f}")
+        else:
+            s = prep.stability_acoustic(self.vp, p.fcut, p.dx, p.dz, p.dt)
+            print("info: Dispersion and stability check")
+            print(f"info: Minimum velocity: {float(np.min(self.vp)):.2f} m/s")
+            print(f"info: Maximum velocity: {float(np.max(self.vp)):.2f} m/s")
+        print(f"info: Maximum frequency: {p.fcut:.2f} Hz")
+        print(f"info: Critical dx for dispersion: {s['dx_lim']:.2f} m")
+        print(f"info: Critical dz for dispersion: {s['dz_lim']:.2f} m")
+        print(f"info: Critical dt for stability: {s['dt_lim']:.5f} s")
+        if s["ok"]:
+            print("info: Dispersion and stability conditions satisfied.")
+        else:
             print("WARNING: Dispersion or stability conditions not satisfied.")
         return s
 

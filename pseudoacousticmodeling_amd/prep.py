@@ -12,7 +12,9 @@ These run once per run on the host (numpy) and feed the C ABI:
 * ``dampening_const``    -- src/Modeling2D.py:110-126 (CPML d0 and f_pico);
 * ``grid_index``         -- source/receiver cell indices, cpp/Survey.cpp:150-158
   (``round``) and src/survey.py:131-138 (``trunc``);
-* ``stability_acoustic`` -- cpp/Modeling.cu:287-292 / src/Modeling2D.py:202-223.
+* ``stability_acoustic`` -- cpp/Modeling.cu:287-292 / src/Modeling2D.py:202-223;
+* ``stability_anisotropic`` -- the VTI / TTI rule, cpp/Modeling.cu:184-327 /
+  src/Modeling2D.py:128-256 (qP phase-velocity scan, 8th-order dt limit).
 """
 from __future__ import annotations
 
@@ -113,6 +115,74 @@ def stability_acoustic(vp, fcut, dx, dz, dt):
     dx_lim = lam / 4.28
     dt_lim = dx_lim / (4.0 * float(np.max(vp)))
     return dict(dx_lim=dx_lim, dz_lim=dx_lim, dt_lim=dt_lim,
+                ok=bool(dx <= dx_lim and dz <= dx_lim and dt <= dt_lim))
+
+
+# sum over the 8th-order second-derivative coefficients c_m (1 - (-1)^m), the
+# stencil's gain at the Nyquist wavenumber (src/Modeling2D.py:144-149,
+# cpp/Modeling.cu:203-208)
+_COEFFS_8TH = (8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0)
+
+
+def stability_sum(coeffs=_COEFFS_8TH):
+    return float(sum(c * (1.0 - (-1.0) ** m) for m, c in enumerate(coeffs, start=1)))
+
+
+def _sd_unit(eps, dl, s2, c2, gate):
+    """Sd on a unit plane wave with sin^2 = s2, cos^2 = c2 (relative to the
+    symmetry axis): -2 (eps - delta) s2 c2 / ((1 + 2 eps) s2^2 + c2^2 +
+    2 (1 + delta) s2 c2); with `gate` 0 where |den| <= 1e-12 (the C++ rule)."""
+    num = -2.0 * (eps - dl) * s2 * c2
+    den = (1.0 + 2.0 * eps) * (s2 * s2) + (c2 * c2) + 2.0 * (1.0 + dl) * s2 * c2
+    if gate:
+        return np.where(np.abs(den) > 1e-12, num / np.where(np.abs(den) > 1e-12, den, 1.0), 0.0).astype(num.dtype)
+    return num / den
+
+
+def stability_anisotropic(vp, eps, delta, fcut, dx, dz, dt, theta=None, rule="py"):
+    """Dispersion and stability limits of the pseudo-acoustic VTI / TTI step:
+    src/Modeling2D.py:128-256 (``rule='py'``, float64) and
+    cpp/Modeling.cu:184-327 (``rule='cpp'``, float32, Sd gated at |den| <= 1e-12).
+
+    * dt_lim = sqrt(2) / (sqrt(S) max(vp sqrt(A))), S = sum c_m (1 - (-1)^m) of the
+      8th-order coefficients, A = (1 + 2 eps + Sk) / dx^2 + (1 + Sk) / dz^2 with
+      Sk the Sd term at the grid's Nyquist wavenumbers (1/dx, 1/dz);
+    * the qP phase velocity vp sqrt((1 + 2 eps) sin^2 + cos^2 + Sd) over 181
+      angles in [0, pi/2] (VTI) or 361 angles phi in [0, pi] relative to the tilt,
+      beta = phi - theta (TTI, theta in radians); its minimum gives
+      dx_lim = dz_lim = vqp_min / fcut / 4.28.
+    Returns vqp_min, vqp_max, eps_max, delta_max, dx_lim, dz_lim, dt_lim, ok."""
+    f = np.float64 if rule == "py" else np.float32
+    gate = rule == "cpp"
+    vp = np.asarray(vp, dtype=f).ravel()
+    eps = np.asarray(eps, dtype=f).ravel()
+    dl = np.asarray(delta, dtype=f).ravel()
+    th = None if theta is None else np.asarray(theta, dtype=f).ravel()
+    one, two = f(1.0), f(2.0)
+    idx2 = (one / f(dx)) * (one / f(dx))
+    idz2 = (one / f(dz)) * (one / f(dz))
+    # Sk at the Nyquist wavenumbers: the unit-wave formula with s2, c2 -> 1/dx^2, 1/dz^2
+    Sk = _sd_unit(eps, dl, idx2, idz2, gate).astype(f)
+    A = ((one + two * eps) + Sk) * idx2 + (one + Sk) * idz2
+    soma = f(stability_sum())
+    dt_lim = float(f(np.sqrt(f(2.0))) / (np.sqrt(soma) * np.max(vp * np.sqrt(A))))
+    if th is None:
+        angles = np.linspace(0.0, 0.5 * np.pi, 181)
+    else:
+        angles = np.linspace(0.0, np.pi, 361)
+    vmin, vmax = np.inf, 0.0
+    for phi in angles:
+        beta = f(phi) if th is None else f(phi) - th
+        sn, cs = np.sin(beta), np.cos(beta)
+        s2, c2 = sn * sn, cs * cs
+        factor = (one + two * eps) * s2 + c2 + _sd_unit(eps, dl, s2, c2, gate).astype(f)
+        v = vp * np.sqrt(factor)
+        vmin = min(vmin, float(np.min(v)))
+        vmax = max(vmax, float(np.max(v)))
+    lam = vmin / fcut
+    dx_lim = lam / 4.28
+    return dict(vqp_min=vmin, vqp_max=vmax, eps_max=float(np.max(eps)), delta_max=float(np.max(dl)),
+                dx_lim=dx_lim, dz_lim=dx_lim, dt_lim=dt_lim,
                 ok=bool(dx <= dx_lim and dz <= dx_lim and dt <= dt_lim))
 
 
