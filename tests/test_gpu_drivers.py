@@ -154,6 +154,43 @@ def test_fwi_iteration_lowers_the_misfit(tmp_path):
     np.testing.assert_allclose(v[:4], inv.vp0[:4], rtol=1e-5)
 
 
+@pytest.mark.parametrize("schedule", ["simultaneous", "hierarchical"])
+def test_multiparameter_fwi_lowers_the_misfit(tmp_path, schedule):
+    """Two iterations of a multiparameter schedule on a VTI + CPML survey
+    (src/Inversion.py:413-940): the misfit falls, the epsilon / delta
+    gradients and models of the released parameters are written, and every
+    model stays in its range."""
+    print(f"case multiparameter fwi {schedule}", flush=True)
+    from pseudoacousticmodeling_amd.inversion import fwi
+    from pseudoacousticmodeling_amd.migration import migration
+
+    path, models = write_survey(str(tmp_path), 1, 1, nshots=3, seed=41, water_rows=4,
+                                extra={"niter": 2, "sigma": 3.0, "multiparameter": True})
+    p, wf = _facade(path)
+    wf.SolveWaveEquation()  # observed data of the true models
+    mig = migration(wf, p)
+    mig.initializeMigrationfields()
+    hist = str(tmp_path / "outputs" / "history.txt")
+    inv = fwi(p, wf, mig, history_file=hist)
+    if schedule == "simultaneous":
+        out = inv.solveFullWaveformInversion()
+    else:
+        out = inv.solveFullWaveformInversionMultiparameterHierarchical()
+    h = np.loadtxt(hist)
+    print(f"  history {h[:, 0].tolist()}", flush=True)
+    assert h.shape == (3, 2) and h[-1, 0] < h[0, 0]
+    for n in ("vp", "epsilon", "delta"):
+        assert os.path.exists(f"{p.estimatedmodelsFolder}fwi_{n}_VTI_Nx{p.nx}_Nz{p.nz}_itr2_freq30.0.bin")
+    # epsilon is active from the first of two iterations in both schedules (hierarchical: progress 0.5)
+    assert os.path.exists(f"{p.gradientsFolder}epsilon_gradient_fwi_iter_1_VTI_Nx{p.nx}_Nz{p.nz}_freq30.0.bin")
+    delta_first = os.path.exists(f"{p.gradientsFolder}delta_gradient_fwi_iter_1_VTI_Nx{p.nx}_Nz{p.nz}_freq30.0.bin")
+    assert delta_first == (schedule == "simultaneous")  # hierarchical: delta from progress 0.7
+    b = inv._bounds()
+    for n, m in out.items():
+        assert np.isfinite(m).all()
+        assert m.min() >= np.float32(b[n][0]) and m.max() <= np.float32(b[n][1])
+
+
 def _run(exe, path):
     binary = os.path.join(BIN, exe)
     assert os.path.exists(binary), "CLI not built (python -m pseudoacousticmodeling_amd.build)"
