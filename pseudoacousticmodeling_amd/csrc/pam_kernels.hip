@@ -94,8 +94,19 @@ static __host__ __device__ inline void psi_threads(int nzp, int nxp, int N, long
     *n = *nx_threads + 2LL * (*nsegz) * (nxp - 8);
 }
 
-__global__ __launch_bounds__(256) void k_psi(const PsiArgs a, long long nx_threads, int nsegz) {
-    psi_body(a, (long long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, nx_threads, nsegz);
+// One launch for every shot.  Blocks go to the XCDs round-robin (block h to
+// XCD h mod 8); block h = x + 8 (s + nshots q) runs segment block g = 8 q + x
+// of shot s, so the segment blocks are spread over the XCDs as they come
+// (the x-strip cell blocks and the heavier z-strip column blocks alike), while
+// the shots of one segment block -- which read the same CPML coefficients --
+// run back to back on one XCD and share its L2.  bps8 = bps rounded up to a
+// multiple of 8; blocks past bps leave.
+__global__ __launch_bounds__(256) void k_psi(const PsiArgs a, long long nx_threads, int nsegz, int bps) {
+    const int h = blockIdx.x;
+    const int q = (h >> 3) / a.nshots, s = (h >> 3) - q * a.nshots;
+    const int g = q * 8 + (h & 7);
+    if (g >= bps) return;  // block-uniform
+    psi_body(a, (long long)g * 256 + threadIdx.x, s, nx_threads, nsegz);
 }
 
 // The psi pass of a group of grids in one launch: block b belongs to grid g
@@ -586,7 +597,9 @@ int launch_psi(const PsiArgs &a, hipStream_t st) {
     long long nx_threads, n;  // the z-columns start on a block boundary
     int nsegz;
     psi_threads(a.nzp, a.nxp, a.N, &nx_threads, &nsegz, &n);
-    hipLaunchKernelGGL(k_psi, dim3((unsigned)((n + 255) / 256), a.nshots), dim3(256), 0, st, a, nx_threads, nsegz);
+    const int bps = (int)((n + 255) / 256);
+    const long long blocks = (long long)(bps + 7) / 8 * 8 * a.nshots;
+    hipLaunchKernelGGL(k_psi, dim3((unsigned)blocks), dim3(256), 0, st, a, nx_threads, nsegz, bps);
     return (int)hipGetLastError();
 }
 int psi_blocks_per_shot(int nzp, int nxp, int N) {
