@@ -298,7 +298,7 @@ def main(argv=None):
     eng.set_option(_lib.PAM_OPT_PROFILE, 0)
     kern_ms, launches = t["kernel_ms"], t["kernel_launches"]
     avg_ms = kern_ms / max(launches, 1)
-    fused = variant >= 4  # the k_march variants update psi in the step kernel
+    fused = variant in (4, 5, 6)  # the k_march variants update psi in the step kernel
     bytes_launch = step_kernel_bytes(w, per_launch, fused)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     print(f"bench: {secs:.3f} s for {args.steps} runs, {per_launch} shots per launch, step kernel variant {variant} avg {avg_ms * 1e3:.1f} us over {launches} launches, "

@@ -129,6 +129,9 @@ struct AdjArgs {
     long long dsstride;
 };
 
+// V > 0: k_step with V cells per lane and TZ rows per wavefront; V == 0: k_tile
+// with TZ rows per block; V == -1: k_march with TZ chunks; V == -2: k_tile_pipe
+// with interior tiles in groups of TZ shots
 struct TileShape {
     int V, TZ;
 };

@@ -556,7 +556,7 @@ TileShape tile_shape(int idx) {
     if (idx < 0 || idx >= kNumTiles) idx = 0;
     return kTiles[idx];
 }
-bool tile_fuses_psi(int idx) { return tile_shape(idx).V < 0; }
+bool tile_fuses_psi(int idx) { return tile_shape(idx).V == -1; }
 
 int launch_step_acoustic_cerjan(int, const StepArgs &, hipStream_t, const hipEvent_t *);
 int launch_step_acoustic_cpml(int, const StepArgs &, hipStream_t, const hipEvent_t *);
@@ -582,7 +582,7 @@ int launch_step_multi(int phys, bool cpml, int tile, const StepArgs *list, const
     return cpml ? launch_multi_tti_cpml(tile, list, boff, ngrid, nblocks, k, pboff, npblocks, st)
                 : launch_multi_tti_cerjan(tile, list, boff, ngrid, nblocks, k, pboff, npblocks, st);
 }
-int march_chunks(int tile) { return tile_shape(tile).V < 0 ? tile_shape(tile).TZ : 0; }
+int march_chunks(int tile) { return tile_shape(tile).V == -1 ? tile_shape(tile).TZ : 0; }
 bool tile_has_multi(int tile) { return tile == 0 || march_chunks(tile) > 0; }
 
 int launch_step(int phys, bool cpml, int tile, const StepArgs &a, hipStream_t st, const hipEvent_t *ev) {

@@ -65,6 +65,34 @@ __device__ __forceinline__ bool tile_inner(const StepArgs &a, int x0, int z0) {
     return x0 >= m && x0 + 64 <= a.nxp - m && z0 >= m && z0 + TZ <= a.nzp - m;
 }
 
+// The derivatives of one row of 4 cells: zw[0..8] the 9-row z-window of U_k
+// at the cells' columns (the row itself at zw[4]), d2x / d1x the undivided x
+// derivatives; c0 = U_k, pzz / pxx second and pz / px first derivatives
+// (pz / px only for VTI / TTI, else 0).  Shared by the tile bodies, so that
+// every instance rounds the same way.
+template <int PHYS>
+__device__ __forceinline__ void tile_derivs(const float (*zw)[4], const float *d2x, const float *d1x,
+                                            const StepArgs &a, float *c0, float *pzz, float *pxx, float *pz,
+                                            float *px) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float cc = zw[4][j];
+        const float u1p = zw[5][j], u1m = zw[3][j];
+        const float u2p = zw[6][j], u2m = zw[2][j];
+        const float u3p = zw[7][j], u3m = zw[1][j];
+        const float u4p = zw[8][j], u4m = zw[0][j];
+        c0[j] = cc;
+        pzz[j] = (KC0 * cc + KC1 * (u1p + u1m) + KC2 * (u2p + u2m) + KC3 * (u3p + u3m) + KC4 * (u4p + u4m)) * a.idz2;
+        pxx[j] = d2x[j] * a.idx2;
+        px[j] = 0.f;
+        pz[j] = 0.f;
+        if (PHYS >= 1) {
+            pz[j] = (KA1 * (u1p - u1m) + KA2 * (u2p - u2m) + KA3 * (u3p - u3m) + KA4 * (u4p - u4m)) * a.idz;
+            px[j] = d1x[j] * a.idx;
+        }
+    }
+}
+
 // One block's work: block b (in [0, nxt * nzt * nshots)) of the launch
 // described by a and the step arguments m.  INNER: the tile satisfies
 // tile_inner, so the per-cell edge tests and the absorbing-layer terms drop
@@ -72,7 +100,17 @@ __device__ __forceinline__ bool tile_inner(const StepArgs &a, int x0, int z0) {
 // the flags this instance implements (the step's flags are masked by it, so
 // the code of the others drops out).
 template <int PHYS, bool CPML, int TZ, int FM, bool INNER = false>
+__device__ __forceinline__ void tile_body_at(const StepArgs &a, const int x0, const int z0, const int s,
+                                             const TileStep m);
+template <int PHYS, bool CPML, int TZ, int FM, bool INNER = false>
 __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const TileStep m) {
+    int x0, z0, s;
+    tile_of<TZ>(a, b, x0, z0, s);
+    tile_body_at<PHYS, CPML, TZ, FM, INNER>(a, x0, z0, s, m);
+}
+template <int PHYS, bool CPML, int TZ, int FM, bool INNER>
+__device__ __forceinline__ void tile_body_at(const StepArgs &a, const int x0, const int z0, const int s,
+                                             const TileStep m) {
     constexpr int TX = 64;
     constexpr int R = TZ / 16;        // rows per thread
     constexpr int ROWS = TZ + 8;      // staged rows (4-row halo each side)
@@ -81,8 +119,6 @@ __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const 
     __shared__ float4 su[ROWS * Q];
     __shared__ float4 sd[(PHYS == 2) ? ROWS * (TX / 4) : 1];
 
-    int x0, z0, s;
-    tile_of<TZ>(a, b, x0, z0, s);
     const int P = a.P, nzp = a.nzp, nxp = a.nxp, N = a.N;
     const long long so = (long long)s * a.sstride;
     const float *__restrict__ cur = m.cur + so;
@@ -193,25 +229,9 @@ __device__ __forceinline__ void tile_body(const StepArgs &a, const int b, const 
         xderivs<4>(w, d2x, d1x);
 
         float c0[4], pzz[4], pxx[4], px[4], pz[4], pxz[4];
+        tile_derivs<PHYS>(zw + r, d2x, d1x, a, c0, pzz, pxx, pz, px);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const float cc = zw[r + 4][j];
-            const float u1p = zw[r + 5][j], u1m = zw[r + 3][j];
-            const float u2p = zw[r + 6][j], u2m = zw[r + 2][j];
-            const float u3p = zw[r + 7][j], u3m = zw[r + 1][j];
-            const float u4p = zw[r + 8][j], u4m = zw[r + 0][j];
-            c0[j] = cc;
-            pzz[j] = (KC0 * cc + KC1 * (u1p + u1m) + KC2 * (u2p + u2m) + KC3 * (u3p + u3m) + KC4 * (u4p + u4m)) *
-                     a.idz2;
-            pxx[j] = d2x[j] * a.idx2;
-            px[j] = 0.f;
-            pz[j] = 0.f;
-            pxz[j] = 0.f;
-            if (PHYS >= 1) {
-                pz[j] = (KA1 * (u1p - u1m) + KA2 * (u2p - u2m) + KA3 * (u3p - u3m) + KA4 * (u4p - u4m)) * a.idz;
-                px[j] = d1x[j] * a.idx;
-            }
-        }
+        for (int j = 0; j < 4; ++j) pxz[j] = 0.f;
         if (PHYS == 2) {
             float q[9][4];
 #pragma unroll
@@ -435,6 +455,295 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(((FM == F_F
     tile_body<PHYS, CPML, TZ, FM>(a, b, m);
 }
 
+// ------------------------------------------------ the pipelined 16-row tile
+// Interior tiles (no rim, no absorbing layer) of a launch are taken by blocks
+// that advance one tile for a group of SG shots in turn, the shots of the
+// group being the tile's consecutive shots.  The model coefficients of the
+// tile are loaded once for the group.  While shot i is computed, the U_k tile
+// of shot i+1 streams from HBM straight into the other of two LDS buffers
+// (gfx950 global_load_lds_dwordx4: no registers hold it in flight) and its
+// U_{k-1} into registers; shot i's result is stored after that prefetch is
+// issued, so the one wait per shot (all memory operations of the wave, then
+// the block barrier) only waits on operations issued before the shot's
+// arithmetic.  The other tiles run the general body, one (tile, shot) per
+// block, in the same launch.  Each cell is computed by the same arithmetic as
+// in k_tile, so the results are those of variant 0 bit for bit.
+typedef __attribute__((address_space(3))) void lds_void;
+
+// the interior-tile rectangle [ix0, ix1) x [iz0, iz1) of the 16-row tile grid
+// (tile_inner<16> holds exactly there), and the edge tiles around it
+struct TileSplit {
+    int ix0, ix1, iz0, iz1, nin, nedge;
+};
+__host__ __device__ inline TileSplit tile_split(int nxp, int nzp, int N, int nxt, int nzt) {
+    const int m = N > 4 ? N : 4;
+    TileSplit t;
+    t.ix0 = (m + 63) / 64;
+    t.ix1 = (nxp - m) / 64;
+    t.iz0 = (m + 15) / 16;
+    t.iz1 = (nzp - m) / 16;
+    if (t.ix1 < t.ix0) t.ix1 = t.ix0;
+    if (t.iz1 < t.iz0) t.iz1 = t.iz0;
+    t.nin = (t.ix1 - t.ix0) * (t.iz1 - t.iz0);
+    t.nedge = nxt * nzt - t.nin;
+    return t;
+}
+// interior tile i (bands of 8 tile columns, x fastest within a band, as tile_of)
+__device__ __forceinline__ void inner_tile_xz(const TileSplit &t, int i, int &x0, int &z0) {
+    constexpr int kBand = 8;
+    const int nw = t.ix1 - t.ix0, nh = t.iz1 - t.iz0;
+    const int band = i / (kBand * nh);
+    const int rem = i - band * (kBand * nh);
+    const int wb = min(kBand, nw - band * kBand);
+    const int tzi = rem / wb;
+    x0 = (t.ix0 + band * kBand + (rem - tzi * wb)) * 64;
+    z0 = (t.iz0 + tzi) * 16;
+}
+// edge tile i: the rows above the rectangle, the rows below it, then the
+// left and right parts of the rows beside it
+__device__ __forceinline__ void edge_tile_xz(const TileSplit &t, int nxt, int nzt, int i, int &x0, int &z0) {
+    const int top = t.iz0 * nxt, bottom = (nzt - t.iz1) * nxt;
+    int txi, tzi;
+    if (i < top) {
+        tzi = i / nxt;
+        txi = i - tzi * nxt;
+    } else if (i < top + bottom) {
+        const int e = i - top;
+        tzi = t.iz1 + e / nxt;
+        txi = e - (e / nxt) * nxt;
+    } else {
+        const int e = i - top - bottom;
+        const int w = t.ix0 + nxt - t.ix1;
+        const int r = e / w, c = e - r * w;
+        tzi = t.iz0 + r;
+        txi = c < t.ix0 ? c : t.ix1 + (c - t.ix0);
+    }
+    x0 = txi * 64;
+    z0 = tzi * 16;
+}
+
+// s_barrier between two compiler barriers: no instruction, but no memory access
+// is moved across (the caller waits for the accesses the barrier orders)
+__device__ __forceinline__ void pipe_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// one interior tile for shots [s0, s0 + ns) (0 < ns <= SG), forward flags only.
+// The U_k tile (with its halo) and the U_{k-1} cells of the next shot are
+// streamed into LDS while a shot is computed (two buffers), so that no load
+// in the loop writes a register the arithmetic waits for.  The buffers are
+// distinct variables used in turn by an unrolled loop, so that the compiler
+// can tell that the reads of one do not alias the DMA into the other (it
+// would otherwise wait for that DMA before every read).
+template <int PHYS, int FM, int SG>
+__device__ __forceinline__ void pipe_inner(const StepArgs &a, const TileStep m, const int x0, const int z0,
+                                           const int s0, const int ns) {
+    constexpr int TX = 64, ROWS = 24, Q = (TX + 8) / 4, NQ = ROWS * Q;  // 432 float4 per staged tile
+    constexpr int NB = NQ + 256;  // + U_{k-1} of the 16 x 64 cells, one float4 per thread
+    __shared__ float4 su0[NB], su1[NB];
+    __shared__ float4 sd[(PHYS == 2) ? ROWS * (TX / 4) : 1];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int flags = m.flags & FM;
+    const int P = a.P;
+    const int cx = tid & 15, rg = tid >> 4;
+    const int col = x0 + 4 * cx;
+    const int z = z0 + rg;
+    const unsigned ro = (unsigned)(z * P + col);
+    // the tile's model, once for the group
+    float v2[4], e[4] = {0.f, 0.f, 0.f, 0.f}, dl[4] = {0.f, 0.f, 0.f, 0.f}, cs[4] = {0.f, 0.f, 0.f, 0.f},
+          sn[4] = {0.f, 0.f, 0.f, 0.f};
+    vld<4>(v2, at(a.v2, ro));
+    if (PHYS >= 1) {
+        vld<4>(e, at(a.eps, ro));
+        vld<4>(dl, at(a.del, ro));
+    }
+    if (PHYS == 2) {
+        vld<4>(cs, at(a.cs, ro));
+        vld<4>(sn, at(a.sn, ro));
+    }
+    // shot s into LDS buffer buf: U_{k-1} of this thread's cells to slot NQ + tid,
+    // then wave wv moves U_k tile slots [64 wv + 256 u, +64) (every address
+    // inside the grid: the tile is at least max(N, 4) cells inside every edge).
+    // Every wave issues two or three loads.
+    auto stage = [&](int s, float4 *buf) {
+        const long long so = (long long)s * a.sstride;
+        __builtin_amdgcn_global_load_lds(at(m.prv + so, ro), (lds_void *)(buf + NQ + 64 * wv), 16, 0, 0);
+        const float *cur = m.cur + so;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int base = 64 * wv + 256 * u;
+            const int i = base + lane;
+            if (base < NQ && i < NQ) {
+                const int r = i / Q, q = i - r * Q;
+                __builtin_amdgcn_global_load_lds(at(cur, (unsigned)((z0 - 4 + r) * P + x0 - 4 + 4 * q)),
+                                                 (lds_void *)(buf + base), 16, 0, 0);
+            }
+        }
+    };
+    // what the shots need besides the fields, read before the loop (inside it, a
+    // global load waited on would also wait for the DMA in flight): the first
+    // receiver of each of this thread's cells and the one after it (chains of
+    // more receivers on one cell are walked in the loop), the source sample,
+    // and which of this thread's cells hold the source of shot s0 + i (bits
+    // 4i .. 4i+3)
+    int rfirst[4] = {-1, -1, -1, -1}, rsecond[4] = {-1, -1, -1, -1};
+    const int rslot = (flags & F_GATHER) ? a.rrow[z] : -1;
+    if (rslot >= 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            rfirst[j] = a.rhead[(long long)rslot * P + col + j];
+            if (rfirst[j] >= 0) rsecond[j] = a.rnext[rfirst[j]];
+        }
+    }
+    unsigned srcbits = 0;
+    float wsrc = 0.f;
+    if ((flags & F_SRC) && m.k + 1 < a.nt) {
+        wsrc = a.wav[m.k + 1];
+        for (int i = 0; i < ns; ++i) {
+            if (z != a.ssz[s0 + i]) continue;
+            const int sx = a.ssx[s0 + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (col + j == sx) srcbits |= 1u << (4 * i + j);
+        }
+    }
+    float prev_out[4];
+    // shot s0 + i from buffer b; shot s0 + i + 1 is streamed into bn meanwhile
+    auto shot = [&](const int i, const float4 *b, float4 *bn) {
+        const int s = s0 + i;
+        // this wave's DMA of shot s and its LDS reads of the previous shot
+        __builtin_amdgcn_s_waitcnt(0x070);  // vmcnt(0) lgkmcnt(0)
+        // every wave's: buffer b is filled, buffer bn is free.  No fence: a release
+        // fence waits for every LDS write in flight, the DMA of later shots
+        // included; the waits above order the accesses, and the empty asm
+        // statements keep the compiler from moving memory accesses across
+        // the barrier.
+        pipe_barrier();
+        if (i + 1 < ns) stage(s + 1, bn);
+        if (i > 0) vst<4>(at(m.prv + (long long)(s - 1) * a.sstride, ro), prev_out);
+        if (PHYS == 2) {
+            for (int t = tid; t < ROWS * (TX / 4); t += 256) {
+                const int r = t >> 4, c = t & 15;
+                float w[12];
+                const float4 l = b[r * Q + c], mm = b[r * Q + c + 1], h = b[r * Q + c + 2];
+                w[0] = l.x; w[1] = l.y; w[2] = l.z; w[3] = l.w;
+                w[4] = mm.x; w[5] = mm.y; w[6] = mm.z; w[7] = mm.w;
+                w[8] = h.x; w[9] = h.y; w[10] = h.z; w[11] = h.w;
+                float d2[4], d1[4];
+                xderivs<4>(w, d2, d1);
+                sd[t] = make_float4(d1[0], d1[1], d1[2], d1[3]);
+            }
+            __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the stores to sd
+            pipe_barrier();
+        }
+        float zw[9][4];
+#pragma unroll
+        for (int h = 0; h < 9; ++h) {
+            const float4 t = b[(rg + h) * Q + cx + 1];
+            zw[h][0] = t.x; zw[h][1] = t.y; zw[h][2] = t.z; zw[h][3] = t.w;
+        }
+        const int lr = rg + 4;
+        float w[12];
+        {
+            const float4 l = b[lr * Q + cx], mm = b[lr * Q + cx + 1], h = b[lr * Q + cx + 2];
+            w[0] = l.x; w[1] = l.y; w[2] = l.z; w[3] = l.w;
+            w[4] = mm.x; w[5] = mm.y; w[6] = mm.z; w[7] = mm.w;
+            w[8] = h.x; w[9] = h.y; w[10] = h.z; w[11] = h.w;
+        }
+        float pv[4];
+        {
+            const float4 t = b[NQ + tid];
+            pv[0] = t.x; pv[1] = t.y; pv[2] = t.z; pv[3] = t.w;
+        }
+        float d2x[4], d1x[4];
+        xderivs<4>(w, d2x, d1x);
+        float c0[4], pzz[4], pxx[4], px[4], pz[4], pxz[4] = {0.f, 0.f, 0.f, 0.f};
+        tile_derivs<PHYS>(zw, d2x, d1x, a, c0, pzz, pxx, pz, px);
+        if (PHYS == 2) {
+            float q[9][4];
+#pragma unroll
+            for (int h = 0; h < 9; ++h) {
+                if (h == 4) continue;
+                const float4 t = sd[(lr - 4 + h) * (TX / 4) + cx];
+                q[h][0] = t.x; q[h][1] = t.y; q[h][2] = t.z; q[h][3] = t.w;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                pxz[j] = (KA1 * (q[5][j] - q[3][j]) + KA2 * (q[6][j] - q[2][j]) + KA3 * (q[7][j] - q[1][j]) +
+                          KA4 * (q[8][j] - q[0][j])) *
+                         a.idxdz;
+        }
+        if (rslot >= 0) {  // interior cells are undamped
+            float *row = a.seis + (long long)s * a.dstride + (long long)(m.k - a.itlag) * a.nrec;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (rfirst[j] < 0) continue;
+                row[rfirst[j]] = c0[j];
+                for (int rr = rsecond[j]; rr >= 0; rr = a.rnext[rr]) row[rr] = c0[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float op;
+            if (PHYS == 0)
+                prev_out[j] = lf_acoustic(c0[j], pv[j], 1.0f, 1.0f, v2[j], pxx[j] + pzz[j], false, op);
+            else if (PHYS == 1)
+                prev_out[j] = lf_vti(c0[j], pv[j], 1.0f, 1.0f, v2[j], e[j], dl[j], px[j], pz[j], pxx[j], pzz[j],
+                                     false, op);
+            else
+                prev_out[j] = lf_tti(c0[j], pv[j], 1.0f, 1.0f, v2[j], e[j], dl[j], cs[j], sn[j], px[j], pz[j],
+                                     pxx[j], pzz[j], pxz[j], false, op);
+        }
+        if ((srcbits >> (4 * i)) & 15u) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if ((srcbits >> (4 * i + j)) & 1u) prev_out[j] += wsrc;
+        }
+    };
+    stage(s0, su0);
+    // fully unrolled: every buffer reference is static
+#pragma unroll
+    for (int i = 0; i < SG; ++i)
+        if (i < ns) shot(i, (i & 1) ? su1 : su0, (i & 1) ? su0 : su1);
+    vst<4>(at(m.prv + (long long)(s0 + ns - 1) * a.sstride, ro), prev_out);
+}
+
+// Blocks go to the XCDs round-robin (hardware block h to XCD h mod 8); XCD x
+// runs interior blocks [x ni8, (x+1) ni8) first, then edge blocks
+// [x ne8, (x+1) ne8): both kinds are spread over the XCDs, each XCD's share is
+// contiguous (the halo and model reuse of neighbouring tiles stays in its L2),
+// and the longer interior blocks start first.  Interior block b runs tile
+// b / G for shot group b % G (G = ceil(nshots / SG)); edge block e runs edge
+// tile e / nshots for shot e % nshots.
+template <int PHYS, bool CPML, int SG>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PHYS < 2 ? 5 : 1))) void k_tile_pipe(
+    const StepArgs a) {
+    const TileSplit t = tile_split(a.nxp, a.nzp, a.N, a.nxt, a.nzt);
+    const int G = (a.nshots + SG - 1) / SG;
+    const int nib = t.nin * G, neb = t.nedge * a.nshots;
+    const int ni8 = (nib + 7) / 8, ne8 = (neb + 7) / 8;
+    const int h = blockIdx.x, x = h & 7, q = h >> 3;
+    const TileStep m{a.k, a.flags, a.cur, a.prv, a.psix, a.psiz};
+    if (q < ni8) {
+        const int b = x * ni8 + q;
+        if (b >= nib) return;  // block-uniform
+        const int ti = b / G, g = b - ti * G;
+        int x0, z0;
+        inner_tile_xz(t, ti, x0, z0);
+        const int s0 = g * SG;
+        pipe_inner<PHYS, F_FWD_MASK, SG>(a, m, x0, z0, s0, min(SG, a.nshots - s0));
+        return;
+    }
+    const int eb = x * ne8 + (q - ni8);
+    if (eb >= neb) return;  // block-uniform
+    const int ei = eb / a.nshots, s = eb - ei * a.nshots;
+    int x0, z0;
+    edge_tile_xz(t, a.nxt, a.nzt, ei, x0, z0);
+    tile_body_at<PHYS, CPML, 16, F_FWD_MASK, false>(a, x0, z0, s, m);
+}
+
 // A group of grids in one launch: block b belongs to grid g with boff[g] <= b <
 // boff[g + 1], whose arguments are list[g] as of step 0 (its two wavefields and
 // psi halves trade places on odd steps); k is the step of every grid.  The new
@@ -475,8 +784,25 @@ static int launch_tile_multi_t(const StepArgs *list, const int *boff, int ngrid,
     return (int)hipGetLastError();
 }
 
+// the pipelined variant: forward steps (flags within F_FWD_MASK) only; the
+// other passes run variant 0, whose arithmetic it shares
 template <int PHYS, bool CPML, int TZ, int WPE>
-static int launch_tile_t(const StepArgs &a0, hipStream_t st, const hipEvent_t *ev = nullptr) {
+static int launch_tile_t(const StepArgs &a0, hipStream_t st, const hipEvent_t *ev);
+template <int PHYS, bool CPML, int SG>
+static int launch_tile_pipe_t(const StepArgs &a0, hipStream_t st, const hipEvent_t *ev = nullptr) {
+    if ((a0.flags & ~F_FWD_MASK) != 0) return launch_tile_t<PHYS, CPML, 16, 1>(a0, st, ev);
+    StepArgs a = a0;
+    a.nxt = (a.nxp + 63) / 64;
+    a.nzt = (a.nzp + 15) / 16;
+    const TileSplit t = tile_split(a.nxp, a.nzp, a.N, a.nxt, a.nzt);
+    const long long G = (a.nshots + SG - 1) / SG;
+    const long long ni8 = (t.nin * G + 7) / 8, ne8 = ((long long)t.nedge * a.nshots + 7) / 8;
+    launch_timed(k_tile_pipe<PHYS, CPML, SG>, dim3((unsigned)(8 * (ni8 + ne8))), st, ev, a);
+    return (int)hipGetLastError();
+}
+
+template <int PHYS, bool CPML, int TZ, int WPE>
+static int launch_tile_t(const StepArgs &a0, hipStream_t st, const hipEvent_t *ev) {
     StepArgs a = a0;
     a.nxt = (a.nxp + 63) / 64;
     a.nzt = (a.nzp + TZ - 1) / TZ;
