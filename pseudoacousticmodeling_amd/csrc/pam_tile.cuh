@@ -500,17 +500,26 @@ __device__ __forceinline__ void inner_tile_xz(const TileSplit &t, int i, int &x0
     z0 = (t.iz0 + tzi) * 16;
 }
 // edge tile i: the rows above the rectangle, the rows below it, then the
-// left and right parts of the rows beside it
+// left and right parts of the rows beside it.  Above and below, the tiles run
+// in bands of 4 tile columns, x fastest within a band, so that a tile's
+// z-neighbours (which share its halo rows) come a few tiles later, on the
+// same XCD.
+__device__ __forceinline__ void band_xz(int e, int nxt, int rows, int &txi, int &tzi) {
+    constexpr int kEdgeBand = 4;
+    const int band = e / (kEdgeBand * rows);
+    const int rem = e - band * (kEdgeBand * rows);
+    const int wb = min(kEdgeBand, nxt - band * kEdgeBand);
+    tzi = rem / wb;
+    txi = band * kEdgeBand + (rem - tzi * wb);
+}
 __device__ __forceinline__ void edge_tile_xz(const TileSplit &t, int nxt, int nzt, int i, int &x0, int &z0) {
     const int top = t.iz0 * nxt, bottom = (nzt - t.iz1) * nxt;
     int txi, tzi;
     if (i < top) {
-        tzi = i / nxt;
-        txi = i - tzi * nxt;
+        band_xz(i, nxt, t.iz0, txi, tzi);
     } else if (i < top + bottom) {
-        const int e = i - top;
-        tzi = t.iz1 + e / nxt;
-        txi = e - (e / nxt) * nxt;
+        band_xz(i - top, nxt, nzt - t.iz1, txi, tzi);
+        tzi += t.iz1;
     } else {
         const int e = i - top - bottom;
         const int w = t.ix0 + nxt - t.ix1;
