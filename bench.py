@@ -109,6 +109,45 @@ def configs_leg(nt_cap):
     return out
 
 
+def autograd_leg(slots=50):
+    """The acoustic + CPML workload as an FWI gradient through the
+    torch.autograd.Function (pseudoacousticmodeling_amd/autograd.py): 32
+    shots, nt = 3001 + itlag, observed gathers of the true model, the misfit
+    of the true model Gaussian-smoothed (sigma 250 m) differentiated by
+    J.backward() -- the forward modelling, then the exact adjoint with the
+    forward state kept in `slots` float16 revolve slots."""
+    import numpy as np
+    import torch
+    from scipy.ndimage import gaussian_filter
+
+    from pseudoacousticmodeling_amd import _lib, workloads
+    from pseudoacousticmodeling_amd.autograd import misfit
+
+    w = workloads.acoustic_cpml_32()
+    S = len(w["sx"])
+    eng = make_engine(w, S)
+    dobs = eng.forward(0, S).clone()
+    v0 = gaussian_filter(w["vp"].astype(np.float64), 250.0 / w["dx"], mode="nearest").astype(np.float32)
+    eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_REVOLVE_F16)
+    eng.set_option(_lib.PAM_OPT_SNAPSHOTS, slots)
+    vp = torch.tensor(v0, device="cuda:0", requires_grad=True)
+    misfit(vp, eng, dobs).backward()  # tunes, allocates the slots
+    torch.cuda.synchronize()
+    vp.grad = None
+    t0 = time.perf_counter()
+    J = misfit(vp, eng, dobs)
+    J.backward()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"unit": "s/shot", "value": dt / S, "seconds": dt, "shots": S, "nt": w["nt"],
+           "padded": [w["nz"] + 2 * w["N"], w["nx"] + 2 * w["N"]], "storage": f"revolve, {slots} float16 slots",
+           "shots_per_launch": eng.get_option(_lib.PAM_OPT_BATCH) or None, "misfit": float(J.item()),
+           "finite": bool(torch.isfinite(vp.grad).all().item())}
+    eng.close()
+    torch.cuda.empty_cache()
+    return out
+
+
 def group_leg(shots_per_model):
     """Sixteen VTI + CPML models of different sizes (201..601 x 801..2401,
     every pair once), ``shots_per_model`` shots each, nt = 3001 + itlag: all
@@ -235,6 +274,7 @@ def main(argv=None):
                     help="shots advanced per launch (0: the autotuner's choice, halved per lane)")
     ap.add_argument("--lanes", type=int, default=2, help="batches advanced at once on their own streams (1 or 2)")
     ap.add_argument("--no-group", action="store_true", help="skip the uneven-grid group leg")
+    ap.add_argument("--no-autograd", action="store_true", help="skip the autograd FWI gradient leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the other single-grid workloads")
     ap.add_argument("--configs-steps", type=int, default=500,
                     help="recorded samples of the reduced TTI + CPML and acoustic + CPML runs")
@@ -373,6 +413,12 @@ def main(argv=None):
         print(f"bench: uneven-grid group {group['value']:.1f} Gpt-updates/s together, "
               f"{group['per_model_loop']:.1f} model by model", file=sys.stderr, flush=True)
 
+    autograd = None
+    if not args.no_autograd:
+        autograd = autograd_leg()
+        print(f"bench: autograd FWI gradient (acoustic CPML, revolve 50 float16 slots) "
+              f"{autograd['value'] * 1e3:.1f} ms/shot", file=sys.stderr, flush=True)
+
     others = None
     if not args.no_configs:
         others = configs_leg(args.configs_steps)
@@ -386,12 +432,18 @@ def main(argv=None):
         cpu = cpu_baseline(w, args.cpu_seconds)
         print(f"bench: cpu {cpu['value']:.3f} Gpt-updates/s ({cpu['sample']})", file=sys.stderr, flush=True)
 
-    traffic = None
+    # HBM traffic per launch of this variant at this batch, from rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs of the same workload kept in profiles/ (not
+    # measured in this run: counters need a profiler run of their own)
+    traffic, traffic_note = None, "no committed counter run for this variant and batch"
     tpath = os.path.join(ROOT, "profiles", f"traffic_{w['name']}.json")
     if os.path.exists(tpath) and args.nt_data == 5001:
-        rec = json.load(open(tpath))
-        if rec.get("variant") == variant and rec.get("shots_per_launch") == per_launch:
-            traffic = rec["traffic_bytes"]
+        for rec in json.load(open(tpath)).get("launches", []):
+            if rec.get("variant") == variant and rec.get("shots_per_launch") == per_launch:
+                traffic = rec["traffic_bytes"]
+                traffic_note = ("from the committed rocprofv3 counter runs in %s (FETCH_SIZE x %.1f + WRITE_SIZE x "
+                                "%.1f), not measured in this run" % (rec["source"], rec["fetch_scale"],
+                                                                     rec["write_scale"]))
     out = {
         "metric": METRIC, "value": value, "unit": "Gpt-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": secs / args.steps * 1e3, "higher_is_better": True,
@@ -402,7 +454,8 @@ def main(argv=None):
                    "global_batch": S, "nt": w["nt"], "shots_per_launch": per_launch, "lanes": lanes,
                    "parallelism": f"1 GPU, {per_launch} shots per launch, {lanes} batches at once on {lanes} streams"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "step kernel variant %d" % variant,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,
+                     "kernel": "step kernel variant %d" % variant,
                      "bytes_per_launch": bytes_launch, "avg_launch_us": avg_ms * 1e3,
                      "avg_launch_note": "each step launch timed alone (one lane, HIP events)",
                      "aggregate_GBs": aggregate, "aggregate_frac": aggregate / HBM_PEAK_GBS,
@@ -411,6 +464,7 @@ def main(argv=None):
         "plain_loop": plain,
         "fwi_gradient": fwi,
         "uneven_group": group,
+        "autograd_gradient": autograd,
         "other_configs": others,
         "cpu_baseline": cpu,
         "finite": finite,

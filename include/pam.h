@@ -263,10 +263,15 @@ int pam_misfit_noalloc(int nshots, int nt_data, int nrec, const float *dobs, con
  * edge-replicating expansion applied --, or [nzp][nxp] when PAM_PADDED).  It
  * is the transpose of the forward map's Jacobian, including the dependence of
  * the CPML profiles on vp: the backward a torch.autograd.Function needs.  The
- * forward states are kept per step in float32 (PAM_STORE_ALL_F32) or as
+ * forward states are kept per step in float32 (PAM_STORE_ALL_F32), as
  * uniform checkpoints recomputed per segment (PAM_STORE_CHECKPOINT, interval
- * PAM_OPT_CKPT_STEP or ceil(sqrt(nt))).  No counterpart in the reference,
- * whose gradient (pam_gradient) is preconditioned and normalised. */
+ * PAM_OPT_CKPT_STEP or ceil(sqrt(nt))), or in PAM_OPT_SNAPSHOTS binomial
+ * (revolve) slots of the whole state, float32 (PAM_STORE_REVOLVE_F32: the
+ * same gradient bit for bit) or each field scaled by its max |value| and
+ * stored in float16 (PAM_STORE_REVOLVE_F16: the adjoint of a forward pass
+ * restarted from rounded states, so the gradient moves by about the float16
+ * rounding).  No counterpart in the reference, whose gradient (pam_gradient)
+ * is preconditioned and normalised. */
 int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const float *dseis, float *g_vp, int layout);
 
 /* Elapsed GPU milliseconds of the last whole-run call's step loop and, with

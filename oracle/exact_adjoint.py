@@ -272,9 +272,12 @@ def tangent(st: Setup, vp, dvp):
     return forward(st, vp, dvp=dvp)["dseis"]
 
 
-def adjoint(st: Setup, vp, r, states=None):
+def adjoint(st: Setup, vp, r, states=None, prev_states=None):
     """dJ/dvp on the padded grid for dJ/dseis = r ([nt-itlag][nrec]), by the
-    reverse sweep of the forward map (k = nt-1 .. 0)."""
+    reverse sweep of the forward map (k = nt-1 .. 0).  ``states[k]`` is the
+    state of step k; the psi / zeta of step k - 1 it also reads come from
+    ``prev_states[k]`` when given (a recomputed sweep may hold two versions of
+    one step), else from ``states[k - 1]``."""
     vp = np.asarray(vp, dtype=np.float64)
     if states is None:
         states = forward(st, vp, keep=True)["states"]
@@ -311,7 +314,7 @@ def adjoint(st: Setup, vp, r, states=None):
             Q = M * (_d2(C, 1) * idx2 + _d2(C, 0) * idz2 + Sx * (_d1(S["px"], 1) * idx + S["zx"])
                      + Sz * (_d1(S["pz"], 0) * idz + S["zz"]))
             gV += phi * Q
-            prev = states[k - 1] if k > 0 else None
+            prev = (prev_states[k] if prev_states is not None else states[k - 1]) if k > 0 else None
             pxm = prev["px"] if prev else 0.0
             pzm = prev["pz"] if prev else 0.0
             zxm = prev["zx"] if prev else 0.0
@@ -335,6 +338,128 @@ def adjoint(st: Setup, vp, r, states=None):
     if cp:
         g = g + gax * pr["dax"] + gbx * pr["dbx"] + gaz * pr["daz"] + gbz * pr["dbz"]
     return g
+
+
+def _quantize(v, kind):
+    """float16 storage of one field as the library keeps a revolve slot (the
+    quantiser of oracle.quantize_scaled: v / max|v| in float32, rounded to
+    float16, read back times max|v|); 'f32' keeps v."""
+    if kind == "f32":
+        return np.array(v, dtype=np.float64, copy=True)
+    v = np.asarray(v, dtype=np.float32)
+    m = np.float32(np.max(np.abs(v))) if v.size else np.float32(0.0)
+    inv = np.float32(1.0) / m if m >= np.finfo(np.float32).tiny else np.float32(0.0)
+    q = (v * inv).astype(np.float16).astype(np.float32)
+    return (q * m).astype(np.float64)
+
+
+def _revolve_beta(s, r):
+    b = 1.0
+    for i in range(1, r + 1):
+        b = b * (s + i) / i
+    return b
+
+
+def revolve_split(n, s):
+    """Offset of the first checkpoint of an n-step range with s free slots
+    (Griewank's treeverse, as the library splits): for the smallest r with
+    beta(s, r) >= n, min(beta(s, r - 1), n - 1), at least 1."""
+    r = 1
+    while _revolve_beta(s, r) < n:
+        r += 1
+    return max(1, int(min(_revolve_beta(s, r - 1), float(n - 1))))
+
+
+def revolve_states(st: Setup, vp, slots, kind="f16"):
+    """The states the library's reverse sweep reads with binomial (revolve)
+    checkpointing of the whole forward state in ``slots`` slots stored as
+    ``kind`` (no counterpart in the reference): a slot keeps the state before
+    a step -- C_k, C_{k-1} and, for CPML, the previous step's psi and zeta --
+    each field quantised on its own; the steps are reversed from nt - 1 down
+    to 0 in treeverse order, reverse step t reading the state of step t and
+    the psi / zeta of step t - 1, both from the one forward recomputation that
+    reaches t.  Returns (states, prev_states) for ``adjoint``."""
+    vp = np.asarray(vp, dtype=np.float64)
+    M = st.live()
+    V = vp * vp * st.dt * st.dt
+    D = st.damping()
+    Sx, Sz = st.strips()
+    pr = st.profiles(vp) if st.abc == 1 else None
+    idx2, idz2 = 1.0 / (st.dx * st.dx), 1.0 / (st.dz * st.dz)
+    idx, idz = 1.0 / st.dx, 1.0 / st.dz
+    shape = (st.nzp, st.nxp)
+    cp = st.abc == 1
+    names = ("C", "Cp", "px", "pz", "zx", "zz") if cp else ("C", "Cp")
+
+    def step(w, k):
+        """one forward step of the working state w (in place); returns the
+        state of step k (C_k and the psi / zeta step k writes)"""
+        C, Cp = w["C"], w["Cp"]
+        lap = _d2(C, 1) * idx2 + _d2(C, 0) * idz2
+        if cp:
+            ax, bx, az, bz = pr["ax"], pr["bx"], pr["az"], pr["bz"]
+            px_n = Sx * (ax * w["px"] + bx * (_d1(C, 1) * idx))
+            pz_n = Sz * (az * w["pz"] + bz * (_d1(C, 0) * idz))
+            ex = _d1(px_n, 1) * idx
+            ez = _d1(pz_n, 0) * idz
+            zx_n = Sx * (ax * w["zx"] + bx * (_d2(C, 1) * idx2 + ex))
+            zz_n = Sz * (az * w["zz"] + bz * (_d2(C, 0) * idz2 + ez))
+            Q = M * (lap + Sx * (ex + zx_n) + Sz * (ez + zz_n))
+            snap = dict(C=C.copy(), px=px_n, pz=pz_n, zx=zx_n, zz=zz_n)
+            Cn = M * (V * Q + 2.0 * C - Cp)
+            w["px"], w["pz"], w["zx"], w["zz"] = px_n, pz_n, zx_n, zz_n
+        else:
+            Q = M * lap
+            snap = dict(C=C.copy())
+            Cn = D * (M * (V * Q + 2.0 * C - D * Cp))
+        if k + 1 < st.nt:
+            Cn[st.sz, st.sx] += st.src[k + 1]
+        w["Cp"], w["C"] = C, Cn
+        return snap
+
+    w = {n: np.zeros(shape) for n in names}
+    w["C"][st.sz, st.sx] += st.src[0]
+    pos = [0]  # step whose state the working set holds
+    store, states, prevs = {}, [None] * st.nt, [None] * st.nt
+
+    def save(k):
+        store[k] = {n: _quantize(w[n], kind) for n in names}
+
+    def advance(frm, to):
+        if pos[0] != frm:
+            for n in names:
+                w[n] = store[frm][n].copy()
+            pos[0] = frm
+        for k in range(frm, to):
+            step(w, k)
+            pos[0] = k + 1
+
+    def leaf(t):
+        advance(t, t)
+        if t >= 1 and cp:
+            prevs[t] = {n: w[n].copy() for n in ("px", "pz", "zx", "zz")}
+        states[t] = step(w, t)
+        pos[0] = t + 1
+
+    def rev(lo, hi, s):
+        n = hi - lo
+        if n == 1:
+            return leaf(lo)
+        if s == 0:
+            for t in range(hi - 1, lo - 1, -1):
+                advance(lo, t)
+                leaf(t)
+            return
+        m = lo + revolve_split(n, s)
+        advance(lo, m)
+        save(m)
+        rev(m, hi, s - 1)
+        del store[m]
+        rev(lo, m, s)
+
+    save(0)
+    rev(0, st.nt, max(1, min(slots, st.nt)) - 1)
+    return states, (prevs if cp else None)
 
 
 def expand_T(gpad, N):

@@ -464,7 +464,7 @@ def quantize_scaled(v, kind):
         return np.array(v, copy=True)
     v = np.asarray(v, dtype=np.float32)
     m = np.float32(np.max(np.abs(v))) if v.size else np.float32(0.0)
-    inv = np.float32(1.0) / m if m > 0 else np.float32(0.0)
+    inv = np.float32(1.0) / m if m >= np.finfo(np.float32).tiny else np.float32(0.0)
     q = v * inv
     q = q.astype(np.float16).astype(np.float32) if kind == "f16" else _bf16_round(q)
     return q * m
@@ -484,7 +484,7 @@ def quantize_with_scale(v, m, kind):
     arithmetic; m <= 0 stores zeros), read back as q * m."""
     v = np.asarray(v, dtype=np.float32)
     m = np.float32(m)
-    inv = np.float32(1.0) / m if m > 0 else np.float32(0.0)
+    inv = np.float32(1.0) / m if m >= np.finfo(np.float32).tiny else np.float32(0.0)
     q = v * inv
     q = q.astype(np.float16).astype(np.float32) if kind == "f16" else _bf16_round(q)
     return q * m

@@ -2761,6 +2761,140 @@ static int exact_copy_state(pam_ctx *ctx, const FieldSet &f, const float *cur, f
     return PAM_OK;
 }
 
+// Binomial (revolve) checkpointing of the exact reverse sweep: the forward
+// steps run on the second field set, whose whole state (both wavefields and,
+// for CPML, the previous step's psi and zeta) is kept at up to nslot steps in
+// float32 or, with `half`, each field scaled by its max |value| and rounded to
+// float16.  Reverse step t needs the state of step t (U_t with the psi and
+// zeta step t writes) and the psi and zeta of step t - 1: from the working
+// state before step t, the second is copied into Sm, step t is run, and the
+// first is copied into S.  Steps are reversed in treeverse order from
+// nt - 1 down to 0.
+template <class Rev>
+static int exact_revolve(pam_ctx *ctx, StepArgs &a, int tile, int nb, int nslot, bool half, long long blk, float *Sm,
+                         float *S, Rev &reverse_step) {
+    const int nt = ctx->g.nt;
+    const int nf = ckpt_fields(ctx);
+    const long long fl = ctx->plane * nb;  // floats per field of a slot
+    const int es = half ? 2 : 4;
+    const bool cpml = ctx->abc == PAM_CPML;
+    FieldSet &w = ctx->aux;
+    int r = zero_fields(ctx, w, nb, true, true);
+    if (r) return r;
+    KCK(launch_inject_point(w.U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, nb, ctx->st));
+    float *wc = w.U0, *wp = w.U1;
+    int ws = 0;  // step whose state the working set holds
+    std::vector<int> slot_of(nt + 1, -1), flip_of(nslot, 0), pool;
+    for (int i = nslot - 1; i >= 0; --i) pool.push_back(i);
+    auto fields = [&](float **pf) {
+        pf[0] = wc;
+        pf[1] = wp;
+        if (cpml) {
+            const long long in = w.psi_flip ? ctx->psi_half : 0;
+            pf[2] = w.psix + in;
+            pf[3] = w.psiz + in;
+            pf[4] = w.zetax;
+            pf[5] = w.zetaz;
+        }
+    };
+    auto store = [&](int k) -> int {
+        if (pool.empty()) return fail(ctx, PAM_E_STATE, "revolve: no free checkpoint slot");
+        const int sl = pool.back();
+        pool.pop_back();
+        slot_of[k] = sl;
+        flip_of[sl] = w.psi_flip;
+        float *pf[6];
+        fields(pf);
+        for (int i = 0; i < nf; ++i) {
+            char *dst = (char *)ctx->ckpt + ((long long)sl * nf + i) * fl * es;
+            if (half)
+                KCK(launch_pack_half(pf[i], ctx->plane, ctx->plane, nb, dst, ctx->plane,
+                                     ctx->rvbits + ((long long)sl * nf + i) * nb, ctx->st));
+            else
+                HCK(hipMemcpyAsync(dst, pf[i], (size_t)fl * 4, hipMemcpyDeviceToDevice, ctx->st));
+        }
+        return PAM_OK;
+    };
+    auto load = [&](int k) -> int {
+        const int sl = slot_of[k];
+        if (sl < 0) return fail(ctx, PAM_E_STATE, "revolve: step %d has no checkpoint", k);
+        wc = w.U0;
+        wp = w.U1;
+        w.psi_flip = flip_of[sl];
+        float *pf[6];
+        fields(pf);
+        for (int i = 0; i < nf; ++i) {
+            const char *src = (const char *)ctx->ckpt + ((long long)sl * nf + i) * fl * es;
+            if (half)
+                KCK(launch_unpack_half(src, ctx->plane, ctx->plane, nb, ctx->rvbits + ((long long)sl * nf + i) * nb,
+                                       pf[i], ctx->plane, ctx->st));
+            else
+                HCK(hipMemcpyAsync(pf[i], src, (size_t)fl * 4, hipMemcpyDeviceToDevice, ctx->st));
+        }
+        ws = k;
+        return PAM_OK;
+    };
+    auto wstep = [&](int k) -> int {
+        int rr = fwd_step_fs(ctx, w, a, tile, wc, wp, k, nullptr);
+        if (rr) return rr;
+        std::swap(wc, wp);
+        ws = k + 1;
+        return PAM_OK;
+    };
+    auto advance = [&](int from, int to) -> int {
+        if (ws != from) {
+            int rr = load(from);
+            if (rr) return rr;
+        }
+        for (int k = from; k < to; ++k) {
+            int rr = wstep(k);
+            if (rr) return rr;
+        }
+        return PAM_OK;
+    };
+    auto leaf = [&](int t) -> int {
+        int rr = advance(t, t);
+        if (rr) return rr;
+        if (t >= 1) {
+            rr = exact_copy_state(ctx, w, wc, Sm, nb, blk);  // psi and zeta of step t - 1
+            if (rr) return rr;
+        }
+        float *ut = wc;
+        rr = wstep(t);
+        if (rr) return rr;
+        rr = exact_copy_state(ctx, w, ut, S, nb, blk);
+        if (rr) return rr;
+        return reverse_step(t, S, t >= 1 ? Sm : nullptr);
+    };
+    std::function<int(int, int, int)> rev = [&](int lo, int hi, int s) -> int {
+        const int n = hi - lo;
+        int rr;
+        if (n == 1) return leaf(lo);
+        if (s == 0) {
+            for (int t = hi - 1; t >= lo; --t) {
+                rr = advance(lo, t);
+                if (rr) return rr;
+                rr = leaf(t);
+                if (rr) return rr;
+            }
+            return PAM_OK;
+        }
+        const int m = lo + revolve_split(n, s);
+        rr = advance(lo, m);
+        if (rr) return rr;
+        rr = store(m);
+        if (rr) return rr;
+        rr = rev(m, hi, s - 1);
+        if (rr) return rr;
+        pool.push_back(slot_of[m]);
+        slot_of[m] = -1;
+        return rev(lo, m, s);
+    };
+    r = store(0);
+    if (r) return r;
+    return rev(0, nt, nslot - 1);
+}
+
 // Backward of pam_forward (acoustic physics): a forward pass that keeps the
 // state of every step in HBM (or uniform checkpoints of the whole state, the
 // segments recomputed on the second field set, with PAM_STORE_CHECKPOINT),
@@ -2774,29 +2908,37 @@ extern "C" int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const fl
     if (nshots < 1) return fail(ctx, PAM_E_ARG, "nshots must be >= 1");
     if (!dseis || !g_vp) return fail(ctx, PAM_E_ARG, "dseis and g_vp must not be NULL");
     if (layout != PAM_INTERIOR && layout != PAM_PADDED) return fail(ctx, PAM_E_ARG, "layout must be 0 or 1");
-    if (ctx->storage != PAM_STORE_ALL_F32 && ctx->storage != PAM_STORE_CHECKPOINT)
+    const int storage = ctx->storage;
+    if (storage != PAM_STORE_ALL_F32 && storage != PAM_STORE_CHECKPOINT && storage != PAM_STORE_REVOLVE_F32 &&
+        storage != PAM_STORE_REVOLVE_F16)
         return fail(ctx, PAM_E_UNSUPPORTED,
-                    "the exact adjoint keeps float32 state: use PAM_STORE_ALL_F32 or PAM_STORE_CHECKPOINT");
+                    "the exact adjoint keeps the state of every step, uniform checkpoints or revolve slots: use "
+                    "PAM_STORE_ALL_F32, PAM_STORE_CHECKPOINT or PAM_STORE_REVOLVE_F32 / _F16");
     const pam_grid &g = ctx->g;
     const int nt = g.nt;
     const long long nd = (long long)(nt - g.itlag) * ctx->nrec;
     const bool cpml = ctx->abc == PAM_CPML;
-    const bool ckpt = ctx->storage == PAM_STORE_CHECKPOINT;
+    const bool ckpt = storage == PAM_STORE_CHECKPOINT;
+    const bool rv = storage == PAM_STORE_REVOLVE_F32 || storage == PAM_STORE_REVOLVE_F16;
+    const bool rv_half = storage == PAM_STORE_REVOLVE_F16;
+    const int nslot = rv ? std::max(1, std::min(ctx->snapshots, nt)) : 0;  // revolve slots
+    const int nfc = ckpt_fields(ctx);                                          // fields of a checkpoint or slot
     const int nf = cpml ? 5 : 1;           // saved fields per step
     const int nwork = cpml ? 6 : 0;        // mu (x, z), nu (x, z) in and out
     const int nacc = cpml ? 5 : 1;         // dJ/dV, and dJ/da, dJ/db in x and z
     const int C = ckpt ? (ctx->ckpt_step > 0 ? ctx->ckpt_step : (int)std::ceil(std::sqrt((double)nt))) : nt;
     const int nseg = ckpt ? (nt - 1) / C + 1 : 1;
-    const int nbuf = ckpt ? C + 1 : nt;    // slots k0 .. hi of a segment
+    const int nbuf = rv ? 2 : (ckpt ? C + 1 : nt);  // states k0 .. hi of a segment (revolve: t - 1 and t)
     const long long pl = ctx->plane;
     size_t freeb = 0, totalb = 0;
     HCK(hipMemGetInfo(&freeb, &totalb));
     int B = batch_size(ctx, nshots);
     const long long per_shot = pl * 4 * ((long long)nbuf * nf + nwork + nacc) +
-                               (ckpt ? (long long)nseg * ckpt_fields(ctx) * pl * 4 : 0);
-    const long long have = ctx->ex_save_bytes + ctx->ex_work_bytes + ctx->ckpt_bytes;
+                               (ckpt ? (long long)nseg * nfc * pl * 4 : 0) +
+                               (rv ? (long long)nslot * nfc * (pl * (rv_half ? 2 : 4) + 4) : 0);
+    const long long have = ctx->ex_save_bytes + ctx->ex_work_bytes + ctx->ckpt_bytes + ctx->rvbits_bytes;
     long long budget = (long long)((freeb + have) * 0.85);
-    if (ckpt && !ctx->aux.U0) budget -= pl * ctx->max_shots * 4 * (cpml ? 8 : 2);
+    if ((ckpt || rv) && !ctx->aux.U0) budget -= pl * ctx->max_shots * 4 * (cpml ? 8 : 2);
     while (B > 1 && (long long)B * per_shot > budget) --B;
     if ((long long)B * per_shot > budget)
         return fail(ctx, PAM_E_NOMEM, "one shot's saved states (%lld bytes) exceed free device memory", per_shot);
@@ -2808,11 +2950,22 @@ extern "C" int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const fl
     r = ensure_scratch(ctx, (void **)&ctx->ex_gacc, &ctx->ex_gacc_bytes, pl * 4);
     if (r) return r;
     if (ckpt) {
-        r = ensure_scratch(ctx, (void **)&ctx->ckpt, &ctx->ckpt_bytes, (long long)nseg * ckpt_fields(ctx) * pl * B * 4);
+        r = ensure_scratch(ctx, (void **)&ctx->ckpt, &ctx->ckpt_bytes, (long long)nseg * nfc * pl * B * 4);
         if (r) return r;
         r = alloc_fieldset(ctx, ctx->aux);
         if (r) return r;
         ctx->ckpt_flip.assign(nseg, 0);
+    }
+    if (rv) {
+        r = ensure_scratch(ctx, (void **)&ctx->ckpt, &ctx->ckpt_bytes,
+                           (long long)nslot * nfc * pl * B * (rv_half ? 2 : 4));
+        if (r) return r;
+        if (rv_half) {
+            r = ensure_scratch(ctx, (void **)&ctx->rvbits, &ctx->rvbits_bytes, (long long)nslot * nfc * B * 4);
+            if (r) return r;
+        }
+        r = alloc_fieldset(ctx, ctx->aux);
+        if (r) return r;
     }
     HCK(hipMemsetAsync(ctx->ex_gacc, 0, (size_t)pl * 4, ctx->st));
     ctx->loop_ms = ctx->kernel_ms = 0.0;
@@ -2825,7 +2978,8 @@ extern "C" int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const fl
     HCK(hipEventRecord(ctx->e0, ctx->st));
     for (int b0 = 0; b0 < nshots; b0 += B) {
         const int nb = std::min(B, nshots - b0);
-        // ---- forward pass: the state of every step, or checkpoints
+        // ---- forward pass: the state of every step, or checkpoints (revolve:
+        // none, the reverse sweep runs the forward steps it needs)
         r = zero_shot_state(ctx, nb, true, true);
         if (r) return r;
         r = upload_sources(ctx, shot0 + b0, nb);
@@ -2834,7 +2988,7 @@ extern "C" int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const fl
         StepArgs a;
         base_step_args(ctx, a, nb);
         a.flags |= F_SRC;
-        {
+        if (!rv) {
             float *cur = ctx->fs.U0, *prv = ctx->fs.U1;
             int j = 0;
             for (int k = 0; k < nt; ++k) {
@@ -2890,7 +3044,41 @@ extern "C" int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const fl
         }
         e.gV = w + nwork * blk;
         const float *data = dseis + (long long)b0 * nd;
-        for (int seg = nseg - 1; seg >= 0; --seg) {
+        // reverse step t from the saved state of step t (S) and of step t - 1 (Sm,
+        // NULL at t = 0; only its psi and zeta are read)
+        auto reverse_step = [&](int t, const float *S, const float *Sm) -> int {
+            e.X = X;
+            e.Y = Y;
+            e.C = S;
+            if (cpml) {
+                e.psx = S + blk;
+                e.psz = S + 2 * blk;
+                e.zex = S + 3 * blk;
+                e.zez = S + 4 * blk;
+                e.psxm = Sm ? Sm + blk : nullptr;
+                e.pszm = Sm ? Sm + 2 * blk : nullptr;
+                e.zexm = Sm ? Sm + 3 * blk : nullptr;
+                e.zezm = Sm ? Sm + 4 * blk : nullptr;
+                e.nux_in = nxi;
+                e.nuz_in = nzi;
+                e.nux_out = nxo;
+                e.nuz_out = nzo;
+                KCK(launch_exact_mu(e, ctx->st));
+            }
+            KCK(launch_exact_lambda(e, ctx->st));
+            std::swap(nxi, nxo);
+            std::swap(nzi, nzo);
+            if (t >= g.itlag)
+                KCK(launch_exact_rec(Y, pl, ctx->P, ctx->d_rz, ctx->d_rx, ctx->nrec, data, nd, t - g.itlag,
+                                     cpml ? nullptr : ctx->cerx, ctx->cerz, nb, ctx->st));
+            std::swap(X, Y);
+            return PAM_OK;
+        };
+        if (rv) {
+            r = exact_revolve(ctx, a, tile, nb, nslot, rv_half, blk, slot(0), slot(1), reverse_step);
+            if (r) return r;
+        }
+        for (int seg = rv ? -1 : nseg - 1; seg >= 0; --seg) {
             const int lo = ckpt ? seg * C : 0;
             const int hi = ckpt ? std::min(nt - 1, (seg + 1) * C - 1) : nt - 1;
             const int k0 = ckpt ? std::max(0, lo - 1) : 0;
@@ -2907,33 +3095,8 @@ extern "C" int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const fl
                 }
             }
             for (int t = hi; t >= lo; --t) {
-                const float *S = slot(t - k0);
-                const float *Sm = (t >= 1 && t - 1 >= k0) ? slot(t - 1 - k0) : nullptr;
-                e.X = X;
-                e.Y = Y;
-                e.C = S;
-                if (cpml) {
-                    e.psx = S + blk;
-                    e.psz = S + 2 * blk;
-                    e.zex = S + 3 * blk;
-                    e.zez = S + 4 * blk;
-                    e.psxm = Sm ? Sm + blk : nullptr;
-                    e.pszm = Sm ? Sm + 2 * blk : nullptr;
-                    e.zexm = Sm ? Sm + 3 * blk : nullptr;
-                    e.zezm = Sm ? Sm + 4 * blk : nullptr;
-                    e.nux_in = nxi;
-                    e.nuz_in = nzi;
-                    e.nux_out = nxo;
-                    e.nuz_out = nzo;
-                    KCK(launch_exact_mu(e, ctx->st));
-                }
-                KCK(launch_exact_lambda(e, ctx->st));
-                std::swap(nxi, nxo);
-                std::swap(nzi, nzo);
-                if (t >= g.itlag)
-                    KCK(launch_exact_rec(Y, pl, ctx->P, ctx->d_rz, ctx->d_rx, ctx->nrec, data, nd, t - g.itlag,
-                                         cpml ? nullptr : ctx->cerx, ctx->cerz, nb, ctx->st));
-                std::swap(X, Y);
+                r = reverse_step(t, slot(t - k0), (t >= 1 && t - 1 >= k0) ? slot(t - 1 - k0) : nullptr);
+                if (r) return r;
             }
         }
         KCK(launch_exact_finish(e, ctx->vp_pad, g.dx, g.dz, g.dt, ctx->ap.cpml_d0, ctx->ap.cpml_f_pico, ctx->ex_gacc,

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(1024) void k_sb_save(const float *U, long long sstr
         }
         m = red[0];
         if (threadIdx.x == 0) scale[(long long)s * nt + k] = m;
-        inv = (m > 0.0f) ? 1.0f / m : 0.0f;
+        inv = scale_inv(m);
     }
     for (long long i = threadIdx.x; i < W; i += blockDim.x) {
         int iz, ix;

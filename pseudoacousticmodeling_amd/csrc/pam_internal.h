@@ -6,6 +6,11 @@
 
 namespace pam {
 
+// 1 / m for a 16-bit storage scale m = max |v|: 0 when m is 0 or below the
+// smallest normal float (1 / m would overflow to Inf, and 0 * Inf is NaN); the
+// values it scales are then stored as 0
+__host__ __device__ inline float scale_inv(float m) { return (m >= 1.17549435e-38f) ? 1.0f / m : 0.0f; }
+
 // flags of StepArgs::flags (wave-uniform switches inside k_step)
 enum : int {
     F_CERJAN = 1,       // Cerjan damping (cerx/cerz)

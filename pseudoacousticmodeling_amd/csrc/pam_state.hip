@@ -35,7 +35,7 @@ __global__ __launch_bounds__(256) void k_pack_half(const float *x, long long sst
                                                    long long qsstride, const unsigned int *bits) {
     const int s = blockIdx.y;
     const float m = __uint_as_float(bits[s]);
-    const float inv = (m > 0.0f) ? 1.0f / m : 0.0f;
+    const float inv = scale_inv(m);
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         q[(long long)s * qsstride + i] = __float2half(x[(long long)s * sstride + i] * inv);
 }

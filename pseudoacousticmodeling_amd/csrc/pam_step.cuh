@@ -217,7 +217,7 @@ __device__ __forceinline__ void snap_put_at(const StepArgs &a, long long off, in
     float inv = 0.0f;
     if (a.save_kind == SV_F16_INT || a.save_kind == SV_BF16_INT) {
         const float m = a.save_scale[(long long)s * a.nt + k];
-        inv = (m > 0.0f) ? 1.0f / m : 0.0f;
+        inv = scale_inv(m);
     }
     switch (a.save_kind) {
         case SV_F32_INT:
@@ -241,7 +241,7 @@ __device__ __forceinline__ void snap_put(const StepArgs &a, int s, int k, long l
     float inv = 0.0f;
     if (a.save_kind == SV_F16_INT || a.save_kind == SV_BF16_INT) {
         const float m = a.save_scale[(long long)s * a.nt + k];
-        inv = (m > 0.0f) ? 1.0f / m : 0.0f;
+        inv = scale_inv(m);
     }
     switch (a.save_kind) {
         case SV_F32_INT:

@@ -12,7 +12,8 @@ sizes (pseudoacousticmodeling_amd/workloads.py):
   with boundary saving, float32 and bfloat16 rims, against the oracle's
   restatement of the same strategy;
 - acoustic + CPML, 301x1201 (401x1301 padded): the gradient against the
-  oracles;
+  oracles, and the autograd gradient with 50 float16 revolve slots against
+  the float64 oracle, a dot-product test and 20 central differences;
 - sixteen uneven VTI + CPML grids, 201..601 x 801..2401: the smallest and the
   largest against the oracle, and all sixteen with 8 shots each (shortened
   record) as one group equal to each model alone bit for bit.
@@ -204,6 +205,84 @@ def test_acoustic_cpml_32_full_grid_gradient():
     print(f"  gradient rel_l2: float32 oracle {e32:.2e}, float64 oracle {e64:.2e}", flush=True)
     assert np.isfinite(g).all() and np.abs(g).max() > 0
     assert e32 <= TOL and e64 <= TOL
+    eng.close()
+
+
+def test_acoustic_cpml_32_autograd_gradient_with_float16_revolve_slots():
+    """The acoustic + CPML workload's FWI gradient through the
+    torch.autograd.Function, the forward state kept in 50 float16 revolve
+    slots: the misfit of the true model's gathers against those of the true
+    model Gaussian-smoothed (sigma 250 m), one shot, shortened record.  The
+    gradient's error against the exact float64 oracle gradient is printed and
+    bounded by the float16 slots' effect; a dot-product test pairs the GPU
+    adjoint with the oracle's tangent (Born) operator; central differences of
+    the GPU's own misfit check it on 20 random smooth perturbations."""
+    import torch
+    from scipy.ndimage import gaussian_filter
+
+    from oracle import exact_adjoint as EA
+    from pseudoacousticmodeling_amd.autograd import misfit
+    from tests.problems import smooth_field
+    from tests.test_gpu_exact_adjoint import _setup
+
+    w = W.acoustic_cpml_32(nshots=1, nt_data=152)
+    pb = _pb(w)
+    g_ = pb["g"]
+    print(f"case acoustic_cpml_32 autograd revolve f16 padded {g_.nzp}x{g_.nxp} nt={g_.nt}", flush=True)
+    eng = _engine(pb, 1)
+    dobs = eng.forward(0, 1).clone()
+    v0 = gaussian_filter(pb["vp"].astype(np.float64), 25.0, mode="nearest").astype(np.float32)
+    eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_REVOLVE_F16)
+    eng.set_option(_lib.PAM_OPT_SNAPSHOTS, 50)
+    vp = torch.tensor(v0, device="cuda", requires_grad=True)
+    J = misfit(vp, eng, dobs)
+    J.backward()
+    g = vp.grad.detach().cpu().numpy().astype(np.float64)
+    eng.set_model(v0)
+    r = (eng.forward(0, 1) - dobs)[0].cpu().numpy().astype(np.float64)
+    N = g_.N
+    st = _setup(pb, 0)
+    vpad = np.pad(v0.astype(np.float64), N, mode="edge")
+    exact = EA.expand_T(EA.adjoint(st, vpad, r), N)
+    e64 = rel_l2(g, exact)
+    eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_REVOLVE_F32)
+    g32 = eng.forward_adjoint(eng.forward(0, 1) - dobs).cpu().numpy().astype(np.float64)
+    print(f"  gradient rel_l2 vs float64 oracle: float16 slots {e64:.2e}, float32 slots {rel_l2(g32, exact):.2e}",
+          flush=True)
+    # dot-product test: <F'(v0) dv, r> (oracle tangent) = <dv, F'(v0)^T r> (GPU),
+    # relative to |lhs| and to |dv| |F'^T r| (a smooth random dv alone meets the
+    # oscillating gradient with much cancellation, so dv also holds a smoothed
+    # copy of the float32-slot gradient)
+    gs = gaussian_filter(g32, 3.0)
+    dv = smooth_field(np.random.default_rng(400), g_.nz, g_.nx, -1.0, 1.0).astype(np.float64) + gs / np.abs(gs).max()
+    lhs = float(np.sum(EA.tangent(st, vpad, np.pad(dv, N, mode="edge")) * r))
+    scale = float(np.linalg.norm(dv) * np.linalg.norm(exact))
+    dots = {k: (abs(lhs - float(np.sum(dv * v))) / abs(lhs), abs(lhs - float(np.sum(dv * v))) / scale)
+            for k, v in (("float16", g), ("float32", g32))}
+    print(f"  dot-product test: <F' dv, r> = {lhs:.6e}; relative error "
+          + ", ".join(f"{k} slots {a:.1e} ({b:.1e} of |dv||g|)" for k, (a, b) in dots.items()), flush=True)
+
+    def Jgpu(m):
+        eng.set_model(m)
+        d = (eng.forward(0, 1) - dobs).double()
+        return 0.5 * float(torch.sum(d * d))
+
+    errs = []
+    for i in range(20):
+        dm = smooth_field(np.random.default_rng(500 + i), g_.nz, g_.nx, -1.0, 1.0).astype(np.float64)
+        h = 5.0  # m/s
+        fd = (Jgpu((v0 + h * dm).astype(np.float32)) - Jgpu((v0 - h * dm).astype(np.float32))) / (2 * h)
+        errs.append((abs(fd - float(np.sum(g * dm))) / abs(fd),
+                     abs(fd - float(np.sum(g * dm))) / float(np.linalg.norm(dm) * np.linalg.norm(g))))
+    rel = [a for a, _ in errs]
+    nrm = [b for _, b in errs]
+    print(f"  central differences, 20 perturbations: relative error median {np.median(rel):.1e}, max "
+          f"{max(rel):.1e}; of |dm||g| max {max(nrm):.1e}", flush=True)
+    assert np.isfinite(g).all() and np.abs(g).max() > 0
+    assert rel_l2(g32, exact) <= TOL
+    assert e64 <= 5e-3
+    assert dots["float32"][0] <= 1e-3 and dots["float16"][1] <= 5e-3
+    assert max(nrm) <= 5e-3
     eng.close()
 
 

@@ -110,3 +110,17 @@ def test_half_snapshots_round_within_their_format(kind):
         qmax = np.max(np.abs(d32)) / m
         assert np.max(np.abs(d16 - d32)) <= _half_ulp_at(qmax, kind) * m * (1 + 1e-6)
         assert 0 < np.max(np.abs(d16 - d32))
+
+
+def test_a_scale_below_the_smallest_normal_float_stores_zeros():
+    """A field whose max |v| is a subnormal float (CPML memory late in a long
+    run): 1 / max would overflow to Inf and 0 * Inf is NaN, so such a field is
+    stored as zeros (the library's scale_inv does the same)."""
+    from oracle import exact_adjoint as EA
+    from oracle.oracle import quantize_scaled
+
+    v = np.array([1e-39, -5e-40, 0.0, 2e-45], dtype=np.float32)
+    for q in (quantize_scaled(v, "f16"), quantize_scaled(v, "bf16"), EA._quantize(v, "f16")):
+        assert np.all(np.isfinite(q)) and not np.any(q)
+    w = np.array([2e-38, 1e-38, 0.0], dtype=np.float32)  # smallest normals still scale
+    assert np.isfinite(quantize_scaled(w, "f16")).all() and quantize_scaled(w, "f16")[0] == np.float32(2e-38)
