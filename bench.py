@@ -188,6 +188,49 @@ def group_leg(shots_per_model):
             "nt": ws[0]["nt"], "variant": tile, "finite": finite}
 
 
+def group_gradient_leg(shots_per_model=8):
+    """The uneven-grid workload's FWI gradients: sixteen VTI + CPML models of
+    different sizes, ``shots_per_model`` shots each, nt = 3001 + itlag,
+    residuals = gathers of the model scaled by 1.02 minus the model's own,
+    float16-scaled snapshots of every step, float64 gradient norms, one
+    pam_group_gradient call (the members in turn, each batched and on two
+    lanes)."""
+    import numpy as np
+    import torch
+
+    from pseudoacousticmodeling_amd import _lib, workloads
+    from pseudoacousticmodeling_amd.engine import Group
+
+    ws = workloads.uneven_vti_group(shots_per_model)
+    engs = [make_engine(w, shots_per_model) for w in ws]
+    res = []
+    for w, e in zip(ws, engs):
+        dcal = e.forward(0, shots_per_model)
+        e.set_model((w["vp"] * np.float32(1.02)).astype(np.float32), w["eps"], w["delta"], w["theta"])
+        res.append(e.forward(0, shots_per_model) - dcal)
+        e.set_model(w["vp"], w["eps"], w["delta"], w["theta"])
+        e.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_ALL_F16)
+        del dcal
+    grp = Group(engs)
+    grp.gradient(res)  # tunes
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    outs, norms = grp.gradient(res)
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - t0
+    n = shots_per_model * len(ws)
+    out = {"unit": "s/shot-gradient", "value": tg / n, "seconds": tg, "models": len(ws),
+           "shots_per_model": shots_per_model, "shot_gradients": n, "nt": ws[0]["nt"],
+           "storage": "float16-scaled snapshots of every step", "gradient_norm2_fp64": norms,
+           "finite": all(bool(torch.isfinite(o).all().item()) for o in outs) and all(np.isfinite(norms))}
+    grp.close()
+    for e in engs:
+        e.close()
+    del res, outs
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(w, seconds):
     """Time the oracle's OpenMP build (float32 storage, float64 arithmetic,
     separate psi / zeta / update passes as the reference's Numba kernels run
@@ -413,6 +456,13 @@ def main(argv=None):
         print(f"bench: uneven-grid group {group['value']:.1f} Gpt-updates/s together, "
               f"{group['per_model_loop']:.1f} model by model", file=sys.stderr, flush=True)
 
+    group_grad = None
+    if not args.no_group:
+        group_grad = group_gradient_leg()
+        print(f"bench: uneven-grid group FWI gradients {group_grad['value'] * 1e3:.1f} ms/shot-gradient "
+              f"({group_grad['shot_gradients']} shot-gradients, {group_grad['seconds']:.2f} s)",
+              file=sys.stderr, flush=True)
+
     autograd = None
     if not args.no_autograd:
         autograd = autograd_leg()
@@ -465,6 +515,7 @@ def main(argv=None):
         "fwi_gradient": fwi,
         "uneven_group": group,
         "autograd_gradient": autograd,
+        "uneven_group_gradient": group_grad,
         "other_configs": others,
         "cpu_baseline": cpu,
         "finite": finite,

@@ -334,6 +334,20 @@ int pam_group_destroy(pam_group *grp);
 int pam_group_forward(pam_group *grp, const int *nshots, float *const *seis, double *loop_ms);
 /* the step-kernel variant of the last pam_group_forward (-1 before) */
 int pam_group_get_tile(const pam_group *grp);
+/* FWI gradients of the members: member i's pam_gradient g_m for shots
+ * 0 .. nshots[i]-1 and its residual[i] (device [nshots[i]][nt_data][nrec]),
+ * into g_m[i] (device [nz][nx]), each with its own storage, batch and lane
+ * options (float16-scaled snapshots: PAM_STORE_ALL_F16, or
+ * PAM_STORE_REVOLVE_F16 slots).  The members run in turn and each one's run
+ * scratch (snapshots, checkpoints, slots) is released before the next one
+ * starts, so the group needs the largest member's scratch, not the sum.
+ * g_norm2 (host [n], or NULL): sum of g_m[i]^2 over the interior in float64,
+ * summed in a fixed order (the same inputs give the same bits); loop_ms (or
+ * NULL): the members' step-loop GPU time.  Errors report through member 0.
+ * Extends the reference's per-model FWI (src/Inversion.py:55-86, one model
+ * per run) to a list of models of different sizes. */
+int pam_group_gradient(pam_group *grp, const int *nshots, const float *const *residual, float *const *g_m,
+                       double *g_norm2, double *loop_ms);
 
 #ifdef __cplusplus
 }

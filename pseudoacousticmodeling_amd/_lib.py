@@ -36,6 +36,7 @@ EXPORTS = [
     "pam_get_field", "pam_forward", "pam_rtm", "pam_gradient", "pam_misfit", "pam_forward_adjoint",
     "pam_last_timing", "pam_group_create", "pam_group_destroy", "pam_group_forward", "pam_group_get_tile",
     "pam_smooth_model", "pam_mute", "pam_misfit_noalloc", "pam_adjoint_reset", "pam_adjoint_step",
+    "pam_group_gradient",
 ]
 
 
@@ -93,6 +94,8 @@ def load():
     lib.pam_group_destroy.argtypes = [vp]
     lib.pam_group_forward.argtypes = [vp, ctypes.POINTER(I), ctypes.POINTER(vp), ctypes.POINTER(D)]
     lib.pam_group_get_tile.argtypes = [vp]
+    lib.pam_group_gradient.argtypes = [vp, ctypes.POINTER(I), ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                       ctypes.POINTER(D), ctypes.POINTER(D)]
     lib.pam_smooth_model.argtypes = [P, P, P, I, I, D, I, I, vp]
     DP = ctypes.POINTER(D)
     lib.pam_mute.argtypes = [P, I, I, I, DP, DP, DP, DP, D, D, D, D, I, vp]

@@ -1676,6 +1676,71 @@ extern "C" int pam_group_forward(pam_group *g, const int *nshots, float *const *
 
 extern "C" int pam_group_get_tile(const pam_group *g) { return g ? g->last_tile : -1; }
 
+static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, bool grad, float *image, float *ilum,
+                       float *g_eps, float *g_del, float *g_th);
+
+// The scratch of the whole-run passes (snapshots, checkpoints, revolve slots,
+// accumulators, staging) released once the context's streams are idle; the
+// step-loop graphs that bake their addresses in go with them.  The next run
+// allocates again.
+static void release_run_scratch(pam_ctx *ctx) {
+    if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+    for (hipStream_t l : ctx->lane_st)
+        if (l) (void)hipStreamSynchronize(l);
+    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
+    struct Buf {
+        void **p;
+        long long *bytes;
+    } bufs[] = {{(void **)&ctx->save, &ctx->save_bytes},       {(void **)&ctx->ckpt, &ctx->ckpt_bytes},
+                {(void **)&ctx->rvbits, &ctx->rvbits_bytes},   {(void **)&ctx->smax, &ctx->smax_bytes},
+                {(void **)&ctx->tmp, &ctx->tmp_bytes},         {(void **)&ctx->acc_ilum, &ctx->acc_bytes},
+                {(void **)&ctx->sbd, &ctx->sbd_bytes},         {(void **)&ctx->acc_multi, &ctx->acc_multi_bytes},
+                {(void **)&ctx->ilum_tmp, &ctx->ilum_tmp_bytes}, {(void **)&ctx->stage, &ctx->stage_bytes},
+                {(void **)&ctx->ex_save, &ctx->ex_save_bytes}, {(void **)&ctx->ex_work, &ctx->ex_work_bytes},
+                {(void **)&ctx->ex_gacc, &ctx->ex_gacc_bytes}};
+    for (Buf &b : bufs) {
+        dfree(*b.p);
+        *b.p = nullptr;
+        *b.bytes = 0;
+    }
+}
+
+// FWI gradients of every member (pam_gradient's g_m, with each member's own
+// storage, batch and lane options), the members in turn, each member's
+// run-scratch released before the next one starts (so the group needs the
+// largest member's scratch, not the sum); g_norm2[i] = sum of g_m[i]^2 over
+// the interior in float64 (two passes in a fixed order).
+extern "C" int pam_group_gradient(pam_group *g, const int *nshots, const float *const *residual, float *const *g_m,
+                                  double *g_norm2, double *loop_ms) {
+    if (!g || !nshots || !residual || !g_m) return PAM_E_ARG;
+    pam_ctx *c0 = g->ctx[0];
+    pam_ctx *const ctx = c0;  // errors of the group go to member 0
+    const int n = (int)g->ctx.size();
+    for (int i = 0; i < n; ++i) {
+        pam_ctx *c = g->ctx[i];
+        int r = ready(c);
+        if (r) return fail(c0, r, "group member %d: %s", i, c->err.c_str());
+        if (nshots[i] < 1 || nshots[i] > c->nshot)
+            return fail(c0, PAM_E_ARG, "group member %d: nshots %d outside [1, shots]", i, nshots[i]);
+        if (!residual[i] || !g_m[i]) return fail(c0, PAM_E_ARG, "group member %d: residual or g_m is NULL", i);
+    }
+    double ms = 0.0;
+    for (int i = 0; i < n; ++i) {
+        pam_ctx *c = g->ctx[i];
+        int r = adjoint_run(c, 0, nshots[i], residual[i], true, g_m[i], nullptr, nullptr, nullptr, nullptr);
+        if (r) return fail(c0, r, "group member %d: %s", i, c->err.c_str());
+        ms += c->loop_ms;
+        if (g_norm2) {
+            KCK(launch_sumsq(g_m[i], (long long)c->g.nz * c->g.nx, c->d_acc, c->d_acc + kMisfitParts, c->st));
+            HCK(hipMemcpyAsync(&g_norm2[i], c->d_acc + kMisfitParts, sizeof(double), hipMemcpyDeviceToHost, c->st));
+        }
+        release_run_scratch(c);  // synchronizes c's streams
+    }
+    if (loop_ms) *loop_ms = ms;
+    return PAM_OK;
+}
+
 extern "C" int pam_last_timing(const pam_ctx *ctx, double *loop_ms, double *kernel_ms, long long *kernel_launches) {
     if (!ctx) return PAM_E_ARG;
     if (loop_ms) *loop_ms = ctx->loop_ms;
@@ -2057,7 +2122,7 @@ static int ckpt_two_lanes(pam_ctx *ctx, CkptRun &R, const std::vector<int> &size
 //    in treeverse order; each adjoint step's S_t (and D_t) comes from one
 //    recomputed forward step.
 static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, bool grad, float *image, float *ilum,
-                       float *g_eps = nullptr, float *g_del = nullptr, float *g_th = nullptr) {
+                       float *g_eps, float *g_del, float *g_th) {
     int r = ready(ctx);
     if (r) return r;
     ctx->adj_next_t = -2;  // the second field set is reused
@@ -2717,7 +2782,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
 
 extern "C" int pam_rtm(pam_ctx *ctx, int shot0, int nshots, const float *data, float *image, float *ilum) {
     if (!ctx) return PAM_E_ARG;
-    return adjoint_run(ctx, shot0, nshots, data, false, image, ilum);
+    return adjoint_run(ctx, shot0, nshots, data, false, image, ilum, nullptr, nullptr, nullptr);
 }
 
 extern "C" int pam_gradient(pam_ctx *ctx, int shot0, int nshots, const float *residual, float *g_m, float *g_eps,

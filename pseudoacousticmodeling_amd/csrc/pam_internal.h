@@ -195,6 +195,9 @@ constexpr int kMisfitParts = 1024;
 int misfit_parts(long long n);
 int launch_residual(const float *dobs, const float *dcal, float *res, long long n, double *part, double *out,
                     hipStream_t st);
+// *out = sum of x[i]^2 over n floats in float64, two passes in a fixed order
+// (per-block partials into part[misfit_parts(n)], then one block)
+int launch_sumsq(const float *x, long long n, double *part, double *out, hipStream_t st);
 // the device address of the allocation-free entry points' scratch array on the
 // calling thread's current device (looked up once per device)
 int pam_noalloc_scratch(double **out);

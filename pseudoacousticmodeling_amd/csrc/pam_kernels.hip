@@ -441,6 +441,14 @@ __global__ void k_residual(const float *dobs, const float *dcal, float *res, lon
     const double b = block_sum256(sum);
     if (threadIdx.x == 0) part[blockIdx.x] = b;
 }
+// per-block partial sums of x^2 in float64 (the same fixed order as k_residual)
+__global__ void k_sumsq(const float *x, long long n, double *part) {
+    double sum = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        sum += (double)x[i] * (double)x[i];
+    const double b = block_sum256(sum);
+    if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
 // *out = the sum of the nparts partials (one block, fixed order)
 __global__ void k_sum_parts(const double *part, int nparts, double *out) {
     double sum = 0.0;
@@ -700,6 +708,12 @@ int launch_residual(const float *dobs, const float *dcal, float *res, long long 
                     hipStream_t st) {
     const int blocks = misfit_parts(n);
     hipLaunchKernelGGL(k_residual, dim3((unsigned)blocks), dim3(256), 0, st, dobs, dcal, res, n, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, (const double *)part, blocks, out);
+    return (int)hipGetLastError();
+}
+int launch_sumsq(const float *x, long long n, double *part, double *out, hipStream_t st) {
+    const int blocks = misfit_parts(n);
+    hipLaunchKernelGGL(k_sumsq, dim3((unsigned)blocks), dim3(256), 0, st, x, n, part);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, (const double *)part, blocks, out);
     return (int)hipGetLastError();
 }

@@ -210,6 +210,25 @@ class Group:
         self.loop_ms = ms.value
         return outs
 
+    def gradient(self, residuals, nshots=None):
+        """FWI gradients g_m of every engine for its residual (device
+        [nshots[i]][nt_data][nrec]; nshots defaults to each residual's first
+        dimension), the members in turn (pam_group_gradient).  Returns (list of
+        device [nz][nx] tensors, list of float64 |g_m|^2)."""
+        n = len(self.engines)
+        if nshots is None:
+            nshots = [int(r.shape[0]) for r in residuals]
+        res = [_f32(r, e.device) for r, e in zip(residuals, self.engines)]
+        outs = [torch.empty((e.nz, e.nx), dtype=torch.float32, device=e.device) for e in self.engines]
+        ns = (ctypes.c_int * n)(*[int(k) for k in nshots])
+        rp = (ctypes.c_void_p * n)(*[r.data_ptr() for r in res])
+        gp = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outs])
+        norms = (ctypes.c_double * n)()
+        ms = ctypes.c_double()
+        L.check(self.engines[0].ctx, self.lib.pam_group_gradient(self.grp, ns, rp, gp, norms, ctypes.byref(ms)))
+        self.loop_ms = ms.value
+        return outs, [float(v) for v in norms]
+
     def tile(self):
         return self.lib.pam_group_get_tile(self.grp)
 

@@ -319,3 +319,42 @@ def test_uneven_vti_group_of_sixteen_equals_each_model_alone():
     grp.close()
     for e in engs:
         e.close()
+
+
+def test_uneven_vti_group_gradients_with_float16_snapshots():
+    """The uneven-grid workload's FWI gradients as one group call (members in
+    turn, float16-scaled snapshots, float64 gradient norms): the smallest
+    model's gradient against the float64 oracle, with float16 snapshots
+    (printed; bounded by that storage's error) and with float32 checkpoints
+    (within 1e-4); every model's norm equals the float64 sum of squares of the
+    gradient returned."""
+    import torch
+
+    from pseudoacousticmodeling_amd.engine import Group
+
+    ws = W.uneven_vti_group(shots_per_model=1, nt_data=82)
+    pbs = [_pb(w) for w in ws]
+    print(f"case uneven group gradients: {len(ws)} models x 1 shot, nt={pbs[0]['g'].nt}", flush=True)
+    engs = [_engine(pb, 1) for pb in pbs]
+    res = [torch.as_tensor(_residual(pb, e, 1), device="cuda") for pb, e in zip(pbs, engs)]
+    for e in engs:
+        e.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_ALL_F16)
+    grp = Group(engs)
+    outs, norms = grp.gradient(res)
+    for i, (o, nrm) in enumerate(zip(outs, norms)):
+        o = o.cpu().numpy()
+        assert np.isfinite(o).all() and np.abs(o).max() > 0, ws[i]["name"]
+        n64 = float(np.sum(o.astype(np.float64) ** 2))
+        assert abs(nrm - n64) <= 1e-12 * n64
+    ref64 = _adjoint_oracle(pbs[0], res[0].cpu().numpy(), True, "f64")["image"]
+    e16 = rel_l2(outs[0].cpu().numpy(), ref64)
+    engs[0].set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_CHECKPOINT)
+    g32 = engs[0].gradient(res[0])["g_m"].cpu().numpy()
+    e32 = rel_l2(g32, ref64)
+    print(f"  {ws[0]['name']}: gradient rel_l2 vs float64 oracle, float16 snapshots {e16:.2e}, "
+          f"float32 checkpoints {e32:.2e}", flush=True)
+    assert e32 <= TOL
+    assert e16 <= 2e-3
+    grp.close()
+    for e in engs:
+        e.close()

@@ -86,3 +86,37 @@ def test_group_in_sub_groups_equals_each_model_alone(lanes):
         for i, (o, r) in enumerate(zip(outs, refs)):
             assert np.array_equal(o, r), f"member {i} differs (call {call})"
     grp.close()
+
+
+@pytest.mark.parametrize("storage", [_lib.PAM_STORE_CHECKPOINT, _lib.PAM_STORE_ALL_F16, _lib.PAM_STORE_REVOLVE_F16])
+def test_group_gradient_equals_each_model_alone(storage):
+    """pam_group_gradient: every member's g_m is the one its own pam_gradient
+    gives, bit for bit, with its storage option (float16-scaled snapshots or
+    revolve slots included), on a second call too (the run scratch released
+    between members is allocated again); each |g_m|^2 is the float64 sum of
+    squares of that gradient."""
+    import torch
+
+    from pseudoacousticmodeling_amd.engine import Group
+    from tests.test_gpu_adjoint import _data
+
+    print(f"case group gradient storage={storage}", flush=True)
+    shots = [2, 1, 3]
+    pbs = [make_problem(1, 1, nz=nz, nx=nx, nshots=k, seed=790 + i, nt=181)
+           for i, ((nz, nx), k) in enumerate(zip(SIZES, shots))]
+    engs = [make_engine(pb, max_shots=k) for pb, k in zip(pbs, shots)]
+    res = [torch.as_tensor(_data(pb, k, seed=33 + i), device="cuda")
+           for i, (pb, k) in enumerate(zip(pbs, shots))]
+    for e in engs:
+        e.set_option(_lib.PAM_OPT_STORAGE, storage)
+    refs = [e.gradient(r)["g_m"].cpu().numpy() for e, r in zip(engs, res)]
+    grp = Group(engs)
+    for call in range(2):
+        outs, norms = grp.gradient(res)
+        for i, (o, ref, nrm) in enumerate(zip(outs, refs, norms)):
+            o = o.cpu().numpy()
+            assert np.abs(o).max() > 0
+            assert np.array_equal(o, ref), f"member {i} differs (call {call})"
+            n64 = float(np.sum(o.astype(np.float64) ** 2))
+            assert abs(nrm - n64) <= 1e-12 * n64, (nrm, n64)
+    grp.close()
