@@ -373,19 +373,29 @@ def main(argv=None):
     per_launch = eng.get_option(_lib.PAM_OPT_BATCH)
     upd = workloads.updates(w) * args.steps
     value = upd / secs / 1e9
-    # the dominant kernel's average duration: one more run of the same workload with
-    # HIP events around every step-kernel launch (launched one by one, not replayed)
+    # the dominant kernel's average duration: after one batch's run, the step
+    # kernel launched 200 times back to back on that batch's fields between two
+    # HIP events on its stream (pam_forward with PAM_OPT_PROFILE 2) ...
+    eng.set_option(_lib.PAM_OPT_PROFILE, 2)
+    eng.forward(0, per_launch, out=seis[:per_launch])
+    t = eng.timing()
+    kern_ms, launches = t["kernel_ms"], t["kernel_launches"]
+    avg_ms = kern_ms / max(launches, 1)
+    # ... and one more run of the whole workload with HIP events around every
+    # step-kernel launch (launched one by one; each event pair adds the launch's
+    # completion signalling to its duration)
     eng.set_option(_lib.PAM_OPT_PROFILE, 1)
     eng.forward(0, S, out=seis)
     t = eng.timing()
     eng.set_option(_lib.PAM_OPT_PROFILE, 0)
-    kern_ms, launches = t["kernel_ms"], t["kernel_launches"]
-    avg_ms = kern_ms / max(launches, 1)
+    ev_avg_ms = t["kernel_ms"] / max(t["kernel_launches"], 1)
+    ev_launches = t["kernel_launches"]
     fused = variant in (4, 5, 6)  # the k_march variants update psi in the step kernel
     bytes_launch = step_kernel_bytes(w, per_launch, fused)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    print(f"bench: {secs:.3f} s for {args.steps} runs, {per_launch} shots per launch, step kernel variant {variant} avg {avg_ms * 1e3:.1f} us over {launches} launches, "
-          f"seis finite={finite}", file=sys.stderr, flush=True)
+    print(f"bench: {secs:.3f} s for {args.steps} runs, {per_launch} shots per launch, step kernel variant {variant} "
+          f"avg {avg_ms * 1e3:.1f} us over {launches} back-to-back launches ({ev_avg_ms * 1e3:.1f} us with events "
+          f"around each of {ev_launches}), seis finite={finite}", file=sys.stderr, flush=True)
 
     lanes = eng.get_option(_lib.PAM_OPT_LANES)
     n_launch = -(-S // per_launch) * w["nt"] * args.steps
@@ -409,7 +419,7 @@ def main(argv=None):
         pavg = t["kernel_ms"] / max(t["kernel_launches"], 1)
         plain = {"value": workloads.updates(w) / ps / 1e9, "unit": "Gpt-updates/s", "seconds": ps,
                  "k_step_avg_us": pavg * 1e3,
-                 "k_step_gbs": step_kernel_bytes(w, 1, eng.get_option(_lib.PAM_OPT_TILE) >= 4) / (pavg * 1e-3) / 1e9,
+                 "k_step_gbs": step_kernel_bytes(w, 1, eng.get_option(_lib.PAM_OPT_TILE) in (4, 5, 6)) / (pavg * 1e-3) / 1e9,
                  "batched_speedup": ps / (secs / args.steps)}
         print(f"bench: plain per-shot loop {ps:.3f} s", file=sys.stderr, flush=True)
     fwi = None
@@ -507,7 +517,13 @@ def main(argv=None):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,
                      "kernel": "step kernel variant %d" % variant,
                      "bytes_per_launch": bytes_launch, "avg_launch_us": avg_ms * 1e3,
-                     "avg_launch_note": "each step launch timed alone (one lane, HIP events)",
+                     "avg_launch_note": "the step kernel launched 200 times back to back on one batch's fields "
+                                        "after its run, between two HIP events on its stream (no CPML pre-pass "
+                                        "between the launches)",
+                     "avg_launch_us_events_each": ev_avg_ms * 1e3,
+                     "events_each_note": "every step launch of one more run of the workload bracketed by its own "
+                                         "HIP events (one lane); the event pair adds the launch's completion "
+                                         "signalling to each duration",
                      "aggregate_GBs": aggregate, "aggregate_frac": aggregate / HBM_PEAK_GBS,
                      "aggregate_note": "algorithmic bytes of every step launch of the timed runs / their wall "
                                        "time (lanes overlapping, CPML pre-pass included)"},

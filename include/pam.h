@@ -99,7 +99,11 @@ enum pam_option {
                               gradient needs S_{t+1} and S_{t-1}) */
     PAM_OPT_PROFILE = 9,   /* 1: HIP events around every step kernel of pam_forward (the
                               kernel time pam_last_timing reports; launches are then not
-                              graph-replayed); 0 (default): no per-launch events */
+                              graph-replayed); 2: after the run, the step kernel alone
+                              launched 200 times back to back on the last batch's fields
+                              between two HIP events (no per-launch event overhead; the
+                              fields and that batch's gathers are overwritten); 0
+                              (default): no kernel timing */
     PAM_OPT_SNAPSHOTS = 10, /* checkpoint slots of PAM_STORE_REVOLVE_* (default 50, >= 1) */
     PAM_OPT_LANES = 11,     /* pam_forward: 2 (default) advances two batches at once on two
                                streams the context creates together (default priority),
@@ -276,7 +280,8 @@ int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const float *dseis,
 
 /* Elapsed GPU milliseconds of the last whole-run call's step loop and, with
  * PAM_OPT_PROFILE 1, the summed duration and the number of its step-kernel
- * launches (HIP events around each); with PAM_OPT_PROFILE 0 kernel_ms is the
+ * launches (HIP events around each; with 2, of the back-to-back timing
+ * launches after the run); with PAM_OPT_PROFILE 0 kernel_ms is the
  * loop time and kernel_launches the number of steps.  After pam_rtm /
  * pam_gradient, kernel_launches counts the time steps run (forward,
  * recomputed and adjoint) over all shots' batches. */

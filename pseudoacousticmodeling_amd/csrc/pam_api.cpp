@@ -391,7 +391,8 @@ extern "C" int pam_set_option(pam_ctx *ctx, int key, long long v) {
             ctx->adj_range = (int)v;
             return PAM_OK;
         case PAM_OPT_PROFILE:
-            ctx->profile = v ? 1 : 0;
+            if (v < 0 || v > 2) return fail(ctx, PAM_E_ARG, "profile must be 0, 1 or 2");
+            ctx->profile = (int)v;
             return PAM_OK;
         case PAM_OPT_SNAPSHOTS:
             if (v < 1 || v > 1000000) return fail(ctx, PAM_E_ARG, "snapshots must be in [1, 1e6]");
@@ -1004,6 +1005,48 @@ static int finish_timing(pam_ctx *ctx, long long nk) {
     return PAM_OK;
 }
 
+// PAM_OPT_PROFILE 2: the step kernel alone, kStepTimingReps launches back to
+// back on the fields of the run's last batch (the forward instance: source
+// injection and receiver gather into seis on, steps itlag .. ; no CPML
+// pre-pass between them, psi as the run left it), bracketed by two HIP events
+// on the context's stream: kernel_ms / kernel_launches of pam_last_timing.
+// The fields and the first nb gathers of seis are overwritten.
+static constexpr int kStepTimingReps = 200;
+static int time_step_kernel(pam_ctx *ctx, int nb, int tile, float *seis, long long nd) {
+    StepArgs a;
+    base_step_args(ctx, a, nb);
+    a.flags |= F_SRC | F_GATHER;
+    a.seis = seis;
+    a.dstride = nd;
+    FieldSet &f = ctx->fs;
+    if (ctx->abc == PAM_CPML) {
+        const long long in = f.psi_flip ? ctx->psi_half : 0, out = f.psi_flip ? 0 : ctx->psi_half;
+        a.psix_in = f.psix + in;
+        a.psiz_in = f.psiz + in;
+        a.psix = f.psix + out;
+        a.psiz = f.psiz + out;
+        a.zetax = f.zetax;
+        a.zetaz = f.zetaz;
+    }
+    const int span = std::max(1, ctx->g.nt - ctx->g.itlag);
+    float *c = f.U0, *p = f.U1;
+    for (int i = -4; i < kStepTimingReps; ++i) {  // 4 untimed launches first
+        if (i == 0) HCK(hipEventRecord(ctx->e0, ctx->st));
+        a.cur = c;
+        a.prv = p;
+        a.k = ctx->g.itlag + ((i + 4) % span);
+        KCK(launch_step(ctx->phys, ctx->abc == PAM_CPML, tile, a, ctx->st, nullptr));
+        std::swap(c, p);
+    }
+    HCK(hipEventRecord(ctx->e1, ctx->st));
+    HCK(hipEventSynchronize(ctx->e1));
+    float ms = 0.f;
+    HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
+    ctx->kernel_ms = ms;
+    ctx->launches = kStepTimingReps;
+    return PAM_OK;
+}
+
 // Runs `body` (host code that enqueues the launches of a step loop on the
 // context's stream) through an instantiated hipGraph cached under `key`:
 // captured on first use, replayed afterwards.  The launches must be
@@ -1062,7 +1105,7 @@ static int run_forward_loop(pam_ctx *ctx, StepArgs &a, int tile, bool timed) {
 // the same on field set f and stream st
 static int run_forward_loop_fs(pam_ctx *ctx, FieldSet &f, hipStream_t st, StepArgs &a, int tile, bool timed) {
     const int nt = ctx->g.nt;
-    const bool events = timed && ctx->profile;
+    const bool events = timed && ctx->profile == 1;
     if (events) {
         int r = ensure_events(ctx, 2 * (size_t)nt);
         if (r) return r;
@@ -1275,7 +1318,9 @@ extern "C" int pam_forward(pam_ctx *ctx, int shot0, int nshots, float *seis) {
             HCK(hipMemcpyAsync(seis + (long long)b0 * nd, ctx->stage, (size_t)nb * nd * sizeof(float),
                                hipMemcpyDeviceToDevice, ctx->st));
     }
-    return bad_report(ctx, "pam_forward");
+    r = bad_report(ctx, "pam_forward");
+    if (r == PAM_OK && ctx->profile == 2) r = time_step_kernel(ctx, std::min(B, nshots), tile, seis, nd);
+    return r;
 }
 
 // ------------------------------------------------------------ preprocessing
