@@ -373,11 +373,13 @@ def main(argv=None):
     per_launch = eng.get_option(_lib.PAM_OPT_BATCH)
     upd = workloads.updates(w) * args.steps
     value = upd / secs / 1e9
-    # the dominant kernel's average duration: after one batch's run, the step
-    # kernel launched 200 times back to back on that batch's fields between two
-    # HIP events on its stream (pam_forward with PAM_OPT_PROFILE 2) ...
+    # the dominant kernel's average duration: after one more run of the workload
+    # (one lane, the same shots per launch), the step kernel launched 200 times
+    # back to back on the last batch's fields between two HIP events on its
+    # stream (pam_forward with PAM_OPT_PROFILE 2) ...
     eng.set_option(_lib.PAM_OPT_PROFILE, 2)
-    eng.forward(0, per_launch, out=seis[:per_launch])
+    eng.forward(0, S, out=seis)
+    assert eng.get_option(_lib.PAM_OPT_BATCH) == per_launch
     t = eng.timing()
     kern_ms, launches = t["kernel_ms"], t["kernel_launches"]
     avg_ms = kern_ms / max(launches, 1)
