@@ -392,7 +392,7 @@ def main(argv=None):
     eng.set_option(_lib.PAM_OPT_PROFILE, 0)
     ev_avg_ms = t["kernel_ms"] / max(t["kernel_launches"], 1)
     ev_launches = t["kernel_launches"]
-    fused = variant in (4, 5, 6)  # the k_march variants update psi in the step kernel
+    fused = variant in (4, 5, 6, 10, 11)  # variants that update psi in the step kernel
     bytes_launch = step_kernel_bytes(w, per_launch, fused)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     print(f"bench: {secs:.3f} s for {args.steps} runs, {per_launch} shots per launch, step kernel variant {variant} "
@@ -421,7 +421,7 @@ def main(argv=None):
         pavg = t["kernel_ms"] / max(t["kernel_launches"], 1)
         plain = {"value": workloads.updates(w) / ps / 1e9, "unit": "Gpt-updates/s", "seconds": ps,
                  "k_step_avg_us": pavg * 1e3,
-                 "k_step_gbs": step_kernel_bytes(w, 1, eng.get_option(_lib.PAM_OPT_TILE) in (4, 5, 6)) / (pavg * 1e-3) / 1e9,
+                 "k_step_gbs": step_kernel_bytes(w, 1, eng.get_option(_lib.PAM_OPT_TILE) in (4, 5, 6, 10, 11)) / (pavg * 1e-3) / 1e9,
                  "batched_speedup": ps / (secs / args.steps)}
         print(f"bench: plain per-shot loop {ps:.3f} s", file=sys.stderr, flush=True)
     fwi = None

@@ -86,6 +86,7 @@ struct pam_ctx {
     // step-kernel variant chosen by the autotuner, per batch size
     std::map<int, int> tuned;
     std::map<int, double> tuned_ms;  // its time per shot-step
+    std::map<int, int> tuned_lanes;  // variant per lane batch, timed on two lanes at once
     int auto_batch = 0;              // shots per launch chosen by the autotuner (0: not yet)
     int last_tile = -1;              // variant of the last whole run
     int last_batch = 0;              // shots per launch of the last whole run
@@ -1183,6 +1184,76 @@ static int lanes_join(pam_ctx *ctx, int r0) {
     return PAM_OK;
 }
 
+// The autotuner for runs on two lanes: every variant timed with both lanes
+// stepping Bl shots at once (a few plain steps each, lanes forked from and
+// joined into the context's stream, HIP events on it), so that what one lane's
+// CPML pre-pass and launch gaps leave to the other counts.  The fields of both
+// lanes' slots are cleared before and after.
+static int autotune_lanes(pam_ctx *ctx, int Bl, int *best) {
+    *best = kDefaultTile;
+    int r = ensure_lanes(ctx);
+    if (r) return r;
+    FieldSet lane1 = ctx->fs;
+    const long long off = (long long)Bl * ctx->plane;
+    lane1.U0 += off;
+    lane1.U1 += off;
+    if (ctx->abc == PAM_CPML) {
+        lane1.psix += off;
+        lane1.psiz += off;
+        lane1.zetax += off;
+        lane1.zetaz += off;
+    }
+    FieldSet *fsl[2] = {&ctx->fs, &lane1};
+    StepArgs a[2];
+    for (int l = 0; l < 2; ++l) base_step_args(ctx, a[l], Bl);
+    r = zero_shot_state(ctx, 2 * Bl, true, true);
+    if (r) return r;
+    double best_ms = 1e300;
+    for (int t = 0; t < tile_count(); ++t) {
+        float *c[2] = {fsl[0]->U0, fsl[1]->U0}, *p[2] = {fsl[0]->U1, fsl[1]->U1};
+        auto steps = [&](int k0, int n) -> int {
+            for (int i = 0; i < n; ++i)
+                for (int l = 0; l < 2; ++l) {
+                    int rr = fwd_step_fs(ctx, *fsl[l], a[l], t, c[l], p[l], k0 + i, nullptr, ctx->lane_st[l]);
+                    if (rr) return rr;
+                    std::swap(c[l], p[l]);
+                }
+            return PAM_OK;
+        };
+        r = lanes_fork(ctx);
+        if (r) return r;
+        r = lanes_join(ctx, steps(0, 2));
+        if (r) return r;
+        HCK(hipEventRecord(ctx->e0, ctx->st));
+        r = lanes_fork(ctx);
+        if (r) return r;
+        r = lanes_join(ctx, steps(2, 6));
+        if (r) return r;
+        HCK(hipEventRecord(ctx->e1, ctx->st));
+        HCK(hipEventSynchronize(ctx->e1));
+        float ms = 0.f;
+        HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            *best = t;
+        }
+    }
+    return zero_shot_state(ctx, 2 * Bl, true, true);
+}
+// variant for two lanes of Bl shots: the option if set, else tuned on two lanes
+static int choose_tile_lanes(pam_ctx *ctx, int Bl) {
+    if (ctx->tile >= 0) return ctx->tile;
+    auto it = ctx->tuned_lanes.find(Bl);
+    if (it != ctx->tuned_lanes.end()) return it->second;
+    int best = kDefaultTile;
+    if (autotune_lanes(ctx, Bl, &best) != PAM_OK) {
+        (void)hipGetLastError();
+        best = choose_tile(ctx, Bl, true);
+    }
+    ctx->tuned_lanes[Bl] = best;
+    return best;
+}
+
 // pam_forward on two lanes of Bl shots per launch: batches alternate between
 // the two lane streams, each lane with its own shot slots, source slots,
 // gather staging and cached graph; the lanes start after the work already
@@ -1192,7 +1263,7 @@ static int lanes_join(pam_ctx *ctx, int r0) {
 static int forward_two_lanes(pam_ctx *ctx, int shot0, int nshots, float *seis, int Bl, long long nd) {
     int rl = ensure_lanes(ctx);
     if (rl) return rl;
-    const int tile = choose_tile(ctx, Bl, true);
+    const int tile = choose_tile_lanes(ctx, Bl);
     ctx->last_tile = tile;
     ctx->last_batch = Bl;
     const bool staged = ctx->graph != 0;

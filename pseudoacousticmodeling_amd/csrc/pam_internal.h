@@ -136,7 +136,8 @@ struct AdjArgs {
 
 // V > 0: k_step with V cells per lane and TZ rows per wavefront; V == 0: k_tile
 // with TZ rows per block; V == -1: k_march with TZ chunks; V == -2: k_tile_pipe
-// with interior tiles in groups of TZ shots
+// with interior tiles in groups of TZ shots; V == -3: the same, its edge tiles
+// updating the CPML psi themselves
 struct TileShape {
     int V, TZ;
 };

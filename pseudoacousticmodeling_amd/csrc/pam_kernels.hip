@@ -564,7 +564,7 @@ TileShape tile_shape(int idx) {
     if (idx < 0 || idx >= kNumTiles) idx = 0;
     return kTiles[idx];
 }
-bool tile_fuses_psi(int idx) { return tile_shape(idx).V == -1; }
+bool tile_fuses_psi(int idx) { return tile_shape(idx).V == -1 || tile_shape(idx).V == -3; }
 
 int launch_step_acoustic_cerjan(int, const StepArgs &, hipStream_t, const hipEvent_t *);
 int launch_step_acoustic_cpml(int, const StepArgs &, hipStream_t, const hipEvent_t *);

@@ -52,12 +52,6 @@ __device__ __forceinline__ void march_load_cell(MarchCell<PHYS> &m, const StepAr
     }
 }
 
-__device__ __forceinline__ void f4_to(float *w, float4 t) {
-    w[0] = t.x;
-    w[1] = t.y;
-    w[2] = t.z;
-    w[3] = t.w;
-}
 
 // 4 consecutive cells starting at column c: vector store when all are in the
 // stencil interior [4, nxp-4), else per cell

@@ -78,6 +78,12 @@ __device__ __forceinline__ void vldg(float *d, const float *rowp, int col, int P
         for (int j = 0; j < V; ++j) d[j] = 0.0f;
     }
 }
+__device__ __forceinline__ void f4_to(float *w, float4 t) {
+    w[0] = t.x;
+    w[1] = t.y;
+    w[2] = t.z;
+    w[3] = t.w;
+}
 // row segment with a 4-cell halo each side: w[0..V+7], cell j of the lane at w[4+j]
 template <int V>
 __device__ __forceinline__ void load_row_halo(float *w, const float *rowp, int col, int P) {

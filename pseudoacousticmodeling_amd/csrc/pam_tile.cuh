@@ -546,12 +546,18 @@ __device__ __forceinline__ void pipe_barrier() {
 // distinct variables used in turn by an unrolled loop, so that the compiler
 // can tell that the reads of one do not alias the DMA into the other (it
 // would otherwise wait for that DMA before every read).
+constexpr int kPipeNQ = 24 * 18;         // float4 of a staged 16-row tile with its 4-cell halo
+constexpr int kPipeNB = kPipeNQ + 256;   // + U_{k-1} of the 16 x 64 cells, one float4 per thread
+// The two LDS buffers of the pipelined kernels, shared by their interior and
+// (fused-psi) edge blocks
+static __shared__ float4 g_pipe_b0[kPipeNB], g_pipe_b1[kPipeNB];
+
 template <int PHYS, int FM, int SG>
 __device__ __forceinline__ void pipe_inner(const StepArgs &a, const TileStep m, const int x0, const int z0,
                                            const int s0, const int ns) {
-    constexpr int TX = 64, ROWS = 24, Q = (TX + 8) / 4, NQ = ROWS * Q;  // 432 float4 per staged tile
-    constexpr int NB = NQ + 256;  // + U_{k-1} of the 16 x 64 cells, one float4 per thread
-    __shared__ float4 su0[NB], su1[NB];
+    constexpr int TX = 64, ROWS = 24, Q = (TX + 8) / 4, NQ = kPipeNQ;
+    float4 *const su0 = g_pipe_b0;
+    float4 *const su1 = g_pipe_b1;
     __shared__ float4 sd[(PHYS == 2) ? ROWS * (TX / 4) : 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int flags = m.flags & FM;
@@ -719,6 +725,301 @@ __device__ __forceinline__ void pipe_inner(const StepArgs &a, const TileStep m, 
     vst<4>(at(m.prv + (long long)(s0 + ns - 1) * a.sstride, ro), prev_out);
 }
 
+// One edge tile (x0, z0) of shot s with CPML, forward flags only, the psi of
+// the step computed in the block instead of by a pre-pass: new psi_x on the
+// tile's rows over columns x0-4 .. x0+67 and new psi_z on rows z0-4 .. z0+19
+// (psi = a psi_prev + b dU, U_k staged with an 8-cell halo), written for the
+// tile's own strip cells and kept in LDS for the zeta update, which then runs
+// as in the 16-row tile.  The halo psi is recomputed from the previous step's
+// psi (double-buffered), never read from a neighbour, and every value is
+// formed by the same expressions as in k_psi and the 16-row tile, so the
+// results are theirs bit for bit.
+template <int PHYS>
+__device__ __forceinline__ void edge_fused_body(const StepArgs &a, const TileStep m, const int x0, const int z0,
+                                                const int s) {
+    constexpr int SQ = 20;                  // float4 per staged row: columns x0-8 .. x0+71
+    constexpr int SR = 32;                  // staged rows z0-8 .. z0+23
+    constexpr int NS = SR * SQ;
+    constexpr int NL = (NS + 255) / 256;
+    constexpr int GX = 18;                  // psi_x groups of a row: columns x0-4 .. x0+67
+    static_assert(NS <= kPipeNB && 16 * GX + 24 * 16 <= kPipeNB, "fused edge tile exceeds the pipe buffers");
+    float4 *const su = g_pipe_b0;              // U_k
+    float4 *const pxl = g_pipe_b1;             // new psi_x [16][GX]
+    float4 *const pzl = g_pipe_b1 + 16 * GX;   // new psi_z [24][16], rows z0-4 .. z0+19
+    __shared__ float4 sd[(PHYS == 2) ? 24 * 16 : 1];
+
+    const int P = a.P, nzp = a.nzp, nxp = a.nxp, N = a.N;
+    const long long so = (long long)s * a.sstride;
+    const float *__restrict__ cur = m.cur + so;
+    float *__restrict__ prv = m.prv + so;
+    const float *__restrict__ pxi = a.psix_in + so;
+    const float *__restrict__ pzi = a.psiz_in + so;
+    float *__restrict__ pxo = a.psix + so;
+    float *__restrict__ pzo = a.psiz + so;
+    float *__restrict__ zetax = a.zetax + so;
+    float *__restrict__ zetaz = a.zetaz + so;
+    const int tid = threadIdx.x;
+    const int flags = m.flags & F_FWD_MASK;
+    const int cx = tid & 15, rg = tid >> 4;
+    const int col = x0 + 4 * cx;
+    const int z = z0 + rg;
+    const bool xtile = x0 < N || x0 + 64 > nxp - N;  // the tile meets an x-strip (block-uniform)
+    const bool ztile = z0 < N || z0 + 16 > nzp - N;  // its rows meet a z-strip (block-uniform)
+    auto xstrip = [&](int c) { return (c >= 4 && c < N) || (c >= nxp - N && c < nxp - 4); };
+    auto zstrip = [&](int r) { return (r >= 4 && r < N) || (r >= nzp - N && r < nzp - 4); };
+
+    // ---- U_k rows z0-8 .. z0+23 with an 8-column halo (zero off the grid),
+    // U_{k-1} and the model of this thread's cells
+    float4 st[NL];
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+        const int i = tid + 256 * u;
+        const int r = i / SQ, q = i - r * SQ;
+        const int gz = z0 - 8 + r, gc = x0 - 8 + 4 * q;
+        st[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < NS && gz >= 0 && gz < nzp && gc >= 0 && gc < P)
+            st[u] = *reinterpret_cast<const float4 *>(at(cur, (unsigned)(gz * P + gc)));
+    }
+    float pv[4], v2[4], e[4], dl[4], cs[4], sn[4];
+    const bool zin = z < nzp;
+    const unsigned ro = (unsigned)((zin ? z : 0) * P + col);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) pv[j] = v2[j] = e[j] = dl[j] = cs[j] = sn[j] = 0.f;
+    if (zin) {
+        vld<4>(pv, at(prv, ro));
+        vld<4>(v2, at(a.v2, ro));
+        if (PHYS >= 1) {
+            vld<4>(e, at(a.eps, ro));
+            vld<4>(dl, at(a.del, ro));
+        }
+        if (PHYS == 2) {
+            vld<4>(cs, at(a.cs, ro));
+            vld<4>(sn, at(a.sn, ro));
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+        const int i = tid + 256 * u;
+        if (i < NS) su[i] = st[u];
+    }
+    __syncthreads();
+
+    if (PHYS == 2) {
+        // first x-derivative (undivided) of rows z0-4 .. z0+19
+        for (int i = tid; i < 24 * 16; i += 256) {
+            const int r = i >> 4, c = i & 15;
+            const float4 *rw = su + (r + 4) * SQ + c + 1;
+            float w[12], d2[4], d1[4];
+            f4_to(w, rw[0]);
+            f4_to(w + 4, rw[1]);
+            f4_to(w + 8, rw[2]);
+            xderivs<4>(w, d2, d1);
+            sd[i] = make_float4(d1[0], d1[1], d1[2], d1[3]);
+        }
+    }
+    if (xtile) {
+        // new psi_x of rows z0 .. z0+15, columns x0-4 .. x0+67 (zero off the x-strips)
+        for (int i = tid; i < 16 * GX; i += 256) {
+            const int r = i / GX, g = i - r * GX;
+            const int zz = z0 + r, c0 = x0 - 4 + 4 * g;
+            float nv[4] = {0.f, 0.f, 0.f, 0.f};
+            bool any = false;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) any = any || xstrip(c0 + j);
+            if (any && zz >= 4 && zz < nzp - 4) {
+                const float4 *rw = su + (r + 8) * SQ + g;
+                float w[12], d2[4], d1[4], pin[4], ax[4], bx[4];
+                f4_to(w, rw[0]);
+                f4_to(w + 4, rw[1]);
+                f4_to(w + 8, rw[2]);
+                xderivs<4>(w, d2, d1);
+                const unsigned pi = (unsigned)(zz * P + c0);
+                vld<4>(pin, at(pxi, pi));
+                vld<4>(ax, at(a.cax, pi));
+                vld<4>(bx, at(a.cbx, pi));
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (!xstrip(c0 + j)) continue;
+                    nv[j] = ax[j] * pin[j] + bx[j] * (d1[j] * a.idx);
+                    if (g >= 1 && g <= 16) *at(pxo, pi + j) = nv[j];
+                }
+            }
+            pxl[i] = make_float4(nv[0], nv[1], nv[2], nv[3]);
+        }
+    }
+    if (ztile) {
+        // new psi_z of rows z0-4 .. z0+19, the tile's columns (zero off the z-strips)
+        for (int i = tid; i < 24 * 16; i += 256) {
+            const int r = i >> 4, c = i & 15;
+            const int zz = z0 - 4 + r, cc = x0 + 4 * c;
+            float nv[4] = {0.f, 0.f, 0.f, 0.f};
+            if (zstrip(zz) && cc < nxp - 4) {
+                float q[9][4], pin[4], az[4], bz[4];
+#pragma unroll
+                for (int h = 0; h < 9; ++h) {
+                    if (h == 4) continue;
+                    f4_to(q[h], su[(r + h) * SQ + c + 2]);
+                }
+                const unsigned pi = (unsigned)(zz * P + cc);
+                vld<4>(pin, at(pzi, pi));
+                vld<4>(az, at(a.caz, pi));
+                vld<4>(bz, at(a.cbz, pi));
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (cc + j < 4 || cc + j >= nxp - 4) continue;
+                    const float d1 = KA1 * (q[5][j] - q[3][j]) + KA2 * (q[6][j] - q[2][j]) +
+                                     KA3 * (q[7][j] - q[1][j]) + KA4 * (q[8][j] - q[0][j]);
+                    nv[j] = az[j] * pin[j] + bz[j] * (d1 * a.idz);
+                    if (r >= 4 && r < 20 && zz < nzp) *at(pzo, pi + j) = nv[j];
+                }
+            }
+            pzl[i] = make_float4(nv[0], nv[1], nv[2], nv[3]);
+        }
+    }
+    if (PHYS == 2 || xtile || ztile) __syncthreads();  // block-uniform
+    if (!zin) return;
+
+    // ---- the derivatives of this thread's row
+    float zw[9][4];
+#pragma unroll
+    for (int h = 0; h < 9; ++h) f4_to(zw[h], su[(rg + 4 + h) * SQ + cx + 2]);
+    float w[12];
+    {
+        const float4 *rw = su + (rg + 8) * SQ + cx + 1;
+        f4_to(w, rw[0]);
+        f4_to(w + 4, rw[1]);
+        f4_to(w + 8, rw[2]);
+    }
+    float d2x[4], d1x[4];
+    xderivs<4>(w, d2x, d1x);
+    float c0[4], pzz[4], pxx[4], px[4], pz[4], pxz[4];
+    tile_derivs<PHYS>(zw, d2x, d1x, a, c0, pzz, pxx, pz, px);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) pxz[j] = 0.f;
+    if (PHYS == 2) {
+        float q[9][4];
+#pragma unroll
+        for (int h = 0; h < 9; ++h) {
+            if (h == 4) continue;
+            f4_to(q[h], sd[(rg + h) * 16 + cx]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            pxz[j] = (KA1 * (q[5][j] - q[3][j]) + KA2 * (q[6][j] - q[2][j]) + KA3 * (q[7][j] - q[1][j]) +
+                      KA4 * (q[8][j] - q[0][j])) *
+                     a.idxdz;
+    }
+    // receiver gather of the input field (no Cerjan factors with CPML)
+    if ((flags & F_GATHER) && a.rrow[z] >= 0) {
+        const int slot = a.rrow[z];
+        const int it = m.k - a.itlag;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int rr = a.rhead[(long long)slot * P + col + j];
+            while (rr >= 0) {
+                a.seis[(long long)s * a.dstride + (long long)it * a.nrec + rr] = c0[j];
+                rr = a.rnext[rr];
+            }
+        }
+    }
+    if (z < 4 || z >= nzp - 4) return;
+
+    // ---- CPML: zeta with the new psi, corrected derivatives
+    bool xlive[4], xs[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        xlive[j] = (col + j >= 4) && (col + j < nxp - 4);
+        xs[j] = xstrip(col + j);
+    }
+    const bool zs = zstrip(z) && col < nxp - 4;
+    float PXX[4], PZZ[4], PX[4], PZ[4], dpx[4], zxv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        PXX[j] = pxx[j];
+        PZZ[j] = pzz[j];
+        PX[j] = px[j];
+        PZ[j] = pz[j];
+        dpx[j] = zxv[j] = 0.f;
+    }
+    if (xtile && (xs[0] || xs[1] || xs[2] || xs[3])) {
+        float pw[12], d2p[4], d1p[4], zo[4], ax[4], bx[4];
+        const float4 *pr = pxl + rg * GX + cx;
+        f4_to(pw, pr[0]);
+        f4_to(pw + 4, pr[1]);
+        f4_to(pw + 8, pr[2]);
+        xderivs<4>(pw, d2p, d1p);
+        vld<4>(zo, at(zetax, ro));
+        vld<4>(ax, at(a.cax, ro));
+        vld<4>(bx, at(a.cbx, ro));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (!xs[j]) continue;
+            const float dps = d1p[j] * a.idx;
+            zxv[j] = ax[j] * zo[j] + bx[j] * (pxx[j] + dps);
+            dpx[j] = dps;
+            PX[j] = px[j] + dps;
+            PXX[j] = pxx[j] + dps + zxv[j];
+            *at(zetax, ro + j) = zxv[j];
+        }
+    }
+    float dpz[4] = {0.f, 0.f, 0.f, 0.f}, zzv[4] = {0.f, 0.f, 0.f, 0.f};
+    if (zs) {
+        float ps[9][4], zo[4], az[4], bz[4];
+#pragma unroll
+        for (int h = 0; h < 9; ++h) {
+            if (h == 4) continue;
+            f4_to(ps[h], pzl[(rg + h) * 16 + cx]);
+        }
+        vld<4>(zo, at(zetaz, ro));
+        vld<4>(az, at(a.caz, ro));
+        vld<4>(bz, at(a.cbz, ro));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float dps = (KA1 * (ps[5][j] - ps[3][j]) + KA2 * (ps[6][j] - ps[2][j]) +
+                               KA3 * (ps[7][j] - ps[1][j]) + KA4 * (ps[8][j] - ps[0][j])) *
+                              a.idz;
+            zzv[j] = az[j] * zo[j] + bz[j] * (pzz[j] + dps);
+            dpz[j] = dps;
+            PZ[j] = pz[j] + dps;
+            PZZ[j] = pzz[j] + dps + zzv[j];
+            if (xlive[j]) *at(zetaz, ro + j) = zzv[j];
+        }
+    }
+
+    // ---- the update (no Cerjan factors), the next step's source, the store
+    float out[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float op;
+        if (PHYS == 0) {
+            float sum = pxx[j] + pzz[j];
+            if (xs[j]) sum += dpx[j];
+            if (zs) sum += dpz[j];
+            if (xs[j]) sum += zxv[j];
+            if (zs) sum += zzv[j];
+            out[j] = lf_acoustic(c0[j], pv[j], 1.0f, 1.0f, v2[j], sum, false, op);
+        } else if (PHYS == 1) {
+            out[j] = lf_vti(c0[j], pv[j], 1.0f, 1.0f, v2[j], e[j], dl[j], PX[j], PZ[j], PXX[j], PZZ[j], false, op);
+        } else {
+            out[j] = lf_tti(c0[j], pv[j], 1.0f, 1.0f, v2[j], e[j], dl[j], cs[j], sn[j], PX[j], PZ[j], PXX[j],
+                            PZZ[j], pxz[j], false, op);
+        }
+    }
+    if ((flags & F_SRC) && z == a.ssz[s] && m.k + 1 < a.nt) {
+        const int sx = a.ssx[s];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (col + j == sx) out[j] += a.wav[m.k + 1];
+    }
+    if (xlive[0] && xlive[3]) {
+        vst<4>(at(prv, ro), out);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (xlive[j]) *at(prv, ro + j) = out[j];
+    }
+}
+
 // Blocks go to the XCDs round-robin (hardware block h to XCD h mod 8); XCD x
 // runs interior blocks [x ni8, (x+1) ni8) first, then edge blocks
 // [x ne8, (x+1) ne8): both kinds are spread over the XCDs, each XCD's share is
@@ -726,8 +1027,11 @@ __device__ __forceinline__ void pipe_inner(const StepArgs &a, const TileStep m, 
 // and the longer interior blocks start first.  Interior block b runs tile
 // b / G for shot group b % G (G = ceil(nshots / SG)); edge block e runs edge
 // tile e / nshots for shot e % nshots.
-template <int PHYS, bool CPML, int SG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PHYS < 2 ? 5 : 1))) void k_tile_pipe(
+// (the VTI instance with fused psi runs at 4 waves per SIMD: at 5 its edge body
+// spills, and the 4-wave instance measured faster)
+template <int PHYS, bool CPML, int SG, bool FUSE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    PHYS < 2 ? ((FUSE && CPML && PHYS == 1) ? 4 : 5) : 1))) void k_tile_pipe(
     const StepArgs a) {
     const TileSplit t = tile_split(a.nxp, a.nzp, a.N, a.nxt, a.nzt);
     const int G = (a.nshots + SG - 1) / SG;
@@ -750,7 +1054,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PHYS < 2 ? 
     const int ei = eb / a.nshots, s = eb - ei * a.nshots;
     int x0, z0;
     edge_tile_xz(t, a.nxt, a.nzt, ei, x0, z0);
-    tile_body_at<PHYS, CPML, 16, F_FWD_MASK, false>(a, x0, z0, s, m);
+    if constexpr (CPML && FUSE)
+        edge_fused_body<PHYS>(a, m, x0, z0, s);
+    else
+        tile_body_at<PHYS, CPML, 16, F_FWD_MASK, false>(a, x0, z0, s, m);
 }
 
 // A group of grids in one launch: block b belongs to grid g with boff[g] <= b <
@@ -797,16 +1104,40 @@ static int launch_tile_multi_t(const StepArgs *list, const int *boff, int ngrid,
 // other passes run variant 0, whose arithmetic it shares
 template <int PHYS, bool CPML, int TZ, int WPE>
 static int launch_tile_t(const StepArgs &a0, hipStream_t st, const hipEvent_t *ev);
-template <int PHYS, bool CPML, int SG>
+template <int PHYS, bool CPML, int SG, bool FUSE>
 static int launch_tile_pipe_t(const StepArgs &a0, hipStream_t st, const hipEvent_t *ev = nullptr) {
-    if ((a0.flags & ~F_FWD_MASK) != 0) return launch_tile_t<PHYS, CPML, 16, 1>(a0, st, ev);
+    if ((a0.flags & ~F_FWD_MASK) != 0) {
+        if (CPML && FUSE) {  // the variant that updates psi itself: its pre-pass, then variant 0
+            PsiArgs p;
+            p.nzp = a0.nzp;
+            p.nxp = a0.nxp;
+            p.P = a0.P;
+            p.N = a0.N;
+            p.nshots = a0.nshots;
+            p.sstride = a0.sstride;
+            p.idx = a0.idx;
+            p.idz = a0.idz;
+            p.cur = a0.cur;
+            p.psix_in = a0.psix_in;
+            p.psiz_in = a0.psiz_in;
+            p.psix = a0.psix;
+            p.psiz = a0.psiz;
+            p.cax = a0.cax;
+            p.cbx = a0.cbx;
+            p.caz = a0.caz;
+            p.cbz = a0.cbz;
+            const int r = launch_psi(p, st);
+            if (r) return r;
+        }
+        return launch_tile_t<PHYS, CPML, 16, 1>(a0, st, ev);
+    }
     StepArgs a = a0;
     a.nxt = (a.nxp + 63) / 64;
     a.nzt = (a.nzp + 15) / 16;
     const TileSplit t = tile_split(a.nxp, a.nzp, a.N, a.nxt, a.nzt);
     const long long G = (a.nshots + SG - 1) / SG;
     const long long ni8 = (t.nin * G + 7) / 8, ne8 = ((long long)t.nedge * a.nshots + 7) / 8;
-    launch_timed(k_tile_pipe<PHYS, CPML, SG>, dim3((unsigned)(8 * (ni8 + ne8))), st, ev, a);
+    launch_timed(k_tile_pipe<PHYS, CPML, SG, FUSE>, dim3((unsigned)(8 * (ni8 + ne8))), st, ev, a);
     return (int)hipGetLastError();
 }
 

@@ -87,12 +87,14 @@ def test_every_kernel_variant_matches_oracle(phys, abc):
 
 @pytest.mark.parametrize("phys,abc", CASES)
 def test_pipelined_tile_is_bit_identical(phys, abc):
-    """The pipelined variants 7 and 8 (an interior tile taken for a group of 4
+    """The pipelined variants 7..11 (an interior tile taken for a group of 2, 4
     or 8 shots, each shot's U_k tile streamed into LDS while the one before is
-    computed) compute every cell with the 16-row tile's arithmetic: seismograms
-    and final fields are variant 0's bits, for shot counts that fill the groups
-    and that do not, with receivers inside interior tiles (two on one cell) and
-    sources there and in the absorbing layer."""
+    computed; 10 and 11 also compute the CPML psi in their edge tiles instead
+    of a pre-pass) compute every cell with the 16-row tile's arithmetic:
+    seismograms and final fields (psi included, through the fields it feeds)
+    are variant 0's bits, for shot counts that fill the groups and that do
+    not, with receivers inside interior tiles (two on one cell) and sources
+    there and in the absorbing layer."""
     import torch
 
     from pseudoacousticmodeling_amd import _lib
@@ -108,7 +110,7 @@ def test_pipelined_tile_is_bit_identical(phys, abc):
     pb["sz"] = np.array([(N + 30 + 9 * s) if s % 4 else 6 for s in range(ns)], dtype=np.int32)
     pb["sx"] = np.array([N + 60 + (23 * s) % 200 for s in range(ns)], dtype=np.int32)
     outs = {}
-    for t in (0, 7, 8):
+    for t in (0, 7, 8, 9, 10, 11):
         eng = make_engine(pb, max_shots=ns)
         eng.set_option(_lib.PAM_OPT_TILE, t)
         print(f"case pipelined phys={phys} abc={abc} variant={t}", flush=True)
@@ -122,7 +124,7 @@ def test_pipelined_tile_is_bit_identical(phys, abc):
         outs[t] = (seis, fields, st.cpu().numpy())
         eng.close()
     assert np.isfinite(outs[0][0]).all() and np.abs(outs[0][0]).max() > 0
-    for t in (7, 8):
+    for t in (7, 8, 9, 10, 11):
         assert np.array_equal(outs[t][0], outs[0][0]), f"variant {t} seismograms"
         for w in (0, 1):
             assert np.array_equal(outs[t][1][w], outs[0][1][w]), f"variant {t} field {w}"
