@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
 """Throughput of the 2D finite-difference engine on one MI355X.
 
-Workload (one "step" = one pass of the hot path over one batch): forward
+Workload (one "step" = one pass of the hot path over one batch; --workload
+picks another single-grid workload for the main leg): forward
 modelling of the 64-shot pseudo-acoustic VTI + 50-cell CPML survey
 (nz=351, nx=1701, dx=dz=12.5 m -> 451 x 1801 padded cells, nt = 98 + 5001
 time steps, 1701 receivers per shot), the shots advanced in batches (the
@@ -42,6 +43,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PHYS_NAME = {0: "acoustic", 1: "VTI pseudo-acoustic", 2: "TTI pseudo-acoustic"}
 METRIC = "Gpt-updates/s & %HBM roofline; FWI gradient s/shot, batched vs plain"
 
 
@@ -307,7 +309,9 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nt-data", type=int, default=5001, help="recorded samples (default: the full 5001)")
+    ap.add_argument("--nt-data", type=int, default=0, help="recorded samples (default: the workload's full record)")
+    ap.add_argument("--workload", default="vti_cpml_64", choices=["vti_cpml_64", "tti_cpml_32", "acoustic_cpml_32"],
+                    help="the main leg's workload (default: the headline one)")
     ap.add_argument("--tile", type=int, default=-1, help="step-kernel variant (-1: the autotuner's choice)")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-shot loop measurement")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample length (0: skip)")
@@ -347,7 +351,9 @@ def main(argv=None):
         bandwidth_probe()
         return 0
     _lib.load()
-    w = workloads.vti_cpml_64(nt_data=args.nt_data)
+    build = {"vti_cpml_64": workloads.vti_cpml_64, "tti_cpml_32": workloads.tti_cpml_32,
+             "acoustic_cpml_32": workloads.acoustic_cpml_32}[args.workload]
+    w = build(nt_data=args.nt_data) if args.nt_data > 0 else build()
     S = len(w["sx"])
     print(f"bench: workload {w['name']} {w['nz'] + 2 * w['N']}x{w['nx'] + 2 * w['N']} padded, nt={w['nt']}, "
           f"{S} shots", file=sys.stderr, flush=True)
@@ -499,7 +505,7 @@ def main(argv=None):
     # measured in this run: counters need a profiler run of their own)
     traffic, traffic_note = None, "no committed counter run for this variant and batch"
     tpath = os.path.join(ROOT, "profiles", f"traffic_{w['name']}.json")
-    if os.path.exists(tpath) and args.nt_data == 5001:
+    if os.path.exists(tpath) and args.nt_data == 0:
         for rec in json.load(open(tpath)).get("launches", []):
             if rec.get("variant") == variant and rec.get("shots_per_launch") == per_launch:
                 traffic = rec["traffic_bytes"]
@@ -510,7 +516,7 @@ def main(argv=None):
         "metric": METRIC, "value": value, "unit": "Gpt-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": secs / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{w['name']}: VTI pseudo-acoustic forward modelling + 50-cell CPML, "
+        "config": {"workload": f"{w['name']}: {PHYS_NAME[w['physics']]} forward modelling + 50-cell CPML, "
                                f"{w['nz'] + 2 * w['N']}x{w['nx'] + 2 * w['N']} padded cells, nt={w['nt']}, "
                                f"{S} shots, {len(w['rx'])} receivers per shot",
                    "global_batch": S, "nt": w["nt"], "shots_per_launch": per_launch, "lanes": lanes,
