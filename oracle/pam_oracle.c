@@ -64,10 +64,16 @@ typedef float store;
 #error "define ORACLE_F64, ORACLE_F32 or ORACLE_MIXED"
 #endif
 
+/* Row loops run on OpenMP threads on grids of at least OMP_MIN_CELLS cells;
+ * smaller grids run on the calling thread, where a parallel region costs more
+ * than the loop (and many small calls from several processes at once would
+ * oversubscribe the cores). */
+#define OMP_MIN_CELLS 65536L
 #ifdef _OPENMP
-#define OMP_ROWS _Pragma("omp parallel for schedule(static)")
+#define OMP_STR(x) #x
+#define OMP_ROWS(cells) _Pragma(OMP_STR(omp parallel for schedule(static) if ((cells) >= OMP_MIN_CELLS)))
 #else
-#define OMP_ROWS
+#define OMP_ROWS(cells)
 #endif
 
 #define R(x) ((real)(x))
@@ -232,7 +238,7 @@ void or_set_operator_only(int on) { g_op_only = on; }
 void or_update_acoustic(store *Uf, const store *Uc, const store *vp, int nzp, int nxp, double dz,
                         double dx, double dt) {
     const real idx2 = R(dx) * R(dx), idz2 = R(dz) * R(dz), dt2 = R(dt) * R(dt);
-    OMP_ROWS
+    OMP_ROWS((long)(nzp) * (nxp))
     for (int j = 4; j < nzp - 4; ++j)
         for (int i = 4; i < nxp - 4; ++i) {
             long c = (long)j * nxp + i;
@@ -262,7 +268,7 @@ static inline real sd_term(real e, real dl, real px, real pz) {
 void or_update_vti(store *Uf, const store *Uc, const store *vp, const store *eps, const store *del,
                    int nzp, int nxp, double dz, double dx, double dt) {
     const real dx2 = R(dx) * R(dx), dz2 = R(dz) * R(dz), dt2 = R(dt) * R(dt);
-    OMP_ROWS
+    OMP_ROWS((long)(nzp) * (nxp))
     for (int j = 4; j < nzp - 4; ++j)
         for (int i = 4; i < nxp - 4; ++i) {
             long c = (long)j * nxp + i;
@@ -282,7 +288,7 @@ void or_update_vti(store *Uf, const store *Uc, const store *vp, const store *eps
 void or_update_tti(store *Uf, const store *Uc, const store *vp, const store *eps, const store *del,
                    const store *theta, int nzp, int nxp, double dz, double dx, double dt) {
     const real dx2 = R(dx) * R(dx), dz2 = R(dz) * R(dz), dt2 = R(dt) * R(dt);
-    OMP_ROWS
+    OMP_ROWS((long)(nzp) * (nxp))
     for (int j = 4; j < nzp - 4; ++j)
         for (int i = 4; i < nxp - 4; ++i) {
             long c = (long)j * nxp + i;
@@ -305,7 +311,7 @@ void or_update_tti(store *Uf, const store *Uc, const store *vp, const store *eps
 /* Cerjan damping of one field, src/CudaKernels.py:3-31 / src/utils.py:380-393:
  * left/right factor first, then top/bottom, so corners get the product. */
 void or_cerjan_apply(store *U, int nzp, int nxp, int N, const store *A) {
-    OMP_ROWS
+    OMP_ROWS((long)(nzp) * (nxp))
     for (int j = 0; j < nzp; ++j)
         for (int i = 0; i < nxp; ++i) {
             long c = (long)j * nxp + i;
@@ -322,7 +328,7 @@ void or_cerjan_apply(store *U, int nzp, int nxp, int N, const store *A) {
  * psi = a*psi + b*dU on the strips, from the current field. */
 void or_cpml_psi(store *psix, store *psiz, const store *Uc, const store *ax, const store *bx,
                  const store *az, const store *bz, int nzp, int nxp, int N, double dx, double dz) {
-    OMP_ROWS
+    OMP_ROWS((long)(nzp) * (nxp))
     for (int j = 4; j < nzp - 4; ++j)
         for (int i = 4; i < nxp - 4; ++i) {
             long c = (long)j * nxp + i;
@@ -338,7 +344,7 @@ void or_cpml_psi(store *psix, store *psiz, const store *Uc, const store *ax, con
 void or_cpml_zeta(store *zetax, store *zetaz, const store *psix, const store *psiz, const store *Uc,
                   const store *ax, const store *bx, const store *az, const store *bz, int nzp, int nxp,
                   int N, double dx, double dz) {
-    OMP_ROWS
+    OMP_ROWS((long)(nzp) * (nxp))
     for (int j = 4; j < nzp - 4; ++j)
         for (int i = 4; i < nxp - 4; ++i) {
             long c = (long)j * nxp + i;
@@ -361,7 +367,7 @@ void or_update_acoustic_cpml(store *Uf, const store *Uc, const store *vp, const 
                              const store *psiz, const store *zetax, const store *zetaz, int nzp,
                              int nxp, int N, double dz, double dx, double dt) {
     const real dt2 = R(dt) * R(dt);
-    OMP_ROWS
+    OMP_ROWS((long)(nzp) * (nxp))
     for (int j = 4; j < nzp - 4; ++j)
         for (int i = 4; i < nxp - 4; ++i) {
             long c = (long)j * nxp + i;
@@ -384,7 +390,7 @@ void or_update_vti_cpml(store *Uf, const store *Uc, const store *vp, const store
                         const store *del, const store *psix, const store *psiz, const store *zetax,
                         const store *zetaz, int nzp, int nxp, int N, double dz, double dx, double dt) {
     const real dt2 = R(dt) * R(dt);
-    OMP_ROWS
+    OMP_ROWS((long)(nzp) * (nxp))
     for (int j = 4; j < nzp - 4; ++j)
         for (int i = 4; i < nxp - 4; ++i) {
             long c = (long)j * nxp + i;
@@ -420,7 +426,7 @@ void or_update_tti_cpml(store *Uf, const store *Uc, const store *vp, const store
                         const store *zetax, const store *zetaz, int nzp, int nxp, int N, double dz,
                         double dx, double dt) {
     const real dt2 = R(dt) * R(dt);
-    OMP_ROWS
+    OMP_ROWS((long)(nzp) * (nxp))
     for (int j = 4; j < nzp - 4; ++j)
         for (int i = 4; i < nxp - 4; ++i) {
             long c = (long)j * nxp + i;
@@ -463,7 +469,7 @@ void or_adjoint_vti(store *Uf, const store *Uc, const store *P, const store *vp,
     store *AU = (store *)calloc(n, sizeof(store)), *BU = (store *)calloc(n, sizeof(store));
     store *QX = (store *)calloc(n, sizeof(store)), *QZ = (store *)calloc(n, sizeof(store));
     const real dt2 = R(dt) * R(dt);
-    OMP_ROWS
+    OMP_ROWS((long)(nzp) * (nxp))
     for (int j = 4; j < nzp - 4; ++j)
         for (int i = 4; i < nxp - 4; ++i) {
             long c = (long)j * nxp + i;
@@ -486,7 +492,7 @@ void or_adjoint_vti(store *Uf, const store *Uc, const store *P, const store *vp,
             QX[c] = (store)((pxx + pzz) * Cx * u);
             QZ[c] = (store)((pxx + pzz) * Cz * u);
         }
-    OMP_ROWS
+    OMP_ROWS((long)(nzp) * (nxp))
     for (int j = 4; j < nzp - 4; ++j)
         for (int i = 4; i < nxp - 4; ++i) {
             long c = (long)j * nxp + i;
@@ -510,7 +516,7 @@ void or_adjoint_tti(store *Uf, const store *Uc, const store *P, const store *vp,
     store *HU = (store *)calloc(n, sizeof(store));
     store *QX = (store *)calloc(n, sizeof(store)), *QZ = (store *)calloc(n, sizeof(store));
     const real dt2 = R(dt) * R(dt);
-    OMP_ROWS
+    OMP_ROWS((long)(nzp) * (nxp))
     for (int j = 4; j < nzp - 4; ++j)
         for (int i = 4; i < nxp - 4; ++i) {
             long c = (long)j * nxp + i;
@@ -537,7 +543,7 @@ void or_adjoint_tti(store *Uf, const store *Uc, const store *P, const store *vp,
             QX[c] = (store)((pxx + pzz) * Cx * u);
             QZ[c] = (store)((pxx + pzz) * Cz * u);
         }
-    OMP_ROWS
+    OMP_ROWS((long)(nzp) * (nxp))
     for (int j = 4; j < nzp - 4; ++j)
         for (int i = 4; i < nxp - 4; ++i) {
             long c = (long)j * nxp + i;
@@ -560,7 +566,7 @@ void or_adjoint_tti(store *Uf, const store *Uc, const store *P, const store *vp,
  * in every build, as in the reference kernel. */
 void or_grad_vti(const store *P, const store *adj, store *geps, store *gdel, int nz, int nx, double dx,
                  double dz, const store *eps, const store *del) {
-    OMP_ROWS
+    OMP_ROWS((long)(nz) * (nx))
     for (int j = 4; j < nz - 4; ++j)
         for (int i = 4; i < nx - 4; ++i) {
             long c = (long)j * nx + i;
@@ -589,7 +595,7 @@ void or_grad_vti(const store *P, const store *adj, store *geps, store *gdel, int
 void or_grad_tti(const store *P, const store *adj, store *geps, store *gdel, store *gth, int nz,
                  int nx, double dx, double dz, const store *eps, const store *del,
                  const store *theta) {
-    OMP_ROWS
+    OMP_ROWS((long)(nz) * (nx))
     for (int j = 4; j < nz - 4; ++j)
         for (int i = 4; i < nx - 4; ++i) {
             long c = (long)j * nx + i;
