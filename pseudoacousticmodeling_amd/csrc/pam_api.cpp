@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -577,6 +578,10 @@ static void base_step_args(pam_ctx *ctx, StepArgs &a, int nshots) {
     a.nz = g.nz;
     a.nx = g.nx;
     a.nshots = nshots;
+    {
+        static const int band_env = getenv("PAM_TILE_BAND") ? atoi(getenv("PAM_TILE_BAND")) : 0;
+        a.band = band_env;
+    }
     a.sstride = ctx->plane;
     a.idx2 = 1.0f / (g.dx * g.dx);
     a.idz2 = 1.0f / (g.dz * g.dz);
@@ -1949,6 +1954,9 @@ struct CkptRun {
     int shot0 = 0;
     const float *data = nullptr;
     long long nd = 0, nint = 0, per_step = 0;
+    long long acc_st = 0;  // floats between shots' accumulators
+    int acc_pad = 0;       // accumulators and snapshots padded
+    std::function<int(const float *, int, float *, hipStream_t)> reduce;  // accumulators into an output
     bool grad = false, d2t = false;
     int tile = 0, C = 0, nseg = 0, t_lo = 0, t_hi = 0, nbuf = 0, skind = 0;
     float *image = nullptr, *ilum = nullptr;
@@ -2029,7 +2037,6 @@ static int ckpt_lane_restore(pam_ctx *ctx, CkptLane &L, int j) {
 static int ckpt_tick(pam_ctx *ctx, CkptRun &R, CkptLane &L, bool *done) {
     const pam_grid &g = ctx->g;
     const int nt = g.nt;
-    const long long nint = R.nint;
     *done = false;
     switch (L.phase) {
         case 0: {  // forward pass set-up
@@ -2037,8 +2044,8 @@ static int ckpt_tick(pam_ctx *ctx, CkptRun &R, CkptLane &L, bool *done) {
             if (r) return r;
             r = upload_sources(ctx, R.shot0 + L.b0, L.nb, L.slot0, L.st);
             if (r) return r;
-            HCK(hipMemsetAsync(L.acc_ilum, 0, (size_t)L.nb * nint * 4, L.st));
-            HCK(hipMemsetAsync(L.acc_img, 0, (size_t)L.nb * nint * 4, L.st));
+            HCK(hipMemsetAsync(L.acc_ilum, 0, (size_t)L.nb * R.acc_st * 4, L.st));
+            HCK(hipMemsetAsync(L.acc_img, 0, (size_t)L.nb * R.acc_st * 4, L.st));
             KCK(launch_inject_point(L.F.U0, ctx->plane, ctx->d_ssz + L.slot0, ctx->d_ssx + L.slot0, ctx->P, ctx->wav,
                                     0, L.nb, L.st));
             base_step_args(ctx, L.a, L.nb);
@@ -2050,7 +2057,8 @@ static int ckpt_tick(pam_ctx *ctx, CkptRun &R, CkptLane &L, bool *done) {
             L.a.save_sstride = R.per_step * R.nbuf;
             L.a.ilum = L.acc_ilum;
             L.a.image = L.acc_img;
-            L.a.acc_sstride = nint;
+            L.a.acc_sstride = R.acc_st;
+            L.a.acc_pad = R.acc_pad;
             L.a.ssz = ctx->d_ssz + L.slot0;
             L.a.ssx = ctx->d_ssx + L.slot0;
             L.cur = L.F.U0;
@@ -2088,7 +2096,8 @@ static int ckpt_tick(pam_ctx *ctx, CkptRun &R, CkptLane &L, bool *done) {
             L.b.save_kstride = R.per_step;
             L.b.save_sstride = R.per_step * R.nbuf;
             L.b.image = L.acc_img;
-            L.b.acc_sstride = nint;
+            L.b.acc_sstride = R.acc_st;
+            L.b.acc_pad = R.acc_pad;
             L.b.ssz = ctx->d_ssz + L.slot0;
             L.b.ssx = ctx->d_ssx + L.slot0;
             L.seg = R.nseg - 1;
@@ -2144,8 +2153,10 @@ static int ckpt_tick(pam_ctx *ctx, CkptRun &R, CkptLane &L, bool *done) {
                 if (!R.red_issued[L.batch - 1]) return PAM_OK;
                 HCK(hipStreamWaitEvent(L.st, R.red_ev[L.batch - 1], 0));
             }
-            KCK(launch_reduce_shots(L.acc_img, nint, L.nb, (int)nint, R.image, 1, L.st));
-            KCK(launch_reduce_shots(L.acc_ilum, nint, L.nb, (int)nint, R.ilum, 1, L.st));
+            int r = R.reduce(L.acc_img, L.nb, R.image, L.st);
+            if (r) return r;
+            r = R.reduce(L.acc_ilum, L.nb, R.ilum, L.st);
+            if (r) return r;
             HCK(hipEventRecord(R.red_ev[L.batch], L.st));
             R.red_issued[L.batch] = 1;
             *done = true;
@@ -2173,8 +2184,8 @@ static int ckpt_two_lanes(pam_ctx *ctx, CkptRun &R, const std::vector<int> &size
         L[l].F = fieldset_view(ctx, ctx->fs, l * Bl);
         L[l].A = fieldset_view(ctx, ctx->aux, l * Bl);
         L[l].save = reinterpret_cast<float *>(ctx->save) + (long long)l * Bl * R.per_step * R.nbuf;
-        L[l].acc_ilum = ctx->acc_ilum + (long long)l * Bl * R.nint;
-        L[l].acc_img = ctx->acc_img + (long long)l * Bl * R.nint;
+        L[l].acc_ilum = ctx->acc_ilum + (long long)l * Bl * R.acc_st;
+        L[l].acc_img = ctx->acc_img + (long long)l * Bl * R.acc_st;
         L[l].ckpt = ctx->ckpt + (long long)l * R.nseg * ckpt_fields(ctx) * ctx->plane * Bl;
         L[l].flip.assign(R.nseg, 0);
         L[l].st = ctx->lane_st[l];
@@ -2272,14 +2283,23 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                     "save-all snapshots");
     // boundary saving: the adjoint step reads D_t from a one-step interior
     // buffer (gradient) or S_t from the rebuilt padded field (imaging)
-    const int skind = (sb || rbc) ? (grad ? SV_F32_INT : SV_F32_PAD)
+    int skind = (sb || rbc) ? (grad ? SV_F32_INT : SV_F32_PAD)
                          : linear_adj ? SV_F32_PAD
                          : rv ? SV_F32_INT
                                       : (storage == PAM_STORE_ALL_F16
                                              ? SV_F16_INT
                                              : (storage == PAM_STORE_ALL_BF16 ? SV_BF16_INT : SV_F32_INT));
     const int elem = (half || sb_kind) ? 2 : 4;
-    const long long per_step = rbc ? 1 : sb ? sb_width(g.nz, g.nx) : (skind == SV_F32_PAD) ? ctx->plane : nint;  // per step
+    const bool multi = grad && (g_eps || g_del || g_th);
+    // float32 save-all or checkpoints with the pressure-form adjoint and no
+    // partials: accumulators and snapshots are sized as padded planes, and when
+    // the kernel variant's passes run the 16-row tile family they are laid out
+    // padded (rows 256-byte aligned, one float4 per thread; unaligned interior
+    // rows took twice as long to update); other variants index them as
+    // interior arrays at the same strides
+    const bool pad_ok = !linear_adj && !multi && (storage == PAM_STORE_ALL_F32 || ckpt);
+    const long long acc_st = pad_ok ? ctx->plane : nint;  // floats between shots' accumulators
+    const long long per_step = rbc ? 1 : sb ? sb_width(g.nz, g.nx) : (skind == SV_F32_PAD || pad_ok) ? ctx->plane : nint;  // per step
     // default interval 2 ceil(sqrt(nt)): every step is recomputed once whatever
     // the interval, so a longer one only trades checkpoint copies for segment
     // buffer (2% faster than ceil(sqrt(nt)) on a 451x1801 VTI grid, nt 5099)
@@ -2293,7 +2313,6 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     const int nseg = ckpt ? t_hi / C + 1 : 1;  // segments of backward steps
     const int nbuf = (rv || rbc) ? 1 : ckpt ? C + 2 : nt;  // snapshots held at once
     const int nslot = rv ? std::max(1, std::min(ctx->snapshots, t_hi - t_lo + 1)) : 0;  // revolve slots
-    const bool multi = grad && (g_eps || g_del || g_th);
     const int nmulti = (ctx->phys == 2) ? 3 : 2;
     // the gradient needs S only through S_{t+1} - 2 S_t + S_{t-1}: store that
     // difference (formed by the forward step in float32, the same arithmetic the
@@ -2324,7 +2343,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         Bl = std::max(1, std::min(Bl, std::min(nshots, ctx->max_shots / 2)));
         B = 2 * Bl;
     }
-    const long long per_shot_bytes = per_step * nbuf * elem + 2 * nint * 4 + (linear_adj ? 5 * ctx->plane * 4 : 0) +
+    const long long per_shot_bytes = per_step * nbuf * elem + 2 * acc_st * 4 + (linear_adj ? 5 * ctx->plane * 4 : 0) +
                                      (multi ? nmulti * nint * 4 : 0) + (half ? (long long)nt * 4 : 0) +
                                      (ckpt ? (long long)nseg * ckpt_fields(ctx) * ctx->plane * 4 : 0) +
                                      (sb ? (long long)nt * 4 + nint * 4 : 0) +
@@ -2346,9 +2365,9 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     }
     r = ensure_scratch(ctx, &ctx->save, &ctx->save_bytes, (long long)B * per_step * nbuf * elem);
     if (r) return r;
-    r = ensure_scratch(ctx, (void **)&ctx->acc_ilum, &ctx->acc_bytes, (long long)B * nint * 4 * 2);
+    r = ensure_scratch(ctx, (void **)&ctx->acc_ilum, &ctx->acc_bytes, (long long)B * acc_st * 4 * 2);
     if (r) return r;
-    ctx->acc_img = ctx->acc_ilum + (long long)B * nint;
+    ctx->acc_img = ctx->acc_ilum + (long long)B * acc_st;
     if (linear_adj) {
         r = ensure_scratch(ctx, (void **)&ctx->tmp, &ctx->tmp_bytes, (long long)B * 5 * ctx->plane * 4);
         if (r) return r;
@@ -2408,6 +2427,16 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     int tile0 = choose_tile(ctx, two ? Bl : B, true);
     ctx->steps_run = 0;
     if ((half || sb || rv || rbc) && tile_shape(tile0).V > 0) tile0 = kDefaultTile;  // k_step has no fused max / D_t
+    const bool accp = pad_ok && tile_pads_acc(tile0);
+    if (accp) skind = SV_F32_PAD;
+    // the accumulators of nb shots into the outputs (padded or interior layout)
+    auto reduce_acc = [&](const float *parts, int nb, float *out, hipStream_t st) -> int {
+        if (accp)
+            KCK(launch_reduce_shots_pad(parts, acc_st, nb, g.nz, g.nx, ctx->P, g.n_abc, out, 1, st));
+        else
+            KCK(launch_reduce_shots(parts, acc_st, nb, (int)nint, out, 1, st));
+        return PAM_OK;
+    };
     ctx->last_tile = tile0;
     ctx->last_batch = two ? Bl : B;
     const long long pint = (long long)g.n_abc * ctx->P + g.n_abc;  // interior origin in a padded field
@@ -2418,6 +2447,9 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         R.data = data;
         R.nd = nd;
         R.nint = nint;
+        R.acc_st = acc_st;
+        R.acc_pad = accp ? 1 : 0;
+        R.reduce = reduce_acc;
         R.per_step = per_step;
         R.grad = grad;
         R.d2t = d2t;
@@ -2451,10 +2483,10 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         if (r) return r;
         r = upload_sources(ctx, shot0 + b0, nb);
         if (r) return r;
-        if (skind == SV_F32_PAD && !sb)
+        if (!accp && skind == SV_F32_PAD && !sb)
             HCK(hipMemsetAsync(ctx->save, 0, (size_t)nb * per_step * nbuf * elem, ctx->st));
-        HCK(hipMemsetAsync(ctx->acc_ilum, 0, (size_t)nb * nint * 4, ctx->st));
-        HCK(hipMemsetAsync(ctx->acc_img, 0, (size_t)nb * nint * 4, ctx->st));
+        HCK(hipMemsetAsync(ctx->acc_ilum, 0, (size_t)nb * acc_st * 4, ctx->st));
+        HCK(hipMemsetAsync(ctx->acc_img, 0, (size_t)nb * acc_st * 4, ctx->st));
         if (multi) HCK(hipMemsetAsync(ctx->acc_multi, 0, (size_t)B * nmulti * nint * 4, ctx->st));
         KCK(launch_inject_point(ctx->fs.U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, nb, ctx->st));
         if (half) {
@@ -2479,7 +2511,8 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         a.smax = ctx->smax;
         a.ilum = ctx->acc_ilum;
         a.image = ctx->acc_img;
-        a.acc_sstride = nint;
+        a.acc_sstride = acc_st;
+        a.acc_pad = accp ? 1 : 0;
         if (!rv) {
             float *cur = ctx->fs.U0, *prv = ctx->fs.U1;
             // boundary saving: rims of S_k into step k of the strip buffer
@@ -2607,7 +2640,8 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         }
         b.save_scale = reinterpret_cast<const float *>(ctx->smax);
         b.image = ctx->acc_img;
-        b.acc_sstride = nint;
+        b.acc_sstride = acc_st;
+        b.acc_pad = accp ? 1 : 0;
         AdjArgs c;
         std::memset(&c, 0, sizeof(c));
         if (linear_adj) {
@@ -2861,8 +2895,10 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                 }
             }
         }
-        KCK(launch_reduce_shots(ctx->acc_img, nint, nb, (int)nint, image, 1, ctx->st));
-        KCK(launch_reduce_shots(ctx->acc_ilum, nint, nb, (int)nint, ilum_acc, 1, ctx->st));
+        r = reduce_acc(ctx->acc_img, nb, image, ctx->st);
+        if (r) return r;
+        r = reduce_acc(ctx->acc_ilum, nb, ilum_acc, ctx->st);
+        if (r) return r;
         if (multi)
             for (int m = 0; m < nmulti; ++m)
                 if (out_multi[m])

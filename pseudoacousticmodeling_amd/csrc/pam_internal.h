@@ -53,6 +53,7 @@ struct StepArgs {
     int nzp, nxp, P, N;   // padded grid, row pitch (floats), absorbing width
     int nz, nx;           // interior
     int nxt, nzt, nshots; // x tiles, z tiles, shots in this launch
+    int band;             // tile columns per band of the 16-row tile order (0: 8)
     long long sstride;    // floats between consecutive shots' fields
     float idx2, idz2, idx, idz, idxdz, idt2, dxdz;
     int k;                // step index of the input field
@@ -93,9 +94,13 @@ struct StepArgs {
     const float *save_prev, *save_next;  // (unused; kept for ABI stability)
     float *dsave;              // F_DSAVE target [nshots][nz][nx], dsave_sstride apart
     long long dsave_sstride;
-    // interior accumulators, per shot [nshots][nz][nx]
+    // accumulators, per shot: interior [nshots][nz][nx], or with acc_pad padded
+    // planes [nshots][nzp][P] (acc_sstride apart) whose float32 snapshots
+    // (save_kind SV_F32_PAD) are padded too, read and written one aligned float4
+    // per thread (16-row tile family only)
     float *ilum, *image;
     long long acc_sstride;
+    int acc_pad;
 };
 
 // CPML psi pre-pass
@@ -145,6 +150,9 @@ int tile_count();
 TileShape tile_shape(int idx);
 // true when the step-kernel variant updates the CPML psi itself (no k_psi pre-pass)
 bool tile_fuses_psi(int idx);
+// true for the variants whose passes run the 16-row tile family (k_tile,
+// k_tile_pipe), which read and write padded accumulators and snapshots
+bool tile_pads_acc(int idx);
 
 // launchers (return hipError_t as int)
 // ev: null, or two events that the launch stamps with the kernel's start and end
@@ -184,6 +192,8 @@ int launch_smax0(unsigned int *smax, int nt, const int *ssz, const int *ssx, int
 int launch_inject_rec(float *U, long long sstride, int P, const int *rz, const int *rx, int nrec,
                       const float *data, long long dstride, int row, float dxdz, int nshots, hipStream_t st);
 int launch_pack_padded(const float *U, int nzp, int nxp, int P, float *out, hipStream_t st);
+int launch_reduce_shots_pad(const float *parts, long long sstride, int nshots, int nz, int nx, int P, int N,
+                            float *out, int accumulate, hipStream_t st);
 int launch_reduce_shots(const float *parts, long long sstride, int nshots, int n, float *out, int accumulate,
                         hipStream_t st);
 int launch_normalize(float *image, const float *ilum, int n, int mode, const float *maxv, hipStream_t st);

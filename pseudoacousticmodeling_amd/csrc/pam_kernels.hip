@@ -394,6 +394,19 @@ __global__ void k_reduce_shots(const float *parts, long long sstride, int nshots
     for (int s = 0; s < nshots; ++s) acc += parts[(long long)s * sstride + i];
     out[i] = acc;
 }
+// the same from padded per-shot planes (sstride apart, pitch P, interior at
+// row N, column N) into the interior [nz][nx]: the shots are added in the
+// same order, so the sums are those of k_reduce_shots over interior copies
+__global__ void k_reduce_shots_pad(const float *parts, long long sstride, int nshots, int nz, int nx, int P, int N,
+                                   float *out, int accumulate) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nz * nx) return;
+    const int iz = (int)(i / nx), ix = (int)(i - (long long)iz * nx);
+    const long long pi = (long long)(iz + N) * P + ix + N;
+    float acc = accumulate ? out[i] : 0.0f;
+    for (int s = 0; s < nshots; ++s) acc += parts[(long long)s * sstride + pi];
+    out[i] = acc;
+}
 __global__ void k_maxabs(const float *x, long long n, unsigned int *out) {
     __shared__ float red[256];
     float m = 0.0f;
@@ -565,6 +578,10 @@ TileShape tile_shape(int idx) {
     return kTiles[idx];
 }
 bool tile_fuses_psi(int idx) { return tile_shape(idx).V == -1 || tile_shape(idx).V == -3; }
+bool tile_pads_acc(int idx) {
+    const int v = tile_shape(idx).V;
+    return v == 0 || v == -2 || v == -3;
+}
 
 int launch_step_acoustic_cerjan(int, const StepArgs &, hipStream_t, const hipEvent_t *);
 int launch_step_acoustic_cpml(int, const StepArgs &, hipStream_t, const hipEvent_t *);
@@ -683,6 +700,13 @@ int launch_reduce_shots(const float *parts, long long sstride, int nshots, int n
                         hipStream_t st) {
     hipLaunchKernelGGL(k_reduce_shots, dim3((n + 255) / 256), dim3(256), 0, st, parts, sstride, nshots, (long long)n,
                        out, accumulate);
+    return (int)hipGetLastError();
+}
+int launch_reduce_shots_pad(const float *parts, long long sstride, int nshots, int nz, int nx, int P, int N,
+                            float *out, int accumulate, hipStream_t st) {
+    const long long n = (long long)nz * nx;
+    hipLaunchKernelGGL(k_reduce_shots_pad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, parts, sstride, nshots,
+                       nz, nx, P, N, out, accumulate);
     return (int)hipGetLastError();
 }
 int launch_maxabs(const float *x, int n, float *out, hipStream_t st) {

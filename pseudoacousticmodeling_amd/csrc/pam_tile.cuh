@@ -44,7 +44,7 @@ struct TileStep {
 // x-neighbour right after it), the shots of a tile adjacent.
 template <int TZ>
 __device__ __forceinline__ void tile_of(const StepArgs &a, const int b, int &x0, int &z0, int &s) {
-    constexpr int kBand = 8;
+    const int kBand = a.band > 0 ? a.band : 8;
     const int tile = b / a.nshots;
     s = b - tile * a.nshots;
     const int band = tile / (kBand * a.nzt);
@@ -365,7 +365,19 @@ __device__ __forceinline__ void tile_body_at(const StepArgs &a, const int x0, co
                 }
             }
         }
-        if ((flags & F_SMAX) || ((flags & F_D2T) && (flags & F_SAVE)) || (flags & F_DSAVE)) {
+        if (a.acc_pad && (flags & F_D2T) && (flags & F_SAVE)) {
+            // padded float32 snapshot: D_k of the interior cells (0 elsewhere), one aligned float4
+            if (z >= N && z < nzp - N && col + 3 >= N && col < nxp - N) {
+                float d4[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    d4[j] = 0.0f;
+                    if (col + j >= N && col + j < nxp - N)
+                        d4[j] = (flags & F_D2T_OP) ? opv[j] : out[j] - 2.0f * c0[j] + pv[r][j];
+                }
+                vst<4>(at(reinterpret_cast<float *>(a.save) + snap_off(a, s, m.k), ro), d4);
+            }
+        } else if ((flags & F_SMAX) || ((flags & F_D2T) && (flags & F_SAVE)) || (flags & F_DSAVE)) {
             const bool zint = z >= N && z < nzp - N;
             const int ib = (z - N) * a.nx - N;  // interior index = ib + c (used on interior cells only)
             const long long sk = snap_off(a, s, m.k);
@@ -389,7 +401,47 @@ __device__ __forceinline__ void tile_body_at(const StepArgs &a, const int x0, co
         }
         // ---- snapshots / illumination / imaging of the input field (after the update:
         // fewer values are live here than before it)
-        if ((flags & (F_SAVE | F_ILUM | F_IMAGE | F_GRADM)) && zlive) {
+        if (a.acc_pad && (flags & (F_SAVE | F_ILUM | F_IMAGE | F_GRADM)) && zlive) {
+            // padded accumulators and snapshots: one aligned float4 per thread
+            // (the gradient reads D_k: a padded run always stores D_k)
+            const bool zint = (z >= N) && (z < nzp - N);
+            bool cin[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) cin[j] = zint && col + j >= N && col + j < nxp - N;
+            const long long acc_s = (long long)s * a.acc_sstride;
+            float *const sv = reinterpret_cast<float *>(a.save) + snap_off(a, s, m.k);
+            if ((flags & F_SAVE) && !(flags & F_D2T)) {
+                float S4[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) S4[j] = cin[j] ? c0[j] : 0.0f;
+                if (zint) vst<4>(at(sv, ro), S4);
+            }
+            if (cin[0] || cin[1] || cin[2] || cin[3]) {
+                if (flags & F_ILUM) {
+                    float v[4];
+                    vld<4>(v, at(a.ilum + acc_s, ro));
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (cin[j]) v[j] += c0[j] * c0[j];
+                    vst<4>(at(a.ilum + acc_s, ro), v);
+                }
+                if (flags & (F_IMAGE | F_GRADM)) {
+                    float S4[4], v[4];
+                    vld<4>(S4, at(sv, ro));
+                    vld<4>(v, at(a.image + acc_s, ro));
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (!cin[j]) continue;
+                        if (flags & F_IMAGE) v[j] += S4[j] * c0[j];
+                        if (flags & F_GRADM) {
+                            const float d2t = S4[j] * a.idt2;
+                            v[j] += d2t * c0[j];
+                        }
+                    }
+                    vst<4>(at(a.image + acc_s, ro), v);
+                }
+            }
+        } else if ((flags & (F_SAVE | F_ILUM | F_IMAGE | F_GRADM)) && zlive) {
             const bool zint = (z >= N) && (z < nzp - N);
             const int ib = (z - N) * a.nx - N;  // interior index = ib + c (used on interior cells only)
             const long long acc_s = (long long)s * a.acc_sstride;  // per-shot bases: wave-uniform
