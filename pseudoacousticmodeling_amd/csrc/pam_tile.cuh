@@ -165,6 +165,29 @@ __device__ __forceinline__ void tile_body_at(const StepArgs &a, const int x0, co
             }
         }
     }
+    // the adjoint passes' padded accumulator and snapshot cells, loaded with
+    // the fields so that their latency overlaps the update (the gradient pass
+    // took 71.7 instead of 61.3 us per 13-shot launch on a 451 x 1801 VTI grid
+    // when they were loaded after it; the illumination pass was faster without)
+    constexpr bool kEarly = FM == F_PASS_GRAD || FM == F_PASS_IMAGE;
+    constexpr bool kEarlyS = kEarly && (FM & (F_IMAGE | F_GRADM)) != 0;
+    float eacc[kEarly ? R : 1][4], esv[kEarlyS ? R : 1][4];
+    if (kEarly && a.acc_pad) {
+        const long long acc_s = (long long)s * a.acc_sstride;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int z = zr0 + r;
+            if (z >= N && z < nzp - N && col + 3 >= N && col < nxp - N) {
+                const unsigned ro = (unsigned)(z * P + col);
+                if (FM & F_ILUM) {
+                    vld<4>(eacc[r], at(a.ilum + acc_s, ro));
+                } else {
+                    vld<4>(eacc[r], at(a.image + acc_s, ro));
+                    if (kEarlyS) vld<4>(esv[kEarlyS ? r : 0], at(reinterpret_cast<const float *>(a.save) + snap_off(a, s, m.k), ro));
+                }
+            }
+        }
+    }
 #pragma unroll
     for (int u = 0; u < NL; ++u) {
         const int i = tid + 256 * u;
@@ -419,7 +442,12 @@ __device__ __forceinline__ void tile_body_at(const StepArgs &a, const int x0, co
             if (cin[0] || cin[1] || cin[2] || cin[3]) {
                 if (flags & F_ILUM) {
                     float v[4];
-                    vld<4>(v, at(a.ilum + acc_s, ro));
+                    if (kEarly) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) v[j] = eacc[kEarly ? r : 0][j];
+                    } else {
+                        vld<4>(v, at(a.ilum + acc_s, ro));
+                    }
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
                         if (cin[j]) v[j] += c0[j] * c0[j];
@@ -427,8 +455,16 @@ __device__ __forceinline__ void tile_body_at(const StepArgs &a, const int x0, co
                 }
                 if (flags & (F_IMAGE | F_GRADM)) {
                     float S4[4], v[4];
-                    vld<4>(S4, at(sv, ro));
-                    vld<4>(v, at(a.image + acc_s, ro));
+                    if (kEarlyS) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            S4[j] = esv[kEarlyS ? r : 0][j];
+                            v[j] = eacc[kEarly ? r : 0][j];
+                        }
+                    } else {
+                        vld<4>(S4, at(sv, ro));
+                        vld<4>(v, at(a.image + acc_s, ro));
+                    }
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         if (!cin[j]) continue;
@@ -487,11 +523,12 @@ __device__ __forceinline__ void tile_body_at(const StepArgs &a, const int x0, co
     if ((flags & F_SMAX) && m.k + 1 < a.nt) block_max_to(a.smax + (long long)s * a.nt + m.k + 1, tmax);
 }
 
-// the instances of the 16-row tile for one pass fit 5 waves per SIMD (the
-// gradient one spills 3 VGPRs there and still runs faster than at 4); the
-// instance for every flag runs at 4
+// the forward, illumination and recompute instances of the 16-row tile fit 5
+// waves per SIMD; the adjoint ones, which load their accumulators early, run
+// at 4 (106 VGPRs for the VTI gradient: at 5 it spilled 27), as does the
+// instance for every flag
 template <int PHYS, bool CPML, int TZ, int WPE, int FM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(((FM == F_FWD_MASK || FM == F_PASS_ILUM || FM == F_PASS_IMAGE || FM == F_PASS_GRAD || FM == F_PASS_SAVE) && TZ == 16 && PHYS < 2) ? 5 : WPE))) void k_tile(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(((FM == F_FWD_MASK || FM == F_PASS_ILUM || FM == F_PASS_SAVE) && TZ == 16 && PHYS < 2) ? 5 : WPE))) void k_tile(
     const StepArgs a) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= a.nxt * a.nzt * a.nshots) return;  // block-uniform
