@@ -855,13 +855,8 @@ extern "C" int pam_adjoint_reset(pam_ctx *ctx, int shot, const float *data) {
     if (!data) return fail(ctx, PAM_E_ARG, "data is NULL");
     r = alloc_fieldset(ctx, ctx->aux);
     if (r) return r;
-    const bool linear_adj = ctx->phys >= 1 && ctx->abc == PAM_CERJAN;
     r = ensure_scratch(ctx, (void **)&ctx->adj_S, &ctx->adj_S_bytes, ctx->plane * 4);
     if (r) return r;
-    if (linear_adj) {
-        r = ensure_scratch(ctx, (void **)&ctx->tmp, &ctx->tmp_bytes, 5 * ctx->plane * 4);
-        if (r) return r;
-    }
     // the adjoint field cleared; the CPML memory cleared, or with
     // PAM_OPT_RESET_ADJ_CPML 0 taken over from the forward per-step state as
     // the reference's driver leaves it (src/Migration.py:1000-1039)
@@ -958,8 +953,6 @@ extern "C" int pam_adjoint_step(pam_ctx *ctx, int t, const float *S_t, float *im
         c.sn = ctx->sn;
         c.cerx = ctx->cerx;
         c.cerz = ctx->cerz;
-        c.tmp = ctx->tmp;
-        c.tsstride = 5 * ctx->plane;
         c.rrow = ctx->d_rrow;
         c.rhead = ctx->d_rhead;
         c.rnext = ctx->d_rnext;
@@ -967,8 +960,7 @@ extern "C" int pam_adjoint_step(pam_ctx *ctx, int t, const float *S_t, float *im
         c.dstride = nd;
         c.image = image;
         c.acc_sstride = nint;
-        KCK(launch_adj_products(ctx->phys, c, ctx->st));
-        KCK(launch_adj_update(ctx->phys, c, ctx->st));
+        KCK(launch_adj_step(ctx->phys, c, ctx->st));
     }
     std::swap(ctx->adj_cur, ctx->adj_prv);
     ctx->adj_next_t = t - 1;
@@ -2343,7 +2335,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         Bl = std::max(1, std::min(Bl, std::min(nshots, ctx->max_shots / 2)));
         B = 2 * Bl;
     }
-    const long long per_shot_bytes = per_step * nbuf * elem + 2 * acc_st * 4 + (linear_adj ? 5 * ctx->plane * 4 : 0) +
+    const long long per_shot_bytes = per_step * nbuf * elem + 2 * acc_st * 4 +
                                      (multi ? nmulti * nint * 4 : 0) + (half ? (long long)nt * 4 : 0) +
                                      (ckpt ? (long long)nseg * ckpt_fields(ctx) * ctx->plane * 4 : 0) +
                                      (sb ? (long long)nt * 4 + nint * 4 : 0) +
@@ -2368,10 +2360,6 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     r = ensure_scratch(ctx, (void **)&ctx->acc_ilum, &ctx->acc_bytes, (long long)B * acc_st * 4 * 2);
     if (r) return r;
     ctx->acc_img = ctx->acc_ilum + (long long)B * acc_st;
-    if (linear_adj) {
-        r = ensure_scratch(ctx, (void **)&ctx->tmp, &ctx->tmp_bytes, (long long)B * 5 * ctx->plane * 4);
-        if (r) return r;
-    }
     if (half || sb) {
         r = ensure_scratch(ctx, (void **)&ctx->smax, &ctx->smax_bytes, (long long)B * nt * 4);
         if (r) return r;
@@ -2670,8 +2658,6 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
             c.sn = ctx->sn;
             c.cerx = ctx->cerx;
             c.cerz = ctx->cerz;
-            c.tmp = ctx->tmp;
-            c.tsstride = 5 * ctx->plane;
             c.rrow = ctx->d_rrow;
             c.rhead = ctx->d_rhead;
             c.rnext = ctx->d_rnext;
@@ -2704,8 +2690,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                     c.D_ = grad ? ctx->sbd : nullptr;
                     c.dsstride = nint;
                 }
-                KCK(launch_adj_products(ctx->phys, c, ctx->st));
-                KCK(launch_adj_update(ctx->phys, c, ctx->st));
+                KCK(launch_adj_step(ctx->phys, c, ctx->st));
                 ++ctx->steps_run;
             }
             if (multi) KCK(partials(t, cur));

@@ -125,8 +125,6 @@ struct AdjArgs {
     const float *P_;       // source field S_t, padded (damped), shot 0
     long long psstride;    // stride between shots in P_
     const float *v2, *eps, *del, *cs, *sn, *cerx, *cerz;
-    float *tmp;            // product scratch [nshots][5][nzp][P]
-    long long tsstride;
     const int *rrow, *rhead, *rnext;
     const float *data;
     long long dstride;
@@ -173,8 +171,8 @@ int launch_step_multi(int phys, bool cpml, int tile, const StepArgs *list, const
 bool tile_has_multi(int tile);
 // row chunks of 16 per block of a z-march variant (0 for the other families)
 int march_chunks(int tile);
-int launch_adj_products(int phys, const AdjArgs &a, hipStream_t st);
-int launch_adj_update(int phys, const AdjArgs &a, hipStream_t st);
+// one step of the linearised VTI/TTI adjoint (products and update in one pass)
+int launch_adj_step(int phys, const AdjArgs &a, hipStream_t st);
 
 // small utility kernels
 int launch_expand(const float *m, int nz, int nx, int N, float *out, int P, hipStream_t st);

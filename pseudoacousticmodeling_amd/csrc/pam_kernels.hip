@@ -148,104 +148,79 @@ __global__ __launch_bounds__(256) void k_psi_multi(const StepArgs *__restrict__ 
 }
 
 // ------------------------------------------- linearised VTI/TTI adjoint (Cerjan)
-// Products from the source field P (src/CudaKernels.py:1382-1472, :1534-1633):
-// tmp[0..4] = A*lam, B*lam, Q*Cx*lam, Q*Cz*lam, H*lam on the interior.
-template <int PHYS>
-__global__ __launch_bounds__(256) void k_adj_products(const AdjArgs a) {
-    const int s = blockIdx.z;
-    const int ix = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int iz = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (ix >= a.nxp || iz >= a.nzp) return;
-    const long long i = (long long)iz * a.P + ix;
-    const long long pl = (long long)a.nzp * a.P;
-    float *T = a.tmp + (long long)s * a.tsstride;
-    float o0 = 0.f, o1 = 0.f, o2 = 0.f, o3 = 0.f, o4 = 0.f;
-    if (ix >= 4 && ix < a.nxp - 4 && iz >= 4 && iz < a.nzp - 4) {
-        const float *Pp = a.P_ + (long long)s * a.psstride;
-        const long long P = a.P;
-        const float pxx = (KC0 * Pp[i] + KC1 * (Pp[i + 1] + Pp[i - 1]) + KC2 * (Pp[i + 2] + Pp[i - 2]) +
-                           KC3 * (Pp[i + 3] + Pp[i - 3]) + KC4 * (Pp[i + 4] + Pp[i - 4])) *
-                          a.idx2;
-        const float pzz = (KC0 * Pp[i] + KC1 * (Pp[i + P] + Pp[i - P]) + KC2 * (Pp[i + 2 * P] + Pp[i - 2 * P]) +
-                           KC3 * (Pp[i + 3 * P] + Pp[i - 3 * P]) + KC4 * (Pp[i + 4 * P] + Pp[i - 4 * P])) *
-                          a.idz2;
-        const float px = (KA1 * (Pp[i + 1] - Pp[i - 1]) + KA2 * (Pp[i + 2] - Pp[i - 2]) +
-                          KA3 * (Pp[i + 3] - Pp[i - 3]) + KA4 * (Pp[i + 4] - Pp[i - 4])) *
-                         a.idx;
-        const float pz = (KA1 * (Pp[i + P] - Pp[i - P]) + KA2 * (Pp[i + 2 * P] - Pp[i - 2 * P]) +
-                          KA3 * (Pp[i + 3 * P] - Pp[i - 3 * P]) + KA4 * (Pp[i + 4 * P] - Pp[i - 4 * P])) *
-                         a.idz;
-        const float e = a.eps[i], dl = a.del[i];
-        float X = px, Z = pz, c = 1.f, sn = 0.f;
-        if (PHYS == 2) {
-            c = a.cs[i];
-            sn = a.sn[i];
-            X = px * c - pz * sn;
-            Z = px * sn + pz * c;
-        }
-        const float X2 = X * X, Z2 = Z * Z, X4 = X2 * X2, Z4 = Z2 * Z2;
-        const float num = -2.0f * (e - dl) * X2 * Z2;
-        const float den = (1.0f + 2.0f * e) * X4 + Z4 + 2.0f * (1.0f + dl) * X2 * Z2;
-        float Sd = 0.f, Cx = 0.f, Cz = 0.f;
-        if (fabsf(den) > 1.0e-12f) {
-            Sd = num / den;
-            if (PHYS == 1) {
-                const float fac = 4.0f * (e - dl) * ((1.0f + 2.0f * e) * X4 - Z4) / (den * den);
-                Cx = fac * px * Z2;
-                Cz = -fac * X2 * pz;
-            } else {
-                const float K = (1.0f + 2.0f * e) * X4 - Z4;
-                const float fac = 4.0f * (e - dl) * X * Z * K / (den * den);
-                Cx = fac * pz;
-                Cz = -fac * px;
-            }
-        }
-        const float lam = a.cur[(long long)s * a.sstride + i];
-        const float Q = pxx + pzz;
-        if (PHYS == 1) {
-            o0 = (1.0f + 2.0f * e + Sd) * lam;
-            o1 = (1.0f + Sd) * lam;
-        } else {
-            o0 = ((1.0f + 2.0f * e) * (c * c) + (sn * sn) + Sd) * lam;
-            o1 = ((1.0f + 2.0f * e) * (sn * sn) + (c * c) + Sd) * lam;
-            o4 = (2.0f * e * (2.0f * sn * c)) * lam;
-        }
-        o2 = Q * Cx * lam;
-        o3 = Q * Cz * lam;
+// The products of one cell (src/CudaKernels.py:1382-1472, :1534-1633):
+// o[0..4] = A*lam, B*lam, Q*Cx*lam, Q*Cz*lam, H*lam, where Q, Cx, Cz and the
+// Sd inside A, B come from the source field; Pf(dx, dz) is the source field at
+// the cell's (x + dx, z + dz).
+template <int PHYS, typename PF>
+__device__ __forceinline__ void adj_products_cell(const PF &Pf, const float e, const float dl, const float c_,
+                                                  const float sn_, const float lam, const AdjArgs &a, float o[5]) {
+    const float pxx = (KC0 * Pf(0, 0) + KC1 * (Pf(1, 0) + Pf(-1, 0)) + KC2 * (Pf(2, 0) + Pf(-2, 0)) +
+                       KC3 * (Pf(3, 0) + Pf(-3, 0)) + KC4 * (Pf(4, 0) + Pf(-4, 0))) *
+                      a.idx2;
+    const float pzz = (KC0 * Pf(0, 0) + KC1 * (Pf(0, 1) + Pf(0, -1)) + KC2 * (Pf(0, 2) + Pf(0, -2)) +
+                       KC3 * (Pf(0, 3) + Pf(0, -3)) + KC4 * (Pf(0, 4) + Pf(0, -4))) *
+                      a.idz2;
+    const float px = (KA1 * (Pf(1, 0) - Pf(-1, 0)) + KA2 * (Pf(2, 0) - Pf(-2, 0)) + KA3 * (Pf(3, 0) - Pf(-3, 0)) +
+                      KA4 * (Pf(4, 0) - Pf(-4, 0))) *
+                     a.idx;
+    const float pz = (KA1 * (Pf(0, 1) - Pf(0, -1)) + KA2 * (Pf(0, 2) - Pf(0, -2)) + KA3 * (Pf(0, 3) - Pf(0, -3)) +
+                      KA4 * (Pf(0, 4) - Pf(0, -4))) *
+                     a.idz;
+    float X = px, Z = pz, c = 1.f, sn = 0.f;
+    if (PHYS == 2) {
+        c = c_;
+        sn = sn_;
+        X = px * c - pz * sn;
+        Z = px * sn + pz * c;
     }
-    T[i] = o0;
-    T[pl + i] = o1;
-    T[2 * pl + i] = o2;
-    T[3 * pl + i] = o3;
-    if (PHYS == 2) T[4 * pl + i] = o4;
+    const float X2 = X * X, Z2 = Z * Z, X4 = X2 * X2, Z4 = Z2 * Z2;
+    const float num = -2.0f * (e - dl) * X2 * Z2;
+    const float den = (1.0f + 2.0f * e) * X4 + Z4 + 2.0f * (1.0f + dl) * X2 * Z2;
+    float Sd = 0.f, Cx = 0.f, Cz = 0.f;
+    if (fabsf(den) > 1.0e-12f) {
+        Sd = num / den;
+        if (PHYS == 1) {
+            const float fac = 4.0f * (e - dl) * ((1.0f + 2.0f * e) * X4 - Z4) / (den * den);
+            Cx = fac * px * Z2;
+            Cz = -fac * X2 * pz;
+        } else {
+            const float K = (1.0f + 2.0f * e) * X4 - Z4;
+            const float fac = 4.0f * (e - dl) * X * Z * K / (den * den);
+            Cx = fac * pz;
+            Cz = -fac * px;
+        }
+    }
+    const float Q = pxx + pzz;
+    o[4] = 0.f;
+    if (PHYS == 1) {
+        o[0] = (1.0f + 2.0f * e + Sd) * lam;
+        o[1] = (1.0f + Sd) * lam;
+    } else {
+        o[0] = ((1.0f + 2.0f * e) * (c * c) + (sn * sn) + Sd) * lam;
+        o[1] = ((1.0f + 2.0f * e) * (sn * sn) + (c * c) + Sd) * lam;
+        o[4] = (2.0f * e * (2.0f * sn * c)) * lam;
+    }
+    o[2] = Q * Cx * lam;
+    o[3] = Q * Cz * lam;
 }
 
-// lambda_new = 2 lam - lam_prev + V (dxx(A lam) + dzz(B lam) [- dxz(H lam)] - dx(QCx lam) - dz(QCz lam)),
-// Cerjan damping, data injection of the next adjoint step, imaging with S_t.
-template <int PHYS>
-__global__ __launch_bounds__(256) void k_adj_update(const AdjArgs a) {
-    const int s = blockIdx.z;
-    const int ix = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int iz = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (ix < 4 || ix >= a.nxp - 4 || iz < 4 || iz >= a.nzp - 4) return;
-    const long long P = a.P;
-    const long long i = (long long)iz * P + ix;
-    const long long pl = (long long)a.nzp * P;
-    const float *T0 = a.tmp + (long long)s * a.tsstride;
-    const float *T1 = T0 + pl, *T2 = T0 + 2 * pl, *T3 = T0 + 3 * pl, *T4 = T0 + 4 * pl;
-    const float dxxA = (KC0 * T0[i] + KC1 * (T0[i + 1] + T0[i - 1]) + KC2 * (T0[i + 2] + T0[i - 2]) +
-                        KC3 * (T0[i + 3] + T0[i - 3]) + KC4 * (T0[i + 4] + T0[i - 4])) *
+// The adjoint operator at one cell: dxx(A lam) + dzz(B lam) [- dxz(H lam)] -
+// dx(Q Cx lam) - dz(Q Cz lam); T(k, dx, dz) is product k at (x + dx, z + dz).
+template <int PHYS, typename TF>
+__device__ __forceinline__ float adj_operator(const TF &T, const AdjArgs &a) {
+    const float dxxA = (KC0 * T(0, 0, 0) + KC1 * (T(0, 1, 0) + T(0, -1, 0)) + KC2 * (T(0, 2, 0) + T(0, -2, 0)) +
+                        KC3 * (T(0, 3, 0) + T(0, -3, 0)) + KC4 * (T(0, 4, 0) + T(0, -4, 0))) *
                        a.idx2;
-    const float dzzB = (KC0 * T1[i] + KC1 * (T1[i + P] + T1[i - P]) + KC2 * (T1[i + 2 * P] + T1[i - 2 * P]) +
-                        KC3 * (T1[i + 3 * P] + T1[i - 3 * P]) + KC4 * (T1[i + 4 * P] + T1[i - 4 * P])) *
+    const float dzzB = (KC0 * T(1, 0, 0) + KC1 * (T(1, 0, 1) + T(1, 0, -1)) + KC2 * (T(1, 0, 2) + T(1, 0, -2)) +
+                        KC3 * (T(1, 0, 3) + T(1, 0, -3)) + KC4 * (T(1, 0, 4) + T(1, 0, -4))) *
                        a.idz2;
-    const float dxQ = (KA1 * (T2[i + 1] - T2[i - 1]) + KA2 * (T2[i + 2] - T2[i - 2]) + KA3 * (T2[i + 3] - T2[i - 3]) +
-                       KA4 * (T2[i + 4] - T2[i - 4])) *
+    const float dxQ = (KA1 * (T(2, 1, 0) - T(2, -1, 0)) + KA2 * (T(2, 2, 0) - T(2, -2, 0)) +
+                       KA3 * (T(2, 3, 0) - T(2, -3, 0)) + KA4 * (T(2, 4, 0) - T(2, -4, 0))) *
                       a.idx;
-    const float dzQ = (KA1 * (T3[i + P] - T3[i - P]) + KA2 * (T3[i + 2 * P] - T3[i - 2 * P]) +
-                       KA3 * (T3[i + 3 * P] - T3[i - 3 * P]) + KA4 * (T3[i + 4 * P] - T3[i - 4 * P])) *
+    const float dzQ = (KA1 * (T(3, 0, 1) - T(3, 0, -1)) + KA2 * (T(3, 0, 2) - T(3, 0, -2)) +
+                       KA3 * (T(3, 0, 3) - T(3, 0, -3)) + KA4 * (T(3, 0, 4) - T(3, 0, -4))) *
                       a.idz;
-    float op;
     if (PHYS == 2) {
         float acc = 0.f;
         const float aa[4] = {KA1, KA2, KA3, KA4};
@@ -253,12 +228,18 @@ __global__ __launch_bounds__(256) void k_adj_update(const AdjArgs a) {
         for (int ii = 1; ii <= 4; ++ii)
 #pragma unroll
             for (int jj = 1; jj <= 4; ++jj)
-                acc += aa[ii - 1] * aa[jj - 1] *
-                       (T4[i + jj * P + ii] - T4[i - jj * P + ii] + T4[i - jj * P - ii] - T4[i + jj * P - ii]);
-        op = dxxA + dzzB - acc * a.idxdz - dxQ - dzQ;
-    } else {
-        op = dxxA + dzzB - dxQ - dzQ;
+                acc += aa[ii - 1] * aa[jj - 1] * (T(4, ii, jj) - T(4, ii, -jj) + T(4, -ii, -jj) - T(4, -ii, jj));
+        return dxxA + dzzB - acc * a.idxdz - dxQ - dzQ;
     }
+    return dxxA + dzzB - dxQ - dzQ;
+}
+
+// lambda_new = 2 lam - lam_prev + V op at padded cell (ix, iz) of shot s, with
+// Cerjan damping, the data injection of the next adjoint step and the imaging
+// with S_t (or the gradient with D_t)
+__device__ __forceinline__ void adj_finish(const AdjArgs &a, const int s, const int ix, const int iz, const float op) {
+    const long long P = a.P;
+    const long long i = (long long)iz * P + ix;
     const long long so = (long long)s * a.sstride;
     const float lam = a.cur[so + i];
     const float fx = a.cerx[ix], fz = a.cerz[iz];
@@ -287,6 +268,64 @@ __global__ __launch_bounds__(256) void k_adj_update(const AdjArgs a) {
         } else {
             a.image[(long long)s * a.acc_sstride + ii] += Ps[0] * lam;
         }
+    }
+}
+
+// One adjoint step of the linearised VTI/TTI operator, products and update in
+// one pass: a block takes one 64 x 16 tile of one shot (tiles of a shot group
+// adjacent, so the model is reused from L2; XCD-contiguous ranges), stages the
+// source field with an 8-cell halo in LDS, forms the products over the tile
+// with a 4-cell halo in LDS (zero outside the stencil interior, as the
+// two-pass form wrote them), then updates its cells from them.  No product
+// plane goes through HBM (the two-pass form wrote and re-read 16 or 20 bytes
+// per cell and shot), and each value is formed by the same expressions, so the
+// results are the two-pass form's bit for bit.
+template <int PHYS>
+__global__ __launch_bounds__(256) void k_adj_fused(const AdjArgs a) {
+    constexpr int TX = 64, TZ = 16;
+    constexpr int HX = TX + 8, HZ = TZ + 8;    // products: the tile with a 4-cell halo
+    constexpr int PX = TX + 16, PZ = TZ + 16;  // source field: the tile with an 8-cell halo
+    constexpr int NT = (PHYS == 2) ? 5 : 4;
+    __shared__ float sP[PZ][PX];
+    __shared__ float sT[NT][HZ][HX];
+    const int nxt = (a.nxp + TX - 1) / TX, nzt = (a.nzp + TZ - 1) / TZ;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    if (b >= nxt * nzt * a.nshots) return;  // block-uniform
+    const int tile = b / a.nshots, s = b - tile * a.nshots;
+    const int tzi = tile / nxt, txi = tile - tzi * nxt;
+    const int x0 = txi * TX, z0 = tzi * TZ;
+    const int tid = threadIdx.x;
+    const int P = a.P, nzp = a.nzp, nxp = a.nxp;
+    const float *Pp = a.P_ + (long long)s * a.psstride;
+    const float *lam = a.cur + (long long)s * a.sstride;
+    for (int i = tid; i < PZ * PX; i += 256) {
+        const int r = i / PX, q = i - r * PX;
+        const int gz = z0 - 8 + r, gx = x0 - 8 + q;
+        sP[r][q] = (gz >= 0 && gz < nzp && gx >= 0 && gx < nxp) ? Pp[(long long)gz * P + gx] : 0.0f;
+    }
+    __syncthreads();
+    for (int i = tid; i < HZ * HX; i += 256) {
+        const int r = i / HX, q = i - r * HX;
+        const int gz = z0 - 4 + r, gx = x0 - 4 + q;
+        float o[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+        if (gz >= 4 && gz < nzp - 4 && gx >= 4 && gx < nxp - 4) {
+            const long long gi = (long long)gz * P + gx;
+            const auto Pf = [&](int dx, int dz) { return sP[r + 4 + dz][q + 4 + dx]; };
+            adj_products_cell<PHYS>(Pf, a.eps[gi], a.del[gi], PHYS == 2 ? a.cs[gi] : 1.0f,
+                                    PHYS == 2 ? a.sn[gi] : 0.0f, lam[gi], a, o);
+        }
+#pragma unroll
+        for (int k = 0; k < NT; ++k) sT[k][r][q] = o[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < (TX * TZ) / 256; ++u) {
+        const int c = tid + 256 * u;
+        const int r = c / TX, q = c - r * TX;
+        const int ix = x0 + q, iz = z0 + r;
+        if (ix < 4 || ix >= nxp - 4 || iz < 4 || iz >= nzp - 4) continue;
+        const auto T = [&](int k, int dx, int dz) { return sT[k][r + 4 + dz][q + 4 + dx]; };
+        adj_finish(a, s, ix, iz, adj_operator<PHYS>(T, a));
     }
 }
 
@@ -640,16 +679,11 @@ int launch_psi_multi(const StepArgs *list, const int *pboff, int ngrid, int npbl
     return (int)hipGetLastError();
 }
 
-int launch_adj_products(int phys, const AdjArgs &a, hipStream_t st) {
-    dim3 g((a.nxp + 63) / 64, (a.nzp + 3) / 4, a.nshots);
-    if (phys == 1) hipLaunchKernelGGL(k_adj_products<1>, g, dim3(256), 0, st, a);
-    else hipLaunchKernelGGL(k_adj_products<2>, g, dim3(256), 0, st, a);
-    return (int)hipGetLastError();
-}
-int launch_adj_update(int phys, const AdjArgs &a, hipStream_t st) {
-    dim3 g((a.nxp + 63) / 64, (a.nzp + 3) / 4, a.nshots);
-    if (phys == 1) hipLaunchKernelGGL(k_adj_update<1>, g, dim3(256), 0, st, a);
-    else hipLaunchKernelGGL(k_adj_update<2>, g, dim3(256), 0, st, a);
+int launch_adj_step(int phys, const AdjArgs &a, hipStream_t st) {
+    const long long tiles = (long long)((a.nxp + 63) / 64) * ((a.nzp + 15) / 16) * a.nshots;
+    const dim3 g((unsigned)((tiles + 7) / 8 * 8));  // a multiple of 8 for the XCD remap
+    if (phys == 1) hipLaunchKernelGGL(k_adj_fused<1>, g, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_adj_fused<2>, g, dim3(256), 0, st, a);
     return (int)hipGetLastError();
 }
 
