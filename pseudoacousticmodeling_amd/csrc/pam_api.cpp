@@ -105,7 +105,8 @@ struct pam_ctx {
     long long rvbits_bytes = 0;
     unsigned int *smax = nullptr;  // per-snapshot max |S_k| (compressed storage)
     long long smax_bytes = 0;
-    // adjoint scratch
+    // the temporal-blocking variant's second field pair: U0 planes of every
+    // shot slot, then U1 planes (slot offsets as in fs)
     float *tmp = nullptr;
     long long tmp_bytes = 0;
     void *save = nullptr;
@@ -705,6 +706,41 @@ static int fwd_step(pam_ctx *ctx, StepArgs &a, int tile, float *cur, float *prv,
     return fwd_step_fs(ctx, ctx->fs, a, tile, cur, prv, k, ev);
 }
 
+// The temporal-blocking variant: nlaunch launches of tblock_steps steps from
+// step k0 on the shots of f (a view of the context's field set), alternating
+// between f's pair (cur, prv) and the matching slots of the second pair; with
+// nlaunch even the state ends in (cur, prv) with cur holding the last level,
+// as nlaunch * tblock_steps single steps (an even number) would leave it.
+// (the second pair is allocated by ensure_tblock beforehand: never during a
+// graph capture)
+static int ensure_scratch(pam_ctx *ctx, void **p, long long *have, long long need);
+static int ensure_tblock(pam_ctx *ctx) {
+    return ensure_scratch(ctx, (void **)&ctx->tmp, &ctx->tmp_bytes, 2LL * ctx->max_shots * ctx->plane * 4);
+}
+static int tblock_launches(pam_ctx *ctx, const FieldSet &f, const StepArgs &a, hipStream_t st, float *cur,
+                           float *prv, int k0, int nlaunch) {
+    if (nlaunch & 1) return fail(ctx, PAM_E_STATE, "temporal blocking needs an even number of launches");
+    if (!ctx->tmp || ctx->tmp_bytes < 2LL * ctx->max_shots * ctx->plane * 4)
+        return fail(ctx, PAM_E_STATE, "temporal blocking without its second field pair");
+    const long long slot = f.U0 - ctx->fs.U0;  // floats from the set's first shot slot
+    if (slot < 0 || slot + (long long)a.nshots * ctx->plane > (long long)ctx->max_shots * ctx->plane)
+        return fail(ctx, PAM_E_STATE, "temporal blocking outside the context's field set");
+    float *x0 = ctx->tmp + slot, *x1 = ctx->tmp + (long long)ctx->max_shots * ctx->plane + slot;
+    const int T = tblock_steps(ctx->phys);
+    for (int j = 0; j < nlaunch; ++j) {
+        const int k = k0 + j * T;
+        if ((j & 1) == 0)
+            KCK(launch_tblock(ctx->phys, a, cur, prv, x0, x1, k, T, st));
+        else
+            KCK(launch_tblock(ctx->phys, a, x0, x1, cur, prv, k, T, st));
+        ctx->steps_run += T;
+    }
+    return PAM_OK;
+}
+static bool tblock_usable(const pam_ctx *ctx, int tile, int flags) {
+    return tile_is_tblock(tile) && ctx->abc == PAM_CERJAN && (flags & ~F_FWD_MASK) == 0;
+}
+
 // The autotuner: times every step-kernel variant on the wavefields of `nb`
 // shots (a few plain steps each, HIP events on the context's stream) and keeps
 // the fastest for that batch size.  The fields it steps on are cleared before
@@ -718,30 +754,43 @@ static int autotune(pam_ctx *ctx, int nb, int *best, double *best_per_shot_step)
     base_step_args(ctx, a, nb);
     int r = zero_shot_state(ctx, nb, true, true);
     if (r) return r;
-    double best_ms = 1e300;
+    double best_ms = 1e300;  // per step
     for (int t = 0; t < tile_count(); ++t) {
         float *c = ctx->fs.U0, *p = ctx->fs.U1;
-        for (int i = 0; i < 2; ++i) {
-            r = fwd_step(ctx, a, t, c, p, i, nullptr);
+        int nsteps = 6;
+        if (tile_is_tblock(t)) {  // two launches untimed, two timed
+            if (!tblock_usable(ctx, t, a.flags)) continue;
+            r = ensure_tblock(ctx);
             if (r) return r;
-            std::swap(c, p);
-        }
-        HCK(hipEventRecord(ctx->e0, ctx->st));
-        for (int i = 0; i < 6; ++i) {
-            r = fwd_step(ctx, a, t, c, p, i, nullptr);
+            r = tblock_launches(ctx, ctx->fs, a, ctx->st, c, p, 0, 2);
             if (r) return r;
-            std::swap(c, p);
+            HCK(hipEventRecord(ctx->e0, ctx->st));
+            r = tblock_launches(ctx, ctx->fs, a, ctx->st, c, p, 2 * tblock_steps(ctx->phys), 2);
+            if (r) return r;
+            nsteps = 2 * tblock_steps(ctx->phys);
+        } else {
+            for (int i = 0; i < 2; ++i) {
+                r = fwd_step(ctx, a, t, c, p, i, nullptr);
+                if (r) return r;
+                std::swap(c, p);
+            }
+            HCK(hipEventRecord(ctx->e0, ctx->st));
+            for (int i = 0; i < 6; ++i) {
+                r = fwd_step(ctx, a, t, c, p, i, nullptr);
+                if (r) return r;
+                std::swap(c, p);
+            }
         }
         HCK(hipEventRecord(ctx->e1, ctx->st));
         HCK(hipEventSynchronize(ctx->e1));
         float ms = 0.f;
         HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
-        if (ms < best_ms) {
-            best_ms = ms;
+        if (ms / nsteps < best_ms) {
+            best_ms = ms / nsteps;
             *best = t;
         }
     }
-    *best_per_shot_step = best_ms / (6.0 * nb);
+    *best_per_shot_step = best_ms / nb;
     return zero_shot_state(ctx, nb, true, true);
 }
 
@@ -1110,14 +1159,28 @@ static int run_forward_loop_fs(pam_ctx *ctx, FieldSet &f, hipStream_t st, StepAr
     }
     if (timed) HCK(hipEventRecord(ctx->e0, st));
     const int base_flags = a.flags;
+    // the temporal-blocking variant: an even number of tblock_steps-step
+    // launches after the first nt % (2 tblock_steps) steps taken singly (its
+    // inner steps gather only from itlag on themselves)
+    const bool tb = !events && tblock_usable(ctx, tile, base_flags);
+    const int nsingle = tb ? nt % (2 * tblock_steps(ctx->phys)) : nt;
+    if (tb) {
+        int r = ensure_tblock(ctx);
+        if (r) return r;
+    }
     auto body = [&]() -> int {
         float *cur = f.U0, *prv = f.U1;
-        for (int k = 0; k < nt; ++k) {
+        for (int k = 0; k < nsingle; ++k) {
             a.flags = base_flags;
             if (!(k >= ctx->g.itlag)) a.flags &= ~F_GATHER;
             int r = fwd_step_fs(ctx, f, a, tile, cur, prv, k, events ? &ctx->kev[2 * k] : nullptr, st);
             if (r) return r;
             std::swap(cur, prv);
+        }
+        if (tb) {
+            a.flags = base_flags;
+            int r = tblock_launches(ctx, f, a, st, cur, prv, nsingle, (nt - nsingle) / tblock_steps(ctx->phys));
+            if (r) return r;
         }
         return PAM_OK;
     };
@@ -1205,10 +1268,25 @@ static int autotune_lanes(pam_ctx *ctx, int Bl, int *best) {
     for (int l = 0; l < 2; ++l) base_step_args(ctx, a[l], Bl);
     r = zero_shot_state(ctx, 2 * Bl, true, true);
     if (r) return r;
-    double best_ms = 1e300;
+    double best_ms = 1e300;  // per step
     for (int t = 0; t < tile_count(); ++t) {
         float *c[2] = {fsl[0]->U0, fsl[1]->U0}, *p[2] = {fsl[0]->U1, fsl[1]->U1};
+        const bool tb = tile_is_tblock(t);
+        if (tb && !tblock_usable(ctx, t, a[0].flags)) continue;
+        if (tb) {
+            r = ensure_tblock(ctx);
+            if (r) return r;
+        }
+        // n single steps, or n / tblock_steps launches of the temporal-blocking variant
         auto steps = [&](int k0, int n) -> int {
+            if (tb) {
+                for (int l = 0; l < 2; ++l) {
+                    int rr = tblock_launches(ctx, *fsl[l], a[l], ctx->lane_st[l], c[l], p[l], k0,
+                                             n / tblock_steps(ctx->phys));
+                    if (rr) return rr;
+                }
+                return PAM_OK;
+            }
             for (int i = 0; i < n; ++i)
                 for (int l = 0; l < 2; ++l) {
                     int rr = fwd_step_fs(ctx, *fsl[l], a[l], t, c[l], p[l], k0 + i, nullptr, ctx->lane_st[l]);
@@ -1217,21 +1295,22 @@ static int autotune_lanes(pam_ctx *ctx, int Bl, int *best) {
                 }
             return PAM_OK;
         };
+        const int nwarm = tb ? 2 * tblock_steps(ctx->phys) : 2, ntimed = tb ? 2 * tblock_steps(ctx->phys) : 6;
         r = lanes_fork(ctx);
         if (r) return r;
-        r = lanes_join(ctx, steps(0, 2));
+        r = lanes_join(ctx, steps(0, nwarm));
         if (r) return r;
         HCK(hipEventRecord(ctx->e0, ctx->st));
         r = lanes_fork(ctx);
         if (r) return r;
-        r = lanes_join(ctx, steps(2, 6));
+        r = lanes_join(ctx, steps(nwarm, ntimed));
         if (r) return r;
         HCK(hipEventRecord(ctx->e1, ctx->st));
         HCK(hipEventSynchronize(ctx->e1));
         float ms = 0.f;
         HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
-        if (ms < best_ms) {
-            best_ms = ms;
+        if (ms / ntimed < best_ms) {
+            best_ms = ms / ntimed;
             *best = t;
         }
     }

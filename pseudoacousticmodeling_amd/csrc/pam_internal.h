@@ -171,6 +171,17 @@ int launch_step_multi(int phys, bool cpml, int tile, const StepArgs *list, const
 bool tile_has_multi(int tile);
 // row chunks of 16 per block of a z-march variant (0 for the other families)
 int march_chunks(int tile);
+// steps advanced per launch by the temporal-blocking variant (k_tblock); 8 for
+// acoustic measured slower on a 301 x 501 grid (2.49 against 2.30 us per step:
+// its 132 KB of LDS leave one block per CU)
+inline int tblock_steps(int) { return 4; }
+// nsub steps k0 .. k0 + nsub - 1 (1 <= nsub <= tblock_steps(phys)) of a.nshots shots with
+// the Cerjan boundary and forward flags, from (in_cur, in_prv) = (U_k0, U_k0-1)
+// into (out_cur, out_prv) = (U_k0+nsub, U_k0+nsub-1): a separate buffer pair
+int launch_tblock(int phys, const StepArgs &a, const float *in_cur, const float *in_prv, float *out_cur,
+                  float *out_prv, int k0, int nsub, hipStream_t st);
+// true for the temporal-blocking variant
+bool tile_is_tblock(int idx);
 // one step of the linearised VTI/TTI adjoint (products and update in one pass)
 int launch_adj_step(int phys, const AdjArgs &a, hipStream_t st);
 

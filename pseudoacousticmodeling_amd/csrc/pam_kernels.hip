@@ -619,8 +619,9 @@ TileShape tile_shape(int idx) {
 bool tile_fuses_psi(int idx) { return tile_shape(idx).V == -1 || tile_shape(idx).V == -3; }
 bool tile_pads_acc(int idx) {
     const int v = tile_shape(idx).V;
-    return v == 0 || v == -2 || v == -3;
+    return v == 0 || v == -2 || v == -3 || v == -4;
 }
+bool tile_is_tblock(int idx) { return idx >= 0 && idx < kNumTiles && kTiles[idx].V == -4; }
 
 int launch_step_acoustic_cerjan(int, const StepArgs &, hipStream_t, const hipEvent_t *);
 int launch_step_acoustic_cpml(int, const StepArgs &, hipStream_t, const hipEvent_t *);

@@ -4,7 +4,9 @@
 // wavefront), 4..6 the LDS-ring z-march k_march (7, 14, 28 chunks of 16 rows
 // per block), 7..9 the pipelined 16-row tile k_tile_pipe (interior tiles in
 // groups of 2, 4 or 8 shots), 10..11 the same with groups of 4 or 8 whose
-// edge tiles update the CPML psi themselves (no pre-pass).
+// edge tiles update the CPML psi themselves (no pre-pass), 12 the temporal
+// blocking k_tblock (tblock_steps(phys) steps per launch, Cerjan, whole forward runs
+// only: a single step of it is variant 0's).
 #pragma once
 #include "pam_step.cuh"
 #include "pam_tile.cuh"
@@ -12,7 +14,7 @@
 
 namespace pam {
 
-static constexpr TileShape kTiles[] = {{0, 16}, {0, 32}, {0, 64}, {4, 16}, {-1, 7}, {-1, 14}, {-1, 28}, {-2, 2}, {-2, 4}, {-2, 8}, {-3, 4}, {-3, 8}};
+static constexpr TileShape kTiles[] = {{0, 16}, {0, 32}, {0, 64}, {4, 16}, {-1, 7}, {-1, 14}, {-1, 28}, {-2, 2}, {-2, 4}, {-2, 8}, {-3, 4}, {-3, 8}, {-4, 0}};
 static constexpr int kNumTiles = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
 
 template <int PHYS, bool CPML>
@@ -30,6 +32,7 @@ static int launch_step_p(int tile, const StepArgs &a, hipStream_t st, const hipE
         case 9: return launch_tile_pipe_t<PHYS, CPML, 8, false>(a, st, ev);
         case 10: return launch_tile_pipe_t<PHYS, CPML, 4, true>(a, st, ev);
         case 11: return launch_tile_pipe_t<PHYS, CPML, 8, true>(a, st, ev);
+        case 12: return launch_tile_t<PHYS, CPML, 16, 1>(a, st, ev);
         default: return (int)hipErrorInvalidValue;
     }
 }
