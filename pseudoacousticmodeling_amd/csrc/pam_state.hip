@@ -3,6 +3,8 @@
 // v / m in float16 with m = its max |v|, and read back as q * m.
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+
 #include "pam_internal.h"
 
 namespace pam {
@@ -13,22 +15,25 @@ __global__ void k_zero_u32(unsigned int *p, int n) {
 }
 
 // bits[s] = max(bits[s], max |x_s|) as float bits (non-negative floats order
-// like their bit patterns); grid (blocks, nshots)
+// like their bit patterns); grid (blocks, nshots).  Each block reduces its
+// share in registers, then across its wavefronts, and makes one atomicMax: the
+// blocks of a shot are kept few (kMaxBlocksPerShot), since every atomic on
+// one address is serialised (1024 blocks per shot took 304 us per call on 32
+// shots of 401 x 1344 cells, most of it waiting on the atomics).
+constexpr int kMaxBlocksPerShot = 64;
 __global__ __launch_bounds__(256) void k_shot_maxabs(const float *x, long long sstride, long long n,
                                                      unsigned int *bits) {
-    __shared__ float red[256];
+    __shared__ float red[4];
     const int s = blockIdx.y;
     const float *xs = x + (long long)s * sstride;
     float m = 0.0f;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         m = fmaxf(m, fabsf(xs[i]));
-    red[threadIdx.x] = m;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicMax(bits + s, __float_as_uint(red[0]));
+    if (threadIdx.x == 0) atomicMax(bits + s, __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
 }
 
 __global__ __launch_bounds__(256) void k_pack_half(const float *x, long long sstride, long long n, __half *q,
@@ -56,7 +61,8 @@ static unsigned grid_blocks(long long n) {
 int launch_pack_half(const float *x, long long sstride, long long n, int nshots, void *q, long long qsstride,
                      unsigned int *bits, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_u32, dim3((nshots + 255) / 256), dim3(256), 0, st, bits, nshots);
-    hipLaunchKernelGGL(k_shot_maxabs, dim3(grid_blocks(n), nshots), dim3(256), 0, st, x, sstride, n, bits);
+    const unsigned mb = std::min<unsigned>(grid_blocks(n), (unsigned)kMaxBlocksPerShot);
+    hipLaunchKernelGGL(k_shot_maxabs, dim3(mb, nshots), dim3(256), 0, st, x, sstride, n, bits);
     hipLaunchKernelGGL(k_pack_half, dim3(grid_blocks(n), nshots), dim3(256), 0, st, x, sstride, n,
                        reinterpret_cast<__half *>(q), qsstride, (const unsigned int *)bits);
     return (int)hipGetLastError();
