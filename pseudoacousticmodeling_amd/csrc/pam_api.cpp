@@ -3026,6 +3026,21 @@ extern "C" int pam_misfit(pam_ctx *ctx, int nshots, const float *dobs, const flo
 }
 
 // ------------------------------------------------------------ exact adjoint
+// The saved state a reverse step reads: C_k and, for CPML, the psi and zeta
+// step k wrote (of step k - 1 only psi and zeta are read)
+struct ExState {
+    const float *C = nullptr, *psx = nullptr, *psz = nullptr, *zex = nullptr, *zez = nullptr;
+};
+// the state in a snapshot slot (fields blk floats apart)
+static ExState ex_slot(const float *slot, long long blk) {
+    ExState e;
+    e.C = slot;
+    e.psx = slot + blk;
+    e.psz = slot + 2 * blk;
+    e.zex = slot + 3 * blk;
+    e.zez = slot + 4 * blk;
+    return e;
+}
 // Copy the state of step k held by field set f (C_k = cur, and for CPML the
 // psi and zeta the step just wrote) into a snapshot slot: fields `blk` floats
 // apart, shots one plane apart.
@@ -3047,13 +3062,15 @@ static int exact_copy_state(pam_ctx *ctx, const FieldSet &f, const float *cur, f
 // for CPML, the previous step's psi and zeta) is kept at up to nslot steps in
 // float32 or, with `half`, each field scaled by its max |value| and rounded to
 // float16.  Reverse step t needs the state of step t (U_t with the psi and
-// zeta step t writes) and the psi and zeta of step t - 1: from the working
-// state before step t, the second is copied into Sm, step t is run, and the
-// first is copied into S.  Steps are reversed in treeverse order from
-// nt - 1 down to 0.
+// zeta step t writes) and the psi and zeta of step t - 1: the zeta of step
+// t - 1 is copied into Sm before step t runs (step t overwrites it), and after
+// it U_t, both psi halves (psi_t and psi_{t-1}, double-buffered) and zeta_t are
+// read where they are (copying the ten fields per reverse step cost a quarter
+// of the reverse sweep).  Steps are reversed in treeverse order from nt - 1
+// down to 0.
 template <class Rev>
 static int exact_revolve(pam_ctx *ctx, StepArgs &a, int tile, int nb, int nslot, bool half, long long blk, float *Sm,
-                         float *S, Rev &reverse_step) {
+                         Rev &reverse_step) {
     const int nt = ctx->g.nt;
     const int nf = ckpt_fields(ctx);
     const long long fl = ctx->plane * nb;  // floats per field of a slot
@@ -3136,16 +3153,28 @@ static int exact_revolve(pam_ctx *ctx, StepArgs &a, int tile, int nb, int nslot,
     auto leaf = [&](int t) -> int {
         int rr = advance(t, t);
         if (rr) return rr;
-        if (t >= 1) {
-            rr = exact_copy_state(ctx, w, wc, Sm, nb, blk);  // psi and zeta of step t - 1
-            if (rr) return rr;
+        const size_t fb = (size_t)ctx->plane * nb * sizeof(float);
+        if (t >= 1 && cpml) {  // zeta of step t - 1
+            HCK(hipMemcpyAsync(Sm + 3 * blk, w.zetax, fb, hipMemcpyDeviceToDevice, ctx->st));
+            HCK(hipMemcpyAsync(Sm + 4 * blk, w.zetaz, fb, hipMemcpyDeviceToDevice, ctx->st));
         }
         float *ut = wc;
         rr = wstep(t);
         if (rr) return rr;
-        rr = exact_copy_state(ctx, w, ut, S, nb, blk);
-        if (rr) return rr;
-        return reverse_step(t, S, t >= 1 ? Sm : nullptr);
+        ExState st, sm;
+        st.C = ut;
+        if (cpml) {
+            const long long newest = w.psi_flip ? ctx->psi_half : 0, older = w.psi_flip ? 0 : ctx->psi_half;
+            st.psx = w.psix + newest;
+            st.psz = w.psiz + newest;
+            st.zex = w.zetax;
+            st.zez = w.zetaz;
+            sm.psx = w.psix + older;
+            sm.psz = w.psiz + older;
+            sm.zex = Sm + 3 * blk;
+            sm.zez = Sm + 4 * blk;
+        }
+        return reverse_step(t, st, t >= 1 ? &sm : nullptr);
     };
     std::function<int(int, int, int)> rev = [&](int lo, int hi, int s) -> int {
         const int n = hi - lo;
@@ -3327,19 +3356,19 @@ extern "C" int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const fl
         const float *data = dseis + (long long)b0 * nd;
         // reverse step t from the saved state of step t (S) and of step t - 1 (Sm,
         // NULL at t = 0; only its psi and zeta are read)
-        auto reverse_step = [&](int t, const float *S, const float *Sm) -> int {
+        auto reverse_step = [&](int t, const ExState &S, const ExState *Sm) -> int {
             e.X = X;
             e.Y = Y;
-            e.C = S;
+            e.C = S.C;
             if (cpml) {
-                e.psx = S + blk;
-                e.psz = S + 2 * blk;
-                e.zex = S + 3 * blk;
-                e.zez = S + 4 * blk;
-                e.psxm = Sm ? Sm + blk : nullptr;
-                e.pszm = Sm ? Sm + 2 * blk : nullptr;
-                e.zexm = Sm ? Sm + 3 * blk : nullptr;
-                e.zezm = Sm ? Sm + 4 * blk : nullptr;
+                e.psx = S.psx;
+                e.psz = S.psz;
+                e.zex = S.zex;
+                e.zez = S.zez;
+                e.psxm = Sm ? Sm->psx : nullptr;
+                e.pszm = Sm ? Sm->psz : nullptr;
+                e.zexm = Sm ? Sm->zex : nullptr;
+                e.zezm = Sm ? Sm->zez : nullptr;
                 e.nux_in = nxi;
                 e.nuz_in = nzi;
                 e.nux_out = nxo;
@@ -3356,7 +3385,7 @@ extern "C" int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const fl
             return PAM_OK;
         };
         if (rv) {
-            r = exact_revolve(ctx, a, tile, nb, nslot, rv_half, blk, slot(0), slot(1), reverse_step);
+            r = exact_revolve(ctx, a, tile, nb, nslot, rv_half, blk, slot(0), reverse_step);
             if (r) return r;
         }
         for (int seg = rv ? -1 : nseg - 1; seg >= 0; --seg) {
@@ -3376,7 +3405,10 @@ extern "C" int pam_forward_adjoint(pam_ctx *ctx, int shot0, int nshots, const fl
                 }
             }
             for (int t = hi; t >= lo; --t) {
-                r = reverse_step(t, slot(t - k0), (t >= 1 && t - 1 >= k0) ? slot(t - 1 - k0) : nullptr);
+                const bool has_m = t >= 1 && t - 1 >= k0;
+                ExState sm;
+                if (has_m) sm = ex_slot(slot(t - 1 - k0), blk);
+                r = reverse_step(t, ex_slot(slot(t - k0), blk), has_m ? &sm : nullptr);
                 if (r) return r;
             }
         }
