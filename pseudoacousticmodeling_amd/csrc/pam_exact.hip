@@ -28,8 +28,8 @@
 // the live cells.  The saved forward state of step k (C_k, psi_k, zeta_k) is
 // read back from the snapshot buffers.
 //
-// One thread per padded cell, shots along blockIdx.z; the neighbours come
-// through the L1 / L2 caches.  This pass is the differentiable path, not the
+// One thread per padded cell (blocks of 64 x 4 cells), shots along
+// blockIdx.z; the neighbours come through the L1 / L2 caches.  This pass is the differentiable path, not the
 // throughput path.
 #include <hip/hip_runtime.h>
 
@@ -82,8 +82,8 @@ __device__ __forceinline__ float u_of(const ExactArgs &a, const float *X, const 
 }
 
 __global__ __launch_bounds__(256) void k_exact_mu(const ExactArgs a) {
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iz = blockIdx.y, s = blockIdx.z;
+    const int ix = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int iz = blockIdx.y * 4 + (threadIdx.x >> 6), s = blockIdx.z;
     if (ix >= a.nxp || iz >= a.nzp) return;
     const bool sx = xstrip(a, iz, ix), sz = zstrip(a, iz, ix);
     if (!sx && !sz) return;
@@ -121,8 +121,8 @@ __global__ __launch_bounds__(256) void k_exact_mu(const ExactArgs a) {
 }
 
 __global__ __launch_bounds__(256) void k_exact_lambda(const ExactArgs a) {
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iz = blockIdx.y, s = blockIdx.z;
+    const int ix = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int iz = blockIdx.y * 4 + (threadIdx.x >> 6), s = blockIdx.z;
     if (ix >= a.nxp || iz >= a.nzp) return;
     const long long P = a.P;
     const long long c = (long long)iz * P + ix;
@@ -274,12 +274,15 @@ __global__ void k_expand_T(const float *g, int nzp, int nxp, int P, int N, float
 
 }  // namespace
 
+// blocks of 64 x 4 cells: the z-neighbours of a cell are mostly in its own
+// block, so they come from L1 (rows of 256 cells fetched every z-neighbour
+// through L2)
 int launch_exact_mu(const ExactArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL(k_exact_mu, dim3((a.nxp + 255) / 256, a.nzp, a.nshots), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_exact_mu, dim3((a.nxp + 63) / 64, (a.nzp + 3) / 4, a.nshots), dim3(256), 0, st, a);
     return (int)hipGetLastError();
 }
 int launch_exact_lambda(const ExactArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL(k_exact_lambda, dim3((a.nxp + 255) / 256, a.nzp, a.nshots), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_exact_lambda, dim3((a.nxp + 63) / 64, (a.nzp + 3) / 4, a.nshots), dim3(256), 0, st, a);
     return (int)hipGetLastError();
 }
 int launch_exact_rec(float *Y, long long sstride, int P, const int *rz, const int *rx, int nrec, const float *data,
