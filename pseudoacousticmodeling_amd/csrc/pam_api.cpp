@@ -2827,13 +2827,15 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                 flip_of[sl] = w.psi_flip;
                 float *pf[6];
                 fields(pf);
+                if (rv_half) {
+                    KCK(launch_pack_half(pf, nf, ctx->plane, ctx->plane, nb,
+                                         (char *)ctx->ckpt + (long long)sl * nf * fl * es, fl, ctx->plane,
+                                         ctx->rvbits + (long long)sl * nf * B, B, ctx->st));
+                    return PAM_OK;
+                }
                 for (int i = 0; i < nf; ++i) {
                     char *dst = (char *)ctx->ckpt + ((long long)sl * nf + i) * fl * es;
-                    if (rv_half)
-                        KCK(launch_pack_half(pf[i], ctx->plane, ctx->plane, nb, dst, ctx->plane,
-                                             ctx->rvbits + ((long long)sl * nf + i) * B, ctx->st));
-                    else
-                        HCK(hipMemcpyAsync(dst, pf[i], (size_t)fl * 4, hipMemcpyDeviceToDevice, ctx->st));
+                    HCK(hipMemcpyAsync(dst, pf[i], (size_t)fl * 4, hipMemcpyDeviceToDevice, ctx->st));
                 }
                 return PAM_OK;
             };
@@ -2845,14 +2847,15 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                 w.psi_flip = flip_of[sl];
                 float *pf[6];
                 fields(pf);
-                for (int i = 0; i < nf; ++i) {
-                    const char *src = (const char *)ctx->ckpt + ((long long)sl * nf + i) * fl * es;
-                    if (rv_half)
-                        KCK(launch_unpack_half(src, ctx->plane, ctx->plane, nb,
-                                               ctx->rvbits + ((long long)sl * nf + i) * B, pf[i], ctx->plane,
-                                               ctx->st));
-                    else
+                if (rv_half) {
+                    KCK(launch_unpack_half((const char *)ctx->ckpt + (long long)sl * nf * fl * es, fl, ctx->plane,
+                                           ctx->plane, nb, nf, ctx->rvbits + (long long)sl * nf * B, B, pf, ctx->plane,
+                                           ctx->st));
+                } else {
+                    for (int i = 0; i < nf; ++i) {
+                        const char *src = (const char *)ctx->ckpt + ((long long)sl * nf + i) * fl * es;
                         HCK(hipMemcpyAsync(pf[i], src, (size_t)fl * 4, hipMemcpyDeviceToDevice, ctx->st));
+                    }
                 }
                 ws = k;
                 return PAM_OK;
@@ -3103,13 +3106,14 @@ static int exact_revolve(pam_ctx *ctx, StepArgs &a, int tile, int nb, int nslot,
         flip_of[sl] = w.psi_flip;
         float *pf[6];
         fields(pf);
+        if (half) {
+            KCK(launch_pack_half(pf, nf, ctx->plane, ctx->plane, nb, (char *)ctx->ckpt + (long long)sl * nf * fl * es,
+                                 fl, ctx->plane, ctx->rvbits + (long long)sl * nf * nb, nb, ctx->st));
+            return PAM_OK;
+        }
         for (int i = 0; i < nf; ++i) {
             char *dst = (char *)ctx->ckpt + ((long long)sl * nf + i) * fl * es;
-            if (half)
-                KCK(launch_pack_half(pf[i], ctx->plane, ctx->plane, nb, dst, ctx->plane,
-                                     ctx->rvbits + ((long long)sl * nf + i) * nb, ctx->st));
-            else
-                HCK(hipMemcpyAsync(dst, pf[i], (size_t)fl * 4, hipMemcpyDeviceToDevice, ctx->st));
+            HCK(hipMemcpyAsync(dst, pf[i], (size_t)fl * 4, hipMemcpyDeviceToDevice, ctx->st));
         }
         return PAM_OK;
     };
@@ -3121,13 +3125,14 @@ static int exact_revolve(pam_ctx *ctx, StepArgs &a, int tile, int nb, int nslot,
         w.psi_flip = flip_of[sl];
         float *pf[6];
         fields(pf);
-        for (int i = 0; i < nf; ++i) {
-            const char *src = (const char *)ctx->ckpt + ((long long)sl * nf + i) * fl * es;
-            if (half)
-                KCK(launch_unpack_half(src, ctx->plane, ctx->plane, nb, ctx->rvbits + ((long long)sl * nf + i) * nb,
-                                       pf[i], ctx->plane, ctx->st));
-            else
+        if (half) {
+            KCK(launch_unpack_half((const char *)ctx->ckpt + (long long)sl * nf * fl * es, fl, ctx->plane, ctx->plane, nb,
+                                   nf, ctx->rvbits + (long long)sl * nf * nb, nb, pf, ctx->plane, ctx->st));
+        } else {
+            for (int i = 0; i < nf; ++i) {
+                const char *src = (const char *)ctx->ckpt + ((long long)sl * nf + i) * fl * es;
                 HCK(hipMemcpyAsync(pf[i], src, (size_t)fl * 4, hipMemcpyDeviceToDevice, ctx->st));
+            }
         }
         ws = k;
         return PAM_OK;

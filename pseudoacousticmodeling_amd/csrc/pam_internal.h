@@ -258,12 +258,16 @@ int launch_rbc_source(float *U, long long sstride, int P, float *D, long long ds
 int launch_copy_interior(const float *src, float *dst, long long sstride, int nzp, int P, int N, int nz, int nx,
                          int nshots, hipStream_t st);
 
-// float16 copy of n floats per shot (shots sstride / qsstride apart), scaled
-// by each shot's max |x| (float bits into bits[shot]); and back
-int launch_pack_half(const float *x, long long sstride, long long n, int nshots, void *q, long long qsstride,
-                     unsigned int *bits, hipStream_t st);
-int launch_unpack_half(const void *q, long long qsstride, long long n, int nshots, const unsigned int *bits, float *x,
-                       long long sstride, hipStream_t st);
+// float16 copies of nf (<= kMaxFields) fields x[f] of n floats per shot
+// (shots sstride apart) into q + f qfstride (halfs; shots qsstride apart),
+// each field and shot scaled by its max |x| (float bits into
+// bits[f bfstride + shot]); and back
+constexpr int kMaxFields = 6;
+int launch_pack_half(const float *const *x, int nf, long long sstride, long long n, int nshots, void *q,
+                     long long qfstride, long long qsstride, unsigned int *bits, long long bfstride, hipStream_t st);
+int launch_unpack_half(const void *q, long long qfstride, long long qsstride, long long n, int nshots, int nf,
+                       const unsigned int *bits, long long bfstride, float *const *x, long long sstride,
+                       hipStream_t st);
 
 // preprocessing (pam_prep.hip)
 int launch_slowness(const float *f, float *s, long long n, int parameter, hipStream_t st);
