@@ -579,10 +579,6 @@ static void base_step_args(pam_ctx *ctx, StepArgs &a, int nshots) {
     a.nz = g.nz;
     a.nx = g.nx;
     a.nshots = nshots;
-    {
-        static const int band_env = getenv("PAM_TILE_BAND") ? atoi(getenv("PAM_TILE_BAND")) : 0;
-        a.band = band_env;
-    }
     a.sstride = ctx->plane;
     a.idx2 = 1.0f / (g.dx * g.dx);
     a.idz2 = 1.0f / (g.dz * g.dz);

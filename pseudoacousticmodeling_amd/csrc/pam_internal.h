@@ -53,7 +53,6 @@ struct StepArgs {
     int nzp, nxp, P, N;   // padded grid, row pitch (floats), absorbing width
     int nz, nx;           // interior
     int nxt, nzt, nshots; // x tiles, z tiles, shots in this launch
-    int band;             // tile columns per band of the 16-row tile order (0: 8)
     long long sstride;    // floats between consecutive shots' fields
     float idx2, idz2, idx, idz, idxdz, idt2, dxdz;
     int k;                // step index of the input field

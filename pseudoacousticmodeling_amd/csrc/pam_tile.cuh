@@ -44,7 +44,7 @@ struct TileStep {
 // x-neighbour right after it), the shots of a tile adjacent.
 template <int TZ>
 __device__ __forceinline__ void tile_of(const StepArgs &a, const int b, int &x0, int &z0, int &s) {
-    const int kBand = a.band > 0 ? a.band : 8;
+    constexpr int kBand = 8;
     const int tile = b / a.nshots;
     s = b - tile * a.nshots;
     const int band = tile / (kBand * a.nzt);
