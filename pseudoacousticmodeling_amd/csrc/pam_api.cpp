@@ -742,6 +742,7 @@ static bool tblock_usable(const pam_ctx *ctx, int tile, int flags) {
 // the fastest for that batch size.  The fields it steps on are cleared before
 // and after, so it must run before a shot is primed.
 static constexpr int kDefaultTile = 5;
+static constexpr int kTuneRounds = 3;
 
 static int autotune(pam_ctx *ctx, int nb, int *best, double *best_per_shot_step) {
     *best = kDefaultTile;
@@ -753,36 +754,39 @@ static int autotune(pam_ctx *ctx, int nb, int *best, double *best_per_shot_step)
     double best_ms = 1e300;  // per step
     for (int t = 0; t < tile_count(); ++t) {
         float *c = ctx->fs.U0, *p = ctx->fs.U1;
-        int nsteps = 6;
-        if (tile_is_tblock(t)) {  // two launches untimed, two timed
+        const bool tb = tile_is_tblock(t);
+        if (tb) {
             if (!tblock_usable(ctx, t, a.flags)) continue;
             r = ensure_tblock(ctx);
             if (r) return r;
-            r = tblock_launches(ctx, ctx->fs, a, ctx->st, c, p, 0, 2);
-            if (r) return r;
-            HCK(hipEventRecord(ctx->e0, ctx->st));
-            r = tblock_launches(ctx, ctx->fs, a, ctx->st, c, p, 2 * tblock_steps(ctx->phys), 2);
-            if (r) return r;
-            nsteps = 2 * tblock_steps(ctx->phys);
-        } else {
-            for (int i = 0; i < 2; ++i) {
-                r = fwd_step(ctx, a, t, c, p, i, nullptr);
-                if (r) return r;
-                std::swap(c, p);
-            }
-            HCK(hipEventRecord(ctx->e0, ctx->st));
-            for (int i = 0; i < 6; ++i) {
-                r = fwd_step(ctx, a, t, c, p, i, nullptr);
-                if (r) return r;
-                std::swap(c, p);
-            }
         }
-        HCK(hipEventRecord(ctx->e1, ctx->st));
-        HCK(hipEventSynchronize(ctx->e1));
-        float ms = 0.f;
-        HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
-        if (ms / nsteps < best_ms) {
-            best_ms = ms / nsteps;
+        // warm-up steps, then the best of kTuneRounds timed windows (variants
+        // within a few percent of each other were picked by chance from one)
+        const int nsteps = tb ? 2 * tblock_steps(ctx->phys) : 6;
+        auto steps = [&](int k0, int n) -> int {
+            if (tb) return tblock_launches(ctx, ctx->fs, a, ctx->st, c, p, k0, n / tblock_steps(ctx->phys));
+            for (int i = 0; i < n; ++i) {
+                int rr = fwd_step(ctx, a, t, c, p, k0 + i, nullptr);
+                if (rr) return rr;
+                std::swap(c, p);
+            }
+            return PAM_OK;
+        };
+        r = steps(0, tb ? nsteps : 2);
+        if (r) return r;
+        double t_ms = 1e300;
+        for (int round = 0; round < kTuneRounds; ++round) {
+            HCK(hipEventRecord(ctx->e0, ctx->st));
+            r = steps(nsteps * (round + 1), nsteps);
+            if (r) return r;
+            HCK(hipEventRecord(ctx->e1, ctx->st));
+            HCK(hipEventSynchronize(ctx->e1));
+            float ms = 0.f;
+            HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
+            t_ms = std::min(t_ms, (double)ms / nsteps);
+        }
+        if (t_ms < best_ms) {
+            best_ms = t_ms;
             *best = t;
         }
     }
@@ -1296,17 +1300,21 @@ static int autotune_lanes(pam_ctx *ctx, int Bl, int *best) {
         if (r) return r;
         r = lanes_join(ctx, steps(0, nwarm));
         if (r) return r;
-        HCK(hipEventRecord(ctx->e0, ctx->st));
-        r = lanes_fork(ctx);
-        if (r) return r;
-        r = lanes_join(ctx, steps(nwarm, ntimed));
-        if (r) return r;
-        HCK(hipEventRecord(ctx->e1, ctx->st));
-        HCK(hipEventSynchronize(ctx->e1));
-        float ms = 0.f;
-        HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
-        if (ms / ntimed < best_ms) {
-            best_ms = ms / ntimed;
+        double t_ms = 1e300;  // the best of kTuneRounds windows
+        for (int round = 0; round < kTuneRounds; ++round) {
+            HCK(hipEventRecord(ctx->e0, ctx->st));
+            r = lanes_fork(ctx);
+            if (r) return r;
+            r = lanes_join(ctx, steps(nwarm + round * ntimed, ntimed));
+            if (r) return r;
+            HCK(hipEventRecord(ctx->e1, ctx->st));
+            HCK(hipEventSynchronize(ctx->e1));
+            float ms = 0.f;
+            HCK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
+            t_ms = std::min(t_ms, (double)ms / ntimed);
+        }
+        if (t_ms < best_ms) {
+            best_ms = t_ms;
             *best = t;
         }
     }
