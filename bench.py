@@ -21,7 +21,12 @@ Prints one JSON line:
   fwi_gradient seconds per shot of the FWI gradient of the same survey, all
              64 shots batched vs the per-shot loop
   uneven_group sixteen VTI + CPML models of different sizes (201..601 x
-             801..2401), 4 shots each, advanced as one group
+             801..2401), 4 shots each, advanced as one group; and their FWI
+             gradients, 8 shots each (uneven_group_gradient)
+  autograd_gradient the acoustic + CPML workload's FWI gradient through the
+             torch.autograd.Function, 50 float16 revolve slots
+  linear_adjoint RTM and FWI gradient with the linearised VTI / TTI adjoint
+             (Cerjan boundary), 16 shots, 398 steps
   other_configs acoustic + Cerjan (1 shot, 301 x 501 padded), TTI + CPML (32
              shots, 501 x 2101) and acoustic + CPML (32 shots, 401 x 1301),
              the last two on a reduced step count
@@ -147,6 +152,50 @@ def autograd_leg(slots=50):
            "finite": bool(torch.isfinite(vp.grad).all().item())}
     eng.close()
     torch.cuda.empty_cache()
+    return out
+
+
+def linear_adjoint_leg(nt_data=300, nshots=16):
+    """RTM and FWI gradient with the linearised VTI / TTI adjoint (the
+    reference's adjoint for pseudo-acoustic physics with the Cerjan boundary,
+    src/CudaKernels.py:1382-1633): the VTI and TTI workloads' models and
+    geometry with a 50-cell Cerjan layer (factor 0.0045) instead of CPML,
+    `nshots` shots, the first `nt_data` recorded samples, uniform checkpoints;
+    the residual is 1% of the recorded gather.  Seconds per shot."""
+    import torch
+
+    from pseudoacousticmodeling_amd import _lib, workloads
+    from pseudoacousticmodeling_amd.engine import Engine
+
+    out = {}
+    for build in (workloads.vti_cpml_64, workloads.tti_cpml_32):
+        w = build(nshots=nshots, nt_data=nt_data)
+        S = len(w["sx"])
+        eng = Engine(w["nz"], w["nx"], w["N"], w["dz"], w["dx"], w["dt"], w["nt"], w["itlag"], physics=w["physics"],
+                     abc=0, cerjan_factor=0.0045, max_shots=S)
+        eng.set_model(w["vp"], w["eps"], w["delta"], w["theta"])
+        eng.set_wavelet(w["src"])
+        eng.set_geometry(w["sz"], w["sx"], w["rz"], w["rx"])
+        eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_CHECKPOINT)
+        seis = torch.empty((S, w["nt"] - w["itlag"], len(w["rx"])), dtype=torch.float32, device="cuda:0")
+        eng.forward(0, S, out=seis)
+        res = seis * 0.01
+        rec = {"unit": "s/shot", "shots": S, "nt": w["nt"], "padded": [w["nz"] + 2 * w["N"], w["nx"] + 2 * w["N"]],
+               "storage": "uniform checkpoints in HBM"}
+        for kind in ("rtm", "gradient"):
+            f = eng.rtm if kind == "rtm" else eng.gradient
+            f(res)  # allocates, tunes
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            o = f(res)
+            torch.cuda.synchronize()
+            rec[kind] = (time.perf_counter() - t0) / S
+            arr = o["g_m"] if isinstance(o, dict) else o[0]
+            rec[kind + "_finite"] = bool(torch.isfinite(arr).all().item())
+        out[("vti" if w["physics"] == 1 else "tti") + "_cerjan"] = rec
+        eng.close()
+        del seis, res
+        torch.cuda.empty_cache()
     return out
 
 
@@ -322,6 +371,7 @@ def main(argv=None):
     ap.add_argument("--lanes", type=int, default=2, help="batches advanced at once on their own streams (1 or 2)")
     ap.add_argument("--no-group", action="store_true", help="skip the uneven-grid group leg")
     ap.add_argument("--no-autograd", action="store_true", help="skip the autograd FWI gradient leg")
+    ap.add_argument("--no-linear", action="store_true", help="skip the linearised VTI / TTI adjoint leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the other single-grid workloads")
     ap.add_argument("--configs-steps", type=int, default=500,
                     help="recorded samples of the reduced TTI + CPML and acoustic + CPML runs")
@@ -487,6 +537,13 @@ def main(argv=None):
         print(f"bench: autograd FWI gradient (acoustic CPML, revolve 50 float16 slots) "
               f"{autograd['value'] * 1e3:.1f} ms/shot", file=sys.stderr, flush=True)
 
+    linear = None
+    if not args.no_linear:
+        linear = linear_adjoint_leg()
+        for k, v in linear.items():
+            print(f"bench: linearised adjoint {k}: RTM {v['rtm'] * 1e3:.2f} ms/shot, FWI gradient "
+                  f"{v['gradient'] * 1e3:.2f} ms/shot ({v['shots']} shots, nt={v['nt']})", file=sys.stderr, flush=True)
+
     others = None
     if not args.no_configs:
         others = configs_leg(args.configs_steps)
@@ -539,6 +596,7 @@ def main(argv=None):
         "fwi_gradient": fwi,
         "uneven_group": group,
         "autograd_gradient": autograd,
+        "linear_adjoint": linear,
         "uneven_group_gradient": group_grad,
         "other_configs": others,
         "cpu_baseline": cpu,
