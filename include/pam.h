@@ -80,7 +80,9 @@ enum pam_option {
                               fastest per shot-step (small batches stay in the MALL) */
     PAM_OPT_TILE = 2,      /* step-kernel variant in [0, pam_tile_count()); -1 (default):
                               the autotuner times every variant on the first run of a
-                              batch size and keeps the fastest */
+                              batch size and keeps the fastest.  Variant 12 advances
+                              4 steps per launch (small grids, Cerjan boundary, forward
+                              modelling); elsewhere it runs as variant 0 */
     PAM_OPT_GRAPH = 3,     /* 1 (default): a whole run's step loop is captured once into a
                               hipGraph (per batch of shots, cached while the model, geometry,
                               options and output buffers stay the same) and replayed; 0: the
