@@ -86,26 +86,37 @@ def test_every_kernel_variant_matches_oracle(phys, abc):
 
 
 @pytest.mark.parametrize("phys,abc", CASES)
-def test_pipelined_tile_is_bit_identical(phys, abc):
+@pytest.mark.parametrize("shape", [(120, 330), (123, 363)])
+def test_pipelined_tile_is_bit_identical(phys, abc, shape):
     """The pipelined variants 7..11 (an interior tile taken for a group of 2, 4
     or 8 shots, each shot's U_k tile streamed into LDS while the one before is
     computed; 10 and 11 also compute the CPML psi in their edge tiles instead
-    of a pre-pass) compute every cell with the 16-row tile's arithmetic:
-    seismograms and final fields (psi included, through the fields it feeds)
-    are variant 0's bits, for shot counts that fill the groups and that do
-    not, with receivers inside interior tiles (two on one cell) and sources
-    there and in the absorbing layer."""
+    of a pre-pass, in three kinds of edge tile: meeting a z-strip only, an
+    x-strip only, or both) compute every cell with the 16-row tile's
+    arithmetic: seismograms and final fields (psi included, through the fields
+    it feeds) are variant 0's bits, for shot counts that fill the groups and
+    that do not, with receivers inside interior tiles (two on one cell) and
+    sources there and in the absorbing layer.  The second grid (147 x 387
+    padded cells) has a last tile row and column inside the 4-cell rim, and
+    receivers on the last rows and columns a step updates."""
     import torch
 
     from pseudoacousticmodeling_amd import _lib
 
     ns = 11
-    pb = make_problem(phys, abc, nz=120, nx=330, nshots=ns, seed=80 + 2 * phys + abc, nt=120)
+    nz, nx = shape
+    pb = make_problem(phys, abc, nz=nz, nx=nx, nshots=ns, seed=80 + 2 * phys + abc, nt=120)
     g = pb["g"]
     N = g.N
     rx = np.concatenate([np.arange(N, N + g.nx, 5), np.arange(N + 3, N + g.nx, 7), [N + 101, N + 101]])
     rz = np.concatenate([np.full(len(range(N, N + g.nx, 5)), N + 2), np.full(len(range(N + 3, N + g.nx, 7)), N + 40),
                          [N + 77, N + 77]])
+    if shape == (123, 363):
+        nzp, nxp = g.nz + 2 * N, g.nx + 2 * N
+        edge_rx = np.arange(4, nxp - 4, 9)
+        edge_rz = np.arange(4, nzp - 4, 11)
+        rx = np.concatenate([rx, edge_rx, np.full(len(edge_rz), nxp - 5), [nxp - 5, 5]])
+        rz = np.concatenate([rz, np.full(len(edge_rx), nzp - 5), edge_rz, [nzp - 5, nzp - 5]])
     pb["rx"], pb["rz"] = rx.astype(np.int32), rz.astype(np.int32)
     pb["sz"] = np.array([(N + 30 + 9 * s) if s % 4 else 6 for s in range(ns)], dtype=np.int32)
     pb["sx"] = np.array([N + 60 + (23 * s) % 200 for s in range(ns)], dtype=np.int32)
