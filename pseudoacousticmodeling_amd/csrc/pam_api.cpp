@@ -2366,13 +2366,14 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                                              : (storage == PAM_STORE_ALL_BF16 ? SV_BF16_INT : SV_F32_INT));
     const int elem = (half || sb_kind) ? 2 : 4;
     const bool multi = grad && (g_eps || g_del || g_th);
-    // float32 save-all or checkpoints with the pressure-form adjoint and no
-    // partials: accumulators and snapshots are sized as padded planes, and when
-    // the kernel variant's passes run the 16-row tile family they are laid out
-    // padded (rows 256-byte aligned, one float4 per thread; unaligned interior
-    // rows took twice as long to update); other variants index them as
-    // interior arrays at the same strides
-    const bool pad_ok = !linear_adj && !multi && (storage == PAM_STORE_ALL_F32 || ckpt);
+    // save-all (float32, float16 or bfloat16) or checkpoints with the
+    // pressure-form adjoint and no partials: accumulators and snapshots are
+    // sized as padded planes, and when the kernel variant's passes run the
+    // 16-row tile family they are laid out padded (rows 256-byte aligned, four
+    // cells per thread in one access; unaligned interior rows took twice as
+    // long to update); other variants index them as interior arrays at the
+    // same strides
+    const bool pad_ok = !linear_adj && !multi && (storage == PAM_STORE_ALL_F32 || half || ckpt);
     const long long acc_st = pad_ok ? ctx->plane : nint;  // floats between shots' accumulators
     const long long per_step = rbc ? 1 : sb ? sb_width(g.nz, g.nx) : (skind == SV_F32_PAD || pad_ok) ? ctx->plane : nint;  // per step
     // default interval 2 ceil(sqrt(nt)): every step is recomputed once whatever
@@ -2419,7 +2420,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         B = 2 * Bl;
     }
     const long long per_shot_bytes = per_step * nbuf * elem + 2 * acc_st * 4 +
-                                     (multi ? nmulti * nint * 4 : 0) + (half ? (long long)nt * 4 : 0) +
+                                     (multi ? nmulti * nint * 4 : 0) + (half ? (long long)nt * 4 * kSmaxSlots : 0) +
                                      (ckpt ? (long long)nseg * ckpt_fields(ctx) * ctx->plane * 4 : 0) +
                                      (sb ? (long long)nt * 4 + nint * 4 : 0) +
                                      (rv ? (long long)nslot * ckpt_fields(ctx) * ctx->plane * (rv_half ? 2 : 4) +
@@ -2444,7 +2445,8 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     if (r) return r;
     ctx->acc_img = ctx->acc_ilum + (long long)B * acc_st;
     if (half || sb) {
-        r = ensure_scratch(ctx, (void **)&ctx->smax, &ctx->smax_bytes, (long long)B * nt * 4);
+        // save-all: kSmaxSlots words per shot and step; boundary saving: one
+        r = ensure_scratch(ctx, (void **)&ctx->smax, &ctx->smax_bytes, (long long)B * nt * 4 * (half ? kSmaxSlots : 1));
         if (r) return r;
     }
     if (sb || rbc) {
@@ -2499,7 +2501,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     ctx->steps_run = 0;
     if ((half || sb || rv || rbc) && tile_shape(tile0).V > 0) tile0 = kDefaultTile;  // k_step has no fused max / D_t
     const bool accp = pad_ok && tile_pads_acc(tile0);
-    if (accp) skind = SV_F32_PAD;
+    if (accp) skind = !half ? SV_F32_PAD : (storage == PAM_STORE_ALL_F16 ? SV_F16_PAD : SV_BF16_PAD);
     // the accumulators of nb shots into the outputs (padded or interior layout)
     auto reduce_acc = [&](const float *parts, int nb, float *out, hipStream_t st) -> int {
         if (accp)
@@ -2561,7 +2563,7 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         if (multi) HCK(hipMemsetAsync(ctx->acc_multi, 0, (size_t)B * nmulti * nint * 4, ctx->st));
         KCK(launch_inject_point(ctx->fs.U0, ctx->plane, ctx->d_ssz, ctx->d_ssx, ctx->P, ctx->wav, 0, nb, ctx->st));
         if (half) {
-            HCK(hipMemsetAsync(ctx->smax, 0, (size_t)nb * nt * 4, ctx->st));
+            HCK(hipMemsetAsync(ctx->smax, 0, (size_t)nb * nt * 4 * kSmaxSlots, ctx->st));
             // scale of S_0 (imaging) or a bound of |D_0| (gradient): 3 max|wavelet| bounds
             // |S_1 - 2 S_0| under the CFL condition
             KCK(launch_smax0(ctx->smax, nt, ctx->d_ssz, ctx->d_ssx, g.n_abc, ctx->nzp, ctx->nxp, ctx->wav, nb,

@@ -44,10 +44,22 @@ constexpr int F_PASS_ILUM = F_FWD_MASK | F_ILUM;
 constexpr int F_PASS_SAVE = F_FWD_MASK | F_SAVE | F_D2T;
 constexpr int F_PASS_GRAD = F_CERJAN | F_RECINJ | F_GRADM | F_D2T;
 constexpr int F_PASS_IMAGE = F_CERJAN | F_RECINJ | F_IMAGE;
+// the forward pass of a save-all run: illumination, the snapshot (S_k, or D_k
+// for a gradient) and, for 16-bit snapshots, the max that scales the next one
+constexpr int F_PASS_SAVEALL = F_FWD_MASK | F_ILUM | F_SAVE | F_SMAX | F_D2T;
 constexpr int F_ALL = 0x7fffffff;
 
+// F_SMAX spreads the max of each (shot, step) over kSmaxSlots words: block b
+// updates word b % kSmaxSlots, so that a few dozen blocks rather than every
+// block of a shot contend for one atomic (same-address atomics serialize: about
+// 27 us per step for 1100 blocks a shot on a 451 x 1801 grid); the scale is the
+// max over the words
+constexpr int kSmaxSlots = 32;
+
 // snapshot storage kinds
-enum : int { SV_NONE = 0, SV_F32_INT = 1, SV_F32_PAD = 2, SV_F16_INT = 3, SV_BF16_INT = 4 };
+// (_INT: interior arrays [nz][nx]; _PAD: padded planes [nzp][P], rows 256-byte
+// aligned, four cells per thread in one access)
+enum : int { SV_NONE = 0, SV_F32_INT = 1, SV_F32_PAD = 2, SV_F16_INT = 3, SV_BF16_INT = 4, SV_F16_PAD = 5, SV_BF16_PAD = 6 };
 
 struct StepArgs {
     int nzp, nxp, P, N;   // padded grid, row pitch (floats), absorbing width
@@ -88,8 +100,8 @@ struct StepArgs {
     long long save_sstride;    // elements between shots
     long long save_kstride;    // elements between steps
     int save_k0;               // step held in save slot 0 (slot of step k is k - save_k0)
-    const float *save_scale;   // [nshots][nt] max |S_k| over the interior (compressed kinds)
-    unsigned int *smax;        // F_SMAX target, [nshots][nt] float bits (atomicMax)
+    const float *save_scale;   // [nshots][nt][kSmaxSlots] max |S_k| over the interior (compressed kinds)
+    unsigned int *smax;        // F_SMAX target, the same array as float bits (atomicMax)
     const float *save_prev, *save_next;  // (unused; kept for ABI stability)
     float *dsave;              // F_DSAVE target [nshots][nz][nx], dsave_sstride apart
     long long dsave_sstride;

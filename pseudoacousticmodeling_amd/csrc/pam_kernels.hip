@@ -408,13 +408,14 @@ __global__ void k_inject_point(float *U, long long sstride, const int *ssz, cons
     if (s >= nshots) return;
     U[(long long)s * sstride + (long long)ssz[s] * P + ssx[s]] += wav[k];
 }
-// max |S_0| over the interior: the field at step 0 is the injected source sample
+// max |S_0| over the interior: the field at step 0 is the injected source
+// sample (word 0 of the step's kSmaxSlots; the others are cleared)
 __global__ void k_smax0(unsigned int *smax, int nt, const int *ssz, const int *ssx, int N, int nzp, int nxp,
                         const float *wav, int nshots, float bound) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nshots) return;
     const bool in = ssz[s] >= N && ssz[s] < nzp - N && ssx[s] >= N && ssx[s] < nxp - N;
-    smax[(long long)s * nt] = __float_as_uint(bound > 0.0f ? bound : (in ? fabsf(wav[0]) : 0.0f));
+    smax[(long long)s * nt * kSmaxSlots] = __float_as_uint(bound > 0.0f ? bound : (in ? fabsf(wav[0]) : 0.0f));
 }
 // one thread per shot so duplicate receiver cells accumulate without races
 __global__ void k_inject_rec(float *U, long long sstride, int P, const int *rz, const int *rx, int nrec,

@@ -541,7 +541,7 @@ __device__ __forceinline__ void march_body(const StepArgs &a, const int b, const
         mc = mn;
         __syncthreads();
     }
-    if ((flags & F_SMAX) && m.k + 1 < a.nt) block_max_to(a.smax + (long long)s * a.nt + m.k + 1, tmax);
+    if ((flags & F_SMAX) && m.k + 1 < a.nt) block_max_to(a.smax + ((long long)s * a.nt + m.k + 1) * kSmaxSlots + blockIdx.x % kSmaxSlots, tmax);
 #undef SLOT
 #undef PSLOT
 }

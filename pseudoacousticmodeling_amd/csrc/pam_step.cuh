@@ -179,6 +179,17 @@ __device__ __forceinline__ float sv_load<__hip_bfloat16>(const void *p, long lon
     return __bfloat162float(reinterpret_cast<const __hip_bfloat16 *>(p)[i]) * inv_scale;
 }
 
+// the scale of the compressed snapshot of step k for shot s: the max over its
+// kSmaxSlots words (non-negative floats order like their bits; the loads are
+// wave-uniform)
+__device__ __forceinline__ float snap_scale(const StepArgs &a, int s, int k) {
+    const unsigned *w = reinterpret_cast<const unsigned *>(a.save_scale) + ((long long)s * a.nt + k) * kSmaxSlots;
+    unsigned m = 0u;
+#pragma unroll
+    for (int i = 0; i < kSmaxSlots; ++i) m = max(m, w[i]);
+    return __uint_as_float(m);
+}
+
 // snapshot value of cell (interior index ii, padded index pi) at step k for shot s.
 // Compressed kinds store S / max|S_k| (max over the interior, save_scale).
 __device__ __forceinline__ float snap_get(const StepArgs &a, int s, int k, long long ii, long long pi) {
@@ -189,10 +200,10 @@ __device__ __forceinline__ float snap_get(const StepArgs &a, int s, int k, long 
         case SV_F32_PAD:
             return sv_load<float>(a.save, base + pi, 1.0f);
         case SV_F16_INT:
-            return __half2float(reinterpret_cast<const __half *>(a.save)[base + ii]) * a.save_scale[(long long)s * a.nt + k];
+            return __half2float(reinterpret_cast<const __half *>(a.save)[base + ii]) * snap_scale(a, s, k);
         case SV_BF16_INT:
             return __bfloat162float(reinterpret_cast<const __hip_bfloat16 *>(a.save)[base + ii]) *
-                   a.save_scale[(long long)s * a.nt + k];
+                   snap_scale(a, s, k);
         default:
             return 0.0f;
     }
@@ -210,10 +221,10 @@ __device__ __forceinline__ float snap_get_at(const StepArgs &a, long long off, i
             return *at(reinterpret_cast<const float *>(a.save) + off, pi);
         case SV_F16_INT:
             return __half2float(*at(reinterpret_cast<const __half *>(a.save) + off, ii)) *
-                   a.save_scale[(long long)s * a.nt + k];
+                   snap_scale(a, s, k);
         case SV_BF16_INT:
             return __bfloat162float(*at(reinterpret_cast<const __hip_bfloat16 *>(a.save) + off, ii)) *
-                   a.save_scale[(long long)s * a.nt + k];
+                   snap_scale(a, s, k);
         default:
             return 0.0f;
     }
@@ -222,7 +233,7 @@ __device__ __forceinline__ void snap_put_at(const StepArgs &a, long long off, in
                                             float v) {
     float inv = 0.0f;
     if (a.save_kind == SV_F16_INT || a.save_kind == SV_BF16_INT) {
-        const float m = a.save_scale[(long long)s * a.nt + k];
+        const float m = snap_scale(a, s, k);
         inv = scale_inv(m);
     }
     switch (a.save_kind) {
@@ -246,7 +257,7 @@ __device__ __forceinline__ void snap_put(const StepArgs &a, int s, int k, long l
     const long long base = (long long)s * a.save_sstride + (long long)(k - a.save_k0) * a.save_kstride;
     float inv = 0.0f;
     if (a.save_kind == SV_F16_INT || a.save_kind == SV_BF16_INT) {
-        const float m = a.save_scale[(long long)s * a.nt + k];
+        const float m = snap_scale(a, s, k);
         inv = scale_inv(m);
     }
     switch (a.save_kind) {
@@ -264,6 +275,45 @@ __device__ __forceinline__ void snap_put(const StepArgs &a, int s, int k, long l
             break;
         default:
             break;
+    }
+}
+
+// Four consecutive cells (padded index pi, a multiple of 4) of a padded
+// snapshot in one access: float32 as a float4, 16-bit kinds as 8 bytes, scaled
+// by the step's max as the interior kinds are (the same arithmetic per cell).
+__device__ __forceinline__ void snap_put4_pad(const StepArgs &a, long long off, int s, int k, unsigned pi,
+                                              const float *v) {
+    if (a.save_kind == SV_F32_PAD) {
+        vst<4>(at(reinterpret_cast<float *>(a.save) + off, pi), v);
+        return;
+    }
+    const float inv = scale_inv(snap_scale(a, s, k));
+    unsigned short h[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (a.save_kind == SV_F16_PAD)
+            h[j] = __half_as_ushort(__float2half(v[j] * inv));
+        else
+            h[j] = __bfloat16_as_ushort(__float2bfloat16(v[j] * inv));
+    }
+    const uint2 w = make_uint2((unsigned)h[0] | ((unsigned)h[1] << 16), (unsigned)h[2] | ((unsigned)h[3] << 16));
+    *reinterpret_cast<uint2 *>(at(reinterpret_cast<unsigned short *>(a.save) + off, pi)) = w;
+}
+__device__ __forceinline__ void snap_get4_pad(const StepArgs &a, long long off, int s, int k, unsigned pi, float *v) {
+    if (a.save_kind == SV_F32_PAD) {
+        vld<4>(v, at(reinterpret_cast<const float *>(a.save) + off, pi));
+        return;
+    }
+    const uint2 w = *reinterpret_cast<const uint2 *>(at(reinterpret_cast<const unsigned short *>(a.save) + off, pi));
+    const float sc = snap_scale(a, s, k);
+    const unsigned short h[4] = {(unsigned short)(w.x & 0xffffu), (unsigned short)(w.x >> 16),
+                                 (unsigned short)(w.y & 0xffffu), (unsigned short)(w.y >> 16)};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (a.save_kind == SV_F16_PAD)
+            v[j] = __half2float(__ushort_as_half(h[j])) * sc;
+        else
+            v[j] = __bfloat162float(__ushort_as_bfloat16(h[j])) * sc;
     }
 }
 

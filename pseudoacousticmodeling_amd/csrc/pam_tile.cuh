@@ -183,7 +183,7 @@ __device__ __forceinline__ void tile_body_at(const StepArgs &a, const int x0, co
                     vld<4>(eacc[r], at(a.ilum + acc_s, ro));
                 } else {
                     vld<4>(eacc[r], at(a.image + acc_s, ro));
-                    if (kEarlyS) vld<4>(esv[kEarlyS ? r : 0], at(reinterpret_cast<const float *>(a.save) + snap_off(a, s, m.k), ro));
+                    if (kEarlyS) snap_get4_pad(a, snap_off(a, s, m.k), s, m.k, ro, esv[kEarlyS ? r : 0]);
                 }
             }
         }
@@ -389,16 +389,19 @@ __device__ __forceinline__ void tile_body_at(const StepArgs &a, const int x0, co
             }
         }
         if (a.acc_pad && (flags & F_D2T) && (flags & F_SAVE)) {
-            // padded float32 snapshot: D_k of the interior cells (0 elsewhere), one aligned float4
+            // padded snapshot: D_k of the interior cells (0 elsewhere), four cells in one
+            // access; with F_SMAX (16-bit kinds) also max |D_k| over them
             if (z >= N && z < nzp - N && col + 3 >= N && col < nxp - N) {
                 float d4[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     d4[j] = 0.0f;
-                    if (col + j >= N && col + j < nxp - N)
+                    if (col + j >= N && col + j < nxp - N) {
                         d4[j] = (flags & F_D2T_OP) ? opv[j] : out[j] - 2.0f * c0[j] + pv[r][j];
+                        if (flags & F_SMAX) tmax = fmaxf(tmax, fabsf(d4[j]));
+                    }
                 }
-                vst<4>(at(reinterpret_cast<float *>(a.save) + snap_off(a, s, m.k), ro), d4);
+                snap_put4_pad(a, snap_off(a, s, m.k), s, m.k, ro, d4);
             }
         } else if ((flags & F_SMAX) || ((flags & F_D2T) && (flags & F_SAVE)) || (flags & F_DSAVE)) {
             const bool zint = z >= N && z < nzp - N;
@@ -432,12 +435,12 @@ __device__ __forceinline__ void tile_body_at(const StepArgs &a, const int x0, co
 #pragma unroll
             for (int j = 0; j < 4; ++j) cin[j] = zint && col + j >= N && col + j < nxp - N;
             const long long acc_s = (long long)s * a.acc_sstride;
-            float *const sv = reinterpret_cast<float *>(a.save) + snap_off(a, s, m.k);
+            const long long sk = snap_off(a, s, m.k);
             if ((flags & F_SAVE) && !(flags & F_D2T)) {
                 float S4[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) S4[j] = cin[j] ? c0[j] : 0.0f;
-                if (zint) vst<4>(at(sv, ro), S4);
+                if (zint) snap_put4_pad(a, sk, s, m.k, ro, S4);
             }
             if (cin[0] || cin[1] || cin[2] || cin[3]) {
                 if (flags & F_ILUM) {
@@ -462,7 +465,7 @@ __device__ __forceinline__ void tile_body_at(const StepArgs &a, const int x0, co
                             v[j] = eacc[kEarly ? r : 0][j];
                         }
                     } else {
-                        vld<4>(S4, at(sv, ro));
+                        snap_get4_pad(a, sk, s, m.k, ro, S4);
                         vld<4>(v, at(a.image + acc_s, ro));
                     }
 #pragma unroll
@@ -520,7 +523,7 @@ __device__ __forceinline__ void tile_body_at(const StepArgs &a, const int x0, co
             }
         }
     }
-    if ((flags & F_SMAX) && m.k + 1 < a.nt) block_max_to(a.smax + (long long)s * a.nt + m.k + 1, tmax);
+    if ((flags & F_SMAX) && m.k + 1 < a.nt) block_max_to(a.smax + ((long long)s * a.nt + m.k + 1) * kSmaxSlots + blockIdx.x % kSmaxSlots, tmax);
 }
 
 // the forward, illumination and recompute instances of the 16-row tile fit 5
@@ -1259,6 +1262,10 @@ static int launch_tile_t(const StepArgs &a0, hipStream_t st, const hipEvent_t *e
         }
         if (within(F_PASS_IMAGE)) {
             launch_timed(k_tile<PHYS, CPML, TZ, WPE, F_PASS_IMAGE>, grid, st, ev, a);
+            return (int)hipGetLastError();
+        }
+        if (within(F_PASS_SAVEALL)) {
+            launch_timed(k_tile<PHYS, CPML, TZ, WPE, F_PASS_SAVEALL>, grid, st, ev, a);
             return (int)hipGetLastError();
         }
     }
