@@ -244,8 +244,7 @@ def group_gradient_leg(shots_per_model=8):
     different sizes, ``shots_per_model`` shots each, nt = 3001 + itlag,
     residuals = gathers of the model scaled by 1.02 minus the model's own,
     float16-scaled snapshots of every step, float64 gradient norms, one
-    pam_group_gradient call (the members in turn, each batched and on two
-    lanes)."""
+    pam_group_gradient call (the members in turn, each batched)."""
     import numpy as np
     import torch
 
