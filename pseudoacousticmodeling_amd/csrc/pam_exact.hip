@@ -164,15 +164,20 @@ __global__ __launch_bounds__(256) void k_exact_lambda(const ExactArgs a) {
     const float nz_k = szs ? q + a.caz[c] * nzi[c] : 0.0f;
     float sx = C0 * (q + a.cbx[c] * nx_k), sz = C0 * (q + a.cbz[c] * nz_k);
     float tx = 0.0f, tz = 0.0f;
+    // mu vanishes off the strips (cleared, and written on strip cells only), so
+    // the D^T (b mu) terms of a cell more than 4 cells from every x (z) strip
+    // are zero and are not read
+    const bool nearx = ix < a.N + 4 || ix >= a.nxp - a.N - 4;
+    const bool nearz = iz < a.N + 4 || iz >= a.nzp - a.N - 4;
 #pragma unroll
     for (int m = 1; m <= 4; ++m) {
         sx += kc(m) * (u_of(a, X, nxi, a.cax, a.cbx, iz, ix + m, c + m, xstrip(a, iz, ix + m)) +
                        u_of(a, X, nxi, a.cax, a.cbx, iz, ix - m, c - m, xstrip(a, iz, ix - m)));
         sz += kc(m) * (u_of(a, X, nzi, a.caz, a.cbz, iz + m, ix, c + m * P, zstrip(a, iz + m, ix)) +
                        u_of(a, X, nzi, a.caz, a.cbz, iz - m, ix, c - m * P, zstrip(a, iz - m, ix)));
-        // D^T (b mu) = -D (b mu); mu vanishes off the strips
-        tx += ka(m) * (a.cbx[c - m] * mux[c - m] - a.cbx[c + m] * mux[c + m]);
-        tz += ka(m) * (a.cbz[c - m * P] * muz[c - m * P] - a.cbz[c + m * P] * muz[c + m * P]);
+        // D^T (b mu) = -D (b mu)
+        if (nearx) tx += ka(m) * (a.cbx[c - m] * mux[c - m] - a.cbx[c + m] * mux[c + m]);
+        if (nearz) tz += ka(m) * (a.cbz[c - m * P] * muz[c - m * P] - a.cbz[c + m * P] * muz[c + m * P]);
     }
     Y[c] = sx * a.idx2 + sz * a.idz2 + 2.0f * phi - Y[c] + tx * a.idx + tz * a.idz;
     a.nux_out[so + c] = nx_k;
