@@ -27,6 +27,9 @@ Prints one JSON line:
              torch.autograd.Function, 50 float16 revolve slots
   linear_adjoint RTM and FWI gradient with the linearised VTI / TTI adjoint
              (Cerjan boundary), 16 shots, 398 steps
+  tti_rtm    RTM of the TTI + CPML workload's 32 shots with boundary saving
+             (float32 and scaled bfloat16 rims) and with uniform checkpoints,
+             illumination-normalised, on a reduced step count
   other_configs acoustic + Cerjan (1 shot, 301 x 501 padded), TTI + CPML (32
              shots, 501 x 2101) and acoustic + CPML (32 shots, 401 x 1301),
              the last two on a reduced step count
@@ -113,6 +116,43 @@ def configs_leg(nt_cap):
         eng.close()
         del seis
         torch.cuda.empty_cache()
+    return out
+
+
+def tti_rtm_leg(nt_data=500, nshots=32):
+    """The TTI workload as its configuration states it: RTM of its 32 shots
+    (TTI + CPML, 501 x 2101 padded) with the source field rebuilt backwards
+    from the rims saved by the forward pass, in float32 and in bfloat16
+    scaled per step (boundary saving), and with uniform checkpoints for
+    comparison; zero-lag image normalised by the source illumination; the
+    first `nt_data` recorded samples (the passes' per-step cost does not
+    depend on the step count); the data are the model's own gathers.
+    Seconds per shot of the whole RTM (forward pass, rebuild or recompute,
+    adjoint pass and imaging)."""
+    import torch
+
+    from pseudoacousticmodeling_amd import _lib, workloads
+
+    w = workloads.tti_cpml_32(nshots=nshots, nt_data=nt_data)
+    S = len(w["sx"])
+    eng = make_engine(w, S)
+    data = eng.forward(0, S)
+    out = {"unit": "s/shot", "shots": S, "nt": w["nt"], "nt_full": workloads.tti_cpml_32(nshots=1)["nt"],
+           "padded": [w["nz"] + 2 * w["N"], w["nx"] + 2 * w["N"]], "normalised": True}
+    for name, storage in (("boundary_f32_rims", _lib.PAM_STORE_BOUNDARY_F32),
+                          ("boundary_bf16_rims", _lib.PAM_STORE_BOUNDARY_BF16),
+                          ("uniform_checkpoints", _lib.PAM_STORE_CHECKPOINT)):
+        eng.set_option(_lib.PAM_OPT_STORAGE, storage)
+        eng.rtm(data)  # allocates, tunes
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        img, _ = eng.rtm(data)
+        torch.cuda.synchronize()
+        out[name] = (time.perf_counter() - t0) / S
+        out[name + "_finite"] = bool(torch.isfinite(img).all().item())
+    eng.close()
+    del data
+    torch.cuda.empty_cache()
     return out
 
 
@@ -372,6 +412,7 @@ def main(argv=None):
     ap.add_argument("--no-autograd", action="store_true", help="skip the autograd FWI gradient leg")
     ap.add_argument("--no-linear", action="store_true", help="skip the linearised VTI / TTI adjoint leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the other single-grid workloads")
+    ap.add_argument("--no-tti-rtm", action="store_true", help="skip the TTI RTM (boundary saving) leg")
     ap.add_argument("--configs-steps", type=int, default=500,
                     help="recorded samples of the reduced TTI + CPML and acoustic + CPML runs")
     ap.add_argument("--no-graph", action="store_true",
@@ -543,6 +584,13 @@ def main(argv=None):
             print(f"bench: linearised adjoint {k}: RTM {v['rtm'] * 1e3:.2f} ms/shot, FWI gradient "
                   f"{v['gradient'] * 1e3:.2f} ms/shot ({v['shots']} shots, nt={v['nt']})", file=sys.stderr, flush=True)
 
+    tti_rtm = None
+    if not args.no_tti_rtm:
+        tti_rtm = tti_rtm_leg()
+        print(f"bench: TTI RTM (32 shots, nt={tti_rtm['nt']}): float32 rims {tti_rtm['boundary_f32_rims'] * 1e3:.1f}, "
+              f"bfloat16 rims {tti_rtm['boundary_bf16_rims'] * 1e3:.1f}, uniform checkpoints "
+              f"{tti_rtm['uniform_checkpoints'] * 1e3:.1f} ms/shot", file=sys.stderr, flush=True)
+
     others = None
     if not args.no_configs:
         others = configs_leg(args.configs_steps)
@@ -597,6 +645,7 @@ def main(argv=None):
         "autograd_gradient": autograd,
         "linear_adjoint": linear,
         "uneven_group_gradient": group_grad,
+        "tti_rtm": tti_rtm,
         "other_configs": others,
         "cpu_baseline": cpu,
         "finite": finite,
