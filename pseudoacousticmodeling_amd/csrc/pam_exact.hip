@@ -202,6 +202,112 @@ __global__ __launch_bounds__(256) void k_exact_lambda(const ExactArgs a) {
     a.gV[so + c] += phi * Q;
 }
 
+// The CPML case of k_exact_lambda on tiles of 64 x 16 cells: the values a
+// cell takes from its neighbours -- u_x = V X (+ b (V X + a nu_x) on x-strip
+// cells), u_z alike, the saved field C, and the b mu products near the strips --
+// are staged in LDS once per block over the tile and a 4-cell halo, instead of
+// being read through the caches by every neighbour; each value, and each
+// cell's result, is formed by the same expressions as in k_exact_lambda.
+// Thread t updates column t mod 64 of rows 4 (t / 64) .. 4 (t / 64) + 3.
+constexpr int kLamTZ = 16;
+__device__ __forceinline__ float d2l(const float *f, int l, int st) {
+    return C0 * f[l] + C1 * (f[l + st] + f[l - st]) + C2 * (f[l + 2 * st] + f[l - 2 * st]) +
+           C3 * (f[l + 3 * st] + f[l - 3 * st]) + C4 * (f[l + 4 * st] + f[l - 4 * st]);
+}
+__device__ __forceinline__ float d1l(const float *f, int l, int st) {
+    return A1 * (f[l + st] - f[l - st]) + A2 * (f[l + 2 * st] - f[l - 2 * st]) +
+           A3 * (f[l + 3 * st] - f[l - 3 * st]) + A4 * (f[l + 4 * st] - f[l - 4 * st]);
+}
+__global__ __launch_bounds__(256) void k_exact_lambda_cpml(const ExactArgs a) {
+    constexpr int W = 64 + 8, H = kLamTZ + 8, NS = W * H;
+    __shared__ float sux[NS], suz[NS], sC[NS], sbx[NS], sbz[NS];
+    const int x0 = blockIdx.x * 64, z0 = blockIdx.y * kLamTZ, s = blockIdx.z, tid = threadIdx.x;
+    const long long P = a.P;
+    const long long so = (long long)s * a.sstride, fo = (long long)s * a.fstride;
+    const float *X = a.X + so;
+    float *Y = a.Y + so;
+    const float *C = a.C + fo;
+    const float *nxi = a.nux_in + so, *nzi = a.nuz_in + so;
+    const float *mux = a.mux + so, *muz = a.muz + so;
+    // the tile has cells within 4 of an x (z) strip: the b mu terms (mu is zero
+    // off the strips, so other cells' terms are zero and not read)
+    const bool tnx = x0 < a.N + 4 || x0 + 63 >= a.nxp - a.N - 4;
+    const bool tnz = z0 < a.N + 4 || z0 + kLamTZ - 1 >= a.nzp - a.N - 4;
+    for (int i = tid; i < NS; i += 256) {
+        const int r = i / W, cc = i - r * W;
+        const int iz = z0 - 4 + r, ix = x0 - 4 + cc;
+        float ux = 0.0f, uz = 0.0f, cv = 0.0f, bx = 0.0f, bz = 0.0f;
+        if (iz >= 0 && iz < a.nzp && ix >= 0 && ix < a.nxp) {
+            const long long c = (long long)iz * P + ix;
+            ux = u_of(a, X, nxi, a.cax, a.cbx, iz, ix, c, xstrip(a, iz, ix));
+            uz = u_of(a, X, nzi, a.caz, a.cbz, iz, ix, c, zstrip(a, iz, ix));
+            cv = C[c];
+            if (tnx) bx = a.cbx[c] * mux[c];
+            if (tnz) bz = a.cbz[c] * muz[c];
+        }
+        sux[i] = ux;
+        suz[i] = uz;
+        sC[i] = cv;
+        sbx[i] = bx;
+        sbz[i] = bz;
+    }
+    __syncthreads();
+    const int lx = tid & 63, ix = x0 + lx;
+    if (ix >= a.nxp) return;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+        const int lz = (tid >> 6) * 4 + rr, iz = z0 + lz;
+        if (iz >= a.nzp) break;
+        const long long c = (long long)iz * P + ix;
+        if (!live_at(a, iz, ix)) {
+            Y[c] = 0.0f;
+            a.nux_out[so + c] = 0.0f;
+            a.nuz_out[so + c] = 0.0f;
+            continue;
+        }
+        const int l = (lz + 4) * W + lx + 4;
+        const float qxx = d2l(sC, l, 1) * a.idx2, qzz = d2l(sC, l, W) * a.idz2;
+        const bool sxs = xstrip(a, iz, ix), szs = zstrip(a, iz, ix);
+        const float phi = X[c];
+        const float q = a.v2[c] * phi;
+        const float nx_k = sxs ? q + a.cax[c] * nxi[c] : 0.0f;
+        const float nz_k = szs ? q + a.caz[c] * nzi[c] : 0.0f;
+        float sx = C0 * (q + a.cbx[c] * nx_k), sz = C0 * (q + a.cbz[c] * nz_k);
+        float tx = 0.0f, tz = 0.0f;
+        const bool nearx = ix < a.N + 4 || ix >= a.nxp - a.N - 4;
+        const bool nearz = iz < a.N + 4 || iz >= a.nzp - a.N - 4;
+#pragma unroll
+        for (int m = 1; m <= 4; ++m) {
+            sx += kc(m) * (sux[l + m] + sux[l - m]);
+            sz += kc(m) * (suz[l + m * W] + suz[l - m * W]);
+            // D^T (b mu) = -D (b mu)
+            if (nearx) tx += ka(m) * (sbx[l - m] - sbx[l + m]);
+            if (nearz) tz += ka(m) * (sbz[l - m * W] - sbz[l + m * W]);
+        }
+        Y[c] = sx * a.idx2 + sz * a.idz2 + 2.0f * phi - Y[c] + tx * a.idx + tz * a.idz;
+        a.nux_out[so + c] = nx_k;
+        a.nuz_out[so + c] = nz_k;
+        float Q = qxx + qzz;
+        if (sxs) {
+            const float *px = a.psx + fo;
+            const float dpx = d1s(px, c, 1) * a.idx;
+            Q += dpx + a.zex[fo + c];
+            const float pm = a.psxm ? a.psxm[fo + c] : 0.0f, zm = a.zexm ? a.zexm[fo + c] : 0.0f;
+            a.gax[so + c] += mux[c] * pm + nx_k * zm;
+            a.gbx[so + c] += mux[c] * (d1l(sC, l, 1) * a.idx) + nx_k * (qxx + dpx);
+        }
+        if (szs) {
+            const float *pz = a.psz + fo;
+            const float dpz = d1s(pz, c, P) * a.idz;
+            Q += dpz + a.zez[fo + c];
+            const float pm = a.pszm ? a.pszm[fo + c] : 0.0f, zm = a.zezm ? a.zezm[fo + c] : 0.0f;
+            a.gaz[so + c] += muz[c] * pm + nz_k * zm;
+            a.gbz[so + c] += muz[c] * (d1l(sC, l, W) * a.idz) + nz_k * (qzz + dpz);
+        }
+        a.gV[so + c] += phi * Q;
+    }
+}
+
 // one thread per (shot, receiver); duplicate cells accumulate through atomics
 __global__ void k_exact_rec(float *Y, long long sstride, int P, const int *rz, const int *rx, int nrec,
                             const float *data, long long dstride, int row, const float *cerx, const float *cerz,
@@ -287,7 +393,11 @@ int launch_exact_mu(const ExactArgs &a, hipStream_t st) {
     return (int)hipGetLastError();
 }
 int launch_exact_lambda(const ExactArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL(k_exact_lambda, dim3((a.nxp + 63) / 64, (a.nzp + 3) / 4, a.nshots), dim3(256), 0, st, a);
+    if (a.cpml)
+        hipLaunchKernelGGL(k_exact_lambda_cpml, dim3((a.nxp + 63) / 64, (a.nzp + kLamTZ - 1) / kLamTZ, a.nshots),
+                           dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_exact_lambda, dim3((a.nxp + 63) / 64, (a.nzp + 3) / 4, a.nshots), dim3(256), 0, st, a);
     return (int)hipGetLastError();
 }
 int launch_exact_rec(float *Y, long long sstride, int P, const int *rz, const int *rx, int nrec, const float *data,
