@@ -345,12 +345,17 @@ int pam_group_get_tile(const pam_group *grp);
  * 0 .. nshots[i]-1 and its residual[i] (device [nshots[i]][nt_data][nrec]),
  * into g_m[i] (device [nz][nx]), each with its own storage, batch and lane
  * options (float16-scaled snapshots: PAM_STORE_ALL_F16, or
- * PAM_STORE_REVOLVE_F16 slots).  The members run in turn and each one's run
- * scratch (snapshots, checkpoints, slots) is released before the next one
- * starts, so the group needs the largest member's scratch, not the sum.
- * g_norm2 (host [n], or NULL): sum of g_m[i]^2 over the interior in float64,
- * summed in a fixed order (the same inputs give the same bits); loop_ms (or
- * NULL): the members' step-loop GPU time.  Errors report through member 0.
+ * PAM_STORE_REVOLVE_F16 slots).  Two host workers take the members (the
+ * largest first), each on a stream of the group's own, so that two members
+ * share the device; a worker passes its run scratch (snapshots, checkpoints,
+ * slots) from one member to the next and all of it is released at the end, so
+ * the group needs about two members' scratch at a time, not the sum.  Each
+ * gradient is the one the member computes alone.  The workers start after the
+ * work queued on the members' streams, and member 0's stream is ordered after
+ * them.  g_norm2 (host [n], or NULL): sum of g_m[i]^2 over the interior in
+ * float64, summed in a fixed order (the same inputs give the same bits);
+ * loop_ms (or NULL): the wall time of the members' runs.  Errors report
+ * through member 0.
  * Extends the reference's per-model FWI (src/Inversion.py:55-86, one model
  * per run) to a list of models of different sizes. */
 int pam_group_gradient(pam_group *grp, const int *nshots, const float *const *residual, float *const *g_m,

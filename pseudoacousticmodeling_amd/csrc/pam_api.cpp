@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -12,6 +14,7 @@
 #include <functional>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pam.h"
@@ -1631,6 +1634,10 @@ struct pam_group {
     // live and die with the group (a new group over the same members must not
     // replay a graph that reads a destroyed group's tables)
     std::map<std::string, hipGraphExec_t> graphs;
+    // the two streams the gradient workers run their members on (default
+    // priority, created on first use) and the fork / join event
+    hipStream_t wst[2] = {nullptr, nullptr};
+    hipEvent_t wev = nullptr;
 };
 
 extern "C" int pam_group_create(pam_ctx *const *ctxs, int n, pam_group **out) {
@@ -1669,6 +1676,13 @@ extern "C" int pam_group_destroy(pam_group *g) {
     }
     for (auto &kv : g->graphs) (void)hipGraphExecDestroy(kv.second);
     g->graphs.clear();
+    for (hipStream_t &w : g->wst)
+        if (w) {
+            (void)hipStreamSynchronize(w);
+            (void)hipStreamDestroy(w);
+            w = nullptr;
+        }
+    if (g->wev) (void)hipEventDestroy(g->wev);
     dfree(g->d_list);
     dfree(g->d_boff);
     dfree(g->d_pboff);
@@ -1879,34 +1893,65 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
 // accumulators, staging) released once the context's streams are idle; the
 // step-loop graphs that bake their addresses in go with them.  The next run
 // allocates again.
+struct ScratchBuf {
+    void **p;
+    long long *bytes;
+};
+constexpr int kScratchBufs = 13;
+static void run_scratch_bufs(pam_ctx *ctx, ScratchBuf *b) {
+    const ScratchBuf l[kScratchBufs] = {
+        {(void **)&ctx->save, &ctx->save_bytes},         {(void **)&ctx->ckpt, &ctx->ckpt_bytes},
+        {(void **)&ctx->rvbits, &ctx->rvbits_bytes},     {(void **)&ctx->smax, &ctx->smax_bytes},
+        {(void **)&ctx->tmp, &ctx->tmp_bytes},           {(void **)&ctx->acc_ilum, &ctx->acc_bytes},
+        {(void **)&ctx->sbd, &ctx->sbd_bytes},           {(void **)&ctx->acc_multi, &ctx->acc_multi_bytes},
+        {(void **)&ctx->ilum_tmp, &ctx->ilum_tmp_bytes}, {(void **)&ctx->stage, &ctx->stage_bytes},
+        {(void **)&ctx->ex_save, &ctx->ex_save_bytes},   {(void **)&ctx->ex_work, &ctx->ex_work_bytes},
+        {(void **)&ctx->ex_gacc, &ctx->ex_gacc_bytes}};
+    for (int i = 0; i < kScratchBufs; ++i) b[i] = l[i];
+}
+static void drop_graphs(pam_ctx *ctx) {
+    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
+}
 static void release_run_scratch(pam_ctx *ctx) {
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
     for (hipStream_t l : ctx->lane_st)
         if (l) (void)hipStreamSynchronize(l);
-    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
-    ctx->graphs.clear();
-    struct Buf {
-        void **p;
-        long long *bytes;
-    } bufs[] = {{(void **)&ctx->save, &ctx->save_bytes},       {(void **)&ctx->ckpt, &ctx->ckpt_bytes},
-                {(void **)&ctx->rvbits, &ctx->rvbits_bytes},   {(void **)&ctx->smax, &ctx->smax_bytes},
-                {(void **)&ctx->tmp, &ctx->tmp_bytes},         {(void **)&ctx->acc_ilum, &ctx->acc_bytes},
-                {(void **)&ctx->sbd, &ctx->sbd_bytes},         {(void **)&ctx->acc_multi, &ctx->acc_multi_bytes},
-                {(void **)&ctx->ilum_tmp, &ctx->ilum_tmp_bytes}, {(void **)&ctx->stage, &ctx->stage_bytes},
-                {(void **)&ctx->ex_save, &ctx->ex_save_bytes}, {(void **)&ctx->ex_work, &ctx->ex_work_bytes},
-                {(void **)&ctx->ex_gacc, &ctx->ex_gacc_bytes}};
-    for (Buf &b : bufs) {
+    drop_graphs(ctx);
+    ScratchBuf bufs[kScratchBufs];
+    run_scratch_bufs(ctx, bufs);
+    for (ScratchBuf &b : bufs) {
         dfree(*b.p);
         *b.p = nullptr;
         *b.bytes = 0;
     }
 }
+// The run scratch of `from` (idle) and of `to` trade places, so that `to`
+// starts with the buffers `from` sized, without a free (hipFree synchronizes
+// the whole device); the graphs of both, which may read either's buffers, are
+// dropped.
+static void swap_run_scratch(pam_ctx *from, pam_ctx *to) {
+    drop_graphs(from);
+    drop_graphs(to);
+    ScratchBuf a[kScratchBufs], b[kScratchBufs];
+    run_scratch_bufs(from, a);
+    run_scratch_bufs(to, b);
+    for (int i = 0; i < kScratchBufs; ++i) {
+        std::swap(*a[i].p, *b[i].p);
+        std::swap(*a[i].bytes, *b[i].bytes);
+    }
+}
 
 // FWI gradients of every member (pam_gradient's g_m, with each member's own
-// storage, batch and lane options), the members in turn, each member's
-// run-scratch released before the next one starts (so the group needs the
-// largest member's scratch, not the sum); g_norm2[i] = sum of g_m[i]^2 over
-// the interior in float64 (two passes in a fixed order).
+// storage, batch and lane options).  Two host workers take the members, the
+// largest first, each on a stream of its own, so that two members' passes
+// share the device (a small member alone leaves most of it idle); a worker
+// hands its run scratch from one member to the next rather than freeing it
+// (a free would synchronize the device and so the other worker), and every
+// member's scratch is released at the end: the group needs about two members'
+// scratch at a time, not the sum.  Each member's gradient is computed as when
+// it runs alone.  g_norm2[i] = sum of g_m[i]^2 over the interior in float64
+// (two passes in a fixed order).
 extern "C" int pam_group_gradient(pam_group *g, const int *nshots, const float *const *residual, float *const *g_m,
                                   double *g_norm2, double *loop_ms) {
     if (!g || !nshots || !residual || !g_m) return PAM_E_ARG;
@@ -1921,19 +1966,71 @@ extern "C" int pam_group_gradient(pam_group *g, const int *nshots, const float *
             return fail(c0, PAM_E_ARG, "group member %d: nshots %d outside [1, shots]", i, nshots[i]);
         if (!residual[i] || !g_m[i]) return fail(c0, PAM_E_ARG, "group member %d: residual or g_m is NULL", i);
     }
-    double ms = 0.0;
+    int dev = 0;
+    HCK(hipGetDevice(&dev));
+    const int nw = n > 1 ? 2 : 1;
+    for (int w = 0; w < nw; ++w)
+        if (!g->wst[w]) HCK(hipStreamCreate(&g->wst[w]));
+    if (!g->wev) HCK(hipEventCreateWithFlags(&g->wev, hipEventDisableTiming));
+    // the workers' streams start after the work already queued on every member's stream
     for (int i = 0; i < n; ++i) {
-        pam_ctx *c = g->ctx[i];
-        int r = adjoint_run(c, 0, nshots[i], residual[i], true, g_m[i], nullptr, nullptr, nullptr, nullptr);
-        if (r) return fail(c0, r, "group member %d: %s", i, c->err.c_str());
-        ms += c->loop_ms;
-        if (g_norm2) {
-            KCK(launch_sumsq(g_m[i], (long long)c->g.nz * c->g.nx, c->d_acc, c->d_acc + kMisfitParts, c->st));
-            HCK(hipMemcpyAsync(&g_norm2[i], c->d_acc + kMisfitParts, sizeof(double), hipMemcpyDeviceToHost, c->st));
-        }
-        release_run_scratch(c);  // synchronizes c's streams
+        HCK(hipEventRecord(g->wev, g->ctx[i]->st));
+        for (int w = 0; w < nw; ++w) HCK(hipStreamWaitEvent(g->wst[w], g->wev, 0));
     }
-    if (loop_ms) *loop_ms = ms;
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return (long long)g->ctx[x]->plane * nshots[x] > (long long)g->ctx[y]->plane * nshots[y];
+    });
+    std::atomic<int> next(0);
+    std::atomic<bool> failed(false);
+    std::vector<int> rc(n, PAM_OK);
+    auto worker = [&](int w) {
+        (void)hipSetDevice(dev);
+        pam_ctx *held = nullptr;  // the member whose scratch this worker carries
+        for (;;) {
+            const int k = next.fetch_add(1);
+            if (k >= n || failed.load()) break;
+            const int i = order[k];
+            pam_ctx *c = g->ctx[i];
+            if (held) swap_run_scratch(held, c);
+            hipStream_t own = c->st;
+            c->st = g->wst[w];
+            int r = adjoint_run(c, 0, nshots[i], residual[i], true, g_m[i], nullptr, nullptr, nullptr, nullptr);
+            if (!r && g_norm2) {
+                if (launch_sumsq(g_m[i], (long long)c->g.nz * c->g.nx, c->d_acc, c->d_acc + kMisfitParts, c->st))
+                    r = fail(c, PAM_E_HIP, "gradient norm: %s", hipGetErrorString(hipGetLastError()));
+                if (!r && hipMemcpyAsync(&g_norm2[i], c->d_acc + kMisfitParts, sizeof(double), hipMemcpyDeviceToHost,
+                                         c->st) != hipSuccess)
+                    r = fail(c, PAM_E_HIP, "gradient norm copy");
+            }
+            if (hipStreamSynchronize(c->st) != hipSuccess && !r) r = fail(c, PAM_E_HIP, "member stream");
+            c->st = own;
+            rc[i] = r;
+            held = c;
+            if (r) {
+                failed.store(true);
+                break;
+            }
+        }
+    };
+    const auto t0 = std::chrono::steady_clock::now();
+    if (nw > 1) {
+        std::thread other(worker, 1);
+        worker(0);
+        other.join();
+    } else {
+        worker(0);
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    for (int w = 0; w < nw; ++w) {
+        HCK(hipEventRecord(g->wev, g->wst[w]));
+        HCK(hipStreamWaitEvent(c0->st, g->wev, 0));
+    }
+    for (int i = 0; i < n; ++i) release_run_scratch(g->ctx[i]);
+    for (int i = 0; i < n; ++i)
+        if (rc[i]) return fail(c0, rc[i], "group member %d: %s", i, g->ctx[i]->err.c_str());
+    if (loop_ms) *loop_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     return PAM_OK;
 }
 
