@@ -1942,6 +1942,16 @@ static void swap_run_scratch(pam_ctx *from, pam_ctx *to) {
     }
 }
 
+// the regions of a whole state's fields (the two wavefields; psi_x, psi_z,
+// zeta_x, zeta_z on their strips), in the order of the revolve slots
+static void state_regions(const pam_ctx *ctx, FieldRegion *reg) {
+    reg[0] = reg[1] = region_all(ctx->nzp, ctx->P);
+    if (ctx->abc == PAM_CPML) {
+        reg[2] = reg[4] = region_xstrips(ctx->nzp, ctx->nxp, ctx->P, ctx->g.n_abc);
+        reg[3] = reg[5] = region_zstrips(ctx->nzp, ctx->P, ctx->g.n_abc);
+    }
+}
+
 // FWI gradients of every member (pam_gradient's g_m, with each member's own
 // storage, batch and lane options).  Two host workers take the members, the
 // largest first, each on a stream of its own, so that two members' passes
@@ -2931,9 +2941,11 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                 float *pf[6];
                 fields(pf);
                 if (rv_half) {
+                    FieldRegion reg[6];
+                    state_regions(ctx, reg);
                     KCK(launch_pack_half(pf, nf, ctx->plane, ctx->plane, nb,
                                          (char *)ctx->ckpt + (long long)sl * nf * fl * es, fl, ctx->plane,
-                                         ctx->rvbits + (long long)sl * nf * B, B, ctx->st));
+                                         ctx->rvbits + (long long)sl * nf * B, B, ctx->st, reg));
                     return PAM_OK;
                 }
                 for (int i = 0; i < nf; ++i) {
@@ -2951,9 +2963,11 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
                 float *pf[6];
                 fields(pf);
                 if (rv_half) {
+                    FieldRegion reg[6];
+                    state_regions(ctx, reg);
                     KCK(launch_unpack_half((const char *)ctx->ckpt + (long long)sl * nf * fl * es, fl, ctx->plane,
                                            ctx->plane, nb, nf, ctx->rvbits + (long long)sl * nf * B, B, pf, ctx->plane,
-                                           ctx->st));
+                                           ctx->st, reg));
                 } else {
                     for (int i = 0; i < nf; ++i) {
                         const char *src = (const char *)ctx->ckpt + ((long long)sl * nf + i) * fl * es;
@@ -3210,8 +3224,10 @@ static int exact_revolve(pam_ctx *ctx, StepArgs &a, int tile, int nb, int nslot,
         float *pf[6];
         fields(pf);
         if (half) {
+            FieldRegion reg[6];
+            state_regions(ctx, reg);
             KCK(launch_pack_half(pf, nf, ctx->plane, ctx->plane, nb, (char *)ctx->ckpt + (long long)sl * nf * fl * es,
-                                 fl, ctx->plane, ctx->rvbits + (long long)sl * nf * nb, nb, ctx->st));
+                                 fl, ctx->plane, ctx->rvbits + (long long)sl * nf * nb, nb, ctx->st, reg));
             return PAM_OK;
         }
         for (int i = 0; i < nf; ++i) {
@@ -3229,8 +3245,10 @@ static int exact_revolve(pam_ctx *ctx, StepArgs &a, int tile, int nb, int nslot,
         float *pf[6];
         fields(pf);
         if (half) {
+            FieldRegion reg[6];
+            state_regions(ctx, reg);
             KCK(launch_unpack_half((const char *)ctx->ckpt + (long long)sl * nf * fl * es, fl, ctx->plane, ctx->plane, nb,
-                                   nf, ctx->rvbits + (long long)sl * nf * nb, nb, pf, ctx->plane, ctx->st));
+                                   nf, ctx->rvbits + (long long)sl * nf * nb, nb, pf, ctx->plane, ctx->st, reg));
         } else {
             for (int i = 0; i < nf; ++i) {
                 const char *src = (const char *)ctx->ckpt + ((long long)sl * nf + i) * fl * es;

@@ -274,11 +274,28 @@ int launch_copy_interior(const float *src, float *dst, long long sstride, int nz
 // each field and shot scaled by its max |x| (float bits into
 // bits[f bfstride + shot]); and back
 constexpr int kMaxFields = 6;
+// The cells of a padded field that a pass touches: rows [r0[b], r1[b]) of up
+// to two row bands times float4 groups [g0[c], g1[c]) of up to two column
+// bands (a band with r1 <= r0 or g1 <= g0 is empty).  Off its region a field
+// is zero (the CPML psi and zeta off their strips), so the max, the pack and
+// the unpack skip those cells; the slot keeps the field's layout.
+struct FieldRegion {
+    int r0[2], r1[2], g0[2], g1[2];
+    int p4;  // float4 per row of the field
+};
+// the whole padded plane of nzp rows of P floats
+FieldRegion region_all(int nzp, int P);
+// the x-strip (columns [4, N) and [nxp-N, nxp-4), rows [4, nzp-4)) or
+// z-strip (rows [4, N) and [nzp-N, nzp-4), every column) cells, in float4 groups
+FieldRegion region_xstrips(int nzp, int nxp, int P, int N);
+FieldRegion region_zstrips(int nzp, int P, int N);
+// reg (nullptr: every field whole) gives each field's region; n = nzp P
 int launch_pack_half(const float *const *x, int nf, long long sstride, long long n, int nshots, void *q,
-                     long long qfstride, long long qsstride, unsigned int *bits, long long bfstride, hipStream_t st);
+                     long long qfstride, long long qsstride, unsigned int *bits, long long bfstride, hipStream_t st,
+                     const FieldRegion *reg = nullptr);
 int launch_unpack_half(const void *q, long long qfstride, long long qsstride, long long n, int nshots, int nf,
                        const unsigned int *bits, long long bfstride, float *const *x, long long sstride,
-                       hipStream_t st);
+                       hipStream_t st, const FieldRegion *reg = nullptr);
 
 // preprocessing (pam_prep.hip)
 int launch_slowness(const float *f, float *s, long long n, int parameter, hipStream_t st);

@@ -138,17 +138,173 @@ static bool four_wide(const void *const *x, int nf, long long sstride, long long
     return true;
 }
 
+// The same three passes over each field's region only (FieldRegion): block
+// (j, shot, field) takes region rows j, j + gridDim.x, ..., its threads the
+// float4 groups of a row; every value goes through the same arithmetic as in
+// the whole-field kernels.
+struct Regions {
+    FieldRegion r[kMaxFields];
+};
+__device__ __forceinline__ int band_len(int a, int b) { return b > a ? b - a : 0; }
+// the row of region row j and the float4 group of region group k
+__device__ __forceinline__ int region_row(const FieldRegion &R, int j) {
+    const int nr0 = band_len(R.r0[0], R.r1[0]);
+    return j < nr0 ? R.r0[0] + j : R.r0[1] + (j - nr0);
+}
+__device__ __forceinline__ int region_group(const FieldRegion &R, int k) {
+    const int ng0 = band_len(R.g0[0], R.g1[0]);
+    return k < ng0 ? R.g0[0] + k : R.g0[1] + (k - ng0);
+}
+__device__ __forceinline__ int region_rows(const FieldRegion &R) {
+    return band_len(R.r0[0], R.r1[0]) + band_len(R.r0[1], R.r1[1]);
+}
+__device__ __forceinline__ int region_groups(const FieldRegion &R) {
+    return band_len(R.g0[0], R.g1[0]) + band_len(R.g0[1], R.g1[1]);
+}
+
+__global__ __launch_bounds__(256) void k_shot_maxabs_r(const FieldPtrs X, long long sstride, const Regions G,
+                                                       unsigned int *bits, long long bfstride) {
+    __shared__ float red[4];
+    const int s = blockIdx.y, f = blockIdx.z;
+    const FieldRegion &R = G.r[f];
+    const int nr = region_rows(R), ng = region_groups(R);
+    const float4 *xs = reinterpret_cast<const float4 *>(X.x[f] + (long long)s * sstride);
+    float m = 0.0f;
+    for (int j = blockIdx.x; j < nr; j += gridDim.x) {
+        const float4 *xr = xs + (long long)region_row(R, j) * R.p4;
+        for (int k = threadIdx.x; k < ng; k += blockDim.x) {
+            const float4 v = xr[region_group(R, k)];
+            m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(bits + (long long)f * bfstride + s, __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
+}
+
+__global__ __launch_bounds__(256) void k_pack_half_r(const FieldPtrs X, long long sstride, const Regions G, __half *q,
+                                                     long long qfstride, long long qsstride, const unsigned int *bits,
+                                                     long long bfstride) {
+    const int s = blockIdx.y, f = blockIdx.z;
+    const FieldRegion &R = G.r[f];
+    const int nr = region_rows(R), ng = region_groups(R);
+    const float inv = scale_inv(__uint_as_float(bits[(long long)f * bfstride + s]));
+    const float4 *xs = reinterpret_cast<const float4 *>(X.x[f] + (long long)s * sstride);
+    uint2 *qs = reinterpret_cast<uint2 *>(q + (long long)f * qfstride + (long long)s * qsstride);
+    for (int j = blockIdx.x; j < nr; j += gridDim.x) {
+        const long long rb = (long long)region_row(R, j) * R.p4;
+        for (int k = threadIdx.x; k < ng; k += blockDim.x) {
+            const long long i = rb + region_group(R, k);
+            const float4 v = xs[i];
+            const unsigned h0 = __half_as_ushort(__float2half(v.x * inv)), h1 = __half_as_ushort(__float2half(v.y * inv));
+            const unsigned h2 = __half_as_ushort(__float2half(v.z * inv)), h3 = __half_as_ushort(__float2half(v.w * inv));
+            qs[i] = make_uint2(h0 | (h1 << 16), h2 | (h3 << 16));
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_unpack_half_r(const __half *q, long long qfstride, long long qsstride,
+                                                       const Regions G, const unsigned int *bits, long long bfstride,
+                                                       const FieldPtrsW X, long long sstride) {
+    const int s = blockIdx.y, f = blockIdx.z;
+    const FieldRegion &R = G.r[f];
+    const int nr = region_rows(R), ng = region_groups(R);
+    const float m = __uint_as_float(bits[(long long)f * bfstride + s]);
+    const uint2 *qs = reinterpret_cast<const uint2 *>(q + (long long)f * qfstride + (long long)s * qsstride);
+    float4 *xs = reinterpret_cast<float4 *>(X.x[f] + (long long)s * sstride);
+    for (int j = blockIdx.x; j < nr; j += gridDim.x) {
+        const long long rb = (long long)region_row(R, j) * R.p4;
+        for (int k = threadIdx.x; k < ng; k += blockDim.x) {
+            const long long i = rb + region_group(R, k);
+            const uint2 w = qs[i];
+            float4 v;
+            v.x = __half2float(__ushort_as_half((unsigned short)(w.x & 0xffffu))) * m;
+            v.y = __half2float(__ushort_as_half((unsigned short)(w.x >> 16))) * m;
+            v.z = __half2float(__ushort_as_half((unsigned short)(w.y & 0xffffu))) * m;
+            v.w = __half2float(__ushort_as_half((unsigned short)(w.y >> 16))) * m;
+            xs[i] = v;
+        }
+    }
+}
+
 static unsigned grid_blocks(long long n) {
     long long b = (n + 255) / 256;
     return (unsigned)(b < 1024 ? (b < 1 ? 1 : b) : 1024);
 }
 
+FieldRegion region_all(int nzp, int P) {
+    FieldRegion r{};
+    r.r0[0] = 0;
+    r.r1[0] = nzp;
+    r.g0[0] = 0;
+    r.g1[0] = P / 4;
+    r.p4 = P / 4;
+    return r;
+}
+FieldRegion region_xstrips(int nzp, int nxp, int P, int N) {
+    FieldRegion r{};
+    r.p4 = P / 4;
+    r.r0[0] = 4;
+    r.r1[0] = nzp - 4;
+    // groups holding columns [4, N) and [nxp-N, nxp-4), one band when they meet
+    const int a0 = 1, a1 = (N + 3) / 4, b0 = (nxp - N) / 4, b1 = (nxp - 4 + 3) / 4;
+    if (b0 <= a1) {
+        r.g0[0] = a0;
+        r.g1[0] = b1;
+    } else {
+        r.g0[0] = a0;
+        r.g1[0] = a1;
+        r.g0[1] = b0;
+        r.g1[1] = b1;
+    }
+    return r;
+}
+FieldRegion region_zstrips(int nzp, int P, int N) {
+    FieldRegion r{};
+    r.p4 = P / 4;
+    r.g0[0] = 0;
+    r.g1[0] = P / 4;
+    if (nzp - N <= N) {
+        r.r0[0] = 4;
+        r.r1[0] = nzp - 4;
+    } else {
+        r.r0[0] = 4;
+        r.r1[0] = N;
+        r.r0[1] = nzp - N;
+        r.r1[1] = nzp - 4;
+    }
+    return r;
+}
+static int region_max_rows(const FieldRegion *reg, int nf) {
+    int m = 1;
+    for (int f = 0; f < nf; ++f) {
+        const FieldRegion &R = reg[f];
+        const int nr = std::max(R.r1[0] - R.r0[0], 0) + std::max(R.r1[1] - R.r0[1], 0);
+        m = std::max(m, nr);
+    }
+    return m;
+}
+
 int launch_pack_half(const float *const *x, int nf, long long sstride, long long n, int nshots, void *q,
-                     long long qfstride, long long qsstride, unsigned int *bits, long long bfstride, hipStream_t st) {
+                     long long qfstride, long long qsstride, unsigned int *bits, long long bfstride, hipStream_t st,
+                     const FieldRegion *reg) {
     if (nf < 1 || nf > kMaxFields) return (int)hipErrorInvalidValue;
     FieldPtrs X{};
     for (int f = 0; f < nf; ++f) X.x[f] = x[f];
     hipLaunchKernelGGL(k_zero_bits, dim3((nshots * nf + 255) / 256), dim3(256), 0, st, bits, bfstride, nshots, nf);
+    if (reg && four_wide(reinterpret_cast<const void *const *>(x), nf, sstride, n, q, qfstride, qsstride)) {
+        Regions G{};
+        for (int f = 0; f < nf; ++f) G.r[f] = reg[f];
+        const unsigned rows = (unsigned)region_max_rows(reg, nf);
+        hipLaunchKernelGGL(k_shot_maxabs_r, dim3(std::min<unsigned>(rows, (unsigned)kMaxBlocksPerShot), nshots, nf),
+                           dim3(256), 0, st, X, sstride, G, bits, bfstride);
+        hipLaunchKernelGGL(k_pack_half_r, dim3(rows, nshots, nf), dim3(256), 0, st, X, sstride, G,
+                           reinterpret_cast<__half *>(q), qfstride, qsstride, (const unsigned int *)bits, bfstride);
+        return (int)hipGetLastError();
+    }
     if (four_wide(reinterpret_cast<const void *const *>(x), nf, sstride, n, q, qfstride, qsstride)) {
         const long long n4 = n / 4;
         const unsigned mb4 = std::min<unsigned>(grid_blocks(n4), (unsigned)kMaxBlocksPerShot);
@@ -166,10 +322,17 @@ int launch_pack_half(const float *const *x, int nf, long long sstride, long long
 
 int launch_unpack_half(const void *q, long long qfstride, long long qsstride, long long n, int nshots, int nf,
                        const unsigned int *bits, long long bfstride, float *const *x, long long sstride,
-                       hipStream_t st) {
+                       hipStream_t st, const FieldRegion *reg) {
     if (nf < 1 || nf > kMaxFields) return (int)hipErrorInvalidValue;
     FieldPtrsW X{};
     for (int f = 0; f < nf; ++f) X.x[f] = x[f];
+    if (reg && four_wide(reinterpret_cast<const void *const *>(x), nf, sstride, n, q, qfstride, qsstride)) {
+        Regions G{};
+        for (int f = 0; f < nf; ++f) G.r[f] = reg[f];
+        hipLaunchKernelGGL(k_unpack_half_r, dim3((unsigned)region_max_rows(reg, nf), nshots, nf), dim3(256), 0, st,
+                           reinterpret_cast<const __half *>(q), qfstride, qsstride, G, bits, bfstride, X, sstride);
+        return (int)hipGetLastError();
+    }
     if (four_wide(reinterpret_cast<const void *const *>(x), nf, sstride, n, q, qfstride, qsstride)) {
         const long long n4 = n / 4;
         hipLaunchKernelGGL(k_unpack_half4, dim3(grid_blocks(n4), nshots, nf), dim3(256), 0, st,
