@@ -27,24 +27,26 @@ namespace pam {
 __device__ __forceinline__ long long sb_w(int nz, int nx) { return 8LL * nx + 8LL * (nz - 8); }
 long long sb_width(int nz, int nx) { return 8LL * nx + 8LL * (nz - 8); }
 
-// interior (iz, ix) of strip element i
-__device__ __forceinline__ void sb_cell(long long i, int nz, int nx, int *iz, int *ix) {
-    const long long tnx = 4LL * nx;
+// interior (iz, ix) of strip element i (32-bit arithmetic: a step's strips
+// hold 8 (nx + nz - 8) elements, far below 2^31 for any grid the plane guard
+// admits)
+__device__ __forceinline__ void sb_cell(int i, int nz, int nx, int *iz, int *ix) {
+    const int tnx = 4 * nx;
     if (i < tnx) {
-        *iz = (int)(i / nx);
-        *ix = (int)(i - (long long)(*iz) * nx);
+        *iz = i / nx;
+        *ix = i - *iz * nx;
     } else if (i < 2 * tnx) {
-        const long long j = i - tnx;
-        const int r = (int)(j / nx);
+        const int j = i - tnx;
+        const int r = j / nx;
         *iz = nz - 4 + r;
-        *ix = (int)(j - (long long)r * nx);
+        *ix = j - r * nx;
     } else {
-        long long j = i - 2 * tnx;
-        const long long side = 4LL * (nz - 8);
+        int j = i - 2 * tnx;
+        const int side = 4 * (nz - 8);
         const bool right = j >= side;
         if (right) j -= side;
-        *iz = 4 + (int)(j >> 2);
-        *ix = (right ? nx - 4 : 0) + (int)(j & 3);
+        *iz = 4 + (j >> 2);
+        *ix = (right ? nx - 4 : 0) + (j & 3);
     }
 }
 
@@ -52,6 +54,20 @@ __device__ __forceinline__ float sb_load(const void *p, int kind, long long i, f
     if (kind == 1) return __half2float(reinterpret_cast<const __half *>(p)[i]) * m;
     if (kind == 2) return __bfloat162float(reinterpret_cast<const __hip_bfloat16 *>(p)[i]) * m;
     return reinterpret_cast<const float *>(p)[i];
+}
+
+// float32 strips of the padded field U (interior origin at (N, N)) into step
+// k of each shot's strip buffer, one element per thread (grid: blocks, shots)
+__global__ __launch_bounds__(256) void k_sb_save_f32(const float *U, long long sstride, int P, int N, int nz, int nx,
+                                                     float *strips, long long strip_sstride, int k) {
+    const int s = blockIdx.y;
+    const int W = (int)sb_w(nz, nx);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W) return;
+    const float *u = U + (long long)s * sstride + (long long)N * P + N;
+    int iz, ix;
+    sb_cell(i, nz, nx, &iz, &ix);
+    strips[(long long)s * strip_sstride + (long long)k * W + i] = u[(long long)iz * P + ix];
 }
 
 // one block per shot: strips of the padded field U (interior origin at
@@ -70,7 +86,7 @@ __global__ __launch_bounds__(1024) void k_sb_save(const float *U, long long sstr
         float m = 0.0f;
         for (long long i = threadIdx.x; i < W; i += blockDim.x) {
             int iz, ix;
-            sb_cell(i, nz, nx, &iz, &ix);
+            sb_cell((int)i, nz, nx, &iz, &ix);
             m = fmaxf(m, fabsf(u[(long long)iz * P + ix]));
         }
         red[threadIdx.x] = m;
@@ -85,7 +101,7 @@ __global__ __launch_bounds__(1024) void k_sb_save(const float *U, long long sstr
     }
     for (long long i = threadIdx.x; i < W; i += blockDim.x) {
         int iz, ix;
-        sb_cell(i, nz, nx, &iz, &ix);
+        sb_cell((int)i, nz, nx, &iz, &ix);
         const float v = u[(long long)iz * P + ix];
         if (kind == 1)
             reinterpret_cast<__half *>(strips)[base + i] = __float2half(v * inv);
@@ -114,7 +130,7 @@ __global__ __launch_bounds__(256) void k_sb_restore(float *U, long long sstride,
     float *u = U + (long long)s * sstride + (long long)N * P + N;  // interior origin
     if (i < W) {
         int iz, ix;
-        sb_cell(i, nz, nx, &iz, &ix);
+        sb_cell((int)i, nz, nx, &iz, &ix);
         const long long base = (long long)s * strip_sstride;
         const float *sc = scale + (long long)s * nt;
         const float v0 = sb_load(strips, kind, base + (long long)L * W + i, kind ? sc[L] : 1.0f);
@@ -125,20 +141,20 @@ __global__ __launch_bounds__(256) void k_sb_restore(float *U, long long sstride,
             D[(long long)s * dsstride + (long long)iz * nx + ix] = v0 - 2.0f * v1 + v2;
         }
     } else if (i < W + nband) {
-        long long j = i - W;
+        int j = (int)(i - W);  // 32-bit: the band holds 8 (nx + 8) + 8 nz cells
         int iz, ix;
-        if (j < 2 * wb) {
-            const bool bot = j >= wb;
-            if (bot) j -= wb;
-            const int r = (int)(j / (nx + 8));
+        if (j < 2 * (int)wb) {
+            const bool bot = j >= (int)wb;
+            if (bot) j -= (int)wb;
+            const int r = j / (nx + 8);
             iz = (bot ? nz : -4) + r;
-            ix = -4 + (int)(j - (long long)r * (nx + 8));
+            ix = -4 + (j - r * (nx + 8));
         } else {
-            j -= 2 * wb;
-            const bool right = j >= 4LL * nz;
-            if (right) j -= 4LL * nz;
-            iz = (int)(j >> 2);
-            ix = (right ? nx : -4) + (int)(j & 3);
+            j -= 2 * (int)wb;
+            const bool right = j >= 4 * nz;
+            if (right) j -= 4 * nz;
+            iz = j >> 2;
+            ix = (right ? nx : -4) + (j & 3);
         }
         u[(long long)iz * P + ix] = 0.0f;
     } else if (i == W + nband && wk >= 0 && wk < nt) {
@@ -183,6 +199,12 @@ __global__ void k_copy_interior(const float *src, float *dst, long long sstride,
 
 int launch_sb_save(const float *U, long long sstride, int P, int N, int nz, int nx, int nshots, int kind,
                    void *strips, long long strip_sstride, int k, float *scale, int nt, hipStream_t st) {
+    if (kind == 0) {  // no scale to find first: the strips of every shot in parallel
+        const long long W = sb_width(nz, nx);
+        hipLaunchKernelGGL(k_sb_save_f32, dim3((unsigned)((W + 255) / 256), nshots), dim3(256), 0, st, U, sstride, P,
+                           N, nz, nx, reinterpret_cast<float *>(strips), strip_sstride, k);
+        return (int)hipGetLastError();
+    }
     hipLaunchKernelGGL(k_sb_save, dim3(nshots), dim3(1024), 0, st, U, sstride, P, N, nz, nx, kind, strips,
                        strip_sstride, k, scale, nt);
     return (int)hipGetLastError();
