@@ -261,8 +261,6 @@ __global__ __launch_bounds__(256) void k_exact_lambda_cpml(const ExactArgs a) {
         const long long c = (long long)iz * P + ix;
         if (!live_at(a, iz, ix)) {
             Y[c] = 0.0f;
-            a.nux_out[so + c] = 0.0f;
-            a.nuz_out[so + c] = 0.0f;
             continue;
         }
         const int l = (lz + 4) * W + lx + 4;
@@ -272,7 +270,9 @@ __global__ __launch_bounds__(256) void k_exact_lambda_cpml(const ExactArgs a) {
         const float q = a.v2[c] * phi;
         const float nx_k = sxs ? q + a.cax[c] * nxi[c] : 0.0f;
         const float nz_k = szs ? q + a.caz[c] * nzi[c] : 0.0f;
-        float sx = C0 * (q + a.cbx[c] * nx_k), sz = C0 * (q + a.cbz[c] * nz_k);
+        // off a strip nu is zero: q alone (without the zero b nu term), and the nu
+        // buffers keep the zeros they were cleared to
+        float sx = C0 * (sxs ? q + a.cbx[c] * nx_k : q), sz = C0 * (szs ? q + a.cbz[c] * nz_k : q);
         float tx = 0.0f, tz = 0.0f;
         const bool nearx = ix < a.N + 4 || ix >= a.nxp - a.N - 4;
         const bool nearz = iz < a.N + 4 || iz >= a.nzp - a.N - 4;
@@ -285,8 +285,8 @@ __global__ __launch_bounds__(256) void k_exact_lambda_cpml(const ExactArgs a) {
             if (nearz) tz += ka(m) * (sbz[l - m * W] - sbz[l + m * W]);
         }
         Y[c] = sx * a.idx2 + sz * a.idz2 + 2.0f * phi - Y[c] + tx * a.idx + tz * a.idz;
-        a.nux_out[so + c] = nx_k;
-        a.nuz_out[so + c] = nz_k;
+        if (sxs) a.nux_out[so + c] = nx_k;
+        if (szs) a.nuz_out[so + c] = nz_k;
         float Q = qxx + qzz;
         if (sxs) {
             const float *px = a.psx + fo;
