@@ -2142,6 +2142,12 @@ struct CkptRun {
     bool grad = false, d2t = false;
     int tile = 0, C = 0, nseg = 0, t_lo = 0, t_hi = 0, nbuf = 0, skind = 0;
     float *image = nullptr, *ilum = nullptr;
+    // save-all runs on the lanes: every step's snapshot (elem bytes per element)
+    // instead of checkpoints; half: 16-bit snapshots scaled by a max found
+    // step by step (smax0: the bound of the first step's scale)
+    bool saveall = false, half = false;
+    int elem = 4;
+    float smax0 = 0.0f;
     std::vector<hipEvent_t> red_ev;   // reduction of batch i done
     std::vector<char> red_issued;     // batch i's reduction enqueued
 };
@@ -2155,6 +2161,7 @@ struct CkptRun {
 struct CkptLane {
     FieldSet F, A;  // the lane's views of the two field sets
     float *save = nullptr, *acc_ilum = nullptr, *acc_img = nullptr, *ckpt = nullptr;
+    unsigned int *smax = nullptr;  // the lane's snapshot scales (16-bit save-all)
     std::vector<int> flip;  // psi buffer of each checkpoint
     hipStream_t st = nullptr;
     int slot0 = 0, Bl = 0;
@@ -2232,6 +2239,18 @@ static int ckpt_tick(pam_ctx *ctx, CkptRun &R, CkptLane &L, bool *done) {
                                     0, L.nb, L.st));
             base_step_args(ctx, L.a, L.nb);
             L.a.flags |= F_SRC | F_ILUM;
+            if (R.saveall) {
+                // every step's snapshot (S_k, or D_k for a gradient), 16-bit ones scaled
+                // by the max the step before finds
+                L.a.flags |= F_SAVE | (R.d2t ? F_D2T : 0) | (R.half ? F_SMAX : 0);
+                if (R.half) {
+                    HCK(hipMemsetAsync(L.smax, 0, (size_t)L.nb * nt * 4 * kSmaxSlots, L.st));
+                    KCK(launch_smax0(L.smax, nt, ctx->d_ssz + L.slot0, ctx->d_ssx + L.slot0, g.n_abc, ctx->nzp,
+                                     ctx->nxp, ctx->wav, L.nb, R.smax0, L.st));
+                    L.a.smax = L.smax;
+                    L.a.save_scale = reinterpret_cast<const float *>(L.smax);
+                }
+            }
             L.a.save_kind = R.skind;
             L.a.save = L.save;
             L.a.save_k0 = 0;
@@ -2251,7 +2270,7 @@ static int ckpt_tick(pam_ctx *ctx, CkptRun &R, CkptLane &L, bool *done) {
             return PAM_OK;
         }
         case 1: {  // forward step k, checkpoint j first when it starts a segment
-            if (L.j < R.nseg && L.k == std::max(0, L.j * R.C - 1)) {
+            if (!R.saveall && L.j < R.nseg && L.k == std::max(0, L.j * R.C - 1)) {
                 int r = ckpt_lane_save(ctx, L, L.j);
                 if (r) return r;
                 ++L.j;
@@ -2282,6 +2301,16 @@ static int ckpt_tick(pam_ctx *ctx, CkptRun &R, CkptLane &L, bool *done) {
             L.b.acc_pad = R.acc_pad;
             L.b.ssz = ctx->d_ssz + L.slot0;
             L.b.ssx = ctx->d_ssx + L.slot0;
+            if (R.saveall) {  // the snapshots of every step are in place: one backward sweep
+                L.b.save_k0 = 0;
+                L.b.save_scale = reinterpret_cast<const float *>(L.smax);
+                L.seg = 0;
+                L.lo = R.t_lo;
+                L.hi = R.t_hi;
+                L.t = R.t_hi;
+                L.phase = 4;
+                return PAM_OK;
+            }
             L.seg = R.nseg - 1;
             L.phase = 2;
             return PAM_OK;
@@ -2364,11 +2393,16 @@ static int ckpt_two_lanes(pam_ctx *ctx, CkptRun &R, const std::vector<int> &size
         L[l].slot0 = l * Bl;
         L[l].Bl = Bl;
         L[l].F = fieldset_view(ctx, ctx->fs, l * Bl);
-        L[l].A = fieldset_view(ctx, ctx->aux, l * Bl);
-        L[l].save = reinterpret_cast<float *>(ctx->save) + (long long)l * Bl * R.per_step * R.nbuf;
+        if (!R.saveall) {
+            L[l].A = fieldset_view(ctx, ctx->aux, l * Bl);
+            L[l].ckpt = ctx->ckpt + (long long)l * R.nseg * ckpt_fields(ctx) * ctx->plane * Bl;
+        }
+        // the lane's share of the snapshot buffer (R.elem bytes per element)
+        L[l].save = reinterpret_cast<float *>(static_cast<char *>(ctx->save) +
+                                              (long long)l * Bl * R.per_step * R.nbuf * R.elem);
+        if (R.half) L[l].smax = ctx->smax + (long long)l * Bl * ctx->g.nt * kSmaxSlots;
         L[l].acc_ilum = ctx->acc_ilum + (long long)l * Bl * R.acc_st;
         L[l].acc_img = ctx->acc_img + (long long)l * Bl * R.acc_st;
-        L[l].ckpt = ctx->ckpt + (long long)l * R.nseg * ckpt_fields(ctx) * ctx->plane * Bl;
         L[l].flip.assign(R.nseg, 0);
         L[l].st = ctx->lane_st[l];
     }
@@ -2519,7 +2553,9 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
     // uniform checkpoints with the pressure-form adjoint and no partials:
     // batches of Bl shots alternate between two lanes, as in pam_forward (an
     // explicit batch is the per-lane one); the buffers hold both lanes' shots
-    bool two = ckpt && !multi && !linear_adj && ctx->lanes == 2 && nshots > 1 && ctx->max_shots >= 2;
+    // (and save-all runs in float32, float16 or bfloat16 on the same lanes)
+    const bool saveall = storage == PAM_STORE_ALL_F32 || half;
+    bool two = (ckpt || saveall) && !multi && !linear_adj && ctx->lanes == 2 && nshots > 1 && ctx->max_shots >= 2;
     int Bl = 0;
     if (two) {
         Bl = ctx->batch > 0 ? ctx->batch : (B + 1) / 2;
@@ -2642,6 +2678,10 @@ static int adjoint_run(pam_ctx *ctx, int shot0, int nshots, const float *data, b
         R.skind = skind;
         R.image = image;
         R.ilum = ilum_acc;
+        R.saveall = saveall;
+        R.half = half;
+        R.elem = elem;
+        R.smax0 = grad ? 3.0f * ctx->wav_absmax : 0.0f;
         // an explicit batch: Bl, Bl, ..., the rest (the batches one lane would
         // run); the automatic one: as even as Bl allows
         std::vector<int> sizes;

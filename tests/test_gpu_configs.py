@@ -123,15 +123,17 @@ def test_vti_cpml_64_all_shots_on_two_lanes_equal_one_shot_at_a_time():
     eng.close()
 
 
-def test_vti_cpml_64_gradient_within_1e4_of_float64_oracle():
-    """The bench's gradient path (uniform checkpoints, two lanes) on the full
-    grid against the float64 oracle's on-the-fly gradient."""
+@pytest.mark.parametrize("storage", [_lib.PAM_STORE_ALL_F16, _lib.PAM_STORE_ALL_F32, _lib.PAM_STORE_CHECKPOINT])
+def test_vti_cpml_64_gradient_within_1e4_of_float64_oracle(storage):
+    """The bench's gradient paths (float16-scaled snapshots of every step, the
+    headline; float32 snapshots; uniform checkpoints -- all on two lanes) on
+    the full grid against the float64 oracle's on-the-fly gradient."""
     w = W.vti_cpml_64(nshots=2, nt_data=602)
     pb = _pb(w)
-    print(f"case vti_cpml_64 gradient nt={pb['g'].nt}, checkpoints, two lanes", flush=True)
+    print(f"case vti_cpml_64 gradient nt={pb['g'].nt}, storage {storage}, two lanes", flush=True)
     eng = _engine(pb, 2)
     res = _residual(pb, eng, 2)
-    eng.set_option(_lib.PAM_OPT_STORAGE, _lib.PAM_STORE_CHECKPOINT)
+    eng.set_option(_lib.PAM_OPT_STORAGE, storage)
     eng.set_option(_lib.PAM_OPT_LANES, 2)
     eng.set_option(_lib.PAM_OPT_BATCH, 1)  # one shot per lane: both lanes busy
     g = eng.gradient(res)["g_m"].cpu().numpy()

@@ -133,3 +133,33 @@ def test_checkpoint_adjoint_on_two_lanes_with_the_automatic_batch():
     ref = _ckpt_run(eng, data, True)
     for g, r in zip(got, ref):
         assert rel_l2(g, r) <= 1e-6
+
+
+@pytest.mark.parametrize("storage", [_lib.PAM_STORE_ALL_F32, _lib.PAM_STORE_ALL_F16, _lib.PAM_STORE_ALL_BF16])
+@pytest.mark.parametrize("phys,abc", [(0, 0), (1, 1), (2, 1)])
+@pytest.mark.parametrize("grad", [True, False])
+def test_save_all_adjoint_on_two_lanes_equals_one_lane(storage, phys, abc, grad):
+    """Save-all runs (float32, and float16 / bfloat16 scaled by the max each
+    step finds) on two lanes: batches of 2 shots alternating between the lanes
+    (7 shots) give the gradient or image and the illumination of the same
+    batches run one after another on one stream, bit for bit, on two calls in
+    a row."""
+    import torch
+
+    from tests.test_gpu_adjoint import _data
+
+    print(f"case save-all lanes storage={storage} phys={phys} abc={abc} grad={grad}", flush=True)
+    pb = make_problem(phys, abc, nshots=7, seed=600 + 2 * phys + abc, nt=181)
+    data = torch.from_numpy(_data(pb, 7, seed=60))
+    eng = make_engine(pb, max_shots=7)
+    eng.set_option(_lib.PAM_OPT_STORAGE, storage)
+    eng.set_option(_lib.PAM_OPT_TILE, 0)
+    eng.set_option(_lib.PAM_OPT_BATCH, 2)
+    eng.set_option(_lib.PAM_OPT_LANES, 1)
+    ref = _ckpt_run(eng, data, grad)
+    eng.set_option(_lib.PAM_OPT_LANES, 2)
+    for call in range(2):
+        got = _ckpt_run(eng, data, grad)
+        for name, g, r in zip(("result", "ilum"), got, ref):
+            assert np.array_equal(g, r), f"{name} differs (call {call})"
+    assert np.isfinite(ref[0]).all() and np.abs(ref[0]).max() > 0
