@@ -219,8 +219,11 @@ __device__ __forceinline__ float d1l(const float *f, int l, int st) {
            A3 * (f[l + 3 * st] - f[l - 3 * st]) + A4 * (f[l + 4 * st] - f[l - 4 * st]);
 }
 __global__ __launch_bounds__(256) void k_exact_lambda_cpml(const ExactArgs a) {
-    constexpr int W = 64 + 8, H = kLamTZ + 8, NS = W * H;
-    __shared__ float sux[NS], suz[NS], sC[NS], sbx[NS], sbz[NS];
+    // C on the tile and a 4-cell halo on every side; the x-neighbour values
+    // (u_x, b mu_x) on the tile's rows with the x halo, the z-neighbour values
+    // (u_z, b mu_z) on its columns with the z halo: 28.4 KB, 5 blocks per CU
+    constexpr int W = 64 + 8, H = kLamTZ + 8, NS = W * H, NX = W * kLamTZ, NZ = 64 * H;
+    __shared__ float sux[NX], suz[NZ], sC[NS], sbx[NX], sbz[NZ];
     const int x0 = blockIdx.x * 64, z0 = blockIdx.y * kLamTZ, s = blockIdx.z, tid = threadIdx.x;
     const long long P = a.P;
     const long long so = (long long)s * a.sstride, fo = (long long)s * a.fstride;
@@ -233,26 +236,90 @@ __global__ __launch_bounds__(256) void k_exact_lambda_cpml(const ExactArgs a) {
     // off the strips, so other cells' terms are zero and not read)
     const bool tnx = x0 < a.N + 4 || x0 + 63 >= a.nxp - a.N - 4;
     const bool tnz = z0 < a.N + 4 || z0 + kLamTZ - 1 >= a.nzp - a.N - 4;
-    for (int i = tid; i < NS; i += 256) {
-        const int r = i / W, cc = i - r * W;
-        const int iz = z0 - 4 + r, ix = x0 - 4 + cc;
-        float ux = 0.0f, uz = 0.0f, cv = 0.0f, bx = 0.0f, bz = 0.0f;
-        if (iz >= 0 && iz < a.nzp && ix >= 0 && ix < a.nxp) {
-            const long long c = (long long)iz * P + ix;
-            ux = u_of(a, X, nxi, a.cax, a.cbx, iz, ix, c, xstrip(a, iz, ix));
-            uz = u_of(a, X, nzi, a.caz, a.cbz, iz, ix, c, zstrip(a, iz, ix));
-            cv = C[c];
-            if (tnx) bx = a.cbx[c] * mux[c];
-            if (tnz) bz = a.cbz[c] * muz[c];
+    // the tile with its halo meets an x (z) strip: u_x (u_z) differs from V X there
+    const bool tsx = x0 - 4 < a.N || x0 + 68 > a.nxp - a.N;
+    const bool tsz = z0 - 4 < a.N || z0 + kLamTZ + 4 > a.nzp - a.N;
+    // Every load of the block is issued before the first is used: the values
+    // of the thread's own cells, then the tile with its halo.  Loads outside
+    // the field read cell 0 and are replaced by zero (no branches, so the
+    // unrolled loads stay in one basic block).
+    const int lx = tid & 63, ix = x0 + lx;
+    float pX[4], pV[4], pY[4], pG[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+        const int iz = z0 + (tid >> 6) * 4 + rr;
+        const bool lv = ix < a.nxp && iz < a.nzp && live_at(a, iz, ix);
+        const long long c = lv ? (long long)iz * P + ix : 0;
+        pX[rr] = X[c];
+        pV[rr] = a.v2[c];
+        pY[rr] = Y[c];
+        pG[rr] = a.gV[so + c];
+    }
+    constexpr int NIT = (NS + 255) / 256;
+    float rX[NIT], rV[NIT], rC[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int i = tid + it * 256, r = i / W, cc = i - r * W;
+        const int iz = z0 - 4 + r, jx = x0 - 4 + cc;
+        const bool in = i < NS && iz >= 0 && iz < a.nzp && jx >= 0 && jx < a.nxp;
+        const long long c = in ? (long long)iz * P + jx : 0;
+        const float xv = X[c], vv = a.v2[c], cv = C[c];
+        rX[it] = in ? xv : 0.0f;
+        rV[it] = in ? vv : 0.0f;
+        rC[it] = in ? cv : 0.0f;
+    }
+    float ux[NIT], uz[NIT], bx[NIT], bz[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const float q = rV[it] * rX[it];  // X vanishes off the live cells
+        ux[it] = q;
+        uz[it] = q;
+        bx[it] = 0.0f;
+        bz[it] = 0.0f;
+    }
+    if (tsx || tnx) {
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int i = tid + it * 256, r = i / W, cc = i - r * W;
+            const int iz = z0 - 4 + r, jx = x0 - 4 + cc;
+            const bool in = i < NS && iz >= 0 && iz < a.nzp && jx >= 0 && jx < a.nxp;
+            const long long c = in ? (long long)iz * P + jx : 0;
+            const float ca = a.cax[c], cb = a.cbx[c], nu = nxi[c], m = mux[c];
+            const float q = ux[it];
+            if (tsx && in && xstrip(a, iz, jx)) ux[it] = q + cb * (q + ca * nu);
+            if (tnx && in) bx[it] = cb * m;
         }
-        sux[i] = ux;
-        suz[i] = uz;
-        sC[i] = cv;
-        sbx[i] = bx;
-        sbz[i] = bz;
+    }
+    if (tsz || tnz) {
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int i = tid + it * 256, r = i / W, cc = i - r * W;
+            const int iz = z0 - 4 + r, jx = x0 - 4 + cc;
+            const bool in = i < NS && iz >= 0 && iz < a.nzp && jx >= 0 && jx < a.nxp;
+            const long long c = in ? (long long)iz * P + jx : 0;
+            const float ca = a.caz[c], cb = a.cbz[c], nu = nzi[c], m = muz[c];
+            const float q = uz[it];
+            if (tsz && in && zstrip(a, iz, jx)) uz[it] = q + cb * (q + ca * nu);
+            if (tnz && in) bz[it] = cb * m;
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int i = tid + it * 256;
+        if (i < NS) {
+            const int r = i / W, cc = i - r * W;
+            sC[i] = rC[it];
+            if (r >= 4 && r < kLamTZ + 4) {
+                sux[i - 4 * W] = ux[it];
+                sbx[i - 4 * W] = bx[it];
+            }
+            if (cc >= 4 && cc < 68) {
+                suz[r * 64 + cc - 4] = uz[it];
+                sbz[r * 64 + cc - 4] = bz[it];
+            }
+        }
     }
     __syncthreads();
-    const int lx = tid & 63, ix = x0 + lx;
     if (ix >= a.nxp) return;
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
@@ -263,11 +330,11 @@ __global__ __launch_bounds__(256) void k_exact_lambda_cpml(const ExactArgs a) {
             Y[c] = 0.0f;
             continue;
         }
-        const int l = (lz + 4) * W + lx + 4;
+        const int l = (lz + 4) * W + lx + 4, lxr = lz * W + lx + 4, lzr = (lz + 4) * 64 + lx;
         const float qxx = d2l(sC, l, 1) * a.idx2, qzz = d2l(sC, l, W) * a.idz2;
         const bool sxs = xstrip(a, iz, ix), szs = zstrip(a, iz, ix);
-        const float phi = X[c];
-        const float q = a.v2[c] * phi;
+        const float phi = pX[rr];
+        const float q = pV[rr] * phi;
         const float nx_k = sxs ? q + a.cax[c] * nxi[c] : 0.0f;
         const float nz_k = szs ? q + a.caz[c] * nzi[c] : 0.0f;
         // off a strip nu is zero: q alone (without the zero b nu term), and the nu
@@ -278,13 +345,13 @@ __global__ __launch_bounds__(256) void k_exact_lambda_cpml(const ExactArgs a) {
         const bool nearz = iz < a.N + 4 || iz >= a.nzp - a.N - 4;
 #pragma unroll
         for (int m = 1; m <= 4; ++m) {
-            sx += kc(m) * (sux[l + m] + sux[l - m]);
-            sz += kc(m) * (suz[l + m * W] + suz[l - m * W]);
+            sx += kc(m) * (sux[lxr + m] + sux[lxr - m]);
+            sz += kc(m) * (suz[lzr + m * 64] + suz[lzr - m * 64]);
             // D^T (b mu) = -D (b mu)
-            if (nearx) tx += ka(m) * (sbx[l - m] - sbx[l + m]);
-            if (nearz) tz += ka(m) * (sbz[l - m * W] - sbz[l + m * W]);
+            if (nearx) tx += ka(m) * (sbx[lxr - m] - sbx[lxr + m]);
+            if (nearz) tz += ka(m) * (sbz[lzr - m * 64] - sbz[lzr + m * 64]);
         }
-        Y[c] = sx * a.idx2 + sz * a.idz2 + 2.0f * phi - Y[c] + tx * a.idx + tz * a.idz;
+        Y[c] = sx * a.idx2 + sz * a.idz2 + 2.0f * phi - pY[rr] + tx * a.idx + tz * a.idz;
         if (sxs) a.nux_out[so + c] = nx_k;
         if (szs) a.nuz_out[so + c] = nz_k;
         float Q = qxx + qzz;
@@ -304,7 +371,7 @@ __global__ __launch_bounds__(256) void k_exact_lambda_cpml(const ExactArgs a) {
             a.gaz[so + c] += muz[c] * pm + nz_k * zm;
             a.gbz[so + c] += muz[c] * (d1l(sC, l, W) * a.idz) + nz_k * (qzz + dpz);
         }
-        a.gV[so + c] += phi * Q;
+        a.gV[so + c] = pG[rr] + phi * Q;
     }
 }
 
