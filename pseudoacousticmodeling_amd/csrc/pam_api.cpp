@@ -3319,10 +3319,12 @@ static int exact_revolve(pam_ctx *ctx, StepArgs &a, int tile, int nb, int nslot,
     auto leaf = [&](int t) -> int {
         int rr = advance(t, t);
         if (rr) return rr;
-        const size_t fb = (size_t)ctx->plane * nb * sizeof(float);
-        if (t >= 1 && cpml) {  // zeta of step t - 1
-            HCK(hipMemcpyAsync(Sm + 3 * blk, w.zetax, fb, hipMemcpyDeviceToDevice, ctx->st));
-            HCK(hipMemcpyAsync(Sm + 4 * blk, w.zetaz, fb, hipMemcpyDeviceToDevice, ctx->st));
+        if (t >= 1 && cpml) {  // zeta of step t - 1, on its strips (the adjoint step reads no other cell)
+            const FieldRegion rg[2] = {region_xstrips(ctx->nzp, ctx->nxp, ctx->P, ctx->g.n_abc),
+                                       region_zstrips(ctx->nzp, ctx->P, ctx->g.n_abc)};
+            const float *zs[2] = {w.zetax, w.zetaz};
+            float *zd[2] = {Sm + 3 * blk, Sm + 4 * blk};
+            HCK((hipError_t)launch_copy_region(zs, zd, 2, ctx->plane, nb, rg, ctx->st));
         }
         float *ut = wc;
         rr = wstep(t);

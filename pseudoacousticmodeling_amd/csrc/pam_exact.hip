@@ -81,9 +81,41 @@ __device__ __forceinline__ float u_of(const ExactArgs &a, const float *X, const 
     return q + cb[c] * (q + ca[c] * nu_in[c]);
 }
 
-__global__ __launch_bounds__(256) void k_exact_mu(const ExactArgs a) {
-    const int ix = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int iz = blockIdx.y * 4 + (threadIdx.x >> 6), s = blockIdx.z;
+// The 64 x 4 tiles that hold strip cells: every tile of the tile rows [0, a0)
+// and [b0, nTR) (the z strips), then the tile columns [0, c0) and [d0, nTC)
+// (the x strips) of the rows between; blockIdx.x runs over them in that order.
+struct StripTiles {
+    int nTR, nTC, a0, b0, c0, d0, rowsA, colsB;
+};
+__host__ StripTiles strip_tiles(const ExactArgs &a) {
+    StripTiles m;
+    m.nTR = (a.nzp + 3) / 4;
+    m.nTC = (a.nxp + 63) / 64;
+    m.a0 = (a.N + 3) / 4;          // rows < N
+    m.b0 = (a.nzp - a.N) / 4;      // rows >= nzp - N
+    if (m.b0 <= m.a0) m.a0 = m.b0 = m.nTR;
+    m.c0 = (a.N + 63) / 64;        // columns < N
+    m.d0 = (a.nxp - a.N) / 64;     // columns >= nxp - N
+    if (m.d0 <= m.c0) m.c0 = m.d0 = m.nTC;
+    m.rowsA = m.a0 + (m.nTR - m.b0);
+    m.colsB = m.c0 + (m.nTC - m.d0);
+    return m;
+}
+
+__global__ __launch_bounds__(256) void k_exact_mu(const ExactArgs a, const StripTiles t) {
+    int k = blockIdx.x, by, bx;
+    if (k < t.rowsA * t.nTC) {
+        const int r = k / t.nTC;
+        bx = k - r * t.nTC;
+        by = r < t.a0 ? r : t.b0 + (r - t.a0);
+    } else {
+        k -= t.rowsA * t.nTC;
+        const int r = k / t.colsB, cc = k - r * t.colsB;
+        by = t.a0 + r;
+        bx = cc < t.c0 ? cc : t.d0 + (cc - t.c0);
+    }
+    const int ix = bx * 64 + (threadIdx.x & 63);
+    const int iz = by * 4 + (threadIdx.x >> 6), s = blockIdx.y;
     if (ix >= a.nxp || iz >= a.nzp) return;
     const bool sx = xstrip(a, iz, ix), sz = zstrip(a, iz, ix);
     if (!sx && !sz) return;
@@ -454,9 +486,12 @@ __global__ void k_expand_T(const float *g, int nzp, int nxp, int P, int N, float
 
 // blocks of 64 x 4 cells: the z-neighbours of a cell are mostly in its own
 // block, so they come from L1 (rows of 256 cells fetched every z-neighbour
-// through L2)
+// through L2); only the tiles that hold strip cells are launched
 int launch_exact_mu(const ExactArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL(k_exact_mu, dim3((a.nxp + 63) / 64, (a.nzp + 3) / 4, a.nshots), dim3(256), 0, st, a);
+    const StripTiles t = strip_tiles(a);
+    const int nb = t.rowsA * t.nTC + (t.nTR - t.rowsA) * t.colsB;
+    if (nb <= 0 || a.nshots <= 0) return 0;
+    hipLaunchKernelGGL(k_exact_mu, dim3(nb, a.nshots), dim3(256), 0, st, a, t);
     return (int)hipGetLastError();
 }
 int launch_exact_lambda(const ExactArgs &a, hipStream_t st) {

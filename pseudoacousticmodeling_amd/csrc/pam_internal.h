@@ -289,6 +289,10 @@ FieldRegion region_all(int nzp, int P);
 // z-strip (rows [4, N) and [nzp-N, nzp-4), every column) cells, in float4 groups
 FieldRegion region_xstrips(int nzp, int nxp, int P, int N);
 FieldRegion region_zstrips(int nzp, int P, int N);
+// y[f] = x[f] on reg[f] of each of nshots shots (shot stride sstride floats);
+// whole fields where the pointers or the stride are not 16-byte multiples
+int launch_copy_region(const float *const *x, float *const *y, int nf, long long sstride, int nshots,
+                       const FieldRegion *reg, hipStream_t st);
 // reg (nullptr: every field whole) gives each field's region; n = nzp P
 int launch_pack_half(const float *const *x, int nf, long long sstride, long long n, int nshots, void *q,
                      long long qfstride, long long qsstride, unsigned int *bits, long long bfstride, hipStream_t st,

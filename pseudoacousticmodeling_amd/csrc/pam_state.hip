@@ -230,6 +230,23 @@ __global__ __launch_bounds__(256) void k_unpack_half_r(const __half *q, long lon
     }
 }
 
+// y = x on the region of each field (float4 groups), shots along y, fields along z
+struct CopyPtrs {
+    const float *x[kMaxFields];
+    float *y[kMaxFields];
+};
+__global__ __launch_bounds__(256) void k_copy_r(const Regions G, const CopyPtrs C, long long sstride) {
+    const int s = blockIdx.y, f = blockIdx.z;
+    const FieldRegion &R = G.r[f];
+    const int nr = region_rows(R), ng = region_groups(R);
+    const float4 *xs = reinterpret_cast<const float4 *>(C.x[f] + (long long)s * sstride);
+    float4 *ys = reinterpret_cast<float4 *>(C.y[f] + (long long)s * sstride);
+    for (int j = blockIdx.x; j < nr; j += gridDim.x) {
+        const long long rb = (long long)region_row(R, j) * R.p4;
+        for (int k = threadIdx.x; k < ng; k += blockDim.x) ys[rb + region_group(R, k)] = xs[rb + region_group(R, k)];
+    }
+}
+
 static unsigned grid_blocks(long long n) {
     long long b = (n + 255) / 256;
     return (unsigned)(b < 1024 ? (b < 1 ? 1 : b) : 1024);
@@ -317,6 +334,32 @@ int launch_pack_half(const float *const *x, int nf, long long sstride, long long
     hipLaunchKernelGGL(k_shot_maxabs, dim3(mb, nshots, nf), dim3(256), 0, st, X, sstride, n, bits, bfstride);
     hipLaunchKernelGGL(k_pack_half, dim3(grid_blocks(n), nshots, nf), dim3(256), 0, st, X, sstride, n,
                        reinterpret_cast<__half *>(q), qfstride, qsstride, (const unsigned int *)bits, bfstride);
+    return (int)hipGetLastError();
+}
+
+int launch_copy_region(const float *const *x, float *const *y, int nf, long long sstride, int nshots,
+                       const FieldRegion *reg, hipStream_t st) {
+    if (nf < 1 || nf > kMaxFields) return (int)hipErrorInvalidValue;
+    bool four = sstride % 4 == 0;
+    for (int f = 0; f < nf; ++f)
+        four = four && reinterpret_cast<uintptr_t>(x[f]) % 16 == 0 && reinterpret_cast<uintptr_t>(y[f]) % 16 == 0;
+    if (!four) {  // whole fields
+        for (int f = 0; f < nf; ++f) {
+            const hipError_t e = hipMemcpyAsync(y[f], x[f], (size_t)sstride * nshots * sizeof(float),
+                                                hipMemcpyDeviceToDevice, st);
+            if (e != hipSuccess) return (int)e;
+        }
+        return 0;
+    }
+    Regions G{};
+    CopyPtrs C{};
+    for (int f = 0; f < nf; ++f) {
+        G.r[f] = reg[f];
+        C.x[f] = x[f];
+        C.y[f] = y[f];
+    }
+    hipLaunchKernelGGL(k_copy_r, dim3((unsigned)region_max_rows(reg, nf), nshots, nf), dim3(256), 0, st, G, C,
+                       sstride);
     return (int)hipGetLastError();
 }
 
