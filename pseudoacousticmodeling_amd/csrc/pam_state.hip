@@ -170,11 +170,21 @@ __global__ __launch_bounds__(256) void k_shot_maxabs_r(const FieldPtrs X, long l
     const int nr = region_rows(R), ng = region_groups(R);
     const float4 *xs = reinterpret_cast<const float4 *>(X.x[f] + (long long)s * sstride);
     float m = 0.0f;
-    for (int j = blockIdx.x; j < nr; j += gridDim.x) {
-        const float4 *xr = xs + (long long)region_row(R, j) * R.p4;
+    // four of the block's rows per pass, their loads issued together; a row
+    // past the region's end reads row j again (the max is unchanged by it)
+    const int G4 = gridDim.x;
+    for (int j = blockIdx.x; j < nr; j += 4 * G4) {
+        const float4 *xr[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xr[u] = xs + (long long)region_row(R, j + u * G4 < nr ? j + u * G4 : j) * R.p4;
         for (int k = threadIdx.x; k < ng; k += blockDim.x) {
-            const float4 v = xr[region_group(R, k)];
-            m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+            const int g = region_group(R, k);
+            float4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = xr[u][g];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                m = fmaxf(m, fmaxf(fmaxf(fabsf(v[u].x), fabsf(v[u].y)), fmaxf(fabsf(v[u].z), fabsf(v[u].w))));
         }
     }
 #pragma unroll
