@@ -122,6 +122,16 @@ __global__ __launch_bounds__(256) void k_exact_mu(const ExactArgs a, const Strip
     const long long c = (long long)iz * a.P + ix;
     const long long so = (long long)s * a.sstride;
     const float *X = a.X + so;
+    // u_of at a neighbour cc of a strip cell (inside the field: a strip cell is
+    // at least 4 cells from every edge), zero where the neighbour is off the
+    // strip; the loads are unconditional, so all neighbours' loads are issued
+    // together
+    auto w_at = [&](bool on, long long cc, const float *ca, const float *cb, const float *nu) {
+        const float v = a.v2[cc], x = X[cc], cav = ca[cc], cbv = cb[cc], nv = nu[cc];
+        const float q = v * x;
+        const float w = q + cbv * (q + cav * nv);
+        return on ? w : 0.0f;
+    };
     if (sx) {
         const float *nu = a.nux_in + so;
         float acc = 0.0f;
@@ -129,8 +139,8 @@ __global__ __launch_bounds__(256) void k_exact_mu(const ExactArgs a, const Strip
         for (int m = 1; m <= 4; ++m) {
             // w(c') = Sx(c') (q + b nu_k)(c'), nu_k = Sx (q + a nu_{k+1})
             const int xm = ix - m, xp = ix + m;
-            const float wm = xstrip(a, iz, xm) ? u_of(a, X, nu, a.cax, a.cbx, iz, xm, c - m, true) : 0.0f;
-            const float wp = xstrip(a, iz, xp) ? u_of(a, X, nu, a.cax, a.cbx, iz, xp, c + m, true) : 0.0f;
+            const float wm = w_at(xstrip(a, iz, xm), c - m, a.cax, a.cbx, nu);
+            const float wp = w_at(xstrip(a, iz, xp), c + m, a.cax, a.cbx, nu);
             acc += ka(m) * (wm - wp);
         }
         float *mu = a.mux + so;
@@ -143,8 +153,8 @@ __global__ __launch_bounds__(256) void k_exact_mu(const ExactArgs a, const Strip
 #pragma unroll
         for (int m = 1; m <= 4; ++m) {
             const int zm = iz - m, zp = iz + m;
-            const float wm = zstrip(a, zm, ix) ? u_of(a, X, nu, a.caz, a.cbz, zm, ix, c - m * P, true) : 0.0f;
-            const float wp = zstrip(a, zp, ix) ? u_of(a, X, nu, a.caz, a.cbz, zp, ix, c + m * P, true) : 0.0f;
+            const float wm = w_at(zstrip(a, zm, ix), c - m * P, a.caz, a.cbz, nu);
+            const float wp = w_at(zstrip(a, zp, ix), c + m * P, a.caz, a.cbz, nu);
             acc += ka(m) * (wm - wp);
         }
         float *mu = a.muz + so;
