@@ -375,13 +375,43 @@ __global__ __launch_bounds__(256) void k_exact_lambda_cpml(const ExactArgs a) {
         const int l = (lz + 4) * W + lx + 4, lxr = lz * W + lx + 4, lzr = (lz + 4) * 64 + lx;
         const float qxx = d2l(sC, l, 1) * a.idx2, qzz = d2l(sC, l, W) * a.idz2;
         const bool sxs = xstrip(a, iz, ix), szs = zstrip(a, iz, ix);
+        // a strip cell's own inputs, each strip's in one block of loads (one
+        // memory round instead of one for nu_k and one for the gradient terms)
+        float xca = 0.0f, xcb = 0.0f, xnu = 0.0f, xdp = 0.0f, xze = 0.0f, xpm = 0.0f, xzm = 0.0f, xga = 0.0f,
+              xgb = 0.0f, xmu = 0.0f;
+        float zca = 0.0f, zcb = 0.0f, znu = 0.0f, zdp = 0.0f, zze = 0.0f, zpm = 0.0f, zzm = 0.0f, zga = 0.0f,
+              zgb = 0.0f, zmu = 0.0f;
+        if (sxs) {
+            xca = a.cax[c];
+            xcb = a.cbx[c];
+            xnu = nxi[c];
+            xze = a.zex[fo + c];
+            xpm = a.psxm ? a.psxm[fo + c] : 0.0f;
+            xzm = a.zexm ? a.zexm[fo + c] : 0.0f;
+            xga = a.gax[so + c];
+            xgb = a.gbx[so + c];
+            xmu = mux[c];
+            xdp = d1s(a.psx + fo, c, 1) * a.idx;
+        }
+        if (szs) {
+            zca = a.caz[c];
+            zcb = a.cbz[c];
+            znu = nzi[c];
+            zze = a.zez[fo + c];
+            zpm = a.pszm ? a.pszm[fo + c] : 0.0f;
+            zzm = a.zezm ? a.zezm[fo + c] : 0.0f;
+            zga = a.gaz[so + c];
+            zgb = a.gbz[so + c];
+            zmu = muz[c];
+            zdp = d1s(a.psz + fo, c, P) * a.idz;
+        }
         const float phi = pX[rr];
         const float q = pV[rr] * phi;
-        const float nx_k = sxs ? q + a.cax[c] * nxi[c] : 0.0f;
-        const float nz_k = szs ? q + a.caz[c] * nzi[c] : 0.0f;
+        const float nx_k = sxs ? q + xca * xnu : 0.0f;
+        const float nz_k = szs ? q + zca * znu : 0.0f;
         // off a strip nu is zero: q alone (without the zero b nu term), and the nu
         // buffers keep the zeros they were cleared to
-        float sx = C0 * (sxs ? q + a.cbx[c] * nx_k : q), sz = C0 * (szs ? q + a.cbz[c] * nz_k : q);
+        float sx = C0 * (sxs ? q + xcb * nx_k : q), sz = C0 * (szs ? q + zcb * nz_k : q);
         float tx = 0.0f, tz = 0.0f;
         const bool nearx = ix < a.N + 4 || ix >= a.nxp - a.N - 4;
         const bool nearz = iz < a.N + 4 || iz >= a.nzp - a.N - 4;
@@ -398,20 +428,14 @@ __global__ __launch_bounds__(256) void k_exact_lambda_cpml(const ExactArgs a) {
         if (szs) a.nuz_out[so + c] = nz_k;
         float Q = qxx + qzz;
         if (sxs) {
-            const float *px = a.psx + fo;
-            const float dpx = d1s(px, c, 1) * a.idx;
-            Q += dpx + a.zex[fo + c];
-            const float pm = a.psxm ? a.psxm[fo + c] : 0.0f, zm = a.zexm ? a.zexm[fo + c] : 0.0f;
-            a.gax[so + c] += mux[c] * pm + nx_k * zm;
-            a.gbx[so + c] += mux[c] * (d1l(sC, l, 1) * a.idx) + nx_k * (qxx + dpx);
+            Q += xdp + xze;
+            a.gax[so + c] = xga + (xmu * xpm + nx_k * xzm);
+            a.gbx[so + c] = xgb + (xmu * (d1l(sC, l, 1) * a.idx) + nx_k * (qxx + xdp));
         }
         if (szs) {
-            const float *pz = a.psz + fo;
-            const float dpz = d1s(pz, c, P) * a.idz;
-            Q += dpz + a.zez[fo + c];
-            const float pm = a.pszm ? a.pszm[fo + c] : 0.0f, zm = a.zezm ? a.zezm[fo + c] : 0.0f;
-            a.gaz[so + c] += muz[c] * pm + nz_k * zm;
-            a.gbz[so + c] += muz[c] * (d1l(sC, l, W) * a.idz) + nz_k * (qzz + dpz);
+            Q += zdp + zze;
+            a.gaz[so + c] = zga + (zmu * zpm + nz_k * zzm);
+            a.gbz[so + c] = zgb + (zmu * (d1l(sC, l, W) * a.idz) + nz_k * (qzz + zdp));
         }
         a.gV[so + c] = pG[rr] + phi * Q;
     }
