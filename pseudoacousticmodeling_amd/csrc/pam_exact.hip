@@ -28,9 +28,11 @@
 // the live cells.  The saved forward state of step k (C_k, psi_k, zeta_k) is
 // read back from the snapshot buffers.
 //
-// One thread per padded cell (blocks of 64 x 4 cells), shots along
-// blockIdx.z; the neighbours come through the L1 / L2 caches.  This pass is the differentiable path, not the
-// throughput path.
+// Shots run along the last grid dimension.  The Cerjan step and the mu step
+// take one thread per padded cell (blocks of 64 x 4 cells; the mu step only
+// over the tiles that hold strip cells), their neighbours coming through the
+// L1 / L2 caches; the CPML lambda step stages 64 x 16 tiles with their halo
+// in LDS (k_exact_lambda_cpml).
 #include <hip/hip_runtime.h>
 
 #include "pam_internal.h"
